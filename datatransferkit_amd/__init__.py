@@ -1,0 +1,11 @@
+"""datatransferkit_amd: MI355X-native meshfree moving-least-squares transfer.
+
+A from-scratch HIP implementation of DataTransferKit's
+MovingLeastSquaresOperator path (kNN search, fused MLS setup with the
+Jacobi pseudo-inverse, CRS apply and transpose) behind a C ABI
+(``include/dtkx/mls.h``), with this Python mirror of the operator surface.
+"""
+from ._lib import DtkxError  # noqa: F401
+from .operators import MovingLeastSquaresOperator, knn, svd_pseudo_inverse  # noqa: F401
+
+__all__ = ["MovingLeastSquaresOperator", "knn", "svd_pseudo_inverse", "DtkxError"]

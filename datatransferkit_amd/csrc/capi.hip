@@ -1,0 +1,672 @@
+// C ABI of the dtkx MLS transfer path (include/dtkx/mls.h).
+//
+// dtkx_mls_create mirrors the MovingLeastSquaresOperator constructor
+// (DTK_MovingLeastSquaresOperator_def.hpp:24-119) and dtkx_mls_apply its
+// apply (:121-141). Preconditions the reference checks with DTK_REQUIRE /
+// DTK_CHECK (dimension 3 :39-42, non-empty source tree :49, value sizes
+// :129-130) are always checked here and reported as status codes.
+#include "apply.hpp"
+#include "common.hpp"
+#include "knn.hpp"
+#include "setup.hpp"
+
+#include <cstring>
+#include <memory>
+#include <string>
+
+struct dtkx_mls
+{
+    dtkx_mls_options opts{};
+    int64_t n_src = 0, n_tgt = 0;
+    int m = 0, kk = 0;
+    hipStream_t stream = nullptr;
+    dtkx::DeviceBuffer<int32_t> offsets, indices, order;
+    dtkx::DeviceBuffer<double> coeffs, a0;
+    dtkx::DeviceBuffer<float> coeffs32;
+    dtkx::DeviceBuffer<double4> src_d, tgt_d;
+    mutable dtkx::CscIndex csc;
+    mutable bool csc_built = false;
+    dtkx_mls_stats stats{};
+};
+
+namespace
+{
+
+thread_local std::string g_last_error;
+
+int fail( int status, std::string const &msg )
+{
+    g_last_error = msg;
+    return status;
+}
+
+template <typename F>
+int guarded( F &&f )
+{
+    try
+    {
+        g_last_error.clear();
+        return f();
+    }
+    catch ( dtkx::Error const &e )
+    {
+        return fail( e.status, e.what() );
+    }
+    catch ( std::bad_alloc const & )
+    {
+        return fail( DTKX_ERR_OUT_OF_MEMORY, "dtkx: host allocation failed" );
+    }
+    catch ( std::exception const &e )
+    {
+        return fail( DTKX_ERR_INVALID_ARGUMENT, e.what() );
+    }
+}
+
+#define REQ( c, st ) DTKX_REQUIRE( c, st )
+
+void validate_options( dtkx_mls_options const &o )
+{
+    REQ( o.dim == 3, DTKX_ERR_DIMENSION );
+    REQ( o.basis == DTKX_BASIS_CONSTANT || o.basis == DTKX_BASIS_LINEAR ||
+             o.basis == DTKX_BASIS_QUADRATIC,
+         DTKX_ERR_INVALID_ARGUMENT );
+    REQ( o.rbf >= DTKX_RBF_WENDLAND0 && o.rbf <= DTKX_RBF_BUHMANN4,
+         DTKX_ERR_INVALID_ARGUMENT );
+    REQ( o.k >= 0, DTKX_ERR_INVALID_ARGUMENT );
+    REQ( o.k <= dtkx::knn_max_k, DTKX_ERR_UNSUPPORTED );
+    REQ( o.precision == DTKX_PRECISION_FP64 ||
+             o.precision == DTKX_PRECISION_FP32_BULK,
+         DTKX_ERR_INVALID_ARGUMENT );
+    REQ( o.geometry == DTKX_GEOMETRY_FLOAT ||
+             o.geometry == DTKX_GEOMETRY_DOUBLE,
+         DTKX_ERR_INVALID_ARGUMENT );
+    REQ( o.solver == DTKX_SOLVER_CLASSICAL_JACOBI ||
+             o.solver == DTKX_SOLVER_CYCLIC_JACOBI,
+         DTKX_ERR_INVALID_ARGUMENT );
+    REQ( o.truncation == DTKX_TRUNCATION_ABSOLUTE ||
+             o.truncation == DTKX_TRUNCATION_RELATIVE,
+         DTKX_ERR_INVALID_ARGUMENT );
+    REQ( o.layout == DTKX_LAYOUT_AOS || o.layout == DTKX_LAYOUT_SOA,
+         DTKX_ERR_INVALID_ARGUMENT );
+    REQ( o.group_size >= 0 && o.group_size <= 16, DTKX_ERR_INVALID_ARGUMENT );
+    REQ( o.max_rotations >= 0, DTKX_ERR_INVALID_ARGUMENT );
+}
+
+// Device view of a caller array: used in place if it is device memory,
+// otherwise copied into a staging buffer.
+template <typename T>
+struct Staged
+{
+    dtkx::DeviceBuffer<T> buf;
+    T const *ptr = nullptr;
+    Staged( T const *p, std::size_t n, hipStream_t s )
+    {
+        if ( n == 0 || !p )
+        {
+            ptr = p;
+            return;
+        }
+        if ( dtkx::is_device_pointer( p ) )
+        {
+            ptr = p;
+            return;
+        }
+        buf.resize( n );
+        DTKX_HIP( hipMemcpyAsync( buf.get(), p, n * sizeof( T ),
+                                  hipMemcpyHostToDevice, s ) );
+        ptr = buf.get();
+    }
+};
+
+// Output view: written in place if device memory, else through a staging
+// buffer copied back by finish().
+template <typename T>
+struct StagedOut
+{
+    dtkx::DeviceBuffer<T> buf;
+    T *user = nullptr;
+    T *ptr = nullptr;
+    std::size_t n = 0;
+    StagedOut( T *p, std::size_t count )
+        : user( p )
+        , n( count )
+    {
+        if ( n == 0 || !p || dtkx::is_device_pointer( p ) )
+        {
+            ptr = p;
+            return;
+        }
+        buf.resize( n );
+        ptr = buf.get();
+    }
+    void finish( hipStream_t s )
+    {
+        if ( buf.get() )
+            DTKX_HIP( hipMemcpyAsync( user, buf.get(), n * sizeof( T ),
+                                      hipMemcpyDeviceToHost, s ) );
+    }
+};
+
+long default_rotation_cap( int m ) { return 200L * m * m + 1000; }
+
+__global__ void offsets_kernel( int32_t *off, int64_t n, int kk )
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if ( i <= n )
+        off[i] = (int32_t)( i * kk );
+}
+
+__global__ void to_f32_kernel( double const *a, float *b, int64_t n )
+{
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if ( i < n )
+        b[i] = (float)a[i];
+}
+
+struct EventTimer
+{
+    hipEvent_t a = nullptr, b = nullptr;
+    hipStream_t s;
+    explicit EventTimer( hipStream_t st )
+        : s( st )
+    {
+        DTKX_HIP( hipEventCreate( &a ) );
+        DTKX_HIP( hipEventCreate( &b ) );
+        DTKX_HIP( hipEventRecord( a, s ) );
+    }
+    double stop_ms()
+    {
+        DTKX_HIP( hipEventRecord( b, s ) );
+        DTKX_HIP( hipEventSynchronize( b ) );
+        float ms = 0;
+        DTKX_HIP( hipEventElapsedTime( &ms, a, b ) );
+        return ms;
+    }
+    ~EventTimer()
+    {
+        if ( a )
+            (void)hipEventDestroy( a );
+        if ( b )
+            (void)hipEventDestroy( b );
+    }
+};
+
+// Shared tail of the two create paths: neighbours are in h->indices.
+void run_setup( dtkx_mls *h, unsigned long long *counters_d )
+{
+    dtkx::SetupParams p;
+    p.m = h->m;
+    p.kk = h->kk;
+    p.rbf = h->opts.rbf;
+    p.geometry = h->opts.geometry;
+    p.truncation = h->opts.truncation;
+    p.rcond = h->opts.rcond;
+    p.max_rot = h->opts.max_rotations > 0 ? h->opts.max_rotations
+                                          : default_rotation_cap( h->m );
+    p.group_size = h->opts.group_size;
+    REQ( h->opts.solver == DTKX_SOLVER_CLASSICAL_JACOBI, DTKX_ERR_UNSUPPORTED );
+    if ( h->opts.store_pinv )
+        h->a0.resize( h->n_tgt * h->m );
+    int g = dtkx::setup_classical(
+        h->src_d.get(), h->tgt_d.get(), h->n_tgt, h->indices.get(),
+        h->order.get(), p, h->coeffs.get(), h->a0.get(), counters_d,
+        h->stream );
+    h->stats.group_size_used = g;
+}
+
+void finish_create( dtkx_mls *h, unsigned long long *counters_d )
+{
+    int64_t const nnz = h->n_tgt * h->kk;
+    h->offsets.resize( h->n_tgt + 1 );
+    offsets_kernel<<<dtkx::ceil_div( h->n_tgt + 1, 256 ), 256, 0,
+                     h->stream>>>( h->offsets.get(), h->n_tgt, h->kk );
+    DTKX_HIP( hipGetLastError() );
+    if ( h->opts.precision == DTKX_PRECISION_FP32_BULK && nnz > 0 )
+    {
+        h->coeffs32.resize( nnz );
+        to_f32_kernel<<<dtkx::ceil_div( nnz, 256 ), 256, 0, h->stream>>>(
+            h->coeffs.get(), h->coeffs32.get(), nnz );
+        DTKX_HIP( hipGetLastError() );
+    }
+    unsigned long long c[3] = {0, 0, 0};
+    DTKX_HIP( hipMemcpyAsync( c, counters_d, sizeof( c ),
+                              hipMemcpyDeviceToHost, h->stream ) );
+    DTKX_HIP( hipStreamSynchronize( h->stream ) );
+    h->stats.n_source = h->n_src;
+    h->stats.n_target = h->n_tgt;
+    h->stats.nnz = nnz;
+    h->stats.basis_size = h->m;
+    h->stats.k = h->kk;
+    h->stats.n_underdetermined = (int64_t)c[0];
+    h->stats.n_unconverged = (int64_t)c[1];
+    h->stats.total_rotations = (int64_t)c[2];
+}
+
+std::unique_ptr<dtkx_mls> make_handle( double const *src, int64_t n_src,
+                                       double const *tgt, int64_t n_tgt,
+                                       dtkx_mls_options const *opts )
+{
+    REQ( opts != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+    validate_options( *opts );
+    REQ( n_src >= 0 && n_tgt >= 0, DTKX_ERR_INVALID_ARGUMENT );
+    REQ( n_src > 0, DTKX_ERR_EMPTY_SOURCE );
+    REQ( src != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+    REQ( n_tgt == 0 || tgt != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+    REQ( n_src < INT32_MAX && n_tgt < INT32_MAX, DTKX_ERR_UNSUPPORTED );
+    auto h = std::make_unique<dtkx_mls>();
+    h->opts = *opts;
+    h->n_src = n_src;
+    h->n_tgt = n_tgt;
+    h->m = opts->basis;
+    h->stream = static_cast<hipStream_t>( opts->stream );
+    int const k = opts->k > 0 ? opts->k : h->m;
+    h->kk = static_cast<int>( k < n_src ? k : n_src );
+    REQ( (int64_t)h->kk * n_tgt < INT32_MAX, DTKX_ERR_UNSUPPORTED );
+    Staged<double> s_src( src, 3 * n_src, h->stream );
+    Staged<double> s_tgt( tgt, 3 * n_tgt, h->stream );
+    h->src_d.resize( n_src );
+    h->tgt_d.resize( std::max<int64_t>( n_tgt, 1 ) );
+    dtkx::points_to_double4( s_src.ptr, n_src, opts->layout, h->src_d.get(),
+                             h->stream );
+    dtkx::points_to_double4( s_tgt.ptr, n_tgt, opts->layout, h->tgt_d.get(),
+                             h->stream );
+    h->indices.resize( std::max<int64_t>( n_tgt * h->kk, 1 ) );
+    h->coeffs.resize( std::max<int64_t>( n_tgt * h->kk, 1 ) );
+    h->order.resize( std::max<int64_t>( n_tgt, 1 ) );
+    // the staging buffers die at scope exit
+    DTKX_HIP( hipStreamSynchronize( h->stream ) );
+    return h;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *dtkx_last_error( void ) { return g_last_error.c_str(); }
+
+const char *dtkx_version( void ) { return "dtkx 0.1 (gfx950)"; }
+
+int dtkx_mls_options_init( dtkx_mls_options *o )
+{
+    if ( !o )
+        return fail( DTKX_ERR_INVALID_ARGUMENT, "dtkx: opts != nullptr" );
+    std::memset( o, 0, sizeof( *o ) );
+    o->dim = 3;
+    o->basis = DTKX_BASIS_LINEAR;
+    o->rbf = DTKX_RBF_WENDLAND0;
+    o->k = 0;
+    o->precision = DTKX_PRECISION_FP64;
+    o->geometry = DTKX_GEOMETRY_FLOAT;
+    o->solver = DTKX_SOLVER_CLASSICAL_JACOBI;
+    o->truncation = DTKX_TRUNCATION_ABSOLUTE;
+    o->rcond = 1e-12;
+    o->max_rotations = 0;
+    o->layout = DTKX_LAYOUT_AOS;
+    o->group_size = 0;
+    o->store_pinv = 0;
+    o->stream = nullptr;
+    return DTKX_SUCCESS;
+}
+
+int dtkx_mls_create( const double *src_xyz, int64_t n_src,
+                     const double *tgt_xyz, int64_t n_tgt,
+                     const dtkx_mls_options *opts, dtkx_mls **out )
+{
+    return guarded( [&]() {
+        REQ( out != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        *out = nullptr;
+        auto h = make_handle( src_xyz, n_src, tgt_xyz, n_tgt, opts );
+        dtkx::DeviceBuffer<unsigned long long> counters( 3 );
+        DTKX_HIP( hipMemsetAsync( counters.get(), 0, 3 * sizeof( uint64_t ),
+                                  h->stream ) );
+        if ( n_tgt > 0 )
+        {
+            EventTimer tk( h->stream );
+            dtkx::KnnInfo info;
+            int const k = opts->k > 0 ? opts->k : h->m;
+            if ( opts->geometry == DTKX_GEOMETRY_FLOAT )
+            {
+                dtkx::DeviceBuffer<float4> sf( n_src ), tf( n_tgt );
+                // float points are rounded from the caller's doubles
+                dtkx::DeviceBuffer<double> tmp;
+                Staged<double> s_src( src_xyz, 3 * n_src, h->stream );
+                Staged<double> s_tgt( tgt_xyz, 3 * n_tgt, h->stream );
+                dtkx::points_to_float4( s_src.ptr, n_src, opts->layout,
+                                        sf.get(), h->stream );
+                dtkx::points_to_float4( s_tgt.ptr, n_tgt, opts->layout,
+                                        tf.get(), h->stream );
+                dtkx::knn_search( opts->geometry, sf.get(), n_src, tf.get(),
+                                  n_tgt, k, h->indices.get(), nullptr,
+                                  h->order.get(), &info, h->stream );
+            }
+            else
+                dtkx::knn_search( opts->geometry, h->src_d.get(), n_src,
+                                  h->tgt_d.get(), n_tgt, k, h->indices.get(),
+                                  nullptr, h->order.get(), &info, h->stream );
+            h->stats.ms_knn = tk.stop_ms();
+            h->stats.tree_height = info.tree_height;
+            EventTimer ts( h->stream );
+            run_setup( h.get(), counters.get() );
+            h->stats.ms_setup = ts.stop_ms();
+        }
+        finish_create( h.get(), counters.get() );
+        *out = h.release();
+        return DTKX_SUCCESS;
+    } );
+}
+
+int dtkx_mls_create_with_neighbors( const double *src_xyz, int64_t n_src,
+                                    const double *tgt_xyz, int64_t n_tgt,
+                                    const int32_t *indices, int32_t k_row,
+                                    const dtkx_mls_options *opts,
+                                    dtkx_mls **out )
+{
+    return guarded( [&]() {
+        REQ( out != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        *out = nullptr;
+        REQ( k_row >= 1 && k_row <= 1024, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( n_tgt == 0 || indices != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        dtkx_mls_options o = *opts;
+        o.k = 0;
+        auto h = make_handle( src_xyz, n_src, tgt_xyz, n_tgt, &o );
+        h->opts.k = k_row;
+        h->kk = k_row;
+        h->indices.resize( std::max<int64_t>( n_tgt * k_row, 1 ) );
+        h->coeffs.resize( std::max<int64_t>( n_tgt * k_row, 1 ) );
+        h->order.release(); // natural target order
+        if ( n_tgt > 0 )
+        {
+            std::size_t const nn = (std::size_t)n_tgt * k_row;
+            // indices must lie in [0, n_src): checked on the host copy
+            std::unique_ptr<int32_t[]> host( new int32_t[nn] );
+            DTKX_HIP( hipMemcpy( host.get(), indices, nn * sizeof( int32_t ),
+                                 hipMemcpyDefault ) );
+            for ( std::size_t i = 0; i < nn; ++i )
+                REQ( host[i] >= 0 && host[i] < n_src,
+                     DTKX_ERR_INVALID_ARGUMENT );
+            DTKX_HIP( hipMemcpyAsync( h->indices.get(), host.get(),
+                                      nn * sizeof( int32_t ),
+                                      hipMemcpyHostToDevice, h->stream ) );
+            DTKX_HIP( hipStreamSynchronize( h->stream ) );
+        }
+        dtkx::DeviceBuffer<unsigned long long> counters( 3 );
+        DTKX_HIP( hipMemsetAsync( counters.get(), 0, 3 * sizeof( uint64_t ),
+                                  h->stream ) );
+        if ( n_tgt > 0 )
+        {
+            EventTimer ts( h->stream );
+            run_setup( h.get(), counters.get() );
+            h->stats.ms_setup = ts.stop_ms();
+        }
+        finish_create( h.get(), counters.get() );
+        *out = h.release();
+        return DTKX_SUCCESS;
+    } );
+}
+
+int dtkx_mls_apply( const dtkx_mls *h, const double *src_vals,
+                    double *tgt_vals, int32_t n_fields, int64_t ld_src,
+                    int64_t ld_tgt )
+{
+    return guarded( [&]() {
+        REQ( h != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( n_fields >= 1, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( ld_src >= h->n_src, DTKX_ERR_SIZE_MISMATCH );
+        REQ( ld_tgt >= h->n_tgt, DTKX_ERR_SIZE_MISMATCH );
+        REQ( src_vals != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( h->n_tgt == 0 || tgt_vals != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        if ( h->n_tgt == 0 )
+            return DTKX_SUCCESS;
+        std::size_t const nin = (std::size_t)( n_fields - 1 ) * ld_src + h->n_src;
+        std::size_t const nout = (std::size_t)( n_fields - 1 ) * ld_tgt + h->n_tgt;
+        Staged<double> x( src_vals, nin, h->stream );
+        StagedOut<double> y( tgt_vals, nout );
+        if ( h->opts.precision == DTKX_PRECISION_FP32_BULK )
+            dtkx::crs_apply_f32_dvals( h->indices.get(), h->coeffs32.get(),
+                                       h->n_tgt, h->kk, x.ptr, ld_src, y.ptr,
+                                       ld_tgt, n_fields, h->stream );
+        else
+            dtkx::crs_apply_f64( h->indices.get(), h->coeffs.get(), h->n_tgt,
+                                 h->kk, x.ptr, ld_src, y.ptr, ld_tgt, n_fields,
+                                 h->stream );
+        y.finish( h->stream );
+        DTKX_HIP( hipStreamSynchronize( h->stream ) );
+        return DTKX_SUCCESS;
+    } );
+}
+
+int dtkx_mls_apply_fp32( const dtkx_mls *h, const float *src_vals,
+                         float *tgt_vals, int32_t n_fields, int64_t ld_src,
+                         int64_t ld_tgt )
+{
+    return guarded( [&]() {
+        REQ( h != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( h->opts.precision == DTKX_PRECISION_FP32_BULK,
+             DTKX_ERR_UNSUPPORTED );
+        REQ( n_fields >= 1, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( ld_src >= h->n_src, DTKX_ERR_SIZE_MISMATCH );
+        REQ( ld_tgt >= h->n_tgt, DTKX_ERR_SIZE_MISMATCH );
+        REQ( src_vals != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( h->n_tgt == 0 || tgt_vals != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        if ( h->n_tgt == 0 )
+            return DTKX_SUCCESS;
+        std::size_t const nin = (std::size_t)( n_fields - 1 ) * ld_src + h->n_src;
+        std::size_t const nout = (std::size_t)( n_fields - 1 ) * ld_tgt + h->n_tgt;
+        Staged<float> x( src_vals, nin, h->stream );
+        StagedOut<float> y( tgt_vals, nout );
+        dtkx::crs_apply_f32( h->indices.get(), h->coeffs32.get(), h->n_tgt,
+                             h->kk, x.ptr, ld_src, y.ptr, ld_tgt, n_fields,
+                             h->stream );
+        y.finish( h->stream );
+        DTKX_HIP( hipStreamSynchronize( h->stream ) );
+        return DTKX_SUCCESS;
+    } );
+}
+
+int dtkx_mls_apply_transpose( const dtkx_mls *h, const double *tgt_bar,
+                              double *src_bar, int32_t n_fields,
+                              int64_t ld_tgt, int64_t ld_src )
+{
+    return guarded( [&]() {
+        REQ( h != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( n_fields >= 1, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( ld_src >= h->n_src, DTKX_ERR_SIZE_MISMATCH );
+        REQ( ld_tgt >= h->n_tgt, DTKX_ERR_SIZE_MISMATCH );
+        REQ( src_bar != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( h->n_tgt == 0 || tgt_bar != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( h->opts.precision == DTKX_PRECISION_FP64, DTKX_ERR_UNSUPPORTED );
+        if ( !h->csc_built )
+        {
+            dtkx::build_csc( h->indices.get(), h->n_tgt * h->kk, h->n_src,
+                             h->csc, h->stream );
+            h->csc_built = true;
+        }
+        std::size_t const nin = (std::size_t)( n_fields - 1 ) * ld_tgt + h->n_tgt;
+        std::size_t const nout = (std::size_t)( n_fields - 1 ) * ld_src + h->n_src;
+        Staged<double> yb( tgt_bar, nin, h->stream );
+        StagedOut<double> xb( src_bar, nout );
+        dtkx::crs_apply_transpose_f64( h->csc, h->coeffs.get(), h->kk,
+                                       h->n_src, yb.ptr, ld_tgt, xb.ptr,
+                                       ld_src, n_fields, h->stream );
+        xb.finish( h->stream );
+        DTKX_HIP( hipStreamSynchronize( h->stream ) );
+        return DTKX_SUCCESS;
+    } );
+}
+
+int dtkx_mls_target_gradient( const dtkx_mls *h, const double *src_vals,
+                              const double *tgt_bar, int32_t n_fields,
+                              int64_t ld_src, int64_t ld_tgt, double *d_tgt )
+{
+    (void)src_vals;
+    (void)tgt_bar;
+    (void)n_fields;
+    (void)ld_src;
+    (void)ld_tgt;
+    (void)d_tgt;
+    return guarded( [&]() -> int {
+        REQ( h != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        dtkx::raise( DTKX_ERR_UNSUPPORTED, "target gradient not built",
+                     __FILE__, __LINE__ );
+    } );
+}
+
+int dtkx_mls_get_graph( const dtkx_mls *h, const int32_t **offsets,
+                        const int32_t **indices, const void **coeffs,
+                        int64_t *nnz )
+{
+    return guarded( [&]() {
+        REQ( h != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        if ( offsets )
+            *offsets = h->offsets.get();
+        if ( indices )
+            *indices = h->indices.get();
+        if ( coeffs )
+            *coeffs = h->opts.precision == DTKX_PRECISION_FP32_BULK
+                          ? static_cast<void const *>( h->coeffs32.get() )
+                          : static_cast<void const *>( h->coeffs.get() );
+        if ( nnz )
+            *nnz = h->n_tgt * h->kk;
+        return DTKX_SUCCESS;
+    } );
+}
+
+int dtkx_mls_get_stats( const dtkx_mls *h, dtkx_mls_stats *stats )
+{
+    return guarded( [&]() {
+        REQ( h != nullptr && stats != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        *stats = h->stats;
+        return DTKX_SUCCESS;
+    } );
+}
+
+int dtkx_mls_destroy( dtkx_mls *h )
+{
+    return guarded( [&]() {
+        delete h;
+        return DTKX_SUCCESS;
+    } );
+}
+
+int dtkx_knn( const double *src_xyz, int64_t n_src, const double *tgt_xyz,
+              int64_t n_tgt, int32_t k, int32_t geometry, int32_t layout,
+              int32_t *indices, double *dist, void *stream )
+{
+    return guarded( [&]() {
+        hipStream_t s = static_cast<hipStream_t>( stream );
+        REQ( n_src > 0, DTKX_ERR_EMPTY_SOURCE );
+        REQ( n_tgt >= 0 && src_xyz != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( n_tgt == 0 || ( tgt_xyz != nullptr && indices != nullptr ),
+             DTKX_ERR_INVALID_ARGUMENT );
+        REQ( k >= 1 && k <= dtkx::knn_max_k, DTKX_ERR_UNSUPPORTED );
+        REQ( geometry == DTKX_GEOMETRY_FLOAT ||
+                 geometry == DTKX_GEOMETRY_DOUBLE,
+             DTKX_ERR_INVALID_ARGUMENT );
+        REQ( layout == DTKX_LAYOUT_AOS || layout == DTKX_LAYOUT_SOA,
+             DTKX_ERR_INVALID_ARGUMENT );
+        if ( n_tgt == 0 )
+            return DTKX_SUCCESS;
+        int const kk = k < n_src ? k : static_cast<int>( n_src );
+        Staged<double> sx( src_xyz, 3 * n_src, s );
+        Staged<double> tx( tgt_xyz, 3 * n_tgt, s );
+        StagedOut<int32_t> oi( indices, (std::size_t)n_tgt * kk );
+        StagedOut<double> od( dist, dist ? (std::size_t)n_tgt * kk : 0 );
+        if ( geometry == DTKX_GEOMETRY_FLOAT )
+        {
+            dtkx::DeviceBuffer<float4> sf( n_src ), tf( n_tgt );
+            dtkx::points_to_float4( sx.ptr, n_src, layout, sf.get(), s );
+            dtkx::points_to_float4( tx.ptr, n_tgt, layout, tf.get(), s );
+            dtkx::knn_search( geometry, sf.get(), n_src, tf.get(), n_tgt, k,
+                              oi.ptr, od.ptr, nullptr, nullptr, s );
+        }
+        else
+        {
+            dtkx::DeviceBuffer<double4> sd( n_src ), td( n_tgt );
+            dtkx::points_to_double4( sx.ptr, n_src, layout, sd.get(), s );
+            dtkx::points_to_double4( tx.ptr, n_tgt, layout, td.get(), s );
+            dtkx::knn_search( geometry, sd.get(), n_src, td.get(), n_tgt, k,
+                              oi.ptr, od.ptr, nullptr, nullptr, s );
+        }
+        oi.finish( s );
+        od.finish( s );
+        DTKX_HIP( hipStreamSynchronize( s ) );
+        return DTKX_SUCCESS;
+    } );
+}
+
+int dtkx_svd_pseudo_inverse( int32_t n, int64_t n_matrices, const double *a,
+                             double *pinv, int64_t *n_underdetermined,
+                             int64_t *n_unconverged, int32_t max_rotations,
+                             void *stream )
+{
+    return guarded( [&]() {
+        hipStream_t s = static_cast<hipStream_t>( stream );
+        REQ( n >= 1, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( n <= 52, DTKX_ERR_UNSUPPORTED );
+        REQ( n_matrices >= 0, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( n_matrices == 0 || ( a != nullptr && pinv != nullptr ),
+             DTKX_ERR_INVALID_ARGUMENT );
+        REQ( max_rotations >= 0, DTKX_ERR_INVALID_ARGUMENT );
+        std::size_t const cnt = (std::size_t)n_matrices * n * n;
+        Staged<double> sa( a, cnt, s );
+        StagedOut<double> sp( pinv, cnt );
+        dtkx::DeviceBuffer<unsigned long long> counters( 3 );
+        DTKX_HIP( hipMemsetAsync( counters.get(), 0, 3 * sizeof( uint64_t ), s ) );
+        long const cap =
+            max_rotations > 0 ? max_rotations : default_rotation_cap( n );
+        dtkx::svd_pinv_batched( n, n_matrices, sa.ptr, sp.ptr, cap,
+                                counters.get(), s );
+        sp.finish( s );
+        unsigned long long c[3];
+        DTKX_HIP( hipMemcpyAsync( c, counters.get(), sizeof( c ),
+                                  hipMemcpyDeviceToHost, s ) );
+        DTKX_HIP( hipStreamSynchronize( s ) );
+        if ( n_underdetermined )
+            *n_underdetermined = (int64_t)c[0];
+        if ( n_unconverged )
+            *n_unconverged = (int64_t)c[1];
+        return DTKX_SUCCESS;
+    } );
+}
+
+// ------------------------------------------------------------- batched
+struct dtkx_mls_batch
+{
+    int unused;
+};
+
+int dtkx_mls_batch_create( int32_t n_problems, const dtkx_mls_problem *problems,
+                           const dtkx_mls_options *opts, dtkx_mls_batch **out )
+{
+    (void)n_problems;
+    (void)problems;
+    (void)opts;
+    if ( out )
+        *out = nullptr;
+    return fail( DTKX_ERR_UNSUPPORTED, "dtkx: batched operators not built" );
+}
+
+int dtkx_mls_batch_apply( const dtkx_mls_batch *b,
+                          const double *const *src_vals,
+                          double *const *tgt_vals )
+{
+    (void)b;
+    (void)src_vals;
+    (void)tgt_vals;
+    return fail( DTKX_ERR_UNSUPPORTED, "dtkx: batched operators not built" );
+}
+
+int dtkx_mls_batch_get_stats( const dtkx_mls_batch *b, dtkx_mls_stats *stats )
+{
+    (void)b;
+    (void)stats;
+    return fail( DTKX_ERR_UNSUPPORTED, "dtkx: batched operators not built" );
+}
+
+int dtkx_mls_batch_destroy( dtkx_mls_batch *b )
+{
+    delete b;
+    return DTKX_SUCCESS;
+}
+
+} // extern "C"

@@ -1,0 +1,122 @@
+// Shared host/device utilities of the dtkx HIP library: error reporting in the
+// reference's DBC message shape, HIP call checking, device buffers, and the
+// point formats used inside the library.
+#ifndef DTKX_COMMON_HPP
+#define DTKX_COMMON_HPP
+
+#include "dtkx/mls.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace dtkx
+{
+
+// Error raised inside the library; translated to a dtkx_status at the C ABI.
+// The message follows the reference's DataTransferKitException output
+// ("<cond>, failed in <file>, line <n>.", packages/Utils/src/DTK_DBC.cpp:40-48).
+struct Error : std::runtime_error
+{
+    int status;
+    Error( int s, std::string const &msg )
+        : std::runtime_error( msg )
+        , status( s )
+    {
+    }
+};
+
+[[noreturn]] void raise( int status, char const *cond, char const *file,
+                         int line );
+
+#define DTKX_REQUIRE( c, status )                                              \
+    do                                                                         \
+    {                                                                          \
+        if ( !( c ) )                                                          \
+            ::dtkx::raise( status, #c, __FILE__, __LINE__ );                   \
+    } while ( 0 )
+
+#define DTKX_HIP( call )                                                       \
+    do                                                                         \
+    {                                                                          \
+        hipError_t e_ = ( call );                                              \
+        if ( e_ != hipSuccess )                                                \
+            ::dtkx::raise( DTKX_ERR_HIP, hipGetErrorString( e_ ), __FILE__,    \
+                           __LINE__ );                                         \
+    } while ( 0 )
+
+// Owning device buffer (hipMalloc/hipFree), move-only.
+template <typename T>
+struct DeviceBuffer
+{
+    T *ptr = nullptr;
+    std::size_t n = 0;
+    DeviceBuffer() = default;
+    explicit DeviceBuffer( std::size_t count ) { resize( count ); }
+    DeviceBuffer( DeviceBuffer const & ) = delete;
+    DeviceBuffer &operator=( DeviceBuffer const & ) = delete;
+    DeviceBuffer( DeviceBuffer &&o ) noexcept
+        : ptr( o.ptr )
+        , n( o.n )
+    {
+        o.ptr = nullptr;
+        o.n = 0;
+    }
+    DeviceBuffer &operator=( DeviceBuffer &&o ) noexcept
+    {
+        if ( this != &o )
+        {
+            release();
+            ptr = o.ptr;
+            n = o.n;
+            o.ptr = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+    ~DeviceBuffer() { release(); }
+    void resize( std::size_t count )
+    {
+        release();
+        if ( count > 0 )
+        {
+            hipError_t e = hipMalloc( &ptr, count * sizeof( T ) );
+            if ( e != hipSuccess )
+            {
+                ptr = nullptr;
+                ::dtkx::raise( DTKX_ERR_OUT_OF_MEMORY, "hipMalloc", __FILE__,
+                               __LINE__ );
+            }
+        }
+        n = count;
+    }
+    void release()
+    {
+        if ( ptr )
+            (void)hipFree( ptr );
+        ptr = nullptr;
+        n = 0;
+    }
+    T *get() const { return ptr; }
+};
+
+// True if p is device-accessible memory (hipMalloc, managed, or registered).
+bool is_device_pointer( void const *p );
+
+// Internal point formats: padded to 16/32 bytes for single-load gathers.
+// float4 for float geometry (kNN), double4 for double geometry and setup.
+void points_to_float4( double const *xyz, int64_t n, int layout, float4 *out,
+                       hipStream_t s );
+void points_to_double4( double const *xyz, int64_t n, int layout,
+                        double4 *out, hipStream_t s );
+
+inline unsigned ceil_div( int64_t a, int64_t b )
+{
+    return static_cast<unsigned>( ( a + b - 1 ) / b );
+}
+
+} // namespace dtkx
+
+#endif

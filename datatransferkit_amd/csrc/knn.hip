@@ -1,0 +1,700 @@
+// LBVH k-nearest-neighbour search for gfx950.
+//
+// Build: scene box -> 63-bit Morton codes (21 bits per axis) -> radix sort
+// of (code, index) -> Karras hierarchy (duplicate codes split by index) ->
+// one bottom-up pass that writes both child boxes into each parent and the
+// subtree heights (the second thread to reach a node continues upward).
+// Query: one lane per target, targets processed in Morton order so a wave
+// walks a compact region; a per-lane stack in LDS; the k best candidates
+// stay sorted in registers as 64-bit keys (float distance bits << 32 |
+// index), so the ordering is (distance, index) with one integer compare.
+// Pruning uses squared distances against a threshold slightly above the
+// square of the current k-th distance, which never drops a candidate that
+// could tie or beat it; the kept set is therefore exactly the k smallest
+// (distance, index) pairs, as brute force gives.
+#include "knn.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+#include <cfloat>
+#include <cstring>
+#include <vector>
+
+namespace dtkx
+{
+namespace
+{
+
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<float>
+{
+    using type = float4;
+};
+template <>
+struct Vec4<double>
+{
+    using type = double4;
+};
+
+// Child code c >= 0: internal node c; c < 0: leaf holding source index ~c.
+template <typename T>
+struct alignas( 16 ) Node
+{
+    T lo[2][3];
+    T hi[2][3];
+    int32_t child[2];
+    int32_t height[2];
+};
+
+constexpr int kBlock = 256;
+
+// ------------------------------------------------------------- scene box
+template <typename T, typename V>
+__global__ void __launch_bounds__( kBlock )
+    bbox_partial( V const *pts, int64_t n, T *partial )
+{
+    T lo[3] = {INFINITY, INFINITY, INFINITY};
+    T hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for ( int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
+          i += (int64_t)gridDim.x * kBlock )
+    {
+        V p = pts[i];
+        lo[0] = fmin( lo[0], p.x );
+        lo[1] = fmin( lo[1], p.y );
+        lo[2] = fmin( lo[2], p.z );
+        hi[0] = fmax( hi[0], p.x );
+        hi[1] = fmax( hi[1], p.y );
+        hi[2] = fmax( hi[2], p.z );
+    }
+    __shared__ T sh[6][kBlock];
+    for ( int d = 0; d < 3; ++d )
+    {
+        sh[d][threadIdx.x] = lo[d];
+        sh[3 + d][threadIdx.x] = hi[d];
+    }
+    __syncthreads();
+    for ( int off = kBlock / 2; off > 0; off >>= 1 )
+    {
+        if ( threadIdx.x < off )
+            for ( int d = 0; d < 3; ++d )
+            {
+                sh[d][threadIdx.x] =
+                    fmin( sh[d][threadIdx.x], sh[d][threadIdx.x + off] );
+                sh[3 + d][threadIdx.x] = fmax(
+                    sh[3 + d][threadIdx.x], sh[3 + d][threadIdx.x + off] );
+            }
+        __syncthreads();
+    }
+    if ( threadIdx.x == 0 )
+        for ( int d = 0; d < 6; ++d )
+            partial[blockIdx.x * 6 + d] = sh[d][0];
+}
+
+template <typename T>
+__global__ void __launch_bounds__( kBlock )
+    bbox_final( T const *partial, int nparts, T *box )
+{
+    __shared__ T sh[6][kBlock];
+    T v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+    for ( int i = threadIdx.x; i < nparts; i += kBlock )
+        for ( int d = 0; d < 3; ++d )
+        {
+            v[d] = fmin( v[d], partial[i * 6 + d] );
+            v[3 + d] = fmax( v[3 + d], partial[i * 6 + 3 + d] );
+        }
+    for ( int d = 0; d < 6; ++d )
+        sh[d][threadIdx.x] = v[d];
+    __syncthreads();
+    for ( int off = kBlock / 2; off > 0; off >>= 1 )
+    {
+        if ( threadIdx.x < off )
+            for ( int d = 0; d < 3; ++d )
+            {
+                sh[d][threadIdx.x] =
+                    fmin( sh[d][threadIdx.x], sh[d][threadIdx.x + off] );
+                sh[3 + d][threadIdx.x] = fmax(
+                    sh[3 + d][threadIdx.x], sh[3 + d][threadIdx.x + off] );
+            }
+        __syncthreads();
+    }
+    if ( threadIdx.x == 0 )
+        for ( int d = 0; d < 6; ++d )
+            box[d] = sh[d][0];
+}
+
+// ---------------------------------------------------------- Morton codes
+__device__ inline uint64_t expand21( uint32_t v )
+{
+    uint64_t x = v & 0x1fffffu;
+    x = ( x | x << 32 ) & 0x1f00000000ffffull;
+    x = ( x | x << 16 ) & 0x1f0000ff0000ffull;
+    x = ( x | x << 8 ) & 0x100f00f00f00f00full;
+    x = ( x | x << 4 ) & 0x10c30c30c30c30c3ull;
+    x = ( x | x << 2 ) & 0x1249249249249249ull;
+    return x;
+}
+
+template <typename T, typename V>
+__global__ void __launch_bounds__( kBlock )
+    morton_kernel( V const *pts, int64_t n, T const *box, uint64_t *codes,
+                   int32_t *ids )
+{
+    int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( i >= n )
+        return;
+    V p = pts[i];
+    double c[3] = {(double)p.x, (double)p.y, (double)p.z};
+    uint32_t q[3];
+    for ( int d = 0; d < 3; ++d )
+    {
+        double lo = box[d], ext = (double)box[3 + d] - lo;
+        double u = ext > 0 ? ( c[d] - lo ) / ext * 2097151.0 : 0.0;
+        u = u < 0 ? 0 : ( u > 2097151.0 ? 2097151.0 : u );
+        q[d] = (uint32_t)u;
+    }
+    codes[i] = ( expand21( q[0] ) << 2 ) | ( expand21( q[1] ) << 1 ) |
+               expand21( q[2] );
+    ids[i] = (int32_t)i;
+}
+
+// ------------------------------------------------------ Karras hierarchy
+// delta(i, j): common prefix length of keys i and j, with the index used
+// to break ties between equal codes; -1 outside [0, n).
+__device__ inline int prefix_delta( uint64_t const *codes, int64_t n,
+                                    int64_t i, int64_t j )
+{
+    if ( j < 0 || j >= n )
+        return -1;
+    uint64_t a = codes[i], b = codes[j];
+    if ( a != b )
+        return __clzll( a ^ b );
+    return 64 + __clz( (uint32_t)i ^ (uint32_t)j );
+}
+
+// parent encodings: (parent << 1) | slot
+template <typename T>
+__global__ void __launch_bounds__( kBlock )
+    karras_kernel( uint64_t const *codes, int64_t n, Node<T> *nodes,
+                   int32_t *parent_internal, int32_t *parent_leaf )
+{
+    int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( i >= n - 1 )
+        return;
+    int d = ( prefix_delta( codes, n, i, i + 1 ) -
+              prefix_delta( codes, n, i, i - 1 ) ) >= 0
+                ? 1
+                : -1;
+    int dmin = prefix_delta( codes, n, i, i - d );
+    int64_t lmax = 2;
+    while ( prefix_delta( codes, n, i, i + lmax * d ) > dmin )
+        lmax <<= 1;
+    int64_t l = 0;
+    for ( int64_t t = lmax >> 1; t >= 1; t >>= 1 )
+        if ( prefix_delta( codes, n, i, i + ( l + t ) * d ) > dmin )
+            l += t;
+    int64_t j = i + l * d;
+    int dnode = prefix_delta( codes, n, i, j );
+    int64_t s = 0, t = l;
+    do
+    {
+        t = ( t + 1 ) >> 1;
+        if ( prefix_delta( codes, n, i, i + ( s + t ) * d ) > dnode )
+            s += t;
+    } while ( t > 1 );
+    int64_t gamma = i + s * d + ( d < 0 ? -1 : 0 );
+    int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+    if ( lo == gamma )
+    {
+        nodes[i].child[0] = ~(int32_t)gamma; // leaf position (rewritten)
+        parent_leaf[gamma] = (int32_t)( i << 1 );
+    }
+    else
+    {
+        nodes[i].child[0] = (int32_t)gamma;
+        parent_internal[gamma] = (int32_t)( i << 1 );
+    }
+    if ( hi == gamma + 1 )
+    {
+        nodes[i].child[1] = ~(int32_t)( gamma + 1 );
+        parent_leaf[gamma + 1] = (int32_t)( ( i << 1 ) | 1 );
+    }
+    else
+    {
+        nodes[i].child[1] = (int32_t)( gamma + 1 );
+        parent_internal[gamma + 1] = (int32_t)( ( i << 1 ) | 1 );
+    }
+}
+
+// Bottom-up: each leaf thread writes its box into its parent's slot; the
+// second thread to arrive at a node forms the union and continues.
+template <typename T, typename V>
+__global__ void __launch_bounds__( kBlock )
+    refit_kernel( V const *pts, int32_t const *sorted_ids, int64_t n,
+                  Node<T> *nodes, int32_t const *parent_internal,
+                  int32_t const *parent_leaf, int32_t *flags, int32_t *height )
+{
+    int64_t p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( p >= n )
+        return;
+    int32_t orig = sorted_ids[p];
+    V pt = pts[orig];
+    T lo[3] = {pt.x, pt.y, pt.z};
+    T hi[3] = {pt.x, pt.y, pt.z};
+    int32_t code = ~orig;
+    int32_t h = 0;
+    int32_t enc = parent_leaf[p];
+    while ( true )
+    {
+        int32_t par = enc >> 1, slot = enc & 1;
+        Node<T> *nd = nodes + par;
+        for ( int d = 0; d < 3; ++d )
+        {
+            nd->lo[slot][d] = lo[d];
+            nd->hi[slot][d] = hi[d];
+        }
+        nd->child[slot] = code;
+        nd->height[slot] = h;
+        __threadfence();
+        if ( atomicAdd( flags + par, 1 ) == 0 )
+            return;
+        __threadfence();
+        int other = 1 - slot;
+        for ( int d = 0; d < 3; ++d )
+        {
+            T olo = __hip_atomic_load( &nd->lo[other][d], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT );
+            T ohi = __hip_atomic_load( &nd->hi[other][d], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT );
+            lo[d] = fmin( lo[d], olo );
+            hi[d] = fmax( hi[d], ohi );
+        }
+        int32_t oh = __hip_atomic_load( &nd->height[other], __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT );
+        h = 1 + ( h > oh ? h : oh );
+        code = par;
+        if ( par == 0 )
+        {
+            *height = h;
+            return;
+        }
+        enc = parent_internal[par];
+    }
+}
+
+// --------------------------------------------------------- k-best lists
+template <int K>
+struct KBestF
+{
+    uint64_t key[K];
+    __device__ void init()
+    {
+#pragma unroll
+        for ( int j = 0; j < K; ++j )
+            key[j] = ~0ull;
+    }
+    // squared-distance threshold above which nothing can enter
+    __device__ float threshold() const
+    {
+        if ( key[K - 1] == ~0ull )
+            return INFINITY;
+        float w = __uint_as_float( (uint32_t)( key[K - 1] >> 32 ) );
+        return w * w * 1.00000095367431640625f; // (1 + 2^-20)
+    }
+    __device__ void insert( float d, int32_t id )
+    {
+        uint64_t c = ( (uint64_t)__float_as_uint( d ) << 32 ) | (uint32_t)id;
+        if ( c < key[K - 1] )
+        {
+#pragma unroll
+            for ( int j = K - 1; j > 0; --j )
+            {
+                uint64_t prev = key[j - 1], cur = key[j];
+                key[j] = c < prev ? prev : ( c < cur ? c : cur );
+            }
+            key[0] = c < key[0] ? c : key[0];
+        }
+    }
+    __device__ int32_t index( int j ) const
+    {
+        return (int32_t)( key[j] & 0xffffffffu );
+    }
+    __device__ double distance( int j ) const
+    {
+        return (double)__uint_as_float( (uint32_t)( key[j] >> 32 ) );
+    }
+};
+
+template <int K>
+struct KBestD
+{
+    double dk[K];
+    int32_t ik[K];
+    __device__ void init()
+    {
+#pragma unroll
+        for ( int j = 0; j < K; ++j )
+        {
+            dk[j] = INFINITY;
+            ik[j] = INT32_MAX;
+        }
+    }
+    __device__ double threshold() const
+    {
+        if ( ik[K - 1] == INT32_MAX && dk[K - 1] == INFINITY )
+            return INFINITY;
+        double w = dk[K - 1];
+        return w * w * 1.0000000000000009; // (1 + 2^-50)
+    }
+    __device__ static bool less( double a, int32_t ia, double b, int32_t ib )
+    {
+        return a < b || ( a == b && ia < ib );
+    }
+    __device__ void insert( double d, int32_t id )
+    {
+        if ( less( d, id, dk[K - 1], ik[K - 1] ) )
+        {
+#pragma unroll
+            for ( int j = K - 1; j > 0; --j )
+            {
+                bool lp = less( d, id, dk[j - 1], ik[j - 1] );
+                bool lc = less( d, id, dk[j], ik[j] );
+                double nd = lp ? dk[j - 1] : ( lc ? d : dk[j] );
+                int32_t ni = lp ? ik[j - 1] : ( lc ? id : ik[j] );
+                dk[j] = nd;
+                ik[j] = ni;
+            }
+            if ( less( d, id, dk[0], ik[0] ) )
+            {
+                dk[0] = d;
+                ik[0] = id;
+            }
+        }
+    }
+    __device__ int32_t index( int j ) const { return ik[j]; }
+    __device__ double distance( int j ) const { return dk[j]; }
+};
+
+template <typename T, int K>
+struct KBest;
+template <int K>
+struct KBest<float, K>
+{
+    using type = KBestF<K>;
+};
+template <int K>
+struct KBest<double, K>
+{
+    using type = KBestD<K>;
+};
+
+// Squared distance from q to its projection onto [lo, hi]: the operation
+// sequence of the point distance (tmp = b - a; s += tmp * tmp), so for a
+// degenerate box it is that distance exactly and otherwise a lower bound.
+template <typename T>
+__device__ inline T box_sq( T const *q, T const *lo, T const *hi )
+{
+#pragma clang fp contract( off )
+    T s = 0;
+#pragma unroll
+    for ( int d = 0; d < 3; ++d )
+    {
+        T c = q[d] < lo[d] ? lo[d] : ( q[d] > hi[d] ? hi[d] : q[d] );
+        T tmp = c - q[d];
+        s += tmp * tmp;
+    }
+    return s;
+}
+
+__device__ inline float sqrt_rn( float s ) { return __fsqrt_rn( s ); }
+__device__ inline double sqrt_rn( double s ) { return __dsqrt_rn( s ); }
+
+constexpr int kQueryBlock = 64;
+
+// One lane per query. STACK > 0: LDS stack of that many entries per lane;
+// STACK == 0: per-lane stack in global memory (gstack, depth entries,
+// laid out [level][lane]).
+template <typename T, int K, int STACK>
+__global__ void __launch_bounds__( kQueryBlock )
+    knn_query_kernel( Node<T> const *nodes, typename Vec4<T>::type const *src,
+                      typename Vec4<T>::type const *tgt, int32_t const *order, int64_t n_tgt, int64_t n_src,
+                      int kk, int32_t *out_idx, double *out_dist,
+                      int2 *gstack, int depth )
+{
+    constexpr int LS = STACK > 0 ? STACK : 1;
+    __shared__ int32_t st_node[LS * kQueryBlock];
+    __shared__ T st_d[LS * kQueryBlock];
+    int64_t qi = blockIdx.x * (int64_t)kQueryBlock + threadIdx.x;
+    if ( qi >= n_tgt )
+        return;
+    int32_t t = order[qi];
+    typename Vec4<T>::type qp = tgt[t];
+    T const q[3] = {qp.x, qp.y, qp.z};
+    typename KBest<T, K>::type best;
+    best.init();
+    T thr = INFINITY;
+
+    if ( n_src == 1 )
+    {
+        // single source: the tree has no internal node
+        typename Vec4<T>::type sp0 = src[0];
+        T const b[3] = {sp0.x, sp0.y, sp0.z};
+        best.insert( sqrt_rn( box_sq( q, b, b ) ), 0 );
+    }
+    else
+    {
+        int sp = 0;
+        int32_t node = 0;
+        while ( node >= 0 )
+        {
+            Node<T> const nd = nodes[node];
+            T s0 = box_sq( q, nd.lo[0], nd.hi[0] );
+            T s1 = box_sq( q, nd.lo[1], nd.hi[1] );
+            int32_t c0 = nd.child[0], c1 = nd.child[1];
+            if ( c0 < 0 && s0 <= thr )
+            {
+                best.insert( sqrt_rn( s0 ), ~c0 );
+                thr = best.threshold();
+            }
+            if ( c1 < 0 && s1 <= thr )
+            {
+                best.insert( sqrt_rn( s1 ), ~c1 );
+                thr = best.threshold();
+            }
+            bool g0 = c0 >= 0 && s0 <= thr;
+            bool g1 = c1 >= 0 && s1 <= thr;
+            int32_t next = -1;
+            if ( g0 && g1 )
+            {
+                bool near0 = s0 <= s1;
+                int32_t far = near0 ? c1 : c0;
+                T fs = near0 ? s1 : s0;
+                next = near0 ? c0 : c1;
+                if ( STACK > 0 )
+                {
+                    st_node[sp * kQueryBlock + threadIdx.x] = far;
+                    st_d[sp * kQueryBlock + threadIdx.x] = fs;
+                }
+                else
+                {
+                    int2 e;
+                    e.x = far;
+                    e.y = __float_as_int( (float)fs );
+                    gstack[(int64_t)sp * n_tgt + qi] = e;
+                }
+                ++sp;
+            }
+            else if ( g0 )
+                next = c0;
+            else if ( g1 )
+                next = c1;
+            else
+            {
+                while ( sp > 0 )
+                {
+                    --sp;
+                    int32_t cand;
+                    T cs;
+                    if ( STACK > 0 )
+                    {
+                        cand = st_node[sp * kQueryBlock + threadIdx.x];
+                        cs = st_d[sp * kQueryBlock + threadIdx.x];
+                    }
+                    else
+                    {
+                        int2 e = gstack[(int64_t)sp * n_tgt + qi];
+                        cand = e.x;
+                        // the float copy of a double bound is rounded; a
+                        // rounded-down copy could prune wrongly, so it is
+                        // only used to order, never to prune, for double
+                        cs = sizeof( T ) == 4 ? (T)__int_as_float( e.y ) : 0;
+                    }
+                    if ( cs <= thr )
+                    {
+                        next = cand;
+                        break;
+                    }
+                }
+            }
+            node = next;
+        }
+    }
+    int64_t row = (int64_t)t * kk;
+    for ( int j = 0; j < kk; ++j )
+    {
+        out_idx[row + j] = best.index( j );
+        if ( out_dist )
+            out_dist[row + j] = best.distance( j );
+    }
+}
+
+template <typename T>
+struct QueryLauncher
+{
+    using V = typename Vec4<T>::type;
+    template <int K>
+    static void run( int stack, Node<T> const *nodes, V const *src,
+                     V const *tgt,
+                     int32_t const *order, int64_t n_tgt, int64_t n_src,
+                     int kk, int32_t *idx, double *dist, int2 *gstack,
+                     int depth, hipStream_t s )
+    {
+        dim3 grid( ceil_div( n_tgt, kQueryBlock ) ), block( kQueryBlock );
+        if ( stack == 32 )
+            knn_query_kernel<T, K, 32><<<grid, block, 0, s>>>(
+                nodes, src, tgt, order, n_tgt, n_src, kk, idx, dist, gstack, depth );
+        else if ( stack == 64 )
+            knn_query_kernel<T, K, 64><<<grid, block, 0, s>>>(
+                nodes, src, tgt, order, n_tgt, n_src, kk, idx, dist, gstack, depth );
+        else
+            knn_query_kernel<T, K, 0><<<grid, block, 0, s>>>(
+                nodes, src, tgt, order, n_tgt, n_src, kk, idx, dist, gstack, depth );
+        DTKX_HIP( hipGetLastError() );
+    }
+    static void dispatch( int k, int stack, Node<T> const *nodes,
+                          V const *src, V const *tgt,
+                          int32_t const *order, int64_t n_tgt, int64_t n_src,
+                          int kk, int32_t *idx, double *dist, int2 *gstack,
+                          int depth, hipStream_t s )
+    {
+#define DTKX_KNN_CASE( KV )                                                    \
+    if ( k <= KV )                                                             \
+    {                                                                          \
+        run<KV>( stack, nodes, src, tgt, order, n_tgt, n_src, kk, idx, dist,        \
+                 gstack, depth, s );                                           \
+        return;                                                                \
+    }
+        DTKX_KNN_CASE( 1 )
+        DTKX_KNN_CASE( 4 )
+        DTKX_KNN_CASE( 8 )
+        DTKX_KNN_CASE( 10 )
+        DTKX_KNN_CASE( 16 )
+        DTKX_KNN_CASE( 24 )
+        DTKX_KNN_CASE( 32 )
+        DTKX_KNN_CASE( 48 )
+        DTKX_KNN_CASE( 64 )
+#undef DTKX_KNN_CASE
+        DTKX_REQUIRE( k <= knn_max_k, DTKX_ERR_UNSUPPORTED );
+    }
+};
+
+template <typename T>
+void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
+               int64_t n_tgt, int k, int32_t *indices, double *dist,
+               int32_t *query_order, KnnInfo *info, hipStream_t s )
+{
+    using V = typename Vec4<T>::type;
+    V const *src = static_cast<V const *>( src_v );
+    V const *tgt = static_cast<V const *>( tgt_v );
+    int const kk = static_cast<int>( k < n_src ? k : n_src );
+
+    // scene box of the sources
+    int nparts = static_cast<int>( std::min<int64_t>(
+        1024, std::max<int64_t>( 1, ceil_div( n_src, kBlock ) ) ) );
+    DeviceBuffer<T> partial( 6 * nparts ), box( 6 );
+    bbox_partial<T, V><<<nparts, kBlock, 0, s>>>( src, n_src, partial.get() );
+    bbox_final<T><<<1, kBlock, 0, s>>>( partial.get(), nparts, box.get() );
+    DTKX_HIP( hipGetLastError() );
+
+    // Morton codes and sort: sources
+    DeviceBuffer<uint64_t> codes( n_src ), codes_sorted( n_src );
+    DeviceBuffer<int32_t> ids( n_src ), ids_sorted( n_src );
+    morton_kernel<T, V><<<ceil_div( n_src, kBlock ), kBlock, 0, s>>>(
+        src, n_src, box.get(), codes.get(), ids.get() );
+    DTKX_HIP( hipGetLastError() );
+    std::size_t temp_bytes = 0;
+    int64_t const nmax = std::max( n_src, n_tgt );
+    DTKX_HIP( hipcub::DeviceRadixSort::SortPairs(
+        nullptr, temp_bytes, codes.get(), codes_sorted.get(), ids.get(),
+        ids_sorted.get(), static_cast<int>( nmax ), 0, 63, s ) );
+    DeviceBuffer<char> temp( temp_bytes );
+    DTKX_HIP( hipcub::DeviceRadixSort::SortPairs(
+        temp.get(), temp_bytes, codes.get(), codes_sorted.get(), ids.get(),
+        ids_sorted.get(), static_cast<int>( n_src ), 0, 63, s ) );
+
+    // hierarchy + refit
+    int64_t const n_int = n_src - 1;
+    DeviceBuffer<Node<T>> nodes( std::max<int64_t>( n_int, 1 ) );
+    DeviceBuffer<int32_t> par_int( std::max<int64_t>( n_int, 1 ) ),
+        par_leaf( n_src ), flags( std::max<int64_t>( n_int, 1 ) ), height( 1 );
+    int h_tree = 0;
+    if ( n_int > 0 )
+    {
+        DTKX_HIP( hipMemsetAsync( flags.get(), 0, sizeof( int32_t ) * n_int, s ) );
+        DTKX_HIP( hipMemsetAsync( height.get(), 0, sizeof( int32_t ), s ) );
+        karras_kernel<T><<<ceil_div( n_int, kBlock ), kBlock, 0, s>>>(
+            codes_sorted.get(), n_src, nodes.get(), par_int.get(),
+            par_leaf.get() );
+        refit_kernel<T, V><<<ceil_div( n_src, kBlock ), kBlock, 0, s>>>(
+            src, ids_sorted.get(), n_src, nodes.get(), par_int.get(),
+            par_leaf.get(), flags.get(), height.get() );
+        DTKX_HIP( hipGetLastError() );
+        DTKX_HIP( hipMemcpyAsync( &h_tree, height.get(), sizeof( int32_t ),
+                                  hipMemcpyDeviceToHost, s ) );
+        DTKX_HIP( hipStreamSynchronize( s ) );
+    }
+    if ( n_tgt == 0 )
+    {
+        if ( info )
+            info->tree_height = h_tree;
+        return;
+    }
+
+    // Morton order of the targets (same scene box)
+    DeviceBuffer<int32_t> order_buf;
+    int32_t *order = query_order;
+    if ( !order )
+    {
+        order_buf.resize( n_tgt );
+        order = order_buf.get();
+    }
+    {
+        DeviceBuffer<uint64_t> tcodes( n_tgt ), tcodes_sorted( n_tgt );
+        DeviceBuffer<int32_t> tids( n_tgt );
+        morton_kernel<T, V><<<ceil_div( n_tgt, kBlock ), kBlock, 0, s>>>(
+            tgt, n_tgt, box.get(), tcodes.get(), tids.get() );
+        DTKX_HIP( hipGetLastError() );
+        DTKX_HIP( hipcub::DeviceRadixSort::SortPairs(
+            temp.get(), temp_bytes, tcodes.get(), tcodes_sorted.get(),
+            tids.get(), order, static_cast<int>( n_tgt ), 0, 63, s ) );
+    }
+
+    // a descent pushes at most one node per internal level below the root
+    int const need = h_tree;
+    int stack = need <= 32 ? 32 : ( need <= 64 ? 64 : 0 );
+    DeviceBuffer<int2> gstack;
+    if ( stack == 0 )
+        gstack.resize( static_cast<std::size_t>( need + 1 ) * n_tgt );
+    QueryLauncher<T>::dispatch( k, stack, nodes.get(), src, tgt, order, n_tgt,
+                                n_src, kk, indices, dist, gstack.get(),
+                                need + 1, s );
+    if ( info )
+    {
+        info->tree_height = h_tree;
+        info->stack_used = stack;
+    }
+    // buffers are released at scope exit; wait so the frees do not race
+    DTKX_HIP( hipStreamSynchronize( s ) );
+}
+
+} // namespace
+
+void knn_search( int geometry, void const *src, int64_t n_src,
+                 void const *tgt, int64_t n_tgt, int k, int32_t *indices,
+                 double *dist, int32_t *query_order, KnnInfo *info,
+                 hipStream_t s )
+{
+    DTKX_REQUIRE( n_src > 0, DTKX_ERR_EMPTY_SOURCE );
+    DTKX_REQUIRE( k >= 1 && k <= knn_max_k, DTKX_ERR_UNSUPPORTED );
+    DTKX_REQUIRE( n_src < INT32_MAX && n_tgt < INT32_MAX,
+                  DTKX_ERR_INVALID_ARGUMENT );
+    if ( geometry == DTKX_GEOMETRY_FLOAT )
+        knn_impl<float>( src, n_src, tgt, n_tgt, k, indices, dist,
+                         query_order, info, s );
+    else
+        knn_impl<double>( src, n_src, tgt, n_tgt, k, indices, dist,
+                          query_order, info, s );
+}
+
+} // namespace dtkx

@@ -1,0 +1,171 @@
+// Device building blocks of the MLS setup, written to round exactly as the
+// reference's serial code does: no FMA contraction, the reference's
+// evaluation order, correctly rounded elementary functions (crmath.hpp).
+#ifndef DTKX_MLS_DEVICE_HPP
+#define DTKX_MLS_DEVICE_HPP
+
+#include "common.hpp"
+#include "crmath.hpp"
+
+#include <cfloat>
+
+namespace dtkx
+{
+namespace dev
+{
+
+// Compactly supported RBFs (DTK_CompactlySupportedRadialBasisFunctions.hpp
+// :45-147), evaluated left to right as written there.
+__device__ inline double rbf_eval( int rbf, double x )
+{
+#pragma clang fp contract( off )
+    switch ( rbf )
+    {
+    case DTKX_RBF_WENDLAND0:
+        return ( 1.0 - x ) * ( 1.0 - x );
+    case DTKX_RBF_WENDLAND2:
+        return ( 1.0 - x ) * ( 1.0 - x ) * ( 1.0 - x ) * ( 1.0 - x ) *
+               ( 4.0 * x + 1.0 );
+    case DTKX_RBF_WENDLAND4:
+        return ( 1.0 - x ) * ( 1.0 - x ) * ( 1.0 - x ) * ( 1.0 - x ) *
+               ( 1.0 - x ) * ( 1.0 - x ) * ( 35.0 * x * x + 18.0 * x + 3.0 );
+    case DTKX_RBF_WENDLAND6:
+        return ( 1.0 - x ) * ( 1.0 - x ) * ( 1.0 - x ) * ( 1.0 - x ) *
+               ( 1.0 - x ) * ( 1.0 - x ) * ( 1.0 - x ) * ( 1.0 - x ) *
+               ( 32.0 * x * x * x + 25.0 * x * x + 8.0 * x + 1.0 );
+    case DTKX_RBF_WU2:
+        return ( 1.0 - x ) * ( 1.0 - x ) * ( 1.0 - x ) * ( 1.0 - x ) *
+               ( 3.0 * x * x * x + 12.0 * x * x + 16.0 * x + 4.0 );
+    case DTKX_RBF_WU4:
+        return ( 1.0 - x ) * ( 1.0 - x ) * ( 1.0 - x ) * ( 1.0 - x ) *
+               ( 1.0 - x ) * ( 1.0 - x ) *
+               ( 5.0 * x * x * x * x * x + 30.0 * x * x * x * x +
+                 72.0 * x * x * x + 82.0 * x * x + 36.0 * x + 6.0 );
+    case DTKX_RBF_BUHMANN2:
+        return 2.0 * x * x * x * x * log( x ) - 7.0 / 2.0 * x * x * x * x +
+               16 / 3.0 * x * x * x - 2 * x * x + 1.0 / 6.0;
+    case DTKX_RBF_BUHMANN3:
+        return x * x * x * x * x * x * x * x -
+               84.0 / 5.0 * x * x * x * x * x * x +
+               1024.0 / 5.0 * x * x * x * x * sqrt( x ) -
+               378.0 * x * x * x * x + 1024.0 / 5.0 * x * x * x * sqrt( x ) -
+               84.0 / 5.0 * x * x + 1.0;
+    default: // DTKX_RBF_BUHMANN4
+        return 99.0 / 35.0 * x * x * x * x * x * x * x * x -
+               132.0 * x * x * x * x * x * x +
+               9216.0 / 35.0 * x * x * x * x * x * sqrt( x ) -
+               11264.0 / 35.0 * x * x * x * x * sqrt( x ) +
+               198.0 * x * x * x * x - 396.0 / 5.0 * x * x + 1.0;
+    }
+}
+
+// Basis row P(l, 0..M-1) of a relative coordinate (DTK_MultivariatePolynomial
+// Basis.hpp:249-276). Float geometry: the coordinate is rounded to float and
+// products are float products, stored as double.
+template <int M, typename T>
+__device__ inline void basis_row_t( T x, T y, T z, double *P )
+{
+#pragma clang fp contract( off )
+    P[0] = 1.;
+    if ( M >= 4 )
+    {
+        P[1] = x;
+        P[2] = y;
+        P[3] = z;
+    }
+    if ( M >= 10 )
+    {
+        P[4] = x * x;
+        P[5] = x * y;
+        P[6] = x * z;
+        P[7] = y * y;
+        P[8] = y * z;
+        P[9] = z * z;
+    }
+}
+
+template <int M>
+__device__ inline void basis_row( int geometry, double const *xr, double *P )
+{
+    if ( geometry == DTKX_GEOMETRY_FLOAT )
+        basis_row_t<M, float>( (float)xr[0], (float)xr[1], (float)xr[2], P );
+    else
+        basis_row_t<M, double>( xr[0], xr[1], xr[2], P );
+}
+
+// |relative coordinate| as computeRadius/computeWeights form it
+// (Impl.hpp:121-125, 162-165): distance(Point{x}, Point{0,0,0}).
+__device__ inline double rel_distance( int geometry, double const *xr )
+{
+#pragma clang fp contract( off )
+    if ( geometry == DTKX_GEOMETRY_FLOAT )
+    {
+        float s = 0.f;
+        for ( int d = 0; d < 3; ++d )
+        {
+            float tmp = 0.f - (float)xr[d];
+            s += tmp * tmp;
+        }
+        return (double)__fsqrt_rn( s );
+    }
+    double s = 0.;
+    for ( int d = 0; d < 3; ++d )
+    {
+        double tmp = 0. - xr[d];
+        s += tmp * tmp;
+    }
+    return __dsqrt_rn( s );
+}
+
+// The 2x2 SVD of DTK_DetailsSVDImpl.hpp:119-162 reduced to what the Jacobi
+// consumes (:258-263): cl = L00, sl = L01, cr = R00 and the sign-fixed sr.
+__device__ inline void svd_2x2_rot( double a00, double a01, double a10,
+                                    double a11, double &cl, double &sl,
+                                    double &cr, double &sr )
+{
+#pragma clang fp contract( off )
+    // AAt = A A^T, AtA = A^T A (mult_2x2 order: two products, one add)
+    double aat00 = a00 * a00 + a01 * a01;
+    double aat01 = a00 * a10 + a01 * a11;
+    double aat10 = a10 * a00 + a11 * a01;
+    double aat11 = a10 * a10 + a11 * a11;
+    double ata00 = a00 * a00 + a10 * a10;
+    double ata01 = a00 * a01 + a10 * a11;
+    double ata10 = a01 * a00 + a11 * a10;
+    double ata11 = a01 * a01 + a11 * a11;
+
+    double phi = 0.5 * crm::atan2( aat01 + aat10, aat00 - aat11 );
+    double cphi = crm::cos( phi );
+    double sphi = crm::sin( phi );
+    double theta = 0.5 * crm::atan2( ata01 + ata10, ata00 - ata11 );
+    double ctheta = crm::cos( theta );
+    double stheta = crm::sin( theta );
+
+    // W = [[ct, -st], [st, ct]]; AW = A W; S = U^T AW with U^T = [[cp, sp],
+    // [-sp, cp]]
+    double w00 = ctheta, w01 = -stheta, w10 = stheta, w11 = ctheta;
+    double aw00 = a00 * w00 + a01 * w10;
+    double aw01 = a00 * w01 + a01 * w11;
+    double aw10 = a10 * w00 + a11 * w10;
+    double aw11 = a10 * w01 + a11 * w11;
+    double ut00 = cphi, ut01 = sphi, ut10 = -sphi, ut11 = cphi;
+    double s00 = ut00 * aw00 + ut01 * aw10;
+    double s11 = ut10 * aw01 + ut11 * aw11;
+    double c00 = copysign( 1., s00 );
+    double c11 = copysign( 1., s11 );
+    // V = W C (C diagonal: the off-diagonal zeros kept for signed zeros)
+    double v00 = w00 * c00 + w01 * 0.0;
+    double v01 = w00 * 0.0 + w01 * c11;
+    double v10 = w10 * c00 + w11 * 0.0;
+    cl = cphi;
+    sl = -sphi;
+    cr = v00;
+    int sg01 = ( v01 > 0 ) - ( v01 < 0 );
+    int sg10 = ( v10 > 0 ) - ( v10 < 0 );
+    sr = ( sg01 == sg10 ) ? -v01 : v01;
+}
+
+} // namespace dev
+} // namespace dtkx
+
+#endif

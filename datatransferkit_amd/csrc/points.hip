@@ -1,0 +1,94 @@
+// Point-format conversion kernels and host utilities shared by all stages.
+#include "common.hpp"
+
+#include <cstdio>
+#include <string>
+
+namespace dtkx
+{
+
+[[noreturn]] void raise( int status, char const *cond, char const *file,
+                         int line )
+{
+    // strip the directory part of the file name
+    std::string f( file );
+    auto pos = f.find_last_of( '/' );
+    if ( pos != std::string::npos )
+        f = f.substr( pos + 1 );
+    throw Error( status, std::string( "dtkx: " ) + cond + ", failed in " + f +
+                             ", line " + std::to_string( line ) + "." );
+}
+
+bool is_device_pointer( void const *p )
+{
+    if ( !p )
+        return false;
+    hipPointerAttribute_t attr;
+    hipError_t e = hipPointerGetAttributes( &attr, p );
+    if ( e != hipSuccess )
+    {
+        (void)hipGetLastError(); // plain host memory: clear the error
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice ||
+           attr.type == hipMemoryTypeManaged ||
+           attr.type == hipMemoryTypeUnified ||
+           ( attr.type == hipMemoryTypeHost && attr.devicePointer != nullptr );
+}
+
+namespace
+{
+
+constexpr int kBlock = 256;
+
+template <typename V, typename T>
+__global__ void __launch_bounds__( kBlock )
+    to_vec4_kernel( double const *xyz, int64_t n, int layout, V *out )
+{
+    int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( i >= n )
+        return;
+    double x, y, z;
+    if ( layout == DTKX_LAYOUT_SOA )
+    {
+        x = xyz[i];
+        y = xyz[n + i];
+        z = xyz[2 * n + i];
+    }
+    else
+    {
+        x = xyz[3 * i];
+        y = xyz[3 * i + 1];
+        z = xyz[3 * i + 2];
+    }
+    V v;
+    v.x = static_cast<T>( x );
+    v.y = static_cast<T>( y );
+    v.z = static_cast<T>( z );
+    v.w = 0;
+    out[i] = v;
+}
+
+} // namespace
+
+void points_to_float4( double const *xyz, int64_t n, int layout, float4 *out,
+                       hipStream_t s )
+{
+    if ( n == 0 )
+        return;
+    to_vec4_kernel<float4, float><<<ceil_div( n, kBlock ), kBlock, 0, s>>>(
+        xyz, n, layout, out );
+    DTKX_HIP( hipGetLastError() );
+}
+
+void points_to_double4( double const *xyz, int64_t n, int layout,
+                        double4 *out, hipStream_t s )
+{
+    if ( n == 0 )
+        return;
+    to_vec4_kernel<double4, double><<<ceil_div( n, kBlock ), kBlock, 0, s>>>(
+        xyz, n, layout, out );
+    DTKX_HIP( hipGetLastError() );
+}
+
+} // namespace dtkx

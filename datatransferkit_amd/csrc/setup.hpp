@@ -1,0 +1,50 @@
+// Fused MLS setup: for each target, gather its neighbours, form relative
+// coordinates, Vandermonde rows, radius, Wendland weights, the moment matrix
+// P^T Phi P, its pseudo-inverse (row 0) and the CRS coefficients, in one
+// kernel. Replaces the chain of separate kernels of the reference's
+// constructor (DTK_MovingLeastSquaresOperator_def.hpp:66-118), whose
+// intermediates (transformed coordinates, P, radius, phi, A, A^+, aux) all
+// go through global memory.
+#ifndef DTKX_SETUP_HPP
+#define DTKX_SETUP_HPP
+
+#include "common.hpp"
+
+namespace dtkx
+{
+
+struct SetupParams
+{
+    int m = 4;
+    int kk = 4;
+    int rbf = DTKX_RBF_WENDLAND0;
+    int geometry = DTKX_GEOMETRY_FLOAT;
+    int truncation = DTKX_TRUNCATION_ABSOLUTE;
+    double rcond = 0.0;
+    long max_rot = 0;
+    int group_size = 0; // classical solver: lanes per system, 0 = choose
+};
+
+// counters[0] = n_underdetermined, [1] = n_unconverged, [2] = rotations
+// (classical) or sweeps (cyclic); accumulated atomically.
+// order (nullable): targets in processing order (a permutation).
+// a0 (nullable): receives row 0 of the pseudo-inverse per target [n][m].
+
+// Classical max-pivot two-sided Jacobi, reference operation order.
+int setup_classical( double4 const *src, double4 const *tgt, int64_t n_tgt,
+                     int32_t const *indices, int32_t const *order,
+                     SetupParams const &p, double *coeffs, double *a0,
+                     unsigned long long *counters, hipStream_t s );
+
+// Lane group size the classical solver uses for (m, kk) when not forced.
+int setup_classical_group( int m, int kk );
+
+// Batched pseudo-inverse of general n x n matrices (n <= 64), one matrix
+// per workgroup (tstSVD path).
+void svd_pinv_batched( int n, int64_t n_mat, double const *a, double *pinv,
+                       long max_rot, unsigned long long *counters,
+                       hipStream_t s );
+
+} // namespace dtkx
+
+#endif

@@ -1,0 +1,236 @@
+"""Host-side mirror of DataTransferKit's MovingLeastSquaresOperator.
+
+Same construction / apply surface as the reference
+(packages/Meshfree/src/DTK_MovingLeastSquaresOperator_decl.hpp:36-63):
+
+    op = MovingLeastSquaresOperator(source_points, target_points,
+                                    rbf="wendland0", basis="linear")
+    op.apply(source_values, target_values)
+
+The template policies of the reference (RBF, polynomial basis) become
+keyword arguments; ``polynomial_basis`` and ``radial_basis_function``
+attributes mirror its public typedefs (:42-45). Points are [n, 3] float64
+arrays (torch CUDA tensors are used in place; numpy arrays or CPU tensors
+are staged by the library). Preconditions the reference checks with
+DTK_REQUIRE raise :class:`DtkxError` here (always on).
+
+Every computation runs in the HIP library (``libdtkx.so``); there is no
+CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import DtkxError, check
+
+
+def _ptr(a):
+    """(pointer, keep-alive) for a torch tensor or numpy array."""
+    if a is None:
+        return None, None
+    if hasattr(a, "data_ptr"):  # torch tensor
+        if not a.is_contiguous():
+            raise DtkxError(_lib.ERR_INVALID_ARGUMENT, "tensor must be contiguous")
+        return ctypes.c_void_p(a.data_ptr()), a
+    arr = np.ascontiguousarray(a)
+    return ctypes.c_void_p(arr.ctypes.data), arr
+
+
+def _dtype_name(a):
+    return str(a.dtype).replace("torch.", "")
+
+
+def _check_points(p, name):
+    shape = tuple(p.shape)
+    if len(shape) != 2:
+        raise DtkxError(_lib.ERR_DIMENSION, f"{name} must be [n, dim]")
+    if _dtype_name(p) != "float64":
+        raise DtkxError(_lib.ERR_INVALID_ARGUMENT, f"{name} must be float64")
+    return shape
+
+
+class MovingLeastSquaresOperator:
+    """MLS transfer operator on one GPU (one-rank communicator)."""
+
+    def __init__(self, source_points, target_points, *, rbf="wendland0",
+                 basis="linear", k=0, precision="fp64", geometry="float",
+                 solver="classical", truncation="absolute", rcond=1e-12,
+                 max_rotations=0, group_size=0, store_pinv=False,
+                 stream=None, neighbors=None):
+        ns, ds = _check_points(source_points, "source_points")
+        nt, dt = _check_points(target_points, "target_points")
+        if ds != dt:
+            raise DtkxError(_lib.ERR_DIMENSION,
+                            "source_points.extent(1) == target_points.extent(1)")
+        o = _lib.default_options()
+        o.dim = ds
+        o.basis = _lib.BASIS[basis]
+        o.rbf = _lib.RBF[rbf]
+        o.k = int(k)
+        o.precision = _lib.PRECISION[precision]
+        o.geometry = _lib.GEOMETRY[geometry]
+        o.solver = _lib.SOLVER[solver]
+        o.truncation = _lib.TRUNCATION[truncation]
+        o.rcond = float(rcond)
+        o.max_rotations = int(max_rotations)
+        o.group_size = int(group_size)
+        o.store_pinv = int(bool(store_pinv))
+        o.stream = ctypes.c_void_p(stream) if stream else None
+        self.polynomial_basis = basis
+        self.radial_basis_function = rbf
+        self.precision = precision
+        self._n_src, self._n_tgt = ns, nt
+        L = _lib.load()
+        sp, ka = _ptr(source_points)
+        tp, kb = _ptr(target_points)
+        h = ctypes.c_void_p()
+        if neighbors is None:
+            check(L.dtkx_mls_create(sp, ns, tp, nt, ctypes.byref(o),
+                                    ctypes.byref(h)))
+        else:
+            nb = neighbors
+            ip, kc = _ptr(nb)
+            check(L.dtkx_mls_create_with_neighbors(
+                sp, ns, tp, nt, ip, int(nb.shape[1]), ctypes.byref(o),
+                ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.load().dtkx_mls_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    # ---------------------------------------------------------------- apply
+    def _fields(self, vals, n, name):
+        shape = tuple(vals.shape)
+        if len(shape) == 1:
+            nf, ld = 1, shape[0]
+        elif len(shape) == 2:
+            nf, ld = shape
+        else:
+            raise DtkxError(_lib.ERR_INVALID_ARGUMENT, f"{name} must be 1-D or 2-D")
+        if ld != n:
+            raise DtkxError(_lib.ERR_SIZE_MISMATCH,
+                            f"{name}.extent(0) == {n} (got {ld})")
+        return nf, ld
+
+    def apply(self, source_values, target_values):
+        """target_values = C source_values (fields along the leading axis)."""
+        nf, lds = self._fields(source_values, self._n_src, "source_values")
+        nf2, ldt = self._fields(target_values, self._n_tgt, "target_values")
+        if nf != nf2:
+            raise DtkxError(_lib.ERR_SIZE_MISMATCH, "field counts differ")
+        fp32 = _dtype_name(source_values) == "float32"
+        L = _lib.load()
+        xp, ka = _ptr(source_values)
+        yp, kb = _ptr(target_values)
+        if fp32:
+            check(L.dtkx_mls_apply_fp32(self._h, xp, yp, nf, lds, ldt))
+        else:
+            check(L.dtkx_mls_apply(self._h, xp, yp, nf, lds, ldt))
+        return target_values
+
+    def apply_transpose(self, target_bar, source_bar):
+        """source_bar = C^T target_bar."""
+        nf, ldt = self._fields(target_bar, self._n_tgt, "target_bar")
+        nf2, lds = self._fields(source_bar, self._n_src, "source_bar")
+        if nf != nf2:
+            raise DtkxError(_lib.ERR_SIZE_MISMATCH, "field counts differ")
+        xp, ka = _ptr(target_bar)
+        yp, kb = _ptr(source_bar)
+        check(_lib.load().dtkx_mls_apply_transpose(self._h, xp, yp, nf, ldt, lds))
+        return source_bar
+
+    # ------------------------------------------------------------ inspection
+    def stats(self) -> dict:
+        s = _lib.Stats()
+        check(_lib.load().dtkx_mls_get_stats(self._h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def graph_pointers(self):
+        """Device pointers (offsets, indices, coeffs) and nnz."""
+        off, idx, co = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        nnz = ctypes.c_int64()
+        check(_lib.load().dtkx_mls_get_graph(
+            self._h, ctypes.byref(off), ctypes.byref(idx), ctypes.byref(co),
+            ctypes.byref(nnz)))
+        return off.value, idx.value, co.value, nnz.value
+
+    def graph(self):
+        """Copies of (offsets, indices [n_tgt, k], coeffs [n_tgt, k]) as torch
+        CUDA tensors."""
+        import torch
+
+        off, idx, co, nnz = self.graph_pointers()
+        nt = self._n_tgt
+        k = nnz // nt if nt else 0
+        offsets = torch.empty(nt + 1, dtype=torch.int32, device="cuda")
+        indices = torch.empty((nt, k), dtype=torch.int32, device="cuda")
+        cdt = torch.float32 if self.precision == "fp32_bulk" else torch.float64
+        coeffs = torch.empty((nt, k), dtype=cdt, device="cuda")
+        _memcpy_d2d(offsets, off, 4 * (nt + 1))
+        if nnz:
+            _memcpy_d2d(indices, idx, 4 * nnz)
+            _memcpy_d2d(coeffs, co, coeffs.element_size() * nnz)
+        return offsets, indices, coeffs
+
+
+def _memcpy_d2d(dst_tensor, src_ptr, nbytes):
+    import torch
+
+    # wrap the raw device pointer without taking ownership
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                              ctypes.c_int]
+    torch.cuda.synchronize()
+    rc = hip.hipMemcpy(ctypes.c_void_p(dst_tensor.data_ptr()),
+                       ctypes.c_void_p(src_ptr), nbytes, 3)  # DeviceToDevice
+    if rc != 0:
+        raise DtkxError(_lib.ERR_HIP, f"hipMemcpy failed ({rc})")
+
+
+def knn(source_points, target_points, k, geometry="float", layout="aos",
+        with_distances=False, stream=None):
+    """k nearest sources of each target: indices [n_tgt, min(k, n_src)]
+    ascending by (distance, index); optionally the distances."""
+    import torch
+
+    ns = source_points.shape[0] if layout == "aos" else source_points.shape[1]
+    nt = target_points.shape[0] if layout == "aos" else target_points.shape[1]
+    kk = min(k, ns)
+    dev = source_points.device if hasattr(source_points, "device") else "cuda"
+    idx = torch.empty((nt, kk), dtype=torch.int32, device=dev)
+    dist = torch.empty((nt, kk), dtype=torch.float64, device=dev) if with_distances else None
+    sp, ka = _ptr(source_points)
+    tp, kb = _ptr(target_points)
+    ip, kc = _ptr(idx)
+    dp, kd = _ptr(dist)
+    check(_lib.load().dtkx_knn(sp, ns, tp, nt, k, _lib.GEOMETRY[geometry],
+                               _lib.LAYOUT[layout], ip, dp,
+                               ctypes.c_void_p(stream) if stream else None))
+    return (idx, dist) if with_distances else idx
+
+
+def svd_pseudo_inverse(matrices, max_rotations=0, stream=None):
+    """Batched pseudo-inverse of [n_mat, n, n] float64 matrices (reference
+    SVDFunctor semantics). Returns (pinv, n_underdetermined, n_unconverged)."""
+    import torch
+
+    n = matrices.shape[-1]
+    nm = matrices.numel() // (n * n) if hasattr(matrices, "numel") else matrices.size // (n * n)
+    out = torch.empty_like(matrices) if hasattr(matrices, "data_ptr") else np.empty_like(matrices)
+    ap, ka = _ptr(matrices)
+    op, kb = _ptr(out)
+    under = ctypes.c_int64()
+    unconv = ctypes.c_int64()
+    check(_lib.load().dtkx_svd_pseudo_inverse(
+        n, nm, ap, op, ctypes.byref(under), ctypes.byref(unconv),
+        int(max_rotations), ctypes.c_void_p(stream) if stream else None))
+    return out, under.value, unconv.value

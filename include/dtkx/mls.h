@@ -1,0 +1,263 @@
+/*
+ * dtkx — MI355X-native meshfree moving-least-squares (MLS) transfer.
+ *
+ * C ABI of the MLS transfer path. Every entry point takes plain pointers and
+ * sizes; no C++ or torch types cross this boundary. Point and value pointers
+ * may be device pointers (hipMalloc / torch CUDA tensors, used in place) or
+ * ordinary host pointers (staged through device memory by the library).
+ *
+ * Reference interface this replaces (DataTransferKit, read-only reference):
+ *   packages/Meshfree/src/DTK_MovingLeastSquaresOperator_decl.hpp:36-63
+ *     template <DeviceType, RBF = Wendland<0>, Basis = Linear3>
+ *     class MovingLeastSquaresOperator : public PointCloudOperator<DeviceType>
+ *       ctor(MPI_Comm, View<Coordinate const**> src, View<Coordinate const**> tgt)
+ *       void apply(View<double const*> src_vals, View<double*> tgt_vals) const
+ *   packages/Meshfree/src/DTK_PointCloudOperator.hpp:24-37 (abstract apply)
+ * The communicator is always a one-rank communicator here (one GPU); the
+ * reference's MPI gather (`fetch`) degenerates to an in-kernel gather.
+ *
+ * Error convention: every function returns a dtkx_status; on failure a
+ * message shaped like the reference's DBC output
+ *   "dtkx: <condition>, failed in <file>, line <n>."
+ * (packages/Utils/src/DTK_DBC.cpp:40-48) is available from
+ * dtkx_last_error() (thread-local). Checks are always on, unlike the
+ * reference's DTK_REQUIRE (active only with HAVE_DTK_DBC).
+ */
+#ifndef DTKX_MLS_H
+#define DTKX_MLS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DTKX_VERSION_MAJOR 0
+#define DTKX_VERSION_MINOR 1
+
+/* ---------------------------------------------------------------- status */
+typedef enum dtkx_status {
+    DTKX_SUCCESS = 0,
+    DTKX_ERR_INVALID_ARGUMENT = 1, /* null pointer, bad enum, negative size */
+    DTKX_ERR_SIZE_MISMATCH = 2,    /* ..._def.hpp:129-130 */
+    DTKX_ERR_DIMENSION = 3,        /* ..._def.hpp:39-42 (only 3D) */
+    DTKX_ERR_EMPTY_SOURCE = 4,     /* ..._def.hpp:49 !search_tree.empty() */
+    DTKX_ERR_HIP = 5,              /* HIP runtime error */
+    DTKX_ERR_NOT_CONVERGED = 6,    /* Jacobi rotation cap reached */
+    DTKX_ERR_UNSUPPORTED = 7,      /* option combination not built */
+    DTKX_ERR_OUT_OF_MEMORY = 8
+} dtkx_status;
+
+/* ------------------------------------------------------------- policies */
+/* Polynomial basis; the value is the basis size m in 3D
+ * (DTK_MultivariatePolynomialBasis.hpp:42-75). */
+typedef enum dtkx_basis {
+    DTKX_BASIS_CONSTANT = 1,  /* {1} */
+    DTKX_BASIS_LINEAR = 4,    /* {1,x,y,z} */
+    DTKX_BASIS_QUADRATIC = 10 /* {1,x,y,z,x^2,xy,xz,y^2,yz,z^2} */
+} dtkx_basis;
+
+/* Compactly supported radial basis functions
+ * (DTK_CompactlySupportedRadialBasisFunctions.hpp:45-147). */
+typedef enum dtkx_rbf {
+    DTKX_RBF_WENDLAND0 = 0,
+    DTKX_RBF_WENDLAND2 = 1,
+    DTKX_RBF_WENDLAND4 = 2,
+    DTKX_RBF_WENDLAND6 = 3,
+    DTKX_RBF_WU2 = 4,
+    DTKX_RBF_WU4 = 5,
+    DTKX_RBF_BUHMANN2 = 6,
+    DTKX_RBF_BUHMANN3 = 7,
+    DTKX_RBF_BUHMANN4 = 8
+} dtkx_rbf;
+
+typedef enum dtkx_precision {
+    /* everything in FP64 (geometry per dtkx_geometry) */
+    DTKX_PRECISION_FP64 = 0,
+    /* FP32 weights, Vandermonde, coefficients and apply (FP32 accumulate);
+     * FP64 moments P^T W P, Jacobi and singular-value truncation */
+    DTKX_PRECISION_FP32_BULK = 1
+} dtkx_precision;
+
+typedef enum dtkx_geometry {
+    /* Reference semantics: coordinates pass through a float point type
+     * before kNN distances, radius, weights and monomials are formed
+     * (DTK_DetailsMovingLeastSquaresOperatorImpl.hpp:40-43,121-125,162-165,
+     * 184-185). */
+    DTKX_GEOMETRY_FLOAT = 0,
+    /* all geometry in double */
+    DTKX_GEOMETRY_DOUBLE = 1
+} dtkx_geometry;
+
+typedef enum dtkx_solver {
+    /* Reference order: classical max-pivot two-sided Jacobi with trig 2x2
+     * SVDs (DTK_DetailsSVDImpl.hpp:119-302). One system per lane group. */
+    DTKX_SOLVER_CLASSICAL_JACOBI = 0,
+    /* Cyclic symmetric Jacobi eigen-solver, one system per lane, registers
+     * only. Same pseudo-inverse up to rounding; much faster. */
+    DTKX_SOLVER_CYCLIC_JACOBI = 1
+} dtkx_solver;
+
+typedef enum dtkx_truncation {
+    /* keep singular values with |s| >= DBL_EPSILON (DTK_DetailsSVDImpl.hpp:291) */
+    DTKX_TRUNCATION_ABSOLUTE = 0,
+    /* keep singular values with |s| >= rcond * max|s| */
+    DTKX_TRUNCATION_RELATIVE = 1
+} dtkx_truncation;
+
+typedef enum dtkx_layout {
+    DTKX_LAYOUT_AOS = 0, /* [n][3] row-major, Kokkos LayoutRight */
+    DTKX_LAYOUT_SOA = 1  /* [3][n], Kokkos LayoutLeft */
+} dtkx_layout;
+
+typedef struct dtkx_mls_options {
+    int32_t dim;        /* spatial dimension; must be 3 */
+    int32_t basis;      /* dtkx_basis; default LINEAR (the reference default) */
+    int32_t rbf;        /* dtkx_rbf; default WENDLAND0 */
+    int32_t k;          /* neighbours per target; 0 => basis size
+                           (..._def.hpp:53-55 hard-wires k = basis size) */
+    int32_t precision;  /* dtkx_precision */
+    int32_t geometry;   /* dtkx_geometry */
+    int32_t solver;     /* dtkx_solver */
+    int32_t truncation; /* dtkx_truncation */
+    double rcond;       /* for DTKX_TRUNCATION_RELATIVE */
+    int32_t max_rotations; /* Jacobi cap: classical = rotations, cyclic =
+                              sweeps; 0 => library default */
+    int32_t layout;     /* dtkx_layout of the point arrays */
+    int32_t group_size; /* lanes per system for the classical solver
+                           (1,2,4,8,16); 0 => autotuned */
+    int32_t store_pinv; /* keep x = A^+ e1 per target for the adjoint
+                           (1) or recompute it (0) */
+    void *stream;       /* hipStream_t; NULL = default stream */
+} dtkx_mls_options;
+
+typedef struct dtkx_mls_stats {
+    int64_t n_source;
+    int64_t n_target;
+    int64_t nnz;
+    int32_t basis_size;
+    int32_t k;                  /* neighbours per row = min(k, n_source) */
+    int64_t n_underdetermined;  /* systems with a truncated singular value
+                                   (the reference's parallel_reduce count,
+                                   Impl.hpp:284-288) */
+    int64_t n_unconverged;      /* systems that hit the Jacobi cap */
+    int64_t total_rotations;    /* classical: rotations; cyclic: sweeps */
+    double ms_knn;              /* stage times of the last create, measured */
+    double ms_setup;            /* with HIP events on the handle's stream   */
+    int32_t group_size_used;
+    int32_t tree_height;
+} dtkx_mls_stats;
+
+typedef struct dtkx_mls dtkx_mls; /* opaque operator handle */
+
+/* ------------------------------------------------------------ generic */
+const char *dtkx_last_error(void);
+const char *dtkx_version(void);
+/* Fill *opts with the defaults (reference semantics: Linear basis,
+ * Wendland<0>, k = basis size, FP64, float geometry, classical Jacobi,
+ * absolute DBL_EPSILON truncation, AoS points). */
+int dtkx_mls_options_init(dtkx_mls_options *opts);
+
+/* ---------------------------------------------------- operator (drop-in) */
+/* MovingLeastSquaresOperator ctor (..._def.hpp:24-119): kNN over the source
+ * points, then the fused setup (relative coordinates, Vandermonde, radius,
+ * weights, moments, pseudo-inverse, coefficients). src/tgt are [n][3]
+ * (AoS) or [3][n] (SoA) doubles. n_tgt may be 0; n_src must be > 0. */
+int dtkx_mls_create(const double *src_xyz, int64_t n_src,
+                    const double *tgt_xyz, int64_t n_tgt,
+                    const dtkx_mls_options *opts, dtkx_mls **out);
+
+/* Same as dtkx_mls_create but with caller-supplied neighbour lists:
+ * row i holds indices[i*k_row .. i*k_row+k_row-1] (0 <= index < n_src).
+ * No kNN is run; the order of a row is the summation order. */
+int dtkx_mls_create_with_neighbors(const double *src_xyz, int64_t n_src,
+                                   const double *tgt_xyz, int64_t n_tgt,
+                                   const int32_t *indices, int32_t k_row,
+                                   const dtkx_mls_options *opts,
+                                   dtkx_mls **out);
+
+/* MovingLeastSquaresOperator::apply (..._def.hpp:121-141):
+ * tgt_vals[f*ld_tgt + i] = sum_j coeff(i,j) * src_vals[f*ld_src + idx(i,j)]
+ * for f < n_fields. n_fields = 1, ld = n reproduces the reference call.
+ * In FP32_BULK precision the values are still double at the boundary. */
+int dtkx_mls_apply(const dtkx_mls *h, const double *src_vals,
+                   double *tgt_vals, int32_t n_fields, int64_t ld_src,
+                   int64_t ld_tgt);
+
+/* FP32 apply: float values, float coefficients, float accumulation.
+ * Requires a handle created with DTKX_PRECISION_FP32_BULK. */
+int dtkx_mls_apply_fp32(const dtkx_mls *h, const float *src_vals,
+                        float *tgt_vals, int32_t n_fields, int64_t ld_src,
+                        int64_t ld_tgt);
+
+/* Transpose apply (no counterpart in the reference's MLS path):
+ * src_bar[f*ld_src + s] = sum_{(i,j): idx(i,j)=s} coeff(i,j) *
+ *                          tgt_bar[f*ld_tgt + i]  (overwrites src_bar). */
+int dtkx_mls_apply_transpose(const dtkx_mls *h, const double *tgt_bar,
+                             double *src_bar, int32_t n_fields,
+                             int64_t ld_tgt, int64_t ld_src);
+
+/* Gradient of L = sum_f <tgt_bar_f, C(t) src_vals_f> with respect to the
+ * target coordinates, neighbour sets frozen; needs DTKX_GEOMETRY_DOUBLE.
+ * d_tgt is [n_tgt][3] (AoS), overwritten. */
+int dtkx_mls_target_gradient(const dtkx_mls *h, const double *src_vals,
+                             const double *tgt_bar, int32_t n_fields,
+                             int64_t ld_src, int64_t ld_tgt, double *d_tgt);
+
+/* CSR view of the operator, device pointers owned by the handle:
+ * offsets[n_tgt+1], indices[nnz], coeffs[nnz] (double, or float for
+ * FP32_BULK). Mirrors the reference's _offset/_indices/_coeffs members
+ * (..._decl.hpp:57-62). */
+int dtkx_mls_get_graph(const dtkx_mls *h, const int32_t **offsets,
+                       const int32_t **indices, const void **coeffs,
+                       int64_t *nnz);
+int dtkx_mls_get_stats(const dtkx_mls *h, dtkx_mls_stats *stats);
+int dtkx_mls_destroy(dtkx_mls *h);
+
+/* ------------------------------------------------------- stage kernels */
+/* k nearest neighbours (ArborX Nearest<Point>{t, k} query,
+ * ..._def.hpp:47-60 + Impl.hpp:28-47). indices is [n_tgt][min(k,n_src)],
+ * each row ascending by (distance, index); dist (optional, may be NULL)
+ * receives the matching distances (float for FLOAT geometry, written as
+ * double). */
+int dtkx_knn(const double *src_xyz, int64_t n_src, const double *tgt_xyz,
+             int64_t n_tgt, int32_t k, int32_t geometry, int32_t layout,
+             int32_t *indices, double *dist, void *stream);
+
+/* Batched pseudo-inverse of n x n general square matrices
+ * (SVDFunctor, DTK_DetailsSVDImpl.hpp:39-309, as driven by tstSVD.cpp):
+ * pinv[m] = V^T E^+ U^T, singular values below DBL_EPSILON dropped.
+ * 1 <= n <= 64. *n_underdetermined receives the reference's reduction. */
+int dtkx_svd_pseudo_inverse(int32_t n, int64_t n_matrices, const double *a,
+                            double *pinv, int64_t *n_underdetermined,
+                            int64_t *n_unconverged, int32_t max_rotations,
+                            void *stream);
+
+/* ------------------------------------------- batched uneven operators */
+typedef struct dtkx_mls_problem {
+    const double *src_xyz; /* device pointers, AoS [n][3] */
+    int64_t n_src;
+    const double *tgt_xyz;
+    int64_t n_tgt;
+    int32_t basis; /* dtkx_basis */
+    int32_t k;     /* 0 => basis size */
+} dtkx_mls_problem;
+
+typedef struct dtkx_mls_batch dtkx_mls_batch;
+
+/* Build many independent operators (rbf/precision/solver from opts). kNN
+ * runs per problem; setup of all problems is one persistent launch. */
+int dtkx_mls_batch_create(int32_t n_problems, const dtkx_mls_problem *problems,
+                          const dtkx_mls_options *opts, dtkx_mls_batch **out);
+/* src_vals[p] / tgt_vals[p]: device pointers per problem (one field). */
+int dtkx_mls_batch_apply(const dtkx_mls_batch *b,
+                         const double *const *src_vals,
+                         double *const *tgt_vals);
+int dtkx_mls_batch_get_stats(const dtkx_mls_batch *b, dtkx_mls_stats *stats);
+int dtkx_mls_batch_destroy(dtkx_mls_batch *b);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#endif /* DTKX_MLS_H */

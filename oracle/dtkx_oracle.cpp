@@ -1,0 +1,1038 @@
+/*
+ * dtkx oracle — TEST INFRASTRUCTURE ONLY.
+ *
+ * A CPU restatement of DataTransferKit's meshfree moving-least-squares path,
+ * used as the correctness checker for the HIP kernels and as the reported CPU
+ * baseline. Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+ * leg load this library; the product (datatransferkit_amd/) never does.
+ *
+ * Each function cites the reference file:line it follows (paths relative to
+ * the DataTransferKit source tree). Abbreviations:
+ *   Impl.hpp = packages/Meshfree/src/DTK_DetailsMovingLeastSquaresOperatorImpl.hpp
+ *   SVD.hpp  = packages/Meshfree/src/DTK_DetailsSVDImpl.hpp
+ *   RBF.hpp  = packages/Meshfree/src/DTK_CompactlySupportedRadialBasisFunctions.hpp
+ *   Basis.hpp= packages/Meshfree/src/DTK_MultivariatePolynomialBasis.hpp
+ *   def.hpp  = packages/Meshfree/src/DTK_MovingLeastSquaresOperator_def.hpp
+ *
+ * The k-nearest-neighbour search of the reference lives in ArborX (an
+ * un-vendored submodule; version unrecoverable, API dates it to ArborX 1.x).
+ * Its semantics are restated here as assumptions:
+ *   - ArborX::Point stores float; a double brace-init rounds each coordinate
+ *     with static_cast<float>;
+ *   - Details::distance(a, b) accumulates tmp = b[d]-a[d]; s += tmp*tmp in
+ *     float over d = 0..2 and returns float sqrt(s);
+ *   - a Nearest<Point>{q, k} query returns min(k, n) results in ascending
+ *     distance. Ties are ordered by source index (this build's convention,
+ *     matching the reference's own brute-force precedent in
+ *     packages/Meshfree/test/tstNearestNeighborExodusGenerator.cpp:117-137,
+ *     whose strict '<' keeps the lowest index).
+ *
+ * Build: see oracle/Makefile (-O3 -ffp-contract=off -fopenmp, no fast-math),
+ * which matches a default x86-64 build of the reference (no FMA contraction).
+ *
+ * Pinning (what checks each function) is listed in tests/test_oracle.py.
+ */
+#include "dtkx/mls.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include <quadmath.h>
+
+namespace
+{
+
+// ------------------------------------------------------- elementary functions
+// The reference calls the platform libm for atan2/cos/sin (SVD.hpp:129-140).
+// glibc 2.35 is not correctly rounded for about 0.1% of arguments, so the
+// oracle offers two modes:
+//   0: platform libm (what the reference's CPU build computes here);
+//   1: correctly rounded: the function evaluated in __float128 (libquadmath,
+//      about 113 bits) and rounded once to double, an implementation that is
+//      independent of the GPU kernels' own correctly rounded routines.
+int g_libm_mode = 0;
+
+inline double j_atan2( double y, double x )
+{
+    return g_libm_mode ? static_cast<double>( atan2q( y, x ) )
+                       : std::atan2( y, x );
+}
+inline double j_cos( double x )
+{
+    return g_libm_mode ? static_cast<double>( cosq( x ) ) : std::cos( x );
+}
+inline double j_sin( double x )
+{
+    return g_libm_mode ? static_cast<double>( sinq( x ) ) : std::sin( x );
+}
+
+// ------------------------------------------------------------------ RBFs
+// RBF.hpp:45-147. Expressions keep the reference's evaluation order.
+double rbf_eval( int rbf, double x )
+{
+    switch ( rbf )
+    {
+    case DTKX_RBF_WENDLAND0: // RBF.hpp:48-51
+        return ( 1.0 - x ) * ( 1.0 - x );
+    case DTKX_RBF_WENDLAND2: // RBF.hpp:57-61
+        return ( 1.0 - x ) * ( 1.0 - x ) * ( 1.0 - x ) * ( 1.0 - x ) *
+               ( 4.0 * x + 1.0 );
+    case DTKX_RBF_WENDLAND4: // RBF.hpp:67-71
+        return ( 1.0 - x ) * ( 1.0 - x ) * ( 1.0 - x ) * ( 1.0 - x ) *
+               ( 1.0 - x ) * ( 1.0 - x ) * ( 35.0 * x * x + 18.0 * x + 3.0 );
+    case DTKX_RBF_WENDLAND6: // RBF.hpp:77-82
+        return ( 1.0 - x ) * ( 1.0 - x ) * ( 1.0 - x ) * ( 1.0 - x ) *
+               ( 1.0 - x ) * ( 1.0 - x ) * ( 1.0 - x ) * ( 1.0 - x ) *
+               ( 32.0 * x * x * x + 25.0 * x * x + 8.0 * x + 1.0 );
+    case DTKX_RBF_WU2: // RBF.hpp:91-95
+        return ( 1.0 - x ) * ( 1.0 - x ) * ( 1.0 - x ) * ( 1.0 - x ) *
+               ( 3.0 * x * x * x + 12.0 * x * x + 16.0 * x + 4.0 );
+    case DTKX_RBF_WU4: // RBF.hpp:101-107
+        return ( 1.0 - x ) * ( 1.0 - x ) * ( 1.0 - x ) * ( 1.0 - x ) *
+               ( 1.0 - x ) * ( 1.0 - x ) *
+               ( 5.0 * x * x * x * x * x + 30.0 * x * x * x * x +
+                 72.0 * x * x * x + 82.0 * x * x + 36.0 * x + 6.0 );
+    case DTKX_RBF_BUHMANN2: // RBF.hpp:116-120
+        return 2.0 * x * x * x * x * std::log( x ) -
+               7.0 / 2.0 * x * x * x * x + 16 / 3.0 * x * x * x - 2 * x * x +
+               1.0 / 6.0;
+    case DTKX_RBF_BUHMANN3: // RBF.hpp:126-133
+        return x * x * x * x * x * x * x * x -
+               84.0 / 5.0 * x * x * x * x * x * x +
+               1024.0 / 5.0 * x * x * x * x * std::sqrt( x ) -
+               378.0 * x * x * x * x +
+               1024.0 / 5.0 * x * x * x * std::sqrt( x ) -
+               84.0 / 5.0 * x * x + 1.0;
+    case DTKX_RBF_BUHMANN4: // RBF.hpp:139-146
+        return 99.0 / 35.0 * x * x * x * x * x * x * x * x -
+               132.0 * x * x * x * x * x * x +
+               9216.0 / 35.0 * x * x * x * x * x * std::sqrt( x ) -
+               11264.0 / 35.0 * x * x * x * x * std::sqrt( x ) +
+               198.0 * x * x * x * x - 396.0 / 5.0 * x * x + 1.0;
+    }
+    return std::numeric_limits<double>::quiet_NaN();
+}
+
+// RadialBasisFunction<RBF>::operator() (RBF.hpp:32-35): rbf(x / radius).
+inline double rbf_radial( int rbf, double radius, double x )
+{
+    return rbf_eval( rbf, x / radius );
+}
+
+// --------------------------------------------------------------- basis
+// Basis.hpp:249-304. T is the coordinate type the point carries: float
+// under the reference's ArborX::Point, double under GEOMETRY_DOUBLE. The
+// products are formed in T and then stored as double.
+template <typename T>
+void basis_eval( int m, int dim, T const *p, double *out )
+{
+    if ( m == 1 )
+    {
+        out[0] = 1.;
+        return;
+    }
+    if ( dim == 3 )
+    {
+        out[0] = 1.;
+        out[1] = p[0];
+        out[2] = p[1];
+        out[3] = p[2];
+        if ( m == 10 )
+        {
+            out[4] = p[0] * p[0];
+            out[5] = p[0] * p[1];
+            out[6] = p[0] * p[2];
+            out[7] = p[1] * p[1];
+            out[8] = p[1] * p[2];
+            out[9] = p[2] * p[2];
+        }
+    }
+    else
+    { // 2D (Basis.hpp:287-304)
+        out[0] = 1.;
+        out[1] = p[0];
+        out[2] = p[1];
+        if ( m == 6 )
+        {
+            out[3] = p[0] * p[0];
+            out[4] = p[0] * p[1];
+            out[5] = p[1] * p[1];
+        }
+    }
+}
+
+// ------------------------------------------------------------ distance
+// ArborX 1.x Details::distance(Point, Point) restated (see header): float
+// accumulation from 0, then sqrt.
+inline float distance_f( float const *a, float const *b )
+{
+    float s = 0.f;
+    for ( int d = 0; d < 3; ++d )
+    {
+        float tmp = b[d] - a[d];
+        s += tmp * tmp;
+    }
+    return std::sqrt( s );
+}
+
+inline double distance_d( double const *a, double const *b )
+{
+    double s = 0.;
+    for ( int d = 0; d < 3; ++d )
+    {
+        double tmp = b[d] - a[d];
+        s += tmp * tmp;
+    }
+    return std::sqrt( s );
+}
+
+// ------------------------------------------------------------------ SVD
+// SVDFunctor (SVD.hpp:39-309) on one n x n matrix held in row-major
+// scratch. E, U, V follow the reference's aux layout semantics.
+int sgn( double x ) { return ( x > 0 ) - ( x < 0 ); } // SVD.hpp:29-33
+
+struct M2
+{
+    double a[2][2];
+};
+
+inline M2 trans_2x2( M2 const &A ) // SVD.hpp:93-97
+{
+    M2 B;
+    B.a[0][0] = A.a[0][0];
+    B.a[0][1] = A.a[1][0];
+    B.a[1][0] = A.a[0][1];
+    B.a[1][1] = A.a[1][1];
+    return B;
+}
+
+inline M2 mult_2x2( M2 const &A, M2 const &B ) // SVD.hpp:109-117
+{
+    M2 C;
+    C.a[0][0] = A.a[0][0] * B.a[0][0] + A.a[0][1] * B.a[1][0];
+    C.a[0][1] = A.a[0][0] * B.a[0][1] + A.a[0][1] * B.a[1][1];
+    C.a[1][0] = A.a[1][0] * B.a[0][0] + A.a[1][1] * B.a[1][0];
+    C.a[1][1] = A.a[1][0] * B.a[0][1] + A.a[1][1] * B.a[1][1];
+    return C;
+}
+
+// svd_2x2 (SVD.hpp:119-162). The singular values E are not used by the
+// caller (SVD.hpp:258-263 only reads L and R), so they are not formed.
+inline void svd_2x2( M2 const &A, M2 &U, M2 &V )
+{
+    M2 At = trans_2x2( A );
+    M2 AAt = mult_2x2( A, At );
+    M2 AtA = mult_2x2( At, A );
+
+    double phi =
+        0.5 * j_atan2( AAt.a[0][1] + AAt.a[1][0], AAt.a[0][0] - AAt.a[1][1] );
+    double cphi = j_cos( phi );
+    double sphi = j_sin( phi );
+    U.a[0][0] = cphi;
+    U.a[0][1] = -sphi;
+    U.a[1][0] = sphi;
+    U.a[1][1] = cphi;
+
+    double theta =
+        0.5 * j_atan2( AtA.a[0][1] + AtA.a[1][0], AtA.a[0][0] - AtA.a[1][1] );
+    double ctheta = j_cos( theta );
+    double stheta = j_sin( theta );
+    M2 W;
+    W.a[0][0] = ctheta;
+    W.a[0][1] = -stheta;
+    W.a[1][0] = stheta;
+    W.a[1][1] = ctheta;
+
+    M2 AW = mult_2x2( A, W );
+    M2 Ut = trans_2x2( U );
+    M2 S = mult_2x2( Ut, AW );
+    M2 C;
+    C.a[0][0] = std::copysign( 1., S.a[0][0] );
+    C.a[0][1] = 0.0;
+    C.a[1][0] = 0.0;
+    C.a[1][1] = std::copysign( 1., S.a[1][1] );
+    V = mult_2x2( W, C );
+}
+
+struct SvdWork
+{
+    int n;
+    std::vector<double> E, U, V;
+    explicit SvdWork( int n_ )
+        : n( n_ )
+        , E( n_ * n_ )
+        , U( n_ * n_ )
+        , V( n_ * n_ )
+    {
+    }
+};
+
+// givens_left / givens_right (SVD.hpp:65-91), row-major n x n.
+inline void givens_left( double *A, int n, double c, double s, int i, int k )
+{
+    for ( int j = 0; j < n; j++ )
+    {
+        double aij = A[i * n + j];
+        double akj = A[k * n + j];
+        A[i * n + j] = c * aij - s * akj;
+        A[k * n + j] = s * aij + c * akj;
+    }
+}
+
+inline void givens_right( double *A, int n, double c, double s, int i, int k )
+{
+    for ( int j = 0; j < n; ++j )
+    {
+        double aji = A[j * n + i];
+        double ajk = A[j * n + k];
+        A[j * n + i] = c * aji - s * ajk;
+        A[j * n + k] = s * aji + c * ajk;
+    }
+}
+
+// norm_F_wo_diag (SVD.hpp:184-195)
+inline double norm_off( double const *A, int n )
+{
+    double norm = 0.0;
+    for ( int i = 0; i < n; i++ )
+        for ( int j = 0; j < n; j++ )
+            norm += ( ( i != j ) ? A[i * n + j] * A[i * n + j] : 0 );
+    return std::sqrt( norm );
+}
+
+// argmax_off_diagonal (SVD.hpp:164-182): first strict maximum, row-major.
+inline void argmax_off( double const *A, int n, int &p, int &q )
+{
+    p = -1;
+    q = -1;
+    double max = -1;
+    for ( int i = 0; i < n; i++ )
+        for ( int j = 0; j < n; j++ )
+            if ( i != j && std::abs( A[i * n + j] ) > max )
+            {
+                p = i;
+                q = j;
+                max = std::abs( A[i * n + j] );
+            }
+}
+
+// The Jacobi loop of SVDFunctor::operator() (SVD.hpp:224-274) with a
+// rotation cap the reference lacks (its loop has none). Returns the number
+// of rotations; *converged tells whether the norm fell below DBL_EPSILON.
+long jacobi_classical( SvdWork &w, double const *A, long max_rot,
+                       bool *converged )
+{
+    int const n = w.n;
+    double *E = w.E.data(), *U = w.U.data(), *V = w.V.data();
+    for ( int i = 0; i < n; i++ )
+        for ( int j = 0; j < n; j++ )
+            E[i * n + j] = A[i * n + j];
+    for ( int i = 0; i < n; i++ )
+        for ( int j = 0; j < n; j++ )
+        {
+            U[i * n + j] = ( i == j ? 1.0 : 0.0 );
+            V[i * n + j] = ( i == j ? 1.0 : 0.0 );
+        }
+    double norm = norm_off( E, n );
+    double const tol = DBL_EPSILON;
+    long rot = 0;
+    while ( norm > tol && rot < max_rot )
+    {
+        int p, q;
+        argmax_off( E, n, p, q );
+        if ( p > q )
+            std::swap( p, q );
+        M2 Apq;
+        Apq.a[0][0] = E[p * n + p];
+        Apq.a[0][1] = E[p * n + q];
+        Apq.a[1][0] = E[q * n + p];
+        Apq.a[1][1] = E[q * n + q];
+        M2 L, R;
+        svd_2x2( Apq, L, R );
+        double cl = L.a[0][0];
+        double sl = L.a[0][1];
+        double cr = R.a[0][0];
+        double sr = ( sgn( R.a[0][1] ) == sgn( R.a[1][0] ) ) ? -R.a[0][1]
+                                                              : R.a[0][1];
+        givens_left( E, n, cl, sl, p, q );
+        givens_right( E, n, cr, sr, p, q );
+        givens_right( U, n, cl, sl, p, q );
+        givens_left( V, n, cr, sr, p, q );
+        norm = norm_off( E, n );
+        ++rot;
+    }
+    *converged = !( norm > tol );
+    return rot;
+}
+
+// Pseudo-inverse entry (SVD.hpp:281-297): value = sum_k V(k,i) U(j,k)/E(k,k)
+// over |E(k,k)| >= tol, k ascending. `relative` replaces the absolute
+// threshold by rcond * max_k |E(k,k)| (a new-build option).
+inline double pinv_entry( SvdWork const &w, int i, int j, double thr,
+                          bool *under )
+{
+    int const n = w.n;
+    double value = 0;
+    for ( int k = 0; k < n; k++ )
+    {
+        double ekk = w.E[k * n + k];
+        if ( std::abs( ekk ) >= thr )
+            value += w.V[k * n + i] * w.U[j * n + k] / ekk;
+        else
+            *under = true;
+    }
+    return value;
+}
+
+inline double truncation_threshold( SvdWork const &w, int truncation,
+                                    double rcond )
+{
+    if ( truncation == DTKX_TRUNCATION_RELATIVE )
+    {
+        double smax = 0.;
+        for ( int k = 0; k < w.n; ++k )
+            smax = std::max( smax, std::abs( w.E[k * w.n + k] ) );
+        return rcond * smax;
+    }
+    return DBL_EPSILON;
+}
+
+long default_max_rotations( int n ) { return 200L * n * n + 1000; }
+
+// ------------------------------------------------------------- kNN (CPU)
+// A k-d tree over the rounded source points with the same (distance,
+// index) order and a pruning bound formed by the same float operation
+// sequence as the point distance, so results equal brute force exactly.
+template <typename T>
+struct KdTree
+{
+    struct Node
+    {
+        T lo[3], hi[3];
+        int begin, end;  // range in perm (leaf) ...
+        int left, right; // children (-1 for leaf)
+    };
+    std::vector<T> pts; // [n][3]
+    std::vector<int> perm;
+    std::vector<Node> nodes;
+    int n = 0;
+    static constexpr int leaf_size = 8;
+
+    void build( std::vector<T> &&p )
+    {
+        pts = std::move( p );
+        n = static_cast<int>( pts.size() / 3 );
+        perm.resize( n );
+        for ( int i = 0; i < n; ++i )
+            perm[i] = i;
+        nodes.clear();
+        nodes.reserve( 2 * ( n / leaf_size + 1 ) + 1 );
+        if ( n > 0 )
+            build_rec( 0, n );
+    }
+
+    int build_rec( int b, int e )
+    {
+        int id = static_cast<int>( nodes.size() );
+        nodes.push_back( Node{} );
+        Node nd;
+        for ( int d = 0; d < 3; ++d )
+        {
+            nd.lo[d] = std::numeric_limits<T>::infinity();
+            nd.hi[d] = -std::numeric_limits<T>::infinity();
+        }
+        for ( int i = b; i < e; ++i )
+            for ( int d = 0; d < 3; ++d )
+            {
+                T v = pts[3 * perm[i] + d];
+                nd.lo[d] = std::min( nd.lo[d], v );
+                nd.hi[d] = std::max( nd.hi[d], v );
+            }
+        nd.begin = b;
+        nd.end = e;
+        nd.left = nd.right = -1;
+        if ( e - b > leaf_size )
+        {
+            int axis = 0;
+            T ext = nd.hi[0] - nd.lo[0];
+            for ( int d = 1; d < 3; ++d )
+                if ( nd.hi[d] - nd.lo[d] > ext )
+                {
+                    ext = nd.hi[d] - nd.lo[d];
+                    axis = d;
+                }
+            int mid = ( b + e ) / 2;
+            std::nth_element( perm.begin() + b, perm.begin() + mid,
+                              perm.begin() + e, [&]( int x, int y ) {
+                                  return pts[3 * x + axis] < pts[3 * y + axis];
+                              } );
+            int l = build_rec( b, mid );
+            int r = build_rec( mid, e );
+            nd.left = l;
+            nd.right = r;
+        }
+        nodes[id] = nd;
+        return id;
+    }
+
+    static T point_dist( T const *q, T const *b )
+    {
+        T s = 0;
+        for ( int d = 0; d < 3; ++d )
+        {
+            T tmp = b[d] - q[d];
+            s += tmp * tmp;
+        }
+        return std::sqrt( s );
+    }
+
+    // Lower bound: distance to the projection onto the box, formed with the
+    // same operation sequence (monotone rounding keeps it <= point_dist).
+    static T box_dist( T const *q, Node const &nd )
+    {
+        T proj[3];
+        for ( int d = 0; d < 3; ++d )
+        {
+            if ( q[d] < nd.lo[d] )
+                proj[d] = nd.lo[d];
+            else if ( q[d] > nd.hi[d] )
+                proj[d] = nd.hi[d];
+            else
+                proj[d] = q[d];
+        }
+        return point_dist( q, proj );
+    }
+
+    // k nearest of q; out_idx/out_dist sized kk = min(k, n), ascending.
+    void query( T const *q, int k, int *out_idx, T *out_dist ) const
+    {
+        int const kk = std::min( k, n );
+        std::vector<T> bd( kk, std::numeric_limits<T>::infinity() );
+        std::vector<int> bi( kk, std::numeric_limits<int>::max() );
+        int cnt = 0;
+        auto less = []( T da, int ia, T db, int ib ) {
+            return da < db || ( da == db && ia < ib );
+        };
+        struct Item
+        {
+            int node;
+            T d;
+        };
+        std::vector<Item> stack;
+        stack.push_back( {0, box_dist( q, nodes[0] )} );
+        while ( !stack.empty() )
+        {
+            Item it = stack.back();
+            stack.pop_back();
+            if ( cnt == kk && it.d > bd[kk - 1] )
+                continue;
+            Node const &nd = nodes[it.node];
+            if ( nd.left < 0 )
+            {
+                for ( int t = nd.begin; t < nd.end; ++t )
+                {
+                    int id = perm[t];
+                    T d = point_dist( q, &pts[3 * id] );
+                    if ( cnt < kk )
+                    {
+                        int pos = cnt++;
+                        while ( pos > 0 && less( d, id, bd[pos - 1], bi[pos - 1] ) )
+                        {
+                            bd[pos] = bd[pos - 1];
+                            bi[pos] = bi[pos - 1];
+                            --pos;
+                        }
+                        bd[pos] = d;
+                        bi[pos] = id;
+                    }
+                    else if ( less( d, id, bd[kk - 1], bi[kk - 1] ) )
+                    {
+                        int pos = kk - 1;
+                        while ( pos > 0 && less( d, id, bd[pos - 1], bi[pos - 1] ) )
+                        {
+                            bd[pos] = bd[pos - 1];
+                            bi[pos] = bi[pos - 1];
+                            --pos;
+                        }
+                        bd[pos] = d;
+                        bi[pos] = id;
+                    }
+                }
+                continue;
+            }
+            T dl = box_dist( q, nodes[nd.left] );
+            T dr = box_dist( q, nodes[nd.right] );
+            // push the farther child first so the nearer is visited first
+            if ( dl <= dr )
+            {
+                stack.push_back( {nd.right, dr} );
+                stack.push_back( {nd.left, dl} );
+            }
+            else
+            {
+                stack.push_back( {nd.left, dl} );
+                stack.push_back( {nd.right, dr} );
+            }
+        }
+        for ( int j = 0; j < kk; ++j )
+        {
+            out_idx[j] = bi[j];
+            out_dist[j] = bd[j];
+        }
+    }
+};
+
+// Read point i of an AoS ([n][3]) or SoA ([3][n]) double array.
+inline void load_point( double const *xyz, int64_t n, int layout, int64_t i,
+                        double *p )
+{
+    for ( int d = 0; d < 3; ++d )
+        p[d] = layout == DTKX_LAYOUT_SOA ? xyz[d * n + i] : xyz[3 * i + d];
+}
+
+template <typename T>
+void knn_impl( double const *src, int64_t ns, double const *tgt, int64_t nt,
+               int k, int layout, int32_t *indices, double *dist )
+{
+    std::vector<T> pts( 3 * ns );
+    for ( int64_t i = 0; i < ns; ++i )
+    {
+        double p[3];
+        load_point( src, ns, layout, i, p );
+        for ( int d = 0; d < 3; ++d )
+            pts[3 * i + d] = static_cast<T>( p[d] );
+    }
+    KdTree<T> tree;
+    tree.build( std::move( pts ) );
+    int const kk = static_cast<int>( std::min<int64_t>( k, ns ) );
+#pragma omp parallel
+    {
+        std::vector<int> idx( kk );
+        std::vector<T> dd( kk );
+#pragma omp for schedule( dynamic, 256 )
+        for ( int64_t t = 0; t < nt; ++t )
+        {
+            double p[3];
+            load_point( tgt, nt, layout, t, p );
+            T q[3] = {static_cast<T>( p[0] ), static_cast<T>( p[1] ),
+                      static_cast<T>( p[2] )};
+            tree.query( q, kk, idx.data(), dd.data() );
+            for ( int j = 0; j < kk; ++j )
+            {
+                indices[t * kk + j] = idx[j];
+                if ( dist )
+                    dist[t * kk + j] = static_cast<double>( dd[j] );
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------- per-target setup
+struct TargetWork
+{
+    int m;
+    std::vector<double> xrel; // [kk][3] relative coordinates (double)
+    std::vector<double> P;    // [kk][m]
+    std::vector<double> phi;  // [kk]
+    std::vector<double> A;    // [m][m]
+    std::vector<double> a0;   // row 0 of A^+
+    SvdWork svd;
+    TargetWork( int m_, int kk )
+        : m( m_ )
+        , xrel( 3 * kk )
+        , P( kk * m_ )
+        , phi( kk )
+        , A( m_ * m_ )
+        , a0( m_ )
+        , svd( m_ )
+    {
+    }
+};
+
+struct SetupCounters
+{
+    long under = 0, unconv = 0, rotations = 0;
+};
+
+// One target of the constructor's stages (def.hpp:66-118) with neighbours
+// given: transform (Impl.hpp:88-97), Vandermonde (Impl.hpp:180-189),
+// radius (Impl.hpp:109-135), weights (Impl.hpp:157-166), moments
+// (Impl.hpp:228-254), SVD pseudo-inverse (Impl.hpp:263-291 + SVD.hpp),
+// coefficients (Impl.hpp:308-334). Writes kk coefficients.
+void setup_target( TargetWork &w, double const *src, int64_t ns,
+                   double const *t, int layout, int32_t const *nbr, int kk,
+                   int rbf, int geometry, int truncation, double rcond,
+                   long max_rot, double *coeff, double *a0_out,
+                   SetupCounters &cnt )
+{
+    int const m = w.m;
+    // transformSourceCoordinates (Impl.hpp:92-95)
+    for ( int l = 0; l < kk; ++l )
+    {
+        double s[3];
+        load_point( src, ns, layout, nbr[l], s );
+        for ( int d = 0; d < 3; ++d )
+            w.xrel[3 * l + d] = s[d] - t[d];
+    }
+    // computeVandermonde (Impl.hpp:184-187) and the distances used by
+    // computeRadius (Impl.hpp:121-125) and computeWeights (Impl.hpp:162-165)
+    std::vector<double> dist( kk );
+    double const zero_d[3] = {0., 0., 0.};
+    float const zero_f[3] = {0.f, 0.f, 0.f};
+    for ( int l = 0; l < kk; ++l )
+    {
+        if ( geometry == DTKX_GEOMETRY_FLOAT )
+        {
+            float pf[3] = {static_cast<float>( w.xrel[3 * l] ),
+                           static_cast<float>( w.xrel[3 * l + 1] ),
+                           static_cast<float>( w.xrel[3 * l + 2] )};
+            basis_eval<float>( m, 3, pf, &w.P[l * m] );
+            dist[l] = distance_f( pf, zero_f );
+        }
+        else
+        {
+            basis_eval<double>( m, 3, &w.xrel[3 * l], &w.P[l * m] );
+            dist[l] = distance_d( &w.xrel[3 * l], zero_d );
+        }
+    }
+    // computeRadius (Impl.hpp:118-134)
+    double distance = 10. * DBL_EPSILON;
+    for ( int l = 0; l < kk; ++l )
+        if ( dist[l] > distance )
+            distance = dist[l];
+    double const radius = 1.1 * distance;
+    // computeWeights (Impl.hpp:161-165)
+    for ( int l = 0; l < kk; ++l )
+        w.phi[l] = rbf_radial( rbf, radius, dist[l] );
+    // computeMoments (Impl.hpp:242-253)
+    for ( int j = 0; j < m; ++j )
+        for ( int k = 0; k < m; ++k )
+        {
+            double tmp = 0.;
+            for ( int l = 0; l < kk; ++l )
+                tmp += w.P[l * m + j] * w.phi[l] * w.P[l * m + k];
+            w.A[j * m + k] = tmp;
+        }
+    // invertMoments -> SVDFunctor (SVD.hpp:224-297), row 0 only
+    bool conv = true;
+    cnt.rotations += jacobi_classical( w.svd, w.A.data(), max_rot, &conv );
+    if ( !conv )
+        ++cnt.unconv;
+    double const thr = truncation_threshold( w.svd, truncation, rcond );
+    bool under = false;
+    for ( int j = 0; j < m; ++j )
+        w.a0[j] = pinv_entry( w.svd, 0, j, thr, &under );
+    if ( under )
+        ++cnt.under;
+    if ( a0_out )
+        for ( int j = 0; j < m; ++j )
+            a0_out[j] = w.a0[j];
+    // computePolynomialCoefficients (Impl.hpp:326-333)
+    for ( int l = 0; l < kk; l++ )
+    {
+        coeff[l] = 0.;
+        for ( int j = 0; j < m; j++ )
+            coeff[l] += w.a0[j] * w.P[l * m + j] * w.phi[l];
+    }
+}
+
+} // namespace
+
+// =================================================================== C API
+extern "C" {
+
+struct dtkx_oracle_counters
+{
+    int64_t n_underdetermined;
+    int64_t n_unconverged;
+    int64_t total_rotations;
+};
+
+// Select the elementary-function mode (see j_atan2 above).
+void dtkx_oracle_set_libm_mode( int mode ) { g_libm_mode = mode ? 1 : 0; }
+int dtkx_oracle_get_libm_mode( void ) { return g_libm_mode; }
+
+// Elementary functions as the oracle evaluates them in the current mode
+// (op: 0 atan2(y, x), 1 sin(x), 2 cos(x)); y is ignored for sin/cos.
+void dtkx_oracle_elementary( int op, double const *y, double const *x,
+                             double *out, int64_t n )
+{
+    for ( int64_t i = 0; i < n; ++i )
+        out[i] = op == 0 ? j_atan2( y[i], x[i] )
+                         : ( op == 1 ? j_sin( x[i] ) : j_cos( x[i] ) );
+}
+
+int dtkx_oracle_num_threads( void )
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void dtkx_oracle_set_num_threads( int n )
+{
+#ifdef _OPENMP
+    if ( n > 0 )
+        omp_set_num_threads( n );
+#else
+    (void)n;
+#endif
+}
+
+// RBF values rbf(r / radius) for n radii (RBF.hpp:22-147).
+int dtkx_oracle_rbf( int rbf, double radius, double const *r, int64_t n,
+                     double *out )
+{
+    if ( rbf < 0 || rbf > DTKX_RBF_BUHMANN4 )
+        return DTKX_ERR_INVALID_ARGUMENT;
+    for ( int64_t i = 0; i < n; ++i )
+        out[i] = rbf_radial( rbf, radius, r[i] );
+    return 0;
+}
+
+// Basis values (Basis.hpp:249-304) for n points of dimension dim (2 or 3);
+// float_point != 0 rounds coordinates through float first (ArborX::Point).
+int dtkx_oracle_basis( int m, int dim, int float_point, double const *pts,
+                       int64_t n, double *out )
+{
+    for ( int64_t i = 0; i < n; ++i )
+    {
+        if ( float_point )
+        {
+            float p[3] = {0.f, 0.f, 0.f};
+            for ( int d = 0; d < dim; ++d )
+                p[d] = static_cast<float>( pts[dim * i + d] );
+            basis_eval<float>( m, dim, p, out + i * m );
+        }
+        else
+        {
+            double p[3] = {0., 0., 0.};
+            for ( int d = 0; d < dim; ++d )
+                p[d] = pts[dim * i + d];
+            basis_eval<double>( m, dim, p, out + i * m );
+        }
+    }
+    return 0;
+}
+
+// The 2x2 SVD of SVD.hpp:119-162 (returns L then R, row-major).
+void dtkx_oracle_svd_2x2( double const *a, double *L, double *R )
+{
+    M2 A, U, V;
+    std::memcpy( A.a, a, sizeof( A.a ) );
+    svd_2x2( A, U, V );
+    std::memcpy( L, U.a, sizeof( U.a ) );
+    std::memcpy( R, V.a, sizeof( V.a ) );
+}
+
+// Batched SVDFunctor (SVD.hpp:197-302) as driven by tstSVD.cpp:87-94 and
+// Impl.hpp:282-288: full n x n pseudo-inverses of general square matrices.
+int dtkx_oracle_svd_pseudo_inverse( int n, int64_t n_matrices, double const *a,
+                                    double *pinv, int truncation, double rcond,
+                                    long max_rot,
+                                    dtkx_oracle_counters *counters )
+{
+    if ( n < 1 )
+        return DTKX_ERR_INVALID_ARGUMENT;
+    if ( max_rot <= 0 )
+        max_rot = default_max_rotations( n );
+    long under = 0, unconv = 0, rots = 0;
+#pragma omp parallel reduction( + : under, unconv, rots )
+    {
+        SvdWork w( n );
+#pragma omp for schedule( dynamic, 1 )
+        for ( int64_t mi = 0; mi < n_matrices; ++mi )
+        {
+            bool conv = true;
+            rots += jacobi_classical( w, a + mi * n * n, max_rot, &conv );
+            if ( !conv )
+                ++unconv;
+            double const thr = truncation_threshold( w, truncation, rcond );
+            bool u = false;
+            for ( int i = 0; i < n; i++ )
+                for ( int j = 0; j < n; j++ )
+                    pinv[mi * n * n + i * n + j] = pinv_entry( w, i, j, thr, &u );
+            if ( u )
+                ++under;
+        }
+    }
+    if ( counters )
+    {
+        counters->n_underdetermined = under;
+        counters->n_unconverged = unconv;
+        counters->total_rotations = rots;
+    }
+    return 0;
+}
+
+// kNN (ArborX Nearest query, see header): rows of min(k, ns) indices,
+// ascending by (distance, index). geometry selects float or double points.
+int dtkx_oracle_knn( double const *src, int64_t ns, double const *tgt,
+                     int64_t nt, int k, int geometry, int layout,
+                     int32_t *indices, double *dist )
+{
+    if ( ns <= 0 || k <= 0 )
+        return DTKX_ERR_INVALID_ARGUMENT;
+    if ( geometry == DTKX_GEOMETRY_FLOAT )
+        knn_impl<float>( src, ns, tgt, nt, k, layout, indices, dist );
+    else
+        knn_impl<double>( src, ns, tgt, nt, k, layout, indices, dist );
+    return 0;
+}
+
+// Brute-force kNN over all sources (O(ns*nt)), same ordering. Used to pin
+// the k-d tree on small inputs.
+int dtkx_oracle_knn_brute( double const *src, int64_t ns, double const *tgt,
+                           int64_t nt, int k, int geometry, int32_t *indices,
+                           double *dist )
+{
+    int const kk = static_cast<int>( std::min<int64_t>( k, ns ) );
+#pragma omp parallel
+    {
+        std::vector<std::pair<double, int>> all( ns );
+#pragma omp for schedule( dynamic, 16 )
+        for ( int64_t t = 0; t < nt; ++t )
+        {
+            for ( int64_t s = 0; s < ns; ++s )
+            {
+                double d;
+                if ( geometry == DTKX_GEOMETRY_FLOAT )
+                {
+                    float a[3], b[3];
+                    for ( int c = 0; c < 3; ++c )
+                    {
+                        a[c] = static_cast<float>( tgt[3 * t + c] );
+                        b[c] = static_cast<float>( src[3 * s + c] );
+                    }
+                    d = distance_f( a, b );
+                }
+                else
+                    d = distance_d( &tgt[3 * t], &src[3 * s] );
+                all[s] = {d, static_cast<int>( s )};
+            }
+            std::partial_sort( all.begin(), all.begin() + kk, all.end() );
+            for ( int j = 0; j < kk; ++j )
+            {
+                indices[t * kk + j] = all[j].second;
+                if ( dist )
+                    dist[t * kk + j] = all[j].first;
+            }
+        }
+    }
+    return 0;
+}
+
+// Setup with given neighbour rows of length kk (def.hpp:66-118 after the
+// search). coeffs is [nt][kk]; a0 (optional) receives row 0 of A^+ per
+// target ([nt][m]).
+int dtkx_oracle_mls_setup( double const *src, int64_t ns, double const *tgt,
+                           int64_t nt, int layout, int32_t const *indices,
+                           int kk, int m, int rbf, int geometry,
+                           int truncation, double rcond, long max_rot,
+                           double *coeffs, double *a0,
+                           dtkx_oracle_counters *counters )
+{
+    if ( m != 1 && m != 4 && m != 10 )
+        return DTKX_ERR_INVALID_ARGUMENT;
+    if ( max_rot <= 0 )
+        max_rot = default_max_rotations( m );
+    long under = 0, unconv = 0, rots = 0;
+#pragma omp parallel reduction( + : under, unconv, rots )
+    {
+        TargetWork w( m, std::max( kk, 1 ) );
+        SetupCounters c;
+#pragma omp for schedule( dynamic, 64 )
+        for ( int64_t t = 0; t < nt; ++t )
+        {
+            double tp[3];
+            load_point( tgt, nt, layout, t, tp );
+            setup_target( w, src, ns, tp, layout, indices + t * kk, kk, rbf,
+                          geometry, truncation, rcond, max_rot,
+                          coeffs + t * kk, a0 ? a0 + t * m : nullptr, c );
+        }
+        under += c.under;
+        unconv += c.unconv;
+        rots += c.rotations;
+    }
+    if ( counters )
+    {
+        counters->n_underdetermined = under;
+        counters->n_unconverged = unconv;
+        counters->total_rotations = rots;
+    }
+    return 0;
+}
+
+// computeTargetValues (Impl.hpp:58-66) for n_fields fields with leading
+// dimensions; the row sum starts at 0 and runs j ascending.
+int dtkx_oracle_apply( int64_t nt, int kk, int32_t const *indices,
+                       double const *coeffs, double const *x, int64_t ld_src,
+                       double *y, int64_t ld_tgt, int n_fields )
+{
+#pragma omp parallel for schedule( static )
+    for ( int64_t i = 0; i < nt; ++i )
+        for ( int f = 0; f < n_fields; ++f )
+        {
+            double acc = 0.;
+            for ( int j = 0; j < kk; ++j )
+                acc += coeffs[i * kk + j] * x[f * ld_src + indices[i * kk + j]];
+            y[f * ld_tgt + i] = acc;
+        }
+    return 0;
+}
+
+// Explicit transpose: xbar[s] = sum over entries with index s of
+// coeff * ybar[row], accumulated in (row, j) ascending order.
+int dtkx_oracle_apply_transpose( int64_t nt, int kk, int32_t const *indices,
+                                 double const *coeffs, double const *ybar,
+                                 int64_t ld_tgt, double *xbar, int64_t ns,
+                                 int64_t ld_src, int n_fields )
+{
+    for ( int f = 0; f < n_fields; ++f )
+    {
+        for ( int64_t s = 0; s < ns; ++s )
+            xbar[f * ld_src + s] = 0.;
+        for ( int64_t i = 0; i < nt; ++i )
+            for ( int j = 0; j < kk; ++j )
+                xbar[f * ld_src + indices[i * kk + j]] +=
+                    coeffs[i * kk + j] * ybar[f * ld_tgt + i];
+    }
+    return 0;
+}
+
+// Full constructor (def.hpp:24-119) + nothing else: kNN, then setup.
+int dtkx_oracle_mls_create( double const *src, int64_t ns, double const *tgt,
+                            int64_t nt, int layout, int k, int m, int rbf,
+                            int geometry, int truncation, double rcond,
+                            long max_rot, int32_t *indices, double *coeffs,
+                            dtkx_oracle_counters *counters )
+{
+    if ( ns <= 0 )
+        return DTKX_ERR_EMPTY_SOURCE;
+    if ( k <= 0 )
+        k = m;
+    int const kk = static_cast<int>( std::min<int64_t>( k, ns ) );
+    if ( nt > 0 )
+    {
+        int rc = dtkx_oracle_knn( src, ns, tgt, nt, k, geometry, layout,
+                                  indices, nullptr );
+        if ( rc )
+            return rc;
+    }
+    return dtkx_oracle_mls_setup( src, ns, tgt, nt, layout, indices, kk, m,
+                                  rbf, geometry, truncation, rcond, max_rot,
+                                  coeffs, nullptr, counters );
+}
+
+} // extern "C"

@@ -1,0 +1,97 @@
+"""The C ABI library loads and exports every function include/dtkx/mls.h
+declares (no compute call: runs without a GPU). Also checks the ctypes
+structures match the header's layout and the option defaults."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import REPO
+
+HEADER = os.path.join(REPO, "include", "dtkx", "mls.h")
+LIB = os.path.join(REPO, "datatransferkit_amd", "libdtkx.so")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(dtkx_\w+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(LIB):
+        pytest.fail(f"{LIB} not built (run __graft_entry__.build())")
+    return ctypes.CDLL(LIB)
+
+
+def test_every_declared_symbol_is_exported(lib):
+    names = declared_functions()
+    assert len(names) >= 18
+    for n in names:
+        assert hasattr(lib, n), n
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r" T (dtkx_\w+)", out))
+    assert set(names) <= exported
+
+
+def test_python_binding_lists_the_header(lib):
+    from datatransferkit_amd import _lib
+
+    assert sorted(_lib.EXPORTED) == declared_functions()
+
+
+def test_struct_layouts_match_header(tmp_path):
+    # compile a C program against the header and print the struct layout
+    src = tmp_path / "layout.c"
+    src.write_text(r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "dtkx/mls.h"
+int main(void) {
+  printf("%zu %zu %zu %zu\n", sizeof(dtkx_mls_options), offsetof(dtkx_mls_options, rcond),
+         offsetof(dtkx_mls_options, stream), sizeof(dtkx_mls_stats));
+  printf("%zu %zu\n", offsetof(dtkx_mls_stats, ms_knn), sizeof(dtkx_mls_problem));
+  return 0;
+}
+""")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(REPO, "include"), "-o", str(exe), str(src)],
+                   check=True)
+    a = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    from datatransferkit_amd import _lib
+
+    assert int(a[0]) == ctypes.sizeof(_lib.Options)
+    assert int(a[1]) == _lib.Options.rcond.offset
+    assert int(a[2]) == _lib.Options.stream.offset
+    assert int(a[3]) == ctypes.sizeof(_lib.Stats)
+    assert int(a[4]) == _lib.Stats.ms_knn.offset
+    assert int(a[5]) == ctypes.sizeof(_lib.Problem)
+
+
+def test_options_defaults_are_reference_semantics(lib):
+    from datatransferkit_amd import _lib
+
+    o = _lib.default_options()
+    assert (o.dim, o.basis, o.rbf, o.k) == (3, 4, 0, 0)  # Wendland<0>, Linear3, k = m
+    assert (o.precision, o.geometry, o.solver, o.truncation) == (0, 0, 0, 0)
+
+
+def test_error_path_without_gpu(lib):
+    # argument validation fails before any device call
+    from datatransferkit_amd import _lib
+
+    o = _lib.default_options()
+    o.dim = 2
+    h = ctypes.c_void_p()
+    L = _lib.load()
+    rc = L.dtkx_mls_create(None, 0, None, 0, ctypes.byref(o), ctypes.byref(h))
+    assert rc == _lib.ERR_DIMENSION
+    assert "o.dim == 3" in L.dtkx_last_error().decode()
+    o.dim = 3
+    rc = L.dtkx_mls_create(None, 0, None, 0, ctypes.byref(o), ctypes.byref(h))
+    assert rc == _lib.ERR_EMPTY_SOURCE
+    assert ", failed in capi.hip, line" in L.dtkx_last_error().decode()
