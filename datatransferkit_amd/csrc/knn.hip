@@ -407,8 +407,11 @@ __device__ inline T box_sq( T const *q, T const *lo, T const *hi )
     return s;
 }
 
-__device__ inline float sqrt_rn( float s ) { return __fsqrt_rn( s ); }
-__device__ inline double sqrt_rn( double s ) { return __dsqrt_rn( s ); }
+// Correctly rounded square roots: llvm.sqrt without fast-math flags (the
+// library is built with -fhip-fp32-correctly-rounded-divide-sqrt). HIP's
+// __fsqrt_rn maps to the native 1-ulp instruction and must not be used.
+__device__ inline float sqrt_rn( float s ) { return __builtin_sqrtf( s ); }
+__device__ inline double sqrt_rn( double s ) { return __builtin_sqrt( s ); }
 
 constexpr int kQueryBlock = 64;
 

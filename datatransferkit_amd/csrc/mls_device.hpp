@@ -106,7 +106,7 @@ __device__ inline double rel_distance( int geometry, double const *xr )
             float tmp = 0.f - (float)xr[d];
             s += tmp * tmp;
         }
-        return (double)__fsqrt_rn( s );
+        return (double)__builtin_sqrtf( s ); // correctly rounded (not __fsqrt_rn)
     }
     double s = 0.;
     for ( int d = 0; d < 3; ++d )
@@ -114,7 +114,7 @@ __device__ inline double rel_distance( int geometry, double const *xr )
         double tmp = 0. - xr[d];
         s += tmp * tmp;
     }
-    return __dsqrt_rn( s );
+    return __builtin_sqrt( s );
 }
 
 // The 2x2 SVD of DTK_DetailsSVDImpl.hpp:119-162 reduced to what the Jacobi

@@ -44,7 +44,7 @@ __device__ inline double norm_off_diag( double const *E )
         for ( int j = 0; j < M; j++ )
             if ( i != j )
                 norm += E[i * M + j] * E[i * M + j];
-    return __dsqrt_rn( norm );
+    return __builtin_sqrt( norm );
 }
 
 template <int M>
@@ -372,7 +372,7 @@ __global__ void __launch_bounds__( kWave )
             for ( int j = 0; j < n; j++ )
                 if ( i != j )
                     norm += E[i * n + j] * E[i * n + j];
-        return __dsqrt_rn( norm );
+        return __builtin_sqrt( norm );
     };
     // the norm and argmax are computed by lane 0 and broadcast through LDS
     __shared__ double sh_norm;
