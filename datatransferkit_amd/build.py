@@ -22,6 +22,7 @@ SOURCES = [
     "points.hip",
     "knn.hip",
     "setup_classical.hip",
+    "setup_cyclic.hip",
     "apply.hip",
     "capi.hip",
 ]

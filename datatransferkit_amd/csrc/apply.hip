@@ -193,15 +193,15 @@ void build_csc( int32_t const *idx, int64_t nnz, int64_t n_src,
                 CscIndex &csc, hipStream_t s )
 {
     DTKX_REQUIRE( nnz < INT32_MAX, DTKX_ERR_UNSUPPORTED );
-    csc.col_offsets.resize( n_src + 1 );
-    csc.entries.resize( std::max<int64_t>( nnz, 1 ) );
+    csc.col_offsets.resize( n_src + 1, s );
+    csc.entries.resize( std::max<int64_t>( nnz, 1 ), s );
     if ( nnz == 0 )
     {
         DTKX_HIP( hipMemsetAsync( csc.col_offsets.get(), 0,
                                   sizeof( int32_t ) * ( n_src + 1 ), s ) );
         return;
     }
-    DeviceBuffer<int32_t> ids( nnz ), keys_sorted( nnz );
+    DeviceBuffer<int32_t> ids( nnz, s ), keys_sorted( nnz, s );
     iota_kernel<<<ceil_div( nnz, kBlock ), kBlock, 0, s>>>( ids.get(), nnz );
     int end_bit = 1;
     while ( ( int64_t( 1 ) << end_bit ) < n_src )
@@ -210,14 +210,13 @@ void build_csc( int32_t const *idx, int64_t nnz, int64_t n_src,
     DTKX_HIP( hipcub::DeviceRadixSort::SortPairs(
         nullptr, temp_bytes, idx, keys_sorted.get(), ids.get(),
         csc.entries.get(), static_cast<int>( nnz ), 0, end_bit, s ) );
-    DeviceBuffer<char> temp( temp_bytes );
+    DeviceBuffer<char> temp( temp_bytes, s );
     DTKX_HIP( hipcub::DeviceRadixSort::SortPairs(
         temp.get(), temp_bytes, idx, keys_sorted.get(), ids.get(),
         csc.entries.get(), static_cast<int>( nnz ), 0, end_bit, s ) );
     col_offsets_kernel<<<ceil_div( nnz + 1, kBlock ), kBlock, 0, s>>>(
         keys_sorted.get(), nnz, n_src, csc.col_offsets.get() );
     DTKX_HIP( hipGetLastError() );
-    DTKX_HIP( hipStreamSynchronize( s ) );
 }
 
 void crs_apply_transpose_f64( CscIndex const &csc, double const *coeffs,

@@ -111,7 +111,7 @@ struct Staged
             ptr = p;
             return;
         }
-        buf.resize( n );
+        buf.resize( n, s );
         DTKX_HIP( hipMemcpyAsync( buf.get(), p, n * sizeof( T ),
                                   hipMemcpyHostToDevice, s ) );
         ptr = buf.get();
@@ -127,7 +127,7 @@ struct StagedOut
     T *user = nullptr;
     T *ptr = nullptr;
     std::size_t n = 0;
-    StagedOut( T *p, std::size_t count )
+    StagedOut( T *p, std::size_t count, hipStream_t s )
         : user( p )
         , n( count )
     {
@@ -136,7 +136,7 @@ struct StagedOut
             ptr = p;
             return;
         }
-        buf.resize( n );
+        buf.resize( n, s );
         ptr = buf.get();
     }
     void finish( hipStream_t s )
@@ -201,12 +201,21 @@ void run_setup( dtkx_mls *h, unsigned long long *counters_d )
     p.geometry = h->opts.geometry;
     p.truncation = h->opts.truncation;
     p.rcond = h->opts.rcond;
+    p.group_size = h->opts.group_size;
+    if ( h->opts.store_pinv )
+        h->a0.resize( h->n_tgt * h->m, h->stream );
+    if ( h->opts.solver == DTKX_SOLVER_CYCLIC_JACOBI )
+    {
+        p.max_rot = h->opts.max_rotations; // sweeps; 0 = default
+        dtkx::setup_cyclic( h->src_d.get(), h->tgt_d.get(), h->n_tgt,
+                            h->indices.get(), h->order.get(), p,
+                            h->coeffs.get(), h->a0.get(), counters_d,
+                            h->stream );
+        h->stats.group_size_used = h->m == 10 ? 2 : 1;
+        return;
+    }
     p.max_rot = h->opts.max_rotations > 0 ? h->opts.max_rotations
                                           : default_rotation_cap( h->m );
-    p.group_size = h->opts.group_size;
-    REQ( h->opts.solver == DTKX_SOLVER_CLASSICAL_JACOBI, DTKX_ERR_UNSUPPORTED );
-    if ( h->opts.store_pinv )
-        h->a0.resize( h->n_tgt * h->m );
     int g = dtkx::setup_classical(
         h->src_d.get(), h->tgt_d.get(), h->n_tgt, h->indices.get(),
         h->order.get(), p, h->coeffs.get(), h->a0.get(), counters_d,
@@ -217,13 +226,13 @@ void run_setup( dtkx_mls *h, unsigned long long *counters_d )
 void finish_create( dtkx_mls *h, unsigned long long *counters_d )
 {
     int64_t const nnz = h->n_tgt * h->kk;
-    h->offsets.resize( h->n_tgt + 1 );
+    h->offsets.resize( h->n_tgt + 1, h->stream );
     offsets_kernel<<<dtkx::ceil_div( h->n_tgt + 1, 256 ), 256, 0,
                      h->stream>>>( h->offsets.get(), h->n_tgt, h->kk );
     DTKX_HIP( hipGetLastError() );
     if ( h->opts.precision == DTKX_PRECISION_FP32_BULK && nnz > 0 )
     {
-        h->coeffs32.resize( nnz );
+        h->coeffs32.resize( nnz, h->stream );
         to_f32_kernel<<<dtkx::ceil_div( nnz, 256 ), 256, 0, h->stream>>>(
             h->coeffs.get(), h->coeffs32.get(), nnz );
         DTKX_HIP( hipGetLastError() );
@@ -264,17 +273,16 @@ std::unique_ptr<dtkx_mls> make_handle( double const *src, int64_t n_src,
     REQ( (int64_t)h->kk * n_tgt < INT32_MAX, DTKX_ERR_UNSUPPORTED );
     Staged<double> s_src( src, 3 * n_src, h->stream );
     Staged<double> s_tgt( tgt, 3 * n_tgt, h->stream );
-    h->src_d.resize( n_src );
-    h->tgt_d.resize( std::max<int64_t>( n_tgt, 1 ) );
+    h->src_d.resize( n_src, h->stream );
+    h->tgt_d.resize( std::max<int64_t>( n_tgt, 1 ), h->stream );
     dtkx::points_to_double4( s_src.ptr, n_src, opts->layout, h->src_d.get(),
                              h->stream );
     dtkx::points_to_double4( s_tgt.ptr, n_tgt, opts->layout, h->tgt_d.get(),
                              h->stream );
-    h->indices.resize( std::max<int64_t>( n_tgt * h->kk, 1 ) );
-    h->coeffs.resize( std::max<int64_t>( n_tgt * h->kk, 1 ) );
-    h->order.resize( std::max<int64_t>( n_tgt, 1 ) );
-    // the staging buffers die at scope exit
-    DTKX_HIP( hipStreamSynchronize( h->stream ) );
+    h->indices.resize( std::max<int64_t>( n_tgt * h->kk, 1 ), h->stream );
+    h->coeffs.resize( std::max<int64_t>( n_tgt * h->kk, 1 ), h->stream );
+    h->order.resize( std::max<int64_t>( n_tgt, 1 ), h->stream );
+    // the staging buffers are freed in stream order at scope exit
     return h;
 }
 
@@ -316,7 +324,7 @@ int dtkx_mls_create( const double *src_xyz, int64_t n_src,
         REQ( out != nullptr, DTKX_ERR_INVALID_ARGUMENT );
         *out = nullptr;
         auto h = make_handle( src_xyz, n_src, tgt_xyz, n_tgt, opts );
-        dtkx::DeviceBuffer<unsigned long long> counters( 3 );
+        dtkx::DeviceBuffer<unsigned long long> counters( 3, h->stream );
         DTKX_HIP( hipMemsetAsync( counters.get(), 0, 3 * sizeof( uint64_t ),
                                   h->stream ) );
         if ( n_tgt > 0 )
@@ -326,15 +334,13 @@ int dtkx_mls_create( const double *src_xyz, int64_t n_src,
             int const k = opts->k > 0 ? opts->k : h->m;
             if ( opts->geometry == DTKX_GEOMETRY_FLOAT )
             {
-                dtkx::DeviceBuffer<float4> sf( n_src ), tf( n_tgt );
-                // float points are rounded from the caller's doubles
-                dtkx::DeviceBuffer<double> tmp;
-                Staged<double> s_src( src_xyz, 3 * n_src, h->stream );
-                Staged<double> s_tgt( tgt_xyz, 3 * n_tgt, h->stream );
-                dtkx::points_to_float4( s_src.ptr, n_src, opts->layout,
-                                        sf.get(), h->stream );
-                dtkx::points_to_float4( s_tgt.ptr, n_tgt, opts->layout,
-                                        tf.get(), h->stream );
+                // float points rounded from the caller's doubles
+                dtkx::DeviceBuffer<float4> sf( n_src, h->stream ),
+                    tf( n_tgt, h->stream );
+                dtkx::double4_to_float4( h->src_d.get(), n_src, sf.get(),
+                                         h->stream );
+                dtkx::double4_to_float4( h->tgt_d.get(), n_tgt, tf.get(),
+                                         h->stream );
                 dtkx::knn_search( opts->geometry, sf.get(), n_src, tf.get(),
                                   n_tgt, k, h->indices.get(), nullptr,
                                   h->order.get(), &info, h->stream );
@@ -371,8 +377,8 @@ int dtkx_mls_create_with_neighbors( const double *src_xyz, int64_t n_src,
         auto h = make_handle( src_xyz, n_src, tgt_xyz, n_tgt, &o );
         h->opts.k = k_row;
         h->kk = k_row;
-        h->indices.resize( std::max<int64_t>( n_tgt * k_row, 1 ) );
-        h->coeffs.resize( std::max<int64_t>( n_tgt * k_row, 1 ) );
+        h->indices.resize( std::max<int64_t>( n_tgt * k_row, 1 ), h->stream );
+        h->coeffs.resize( std::max<int64_t>( n_tgt * k_row, 1 ), h->stream );
         h->order.release(); // natural target order
         if ( n_tgt > 0 )
         {
@@ -389,7 +395,7 @@ int dtkx_mls_create_with_neighbors( const double *src_xyz, int64_t n_src,
                                       hipMemcpyHostToDevice, h->stream ) );
             DTKX_HIP( hipStreamSynchronize( h->stream ) );
         }
-        dtkx::DeviceBuffer<unsigned long long> counters( 3 );
+        dtkx::DeviceBuffer<unsigned long long> counters( 3, h->stream );
         DTKX_HIP( hipMemsetAsync( counters.get(), 0, 3 * sizeof( uint64_t ),
                                   h->stream ) );
         if ( n_tgt > 0 )
@@ -420,7 +426,7 @@ int dtkx_mls_apply( const dtkx_mls *h, const double *src_vals,
         std::size_t const nin = (std::size_t)( n_fields - 1 ) * ld_src + h->n_src;
         std::size_t const nout = (std::size_t)( n_fields - 1 ) * ld_tgt + h->n_tgt;
         Staged<double> x( src_vals, nin, h->stream );
-        StagedOut<double> y( tgt_vals, nout );
+        StagedOut<double> y( tgt_vals, nout, h->stream );
         if ( h->opts.precision == DTKX_PRECISION_FP32_BULK )
             dtkx::crs_apply_f32_dvals( h->indices.get(), h->coeffs32.get(),
                                        h->n_tgt, h->kk, x.ptr, ld_src, y.ptr,
@@ -453,7 +459,7 @@ int dtkx_mls_apply_fp32( const dtkx_mls *h, const float *src_vals,
         std::size_t const nin = (std::size_t)( n_fields - 1 ) * ld_src + h->n_src;
         std::size_t const nout = (std::size_t)( n_fields - 1 ) * ld_tgt + h->n_tgt;
         Staged<float> x( src_vals, nin, h->stream );
-        StagedOut<float> y( tgt_vals, nout );
+        StagedOut<float> y( tgt_vals, nout, h->stream );
         dtkx::crs_apply_f32( h->indices.get(), h->coeffs32.get(), h->n_tgt,
                              h->kk, x.ptr, ld_src, y.ptr, ld_tgt, n_fields,
                              h->stream );
@@ -484,7 +490,7 @@ int dtkx_mls_apply_transpose( const dtkx_mls *h, const double *tgt_bar,
         std::size_t const nin = (std::size_t)( n_fields - 1 ) * ld_tgt + h->n_tgt;
         std::size_t const nout = (std::size_t)( n_fields - 1 ) * ld_src + h->n_src;
         Staged<double> yb( tgt_bar, nin, h->stream );
-        StagedOut<double> xb( src_bar, nout );
+        StagedOut<double> xb( src_bar, nout, h->stream );
         dtkx::crs_apply_transpose_f64( h->csc, h->coeffs.get(), h->kk,
                                        h->n_src, yb.ptr, ld_tgt, xb.ptr,
                                        ld_src, n_fields, h->stream );
@@ -569,11 +575,11 @@ int dtkx_knn( const double *src_xyz, int64_t n_src, const double *tgt_xyz,
         int const kk = k < n_src ? k : static_cast<int>( n_src );
         Staged<double> sx( src_xyz, 3 * n_src, s );
         Staged<double> tx( tgt_xyz, 3 * n_tgt, s );
-        StagedOut<int32_t> oi( indices, (std::size_t)n_tgt * kk );
-        StagedOut<double> od( dist, dist ? (std::size_t)n_tgt * kk : 0 );
+        StagedOut<int32_t> oi( indices, (std::size_t)n_tgt * kk, s );
+        StagedOut<double> od( dist, dist ? (std::size_t)n_tgt * kk : 0, s );
         if ( geometry == DTKX_GEOMETRY_FLOAT )
         {
-            dtkx::DeviceBuffer<float4> sf( n_src ), tf( n_tgt );
+            dtkx::DeviceBuffer<float4> sf( n_src, s ), tf( n_tgt, s );
             dtkx::points_to_float4( sx.ptr, n_src, layout, sf.get(), s );
             dtkx::points_to_float4( tx.ptr, n_tgt, layout, tf.get(), s );
             dtkx::knn_search( geometry, sf.get(), n_src, tf.get(), n_tgt, k,
@@ -581,7 +587,7 @@ int dtkx_knn( const double *src_xyz, int64_t n_src, const double *tgt_xyz,
         }
         else
         {
-            dtkx::DeviceBuffer<double4> sd( n_src ), td( n_tgt );
+            dtkx::DeviceBuffer<double4> sd( n_src, s ), td( n_tgt, s );
             dtkx::points_to_double4( sx.ptr, n_src, layout, sd.get(), s );
             dtkx::points_to_double4( tx.ptr, n_tgt, layout, td.get(), s );
             dtkx::knn_search( geometry, sd.get(), n_src, td.get(), n_tgt, k,
@@ -609,8 +615,8 @@ int dtkx_svd_pseudo_inverse( int32_t n, int64_t n_matrices, const double *a,
         REQ( max_rotations >= 0, DTKX_ERR_INVALID_ARGUMENT );
         std::size_t const cnt = (std::size_t)n_matrices * n * n;
         Staged<double> sa( a, cnt, s );
-        StagedOut<double> sp( pinv, cnt );
-        dtkx::DeviceBuffer<unsigned long long> counters( 3 );
+        StagedOut<double> sp( pinv, cnt, s );
+        dtkx::DeviceBuffer<unsigned long long> counters( 3, s );
         DTKX_HIP( hipMemsetAsync( counters.get(), 0, 3 * sizeof( uint64_t ), s ) );
         long const cap =
             max_rotations > 0 ? max_rotations : default_rotation_cap( n );

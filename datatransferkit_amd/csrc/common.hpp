@@ -47,32 +47,29 @@ struct Error : std::runtime_error
                            __LINE__ );                                         \
     } while ( 0 )
 
-// Owning device buffer (hipMalloc/hipFree), move-only.
+// Owning device buffer, move-only. With a stream it uses the stream-ordered
+// allocator (hipMallocAsync / hipFreeAsync on that stream), which neither
+// synchronises the device nor pays hipMalloc's cost on every call; without
+// one it falls back to hipMalloc / hipFree.
 template <typename T>
 struct DeviceBuffer
 {
     T *ptr = nullptr;
     std::size_t n = 0;
+    hipStream_t stream = nullptr;
+    bool async = false;
     DeviceBuffer() = default;
     explicit DeviceBuffer( std::size_t count ) { resize( count ); }
+    DeviceBuffer( std::size_t count, hipStream_t s ) { resize( count, s ); }
     DeviceBuffer( DeviceBuffer const & ) = delete;
     DeviceBuffer &operator=( DeviceBuffer const & ) = delete;
-    DeviceBuffer( DeviceBuffer &&o ) noexcept
-        : ptr( o.ptr )
-        , n( o.n )
-    {
-        o.ptr = nullptr;
-        o.n = 0;
-    }
+    DeviceBuffer( DeviceBuffer &&o ) noexcept { steal( o ); }
     DeviceBuffer &operator=( DeviceBuffer &&o ) noexcept
     {
         if ( this != &o )
         {
             release();
-            ptr = o.ptr;
-            n = o.n;
-            o.ptr = nullptr;
-            o.n = 0;
+            steal( o );
         }
         return *this;
     }
@@ -91,15 +88,44 @@ struct DeviceBuffer
             }
         }
         n = count;
+        async = false;
+    }
+    void resize( std::size_t count, hipStream_t s )
+    {
+        release();
+        if ( count > 0 )
+        {
+            hipError_t e = hipMallocAsync( (void **)&ptr, count * sizeof( T ), s );
+            if ( e != hipSuccess )
+            {
+                ptr = nullptr;
+                ::dtkx::raise( DTKX_ERR_OUT_OF_MEMORY, "hipMallocAsync",
+                               __FILE__, __LINE__ );
+            }
+        }
+        n = count;
+        stream = s;
+        async = true;
     }
     void release()
     {
         if ( ptr )
-            (void)hipFree( ptr );
+            (void)( async ? hipFreeAsync( ptr, stream ) : hipFree( ptr ) );
         ptr = nullptr;
         n = 0;
     }
     T *get() const { return ptr; }
+
+  private:
+    void steal( DeviceBuffer &o )
+    {
+        ptr = o.ptr;
+        n = o.n;
+        stream = o.stream;
+        async = o.async;
+        o.ptr = nullptr;
+        o.n = 0;
+    }
 };
 
 // True if p is device-accessible memory (hipMalloc, managed, or registered).
@@ -111,6 +137,9 @@ void points_to_float4( double const *xyz, int64_t n, int layout, float4 *out,
                        hipStream_t s );
 void points_to_double4( double const *xyz, int64_t n, int layout,
                         double4 *out, hipStream_t s );
+
+void double4_to_float4( double4 const *in, int64_t n, float4 *out,
+                        hipStream_t s );
 
 inline unsigned ceil_div( int64_t a, int64_t b )
 {

@@ -1,9 +1,11 @@
 // LBVH k-nearest-neighbour search for gfx950.
 //
 // Build: scene box -> 63-bit Morton codes (21 bits per axis) -> radix sort
-// of (code, index) -> Karras hierarchy (duplicate codes split by index) ->
-// one bottom-up pass that writes both child boxes into each parent and the
-// subtree heights (the second thread to reach a node continues upward).
+// of (code, index) -> Karras hierarchy (duplicate codes split by index), in
+// which each internal node also writes both child boxes: exact for small
+// children, the Morton cell of the child's common prefix for large ones (a
+// conservative box needs no bottom-up pass, whose device-scope fences are
+// costly across the XCDs' separate L2 caches).
 // Query: one lane per target, targets processed in Morton order so a wave
 // walks a compact region; a per-lane stack in LDS; the k best candidates
 // stay sorted in registers as 64-bit keys (float distance bits << 32 |
@@ -173,10 +175,69 @@ __device__ inline int prefix_delta( uint64_t const *codes, int64_t n,
     return 64 + __clz( (uint32_t)i ^ (uint32_t)j );
 }
 
-// parent encodings: (parent << 1) | slot
+__device__ inline uint32_t compact21( uint64_t x )
+{
+    x &= 0x1249249249249249ull;
+    x = ( x | x >> 2 ) & 0x10c30c30c30c30c3ull;
+    x = ( x | x >> 4 ) & 0x100f00f00f00f00full;
+    x = ( x | x >> 8 ) & 0x1f0000ff0000ffull;
+    x = ( x | x >> 16 ) & 0x1f00000000ffffull;
+    x = ( x | x >> 32 ) & 0x1fffffull;
+    return (uint32_t)x;
+}
+
+__device__ inline float round_down( double v ) { return __double2float_rd( v ); }
+__device__ inline float round_up( double v ) { return __double2float_ru( v ); }
+__device__ inline double round_down_d( double v ) { return v; }
+
+// Conservative box of every point whose Morton code shares the first
+// `delta` bits of `code` (bit 63 is always 0): the quantisation cell of the
+// prefix, mapped back to world coordinates with a margin and outward
+// rounding, so it contains the points whatever the rounding of the forward
+// quantisation (morton_kernel).
 template <typename T>
+__device__ inline void cell_box( uint64_t code, int delta, T const *scene,
+                                 T *lo, T *hi )
+{
+    int const nb = delta >= 64 ? 63 : ( delta - 1 ); // known bits below 63
+    uint64_t const all = ( 1ull << 63 ) - 1;
+    uint64_t const mask = nb <= 0 ? 0ull : ( all & ~( ( 1ull << ( 63 - nb ) ) - 1 ) );
+    uint64_t const pmin = code & mask;
+    uint64_t const pmax = ( code & mask ) | ( all & ~mask );
+    uint32_t const qmin[3] = {compact21( pmin >> 2 ), compact21( pmin >> 1 ),
+                              compact21( pmin )};
+    uint32_t const qmax[3] = {compact21( pmax >> 2 ), compact21( pmax >> 1 ),
+                              compact21( pmax )};
+    for ( int d = 0; d < 3; ++d )
+    {
+        double const s0 = scene[d], ext = (double)scene[3 + d] - s0;
+        double const h = ext / 2097151.0;
+        double const margin = 1e-9 * ext + 1e-300;
+        double l = s0 + qmin[d] * h - margin;
+        double u = s0 + ( qmax[d] + 1.0 ) * h + margin;
+        if ( sizeof( T ) == 4 )
+        {
+            lo[d] = (T)round_down( l );
+            hi[d] = (T)round_up( u );
+        }
+        else
+        {
+            lo[d] = (T)l;
+            hi[d] = (T)u;
+        }
+    }
+}
+
+// Children of internal node i: ranges, codes, parents and child boxes.
+// A child that is a leaf gets its point as a degenerate box and the code
+// ~source_index; an internal child covering at most kTight points gets its
+// exact box, a larger one the Morton cell of its common prefix.
+constexpr int kTight = 8;
+
+template <typename T, typename V>
 __global__ void __launch_bounds__( kBlock )
-    karras_kernel( uint64_t const *codes, int64_t n, Node<T> *nodes,
+    karras_kernel( uint64_t const *codes, int32_t const *sorted_ids,
+                   V const *pts, int64_t n, T const *scene, Node<T> *nodes,
                    int32_t *parent_internal, int32_t *parent_leaf )
 {
     int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
@@ -203,84 +264,73 @@ __global__ void __launch_bounds__( kBlock )
         if ( prefix_delta( codes, n, i, i + ( s + t ) * d ) > dnode )
             s += t;
     } while ( t > 1 );
-    int64_t gamma = i + s * d + ( d < 0 ? -1 : 0 );
-    int64_t lo = i < j ? i : j, hi = i < j ? j : i;
-    if ( lo == gamma )
+    int64_t const gamma = i + s * d + ( d < 0 ? -1 : 0 );
+    int64_t const first = i < j ? i : j, last = i < j ? j : i;
+    int64_t const rb[2] = {first, gamma + 1};
+    int64_t const re[2] = {gamma, last};
+    Node<T> nd;
+    for ( int c = 0; c < 2; ++c )
     {
-        nodes[i].child[0] = ~(int32_t)gamma; // leaf position (rewritten)
-        parent_leaf[gamma] = (int32_t)( i << 1 );
+        int64_t const b = rb[c], e = re[c];
+        if ( b == e )
+        {
+            int32_t const orig = sorted_ids[b];
+            V const p = pts[orig];
+            nd.lo[c][0] = nd.hi[c][0] = p.x;
+            nd.lo[c][1] = nd.hi[c][1] = p.y;
+            nd.lo[c][2] = nd.hi[c][2] = p.z;
+            nd.child[c] = ~orig;
+            parent_leaf[b] = (int32_t)( ( i << 1 ) | c );
+            continue;
+        }
+        int64_t const child = c == 0 ? gamma : gamma + 1;
+        nd.child[c] = (int32_t)child;
+        parent_internal[child] = (int32_t)( ( i << 1 ) | c );
+        if ( e - b + 1 <= kTight )
+        {
+            T lo[3] = {INFINITY, INFINITY, INFINITY};
+            T hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for ( int64_t q = b; q <= e; ++q )
+            {
+                V const p = pts[sorted_ids[q]];
+                lo[0] = fmin( lo[0], p.x );
+                lo[1] = fmin( lo[1], p.y );
+                lo[2] = fmin( lo[2], p.z );
+                hi[0] = fmax( hi[0], p.x );
+                hi[1] = fmax( hi[1], p.y );
+                hi[2] = fmax( hi[2], p.z );
+            }
+            for ( int q = 0; q < 3; ++q )
+            {
+                nd.lo[c][q] = lo[q];
+                nd.hi[c][q] = hi[q];
+            }
+        }
+        else
+            cell_box<T>( codes[b], prefix_delta( codes, n, b, e ), scene,
+                         nd.lo[c], nd.hi[c] );
     }
-    else
-    {
-        nodes[i].child[0] = (int32_t)gamma;
-        parent_internal[gamma] = (int32_t)( i << 1 );
-    }
-    if ( hi == gamma + 1 )
-    {
-        nodes[i].child[1] = ~(int32_t)( gamma + 1 );
-        parent_leaf[gamma + 1] = (int32_t)( ( i << 1 ) | 1 );
-    }
-    else
-    {
-        nodes[i].child[1] = (int32_t)( gamma + 1 );
-        parent_internal[gamma + 1] = (int32_t)( ( i << 1 ) | 1 );
-    }
+    nd.height[0] = nd.height[1] = 0;
+    nodes[i] = nd;
 }
 
-// Bottom-up: each leaf thread writes its box into its parent's slot; the
-// second thread to arrive at a node forms the union and continues.
-template <typename T, typename V>
+// Depth of every leaf by walking its parents (reads only); the maximum is
+// the tree height that sizes the traversal stack.
 __global__ void __launch_bounds__( kBlock )
-    refit_kernel( V const *pts, int32_t const *sorted_ids, int64_t n,
-                  Node<T> *nodes, int32_t const *parent_internal,
-                  int32_t const *parent_leaf, int32_t *flags, int32_t *height )
+    leaf_depth_kernel( int32_t const *parent_internal,
+                       int32_t const *parent_leaf, int64_t n, int32_t *height )
 {
     int64_t p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
     if ( p >= n )
         return;
-    int32_t orig = sorted_ids[p];
-    V pt = pts[orig];
-    T lo[3] = {pt.x, pt.y, pt.z};
-    T hi[3] = {pt.x, pt.y, pt.z};
-    int32_t code = ~orig;
-    int32_t h = 0;
-    int32_t enc = parent_leaf[p];
-    while ( true )
+    int32_t node = parent_leaf[p] >> 1;
+    int32_t depth = 1;
+    while ( node != 0 && depth < 4096 )
     {
-        int32_t par = enc >> 1, slot = enc & 1;
-        Node<T> *nd = nodes + par;
-        for ( int d = 0; d < 3; ++d )
-        {
-            nd->lo[slot][d] = lo[d];
-            nd->hi[slot][d] = hi[d];
-        }
-        nd->child[slot] = code;
-        nd->height[slot] = h;
-        __threadfence();
-        if ( atomicAdd( flags + par, 1 ) == 0 )
-            return;
-        __threadfence();
-        int other = 1 - slot;
-        for ( int d = 0; d < 3; ++d )
-        {
-            T olo = __hip_atomic_load( &nd->lo[other][d], __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT );
-            T ohi = __hip_atomic_load( &nd->hi[other][d], __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT );
-            lo[d] = fmin( lo[d], olo );
-            hi[d] = fmax( hi[d], ohi );
-        }
-        int32_t oh = __hip_atomic_load( &nd->height[other], __ATOMIC_RELAXED,
-                                        __HIP_MEMORY_SCOPE_AGENT );
-        h = 1 + ( h > oh ? h : oh );
-        code = par;
-        if ( par == 0 )
-        {
-            *height = h;
-            return;
-        }
-        enc = parent_internal[par];
+        node = parent_internal[node] >> 1;
+        ++depth;
     }
+    atomicMax( height, depth );
 }
 
 // --------------------------------------------------------- k-best lists
@@ -595,14 +645,14 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     // scene box of the sources
     int nparts = static_cast<int>( std::min<int64_t>(
         1024, std::max<int64_t>( 1, ceil_div( n_src, kBlock ) ) ) );
-    DeviceBuffer<T> partial( 6 * nparts ), box( 6 );
+    DeviceBuffer<T> partial( 6 * nparts, s ), box( 6, s );
     bbox_partial<T, V><<<nparts, kBlock, 0, s>>>( src, n_src, partial.get() );
     bbox_final<T><<<1, kBlock, 0, s>>>( partial.get(), nparts, box.get() );
     DTKX_HIP( hipGetLastError() );
 
     // Morton codes and sort: sources
-    DeviceBuffer<uint64_t> codes( n_src ), codes_sorted( n_src );
-    DeviceBuffer<int32_t> ids( n_src ), ids_sorted( n_src );
+    DeviceBuffer<uint64_t> codes( n_src, s ), codes_sorted( n_src, s );
+    DeviceBuffer<int32_t> ids( n_src, s ), ids_sorted( n_src, s );
     morton_kernel<T, V><<<ceil_div( n_src, kBlock ), kBlock, 0, s>>>(
         src, n_src, box.get(), codes.get(), ids.get() );
     DTKX_HIP( hipGetLastError() );
@@ -611,27 +661,25 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     DTKX_HIP( hipcub::DeviceRadixSort::SortPairs(
         nullptr, temp_bytes, codes.get(), codes_sorted.get(), ids.get(),
         ids_sorted.get(), static_cast<int>( nmax ), 0, 63, s ) );
-    DeviceBuffer<char> temp( temp_bytes );
+    DeviceBuffer<char> temp( temp_bytes, s );
     DTKX_HIP( hipcub::DeviceRadixSort::SortPairs(
         temp.get(), temp_bytes, codes.get(), codes_sorted.get(), ids.get(),
         ids_sorted.get(), static_cast<int>( n_src ), 0, 63, s ) );
 
     // hierarchy + refit
     int64_t const n_int = n_src - 1;
-    DeviceBuffer<Node<T>> nodes( std::max<int64_t>( n_int, 1 ) );
-    DeviceBuffer<int32_t> par_int( std::max<int64_t>( n_int, 1 ) ),
-        par_leaf( n_src ), flags( std::max<int64_t>( n_int, 1 ) ), height( 1 );
+    DeviceBuffer<Node<T>> nodes( std::max<int64_t>( n_int, 1 ), s );
+    DeviceBuffer<int32_t> par_int( std::max<int64_t>( n_int, 1 ), s ),
+        par_leaf( n_src, s ), height( 1, s );
     int h_tree = 0;
     if ( n_int > 0 )
     {
-        DTKX_HIP( hipMemsetAsync( flags.get(), 0, sizeof( int32_t ) * n_int, s ) );
         DTKX_HIP( hipMemsetAsync( height.get(), 0, sizeof( int32_t ), s ) );
-        karras_kernel<T><<<ceil_div( n_int, kBlock ), kBlock, 0, s>>>(
-            codes_sorted.get(), n_src, nodes.get(), par_int.get(),
-            par_leaf.get() );
-        refit_kernel<T, V><<<ceil_div( n_src, kBlock ), kBlock, 0, s>>>(
-            src, ids_sorted.get(), n_src, nodes.get(), par_int.get(),
-            par_leaf.get(), flags.get(), height.get() );
+        karras_kernel<T, V><<<ceil_div( n_int, kBlock ), kBlock, 0, s>>>(
+            codes_sorted.get(), ids_sorted.get(), src, n_src, box.get(),
+            nodes.get(), par_int.get(), par_leaf.get() );
+        leaf_depth_kernel<<<ceil_div( n_src, kBlock ), kBlock, 0, s>>>(
+            par_int.get(), par_leaf.get(), n_src, height.get() );
         DTKX_HIP( hipGetLastError() );
         DTKX_HIP( hipMemcpyAsync( &h_tree, height.get(), sizeof( int32_t ),
                                   hipMemcpyDeviceToHost, s ) );
@@ -649,12 +697,12 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     int32_t *order = query_order;
     if ( !order )
     {
-        order_buf.resize( n_tgt );
+        order_buf.resize( n_tgt, s );
         order = order_buf.get();
     }
     {
-        DeviceBuffer<uint64_t> tcodes( n_tgt ), tcodes_sorted( n_tgt );
-        DeviceBuffer<int32_t> tids( n_tgt );
+        DeviceBuffer<uint64_t> tcodes( n_tgt, s ), tcodes_sorted( n_tgt, s );
+        DeviceBuffer<int32_t> tids( n_tgt, s );
         morton_kernel<T, V><<<ceil_div( n_tgt, kBlock ), kBlock, 0, s>>>(
             tgt, n_tgt, box.get(), tcodes.get(), tids.get() );
         DTKX_HIP( hipGetLastError() );
@@ -668,7 +716,7 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     int stack = need <= 32 ? 32 : ( need <= 64 ? 64 : 0 );
     DeviceBuffer<int2> gstack;
     if ( stack == 0 )
-        gstack.resize( static_cast<std::size_t>( need + 1 ) * n_tgt );
+        gstack.resize( static_cast<std::size_t>( need + 1 ) * n_tgt, s );
     QueryLauncher<T>::dispatch( k, stack, nodes.get(), src, tgt, order, n_tgt,
                                 n_src, kk, indices, dist, gstack.get(),
                                 need + 1, s );
@@ -677,8 +725,7 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
         info->tree_height = h_tree;
         info->stack_used = stack;
     }
-    // buffers are released at scope exit; wait so the frees do not race
-    DTKX_HIP( hipStreamSynchronize( s ) );
+    // buffers are released at scope exit, ordered on the stream
 }
 
 } // namespace

@@ -69,7 +69,26 @@ __global__ void __launch_bounds__( kBlock )
     out[i] = v;
 }
 
+__global__ void __launch_bounds__( kBlock )
+    d4_to_f4_kernel( double4 const *in, int64_t n, float4 *out )
+{
+    int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( i >= n )
+        return;
+    double4 const v = in[i];
+    out[i] = float4{(float)v.x, (float)v.y, (float)v.z, 0.f};
+}
+
 } // namespace
+
+void double4_to_float4( double4 const *in, int64_t n, float4 *out,
+                        hipStream_t s )
+{
+    if ( n == 0 )
+        return;
+    d4_to_f4_kernel<<<ceil_div( n, kBlock ), kBlock, 0, s>>>( in, n, out );
+    DTKX_HIP( hipGetLastError() );
+}
 
 void points_to_float4( double const *xyz, int64_t n, int layout, float4 *out,
                        hipStream_t s )

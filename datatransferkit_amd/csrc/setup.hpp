@@ -39,6 +39,13 @@ int setup_classical( double4 const *src, double4 const *tgt, int64_t n_tgt,
 // Lane group size the classical solver uses for (m, kk) when not forced.
 int setup_classical_group( int m, int kk );
 
+// Cyclic symmetric Jacobi eigen-solver, registers only (fast path);
+// p.max_rot caps the sweeps (0 = 30).
+void setup_cyclic( double4 const *src, double4 const *tgt, int64_t n_tgt,
+                   int32_t const *indices, int32_t const *order,
+                   SetupParams const &p, double *coeffs, double *a0,
+                   unsigned long long *counters, hipStream_t s );
+
 // Batched pseudo-inverse of general n x n matrices (n <= 64), one matrix
 // per workgroup (tstSVD path).
 void svd_pinv_batched( int n, int64_t n_mat, double const *a, double *pinv,

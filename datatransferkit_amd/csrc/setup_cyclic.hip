@@ -1,0 +1,347 @@
+// Fused MLS setup with a cyclic symmetric Jacobi eigen-solver: the fast
+// FP64 path (DTKX_SOLVER_CYCLIC_JACOBI).
+//
+// One target system per lane (per lane pair for the quadratic basis, see
+// setup_cyclic_kernel). The moment matrix A = P^T Phi P is symmetric
+// positive semi-definite, so A^+ e_0 = V Lambda^+ V^T e_0 with A = V Lambda
+// V^T; the pseudo-inverse is the same as the reference's two-sided SVD
+// (DTK_DetailsSVDImpl.hpp) up to rounding. Every (p, q) index is a
+// compile-time constant (fully unrolled sweeps), so A (upper triangle) and
+// V stay in registers: no LDS, no barriers, no divergence except the
+// per-wave sweep count. Rotations use the Rutishauser form (t = sgn(theta)
+// / (|theta| + sqrt(theta^2 + 1)), c = 1/sqrt(1 + t^2), tau = s/(1 + c)).
+// Sweeps stop when every lane's off-diagonal mass is below
+// (DBL_EPSILON/8)^2 of its diagonal mass, or at the sweep cap.
+// Truncation keeps the reference's semantics: eigenvalues with |l| >=
+// DBL_EPSILON (or rcond * max|l|) contribute.
+// Neighbour data is re-gathered from L2 in three passes (radius, moments,
+// coefficients) instead of being staged.
+#include "mls_device.hpp"
+#include "setup.hpp"
+
+namespace dtkx
+{
+namespace
+{
+
+constexpr int kBlock = 64;
+
+template <int M>
+struct SymIdx
+{
+    // index of (r, c), r <= c, in the packed upper triangle
+    static constexpr int at( int r, int c )
+    {
+        return r <= c ? r * M - r * ( r - 1 ) / 2 + ( c - r )
+                      : c * M - c * ( c - 1 ) / 2 + ( r - c );
+    }
+    static constexpr int size = M * ( M + 1 ) / 2;
+};
+
+template <int M, typename T>
+__device__ inline void basis_fast( T x, T y, T z, double *P )
+{
+    P[0] = 1.;
+    if ( M >= 4 )
+    {
+        P[1] = x;
+        P[2] = y;
+        P[3] = z;
+    }
+    if ( M >= 10 )
+    {
+        P[4] = x * x;
+        P[5] = x * y;
+        P[6] = x * z;
+        P[7] = y * y;
+        P[8] = y * z;
+        P[9] = z * z;
+    }
+}
+
+template <int M>
+__device__ inline void basis_any( int geometry, double const *xr, double *P )
+{
+    if ( geometry == DTKX_GEOMETRY_FLOAT )
+        basis_fast<M, float>( (float)xr[0], (float)xr[1], (float)xr[2], P );
+    else
+        basis_fast<M, double>( xr[0], xr[1], xr[2], P );
+}
+
+// V holds R rows of the eigenvector matrix (all of it when R == M)
+template <int M, int R, int P_, int Q_>
+__device__ inline void rotate( double *A, double *V )
+{
+    using S = SymIdx<M>;
+    double const apq = A[S::at( P_, Q_ )];
+    double const app = A[S::at( P_, P_ )];
+    double const aqq = A[S::at( Q_, Q_ )];
+    double const theta = ( aqq - app ) / ( 2.0 * apq );
+    double const at = fabs( theta );
+    double t = 1.0 / ( at + __builtin_sqrt( fma( theta, theta, 1.0 ) ) );
+    t = theta < 0 ? -t : t;
+    // huge |theta|: theta^2 overflows, t -> 1/(2 theta); apq == 0: no-op
+    t = at > 1e150 ? 0.5 / theta : t;
+    t = ( apq == 0.0 ) ? 0.0 : t;
+    double const c = 1.0 / __builtin_sqrt( fma( t, t, 1.0 ) );
+    double const s = t * c;
+    double const tau = s / ( 1.0 + c );
+    double const h = t * apq;
+    A[S::at( P_, P_ )] = app - h;
+    A[S::at( Q_, Q_ )] = aqq + h;
+    A[S::at( P_, Q_ )] = 0.0;
+#pragma unroll
+    for ( int r = 0; r < M; ++r )
+    {
+        if ( r == P_ || r == Q_ )
+            continue;
+        double const arp = A[S::at( r, P_ )];
+        double const arq = A[S::at( r, Q_ )];
+        A[S::at( r, P_ )] = arp - s * ( arq + tau * arp );
+        A[S::at( r, Q_ )] = arq + s * ( arp - tau * arq );
+    }
+#pragma unroll
+    for ( int r = 0; r < R; ++r )
+    {
+        double const vrp = V[r * M + P_];
+        double const vrq = V[r * M + Q_];
+        V[r * M + P_] = vrp - s * ( vrq + tau * vrp );
+        V[r * M + Q_] = vrq + s * ( vrp - tau * vrq );
+    }
+}
+
+// one cyclic sweep in row order (0,1), (0,2), ..., (M-2, M-1)
+template <int M, int R, int P_, int Q_>
+struct Sweep
+{
+    __device__ static inline void run( double *A, double *V )
+    {
+        rotate<M, R, P_, Q_>( A, V );
+        if constexpr ( Q_ + 1 < M )
+            Sweep<M, R, P_, Q_ + 1>::run( A, V );
+        else if constexpr ( P_ + 2 < M )
+            Sweep<M, R, P_ + 1, P_ + 2>::run( A, V );
+    }
+};
+
+template <int M, int R>
+__device__ inline void sweep_all( double *A, double *V )
+{
+    if constexpr ( M >= 2 )
+        Sweep<M, R, 0, 1>::run( A, V );
+}
+
+template <int M>
+__device__ inline bool converged( double const *A )
+{
+    using S = SymIdx<M>;
+    double off = 0.0, diag = 0.0;
+#pragma unroll
+    for ( int r = 0; r < M; ++r )
+    {
+        diag = fma( A[S::at( r, r )], A[S::at( r, r )], diag );
+#pragma unroll
+        for ( int c = r + 1; c < M; ++c )
+            off = fma( A[S::at( r, c )], A[S::at( r, c )], off );
+    }
+    constexpr double tol = ( DBL_EPSILON / 8 ) * ( DBL_EPSILON / 8 );
+    return !( off > tol * diag );
+}
+
+// H lanes per system (H = 1, or 2 for M = 10 so A and half of V fit the
+// 256 VGPRs): every lane of the pair keeps the whole moment matrix and runs
+// the same rotations on it (identical values, so both agree), and owns
+// R = M / H rows of V.
+template <int M, int H>
+__global__ void __launch_bounds__( kBlock )
+    setup_cyclic_kernel( double4 const *__restrict__ src,
+                         double4 const *__restrict__ tgt, int64_t n_tgt,
+                         int32_t const *__restrict__ indices,
+                         int32_t const *__restrict__ order, int kk, int rbf,
+                         int geometry, int truncation, double rcond,
+                         int max_sweeps, double *__restrict__ coeffs,
+                         double *__restrict__ a0_out,
+                         unsigned long long *__restrict__ counters )
+{
+    using S = SymIdx<M>;
+    constexpr int R = M / H;
+    constexpr int SPB = kBlock / H; // systems per block
+    int const h = threadIdx.x % H;
+    int64_t const slot = blockIdx.x * (int64_t)SPB + threadIdx.x / H;
+    bool const valid = slot < n_tgt;
+    int64_t const t = valid ? ( order ? order[slot] : slot ) : 0;
+    int32_t const *nbr = indices + t * kk;
+    double4 const tp = valid ? tgt[t] : double4{0, 0, 0, 0};
+
+    // radius (Impl.hpp:118-134): the largest neighbour distance
+    double dmax = 10. * DBL_EPSILON;
+    if ( valid )
+        for ( int l = 0; l < kk; ++l )
+        {
+            double4 const sp = src[nbr[l]];
+            double const xr[3] = {sp.x - tp.x, sp.y - tp.y, sp.z - tp.z};
+            double const d = dev::rel_distance( geometry, xr );
+            dmax = d > dmax ? d : dmax;
+        }
+    double const radius = 1.1 * dmax;
+
+    // moments, upper triangle
+    double A[S::size];
+#pragma unroll
+    for ( int e = 0; e < S::size; ++e )
+        A[e] = 0.0;
+    if ( valid )
+        for ( int l = 0; l < kk; ++l )
+        {
+            double4 const sp = src[nbr[l]];
+            double const xr[3] = {sp.x - tp.x, sp.y - tp.y, sp.z - tp.z};
+            double P[M];
+            basis_any<M>( geometry, xr, P );
+            double const phi =
+                dev::rbf_eval( rbf, dev::rel_distance( geometry, xr ) / radius );
+#pragma unroll
+            for ( int r = 0; r < M; ++r )
+            {
+                double const pr = P[r] * phi;
+#pragma unroll
+                for ( int c = r; c < M; ++c )
+                    A[S::at( r, c )] = fma( pr, P[c], A[S::at( r, c )] );
+            }
+        }
+
+    // cyclic Jacobi; this lane owns rows h*R .. h*R + R - 1 of V
+    double V[R * M];
+#pragma unroll
+    for ( int e = 0; e < R * M; ++e )
+        V[e] = ( h * R + e / M == e % M ) ? 1.0 : 0.0;
+    int sweeps = 0;
+    bool done = !valid || converged<M>( A );
+    while ( !__all( done ) && sweeps < max_sweeps )
+    {
+        sweep_all<M, R>( A, V );
+        ++sweeps;
+        done = done || converged<M>( A );
+    }
+
+    // x = A^+ e_0 = sum_k [|l_k| >= thr] V(0,k) V(:,k) / l_k
+    double thr = DBL_EPSILON;
+    if ( truncation == DTKX_TRUNCATION_RELATIVE )
+    {
+        double lmax = 0.;
+#pragma unroll
+        for ( int k = 0; k < M; ++k )
+            lmax = fmax( lmax, fabs( A[S::at( k, k )] ) );
+        thr = rcond * lmax;
+    }
+    double w[M];
+    bool under = false;
+#pragma unroll
+    for ( int k = 0; k < M; ++k )
+    {
+        double const lk = A[S::at( k, k )];
+        bool const keep = fabs( lk ) >= thr;
+        under = under || !keep;
+        // V(0, k) lives in the lane owning row 0 (h == 0)
+        double v0k = V[k];
+        if ( H > 1 )
+            v0k = __shfl( v0k, ( threadIdx.x / H ) * H, kBlock );
+        w[k] = keep ? v0k / lk : 0.0;
+    }
+    double x[M];
+#pragma unroll
+    for ( int r = 0; r < R; ++r )
+    {
+        double acc = 0.0;
+#pragma unroll
+        for ( int k = 0; k < M; ++k )
+            acc = fma( w[k], V[r * M + k], acc );
+        x[r] = acc;
+    }
+    if ( H > 1 )
+    {
+        // gather the partner's rows: x[R..M) from lane h^1 (H == 2)
+        double xo[R];
+#pragma unroll
+        for ( int r = 0; r < R; ++r )
+            xo[r] = __shfl_xor( x[r], 1, kBlock );
+#pragma unroll
+        for ( int r = 0; r < R; ++r )
+        {
+            double const mine = x[r];
+            x[h * R + r] = mine;
+            x[( 1 - h ) * R + r] = xo[r];
+        }
+    }
+
+    if ( !valid )
+        return;
+    if ( a0_out && h == 0 )
+#pragma unroll
+        for ( int j = 0; j < M; ++j )
+            a0_out[t * M + j] = x[j];
+    // coefficients c_l = phi_l (P_l . x), neighbours split over the H lanes
+    for ( int l = h; l < kk; l += H )
+    {
+        double4 const sp = src[nbr[l]];
+        double const xr[3] = {sp.x - tp.x, sp.y - tp.y, sp.z - tp.z};
+        double P[M];
+        basis_any<M>( geometry, xr, P );
+        double const phi =
+            dev::rbf_eval( rbf, dev::rel_distance( geometry, xr ) / radius );
+        double acc = 0.0;
+#pragma unroll
+        for ( int j = 0; j < M; ++j )
+            acc = fma( x[j], P[j], acc );
+        coeffs[t * kk + l] = acc * phi;
+    }
+    if ( counters && h == 0 )
+    {
+        if ( under )
+            atomicAdd( counters + 0, 1ull );
+        if ( !done )
+            atomicAdd( counters + 1, 1ull );
+        atomicAdd( counters + 2, (unsigned long long)sweeps );
+    }
+}
+
+template <int M, int H>
+void launch_cyclic( double4 const *src, double4 const *tgt, int64_t n_tgt,
+                    int32_t const *indices, int32_t const *order,
+                    SetupParams const &p, double *coeffs, double *a0,
+                    unsigned long long *counters, hipStream_t s )
+{
+    int const sweeps = p.max_rot > 0 ? static_cast<int>( p.max_rot ) : 30;
+    setup_cyclic_kernel<M, H><<<ceil_div( n_tgt, kBlock / H ), kBlock, 0, s>>>(
+        src, tgt, n_tgt, indices, order, p.kk, p.rbf, p.geometry, p.truncation,
+        p.rcond, sweeps, coeffs, a0, counters );
+    DTKX_HIP( hipGetLastError() );
+}
+
+} // namespace
+
+void setup_cyclic( double4 const *src, double4 const *tgt, int64_t n_tgt,
+                   int32_t const *indices, int32_t const *order,
+                   SetupParams const &p, double *coeffs, double *a0,
+                   unsigned long long *counters, hipStream_t s )
+{
+    if ( n_tgt == 0 )
+        return;
+    switch ( p.m )
+    {
+    case 1:
+        launch_cyclic<1, 1>( src, tgt, n_tgt, indices, order, p, coeffs, a0,
+                          counters, s );
+        return;
+    case 4:
+        launch_cyclic<4, 1>( src, tgt, n_tgt, indices, order, p, coeffs, a0,
+                          counters, s );
+        return;
+    case 10:
+        launch_cyclic<10, 2>( src, tgt, n_tgt, indices, order, p, coeffs, a0,
+                           counters, s );
+        return;
+    }
+    DTKX_REQUIRE( p.m == 1 || p.m == 4 || p.m == 10, DTKX_ERR_INVALID_ARGUMENT );
+}
+
+} // namespace dtkx
