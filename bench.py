@@ -98,6 +98,8 @@ def main():
     ap.add_argument("--rbf", default="wendland2")
     ap.add_argument("--solver", default="classical")
     ap.add_argument("--precision", default="fp64")
+    ap.add_argument("--group", type=int, default=0,
+                    help="lanes per system for the classical solver (0 = default)")
     ap.add_argument("--cpu-sample", type=int, default=20_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -122,7 +124,7 @@ def main():
     d_out = torch.zeros((args.fields, n), dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream(dev).cuda_stream
     kw = dict(basis="quadratic", rbf=args.rbf, k=args.k, solver=args.solver,
-              precision=args.precision, stream=stream)
+              precision=args.precision, stream=stream, group_size=args.group)
 
     def step():
         op = dtk.MovingLeastSquaresOperator(d_src, d_tgt, **kw)

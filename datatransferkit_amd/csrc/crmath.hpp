@@ -184,31 +184,33 @@ DTKX_HD dd inv_fact( int n )
     }
 }
 
-// atan(j/16) as double-double, j = 0..16
-DTKX_HD dd atan_j16( int j )
-{
-    DTKX_NO_CONTRACT
-    switch ( j )
-    {
-    case 0: return {0.0, 0.0};
-    case 1: return {0x1.ff55bb72cfdeap-5, -0x1.c934d86d23f1dp-60};
-    case 2: return {0x1.fd5ba9aac2f6ep-4, -0x1.cd37686760c17p-59};
-    case 3: return {0x1.7b97b4bce5b02p-3, 0x1.347b0b4f881cap-58};
-    case 4: return {0x1.f5b75f92c80ddp-3, 0x1.8ab6e3cf7afbdp-57};
-    case 5: return {0x1.362773707ebccp-2, -0x1.963a544b672d8p-57};
-    case 6: return {0x1.6f61941e4def1p-2, -0x1.c63aae6f6e918p-56};
-    case 7: return {0x1.a64eec3cc23fdp-2, -0x1.24dec1b50b7ffp-56};
-    case 8: return {0x1.dac670561bb4fp-2, 0x1.a2b7f222f65e2p-56};
-    case 9: return {0x1.0657e94db30d0p-1, -0x1.d5b495f6349e6p-56};
-    case 10: return {0x1.1e00babdefeb4p-1, -0x1.928df287a668fp-58};
-    case 11: return {0x1.345f01cce37bbp-1, 0x1.1021137c71102p-55};
-    case 12: return {0x1.4978fa3269ee1p-1, 0x1.2419a87f2a458p-56};
-    case 13: return {0x1.5d58987169b18p-1, 0x1.0028e4bc5e7cap-57};
-    case 14: return {0x1.700a7c5784634p-1, -0x1.8c34d25aadef6p-56};
-    case 15: return {0x1.819d0b7158a4dp-1, -0x1.bf76229d3b917p-56};
-    default: return {0x1.921fb54442d18p-1, 0x1.1a62633145c07p-55};
-    }
-}
+// atan(j/16) as double-double, j = 0..16 (a table in constant memory: an
+// index that varies across lanes loads one entry instead of selecting among
+// all of them in registers)
+#if defined( __HIPCC__ )
+__device__ __constant__
+#endif
+static const double kAtanJ16[17][2] = {
+    {0.0, 0.0},
+    {0x1.ff55bb72cfdeap-5, -0x1.c934d86d23f1dp-60},
+    {0x1.fd5ba9aac2f6ep-4, -0x1.cd37686760c17p-59},
+    {0x1.7b97b4bce5b02p-3, 0x1.347b0b4f881cap-58},
+    {0x1.f5b75f92c80ddp-3, 0x1.8ab6e3cf7afbdp-57},
+    {0x1.362773707ebccp-2, -0x1.963a544b672d8p-57},
+    {0x1.6f61941e4def1p-2, -0x1.c63aae6f6e918p-56},
+    {0x1.a64eec3cc23fdp-2, -0x1.24dec1b50b7ffp-56},
+    {0x1.dac670561bb4fp-2, 0x1.a2b7f222f65e2p-56},
+    {0x1.0657e94db30d0p-1, -0x1.d5b495f6349e6p-56},
+    {0x1.1e00babdefeb4p-1, -0x1.928df287a668fp-58},
+    {0x1.345f01cce37bbp-1, 0x1.1021137c71102p-55},
+    {0x1.4978fa3269ee1p-1, 0x1.2419a87f2a458p-56},
+    {0x1.5d58987169b18p-1, 0x1.0028e4bc5e7cap-57},
+    {0x1.700a7c5784634p-1, -0x1.8c34d25aadef6p-56},
+    {0x1.819d0b7158a4dp-1, -0x1.bf76229d3b917p-56},
+    {0x1.921fb54442d18p-1, 0x1.1a62633145c07p-55}
+};
+
+DTKX_HD dd atan_j16( int j ) { return {kAtanJ16[j][0], kAtanJ16[j][1]}; }
 
 // 1/(2n+1) as double-double, n = 1..4 (higher terms use double)
 DTKX_HD dd inv_odd( int n )
@@ -287,9 +289,34 @@ DTKX_HD double sin_cos_q( double x, int shift )
 }
 
 DTKX_HD double sin( double x ) { return sin_cos_q( x, 0 ); }
-    DTKX_NO_CONTRACT
 DTKX_HD double cos( double x ) { return sin_cos_q( x, 1 ); }
+
+// sin and cos of one argument sharing the reduction; each result is the
+// same correctly rounded value sin() and cos() return.
+DTKX_HD void sincos( double x, double *s, double *c )
+{
     DTKX_NO_CONTRACT
+    if ( !( std::fabs( x ) < 1.6e6 ) )
+    {
+        *s = std::sin( x );
+        *c = std::cos( x );
+        return;
+    }
+    int q;
+    dd const r = reduce_pio2( x, q );
+    dd const ps = sin_poly( r );
+    dd const pc = cos_poly( r );
+    // sin(x) uses quadrant q, cos(x) = sin(x + pi/2) quadrant q + 1
+    int const qs = q & 3, qc = ( q + 1 ) & 3;
+    dd vs = ( qs & 1 ) ? pc : ps;
+    if ( qs & 2 )
+        vs = dd_neg( vs );
+    dd vc = ( qc & 1 ) ? pc : ps;
+    if ( qc & 2 )
+        vc = dd_neg( vc );
+    *s = std::fabs( x ) < 0x1p-27 ? x : vs.hi + vs.lo;
+    *c = vc.hi + vc.lo;
+}
 
 // ---------------------------------------------------------------- atan2
 // atan(t) for a double-double 0 <= t <= 1.

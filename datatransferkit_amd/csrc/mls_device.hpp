@@ -119,50 +119,69 @@ __device__ inline double rel_distance( int geometry, double const *xr )
 
 // The 2x2 SVD of DTK_DetailsSVDImpl.hpp:119-162 reduced to what the Jacobi
 // consumes (:258-263): cl = L00, sl = L01, cr = R00 and the sign-fixed sr.
+// Split in two so the two independent angle chains can run on two lanes:
+// svd_2x2_angle_args gives the atan2 arguments of phi (AA^T, second = false)
+// or theta (A^T A, second = true); svd_2x2_from_angles finishes.
+__device__ inline void svd_2x2_angle_args( double a00, double a01, double a10,
+                                           double a11, bool second, double &y,
+                                           double &x )
+{
+#pragma clang fp contract( off )
+    // mult_2x2 order: two products, one add
+    double const aat00 = a00 * a00 + a01 * a01;
+    double const aat01 = a00 * a10 + a01 * a11;
+    double const aat10 = a10 * a00 + a11 * a01;
+    double const aat11 = a10 * a10 + a11 * a11;
+    double const ata00 = a00 * a00 + a10 * a10;
+    double const ata01 = a00 * a01 + a10 * a11;
+    double const ata10 = a01 * a00 + a11 * a10;
+    double const ata11 = a01 * a01 + a11 * a11;
+    y = second ? ata01 + ata10 : aat01 + aat10;
+    x = second ? ata00 - ata11 : aat00 - aat11;
+}
+
+__device__ inline void svd_2x2_from_angles( double a00, double a01, double a10,
+                                            double a11, double cphi,
+                                            double sphi, double ctheta,
+                                            double stheta, double &cl,
+                                            double &sl, double &cr, double &sr )
+{
+#pragma clang fp contract( off )
+    // W = [[ct, -st], [st, ct]]; AW = A W; S = U^T AW with U^T = [[cp, sp],
+    // [-sp, cp]]
+    double const w00 = ctheta, w01 = -stheta, w10 = stheta, w11 = ctheta;
+    double const aw00 = a00 * w00 + a01 * w10;
+    double const aw01 = a00 * w01 + a01 * w11;
+    double const aw10 = a10 * w00 + a11 * w10;
+    double const aw11 = a10 * w01 + a11 * w11;
+    double const ut00 = cphi, ut01 = sphi, ut10 = -sphi, ut11 = cphi;
+    double const s00 = ut00 * aw00 + ut01 * aw10;
+    double const s11 = ut10 * aw01 + ut11 * aw11;
+    double const c00 = copysign( 1., s00 );
+    double const c11 = copysign( 1., s11 );
+    // V = W C (C diagonal: the off-diagonal zeros kept for signed zeros)
+    double const v00 = w00 * c00 + w01 * 0.0;
+    double const v01 = w00 * 0.0 + w01 * c11;
+    double const v10 = w10 * c00 + w11 * 0.0;
+    cl = cphi;
+    sl = -sphi;
+    cr = v00;
+    int const sg01 = ( v01 > 0 ) - ( v01 < 0 );
+    int const sg10 = ( v10 > 0 ) - ( v10 < 0 );
+    sr = ( sg01 == sg10 ) ? -v01 : v01;
+}
+
 __device__ inline void svd_2x2_rot( double a00, double a01, double a10,
                                     double a11, double &cl, double &sl,
                                     double &cr, double &sr )
 {
-#pragma clang fp contract( off )
-    // AAt = A A^T, AtA = A^T A (mult_2x2 order: two products, one add)
-    double aat00 = a00 * a00 + a01 * a01;
-    double aat01 = a00 * a10 + a01 * a11;
-    double aat10 = a10 * a00 + a11 * a01;
-    double aat11 = a10 * a10 + a11 * a11;
-    double ata00 = a00 * a00 + a10 * a10;
-    double ata01 = a00 * a01 + a10 * a11;
-    double ata10 = a01 * a00 + a11 * a10;
-    double ata11 = a01 * a01 + a11 * a11;
-
-    double phi = 0.5 * crm::atan2( aat01 + aat10, aat00 - aat11 );
-    double cphi = crm::cos( phi );
-    double sphi = crm::sin( phi );
-    double theta = 0.5 * crm::atan2( ata01 + ata10, ata00 - ata11 );
-    double ctheta = crm::cos( theta );
-    double stheta = crm::sin( theta );
-
-    // W = [[ct, -st], [st, ct]]; AW = A W; S = U^T AW with U^T = [[cp, sp],
-    // [-sp, cp]]
-    double w00 = ctheta, w01 = -stheta, w10 = stheta, w11 = ctheta;
-    double aw00 = a00 * w00 + a01 * w10;
-    double aw01 = a00 * w01 + a01 * w11;
-    double aw10 = a10 * w00 + a11 * w10;
-    double aw11 = a10 * w01 + a11 * w11;
-    double ut00 = cphi, ut01 = sphi, ut10 = -sphi, ut11 = cphi;
-    double s00 = ut00 * aw00 + ut01 * aw10;
-    double s11 = ut10 * aw01 + ut11 * aw11;
-    double c00 = copysign( 1., s00 );
-    double c11 = copysign( 1., s11 );
-    // V = W C (C diagonal: the off-diagonal zeros kept for signed zeros)
-    double v00 = w00 * c00 + w01 * 0.0;
-    double v01 = w00 * 0.0 + w01 * c11;
-    double v10 = w10 * c00 + w11 * 0.0;
-    cl = cphi;
-    sl = -sphi;
-    cr = v00;
-    int sg01 = ( v01 > 0 ) - ( v01 < 0 );
-    int sg10 = ( v10 > 0 ) - ( v10 < 0 );
-    sr = ( sg01 == sg10 ) ? -v01 : v01;
+    double y, x, cphi, sphi, ctheta, stheta;
+    svd_2x2_angle_args( a00, a01, a10, a11, false, y, x );
+    crm::sincos( 0.5 * crm::atan2( y, x ), &sphi, &cphi );
+    svd_2x2_angle_args( a00, a01, a10, a11, true, y, x );
+    crm::sincos( 0.5 * crm::atan2( y, x ), &stheta, &ctheta );
+    svd_2x2_from_angles( a00, a01, a10, a11, cphi, sphi, ctheta, stheta, cl,
+                         sl, cr, sr );
 }
 
 } // namespace dev

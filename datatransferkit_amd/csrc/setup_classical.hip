@@ -33,40 +33,45 @@ namespace
 
 constexpr int kWave = 64;
 
+// neighbours staged in LDS at a time: M of them fill exactly the space of
+// U and V(:,0) they share
+constexpr int chunk_of( int m ) { return m == 1 ? 16 : m; }
+
+// Per-system LDS (doubles): E [M*M], a region W shared by the staged
+// Vandermonde chunk (P [chunk*M] + weights [chunk]) during the moments and
+// by U [M*M] + V(:,0) [M] during the Jacobi, and A^+ row 0 [M].
+constexpr int region_w( int m )
+{
+    return m * m + m > chunk_of( m ) * ( m + 1 ) ? m * m + m
+                                                 : chunk_of( m ) * ( m + 1 );
+}
+constexpr int per_system( int m ) { return m * m + region_w( m ) + m; }
+
+// Exact off-diagonal norm of the reference (row-major sequential sum, then
+// a correctly rounded sqrt), compared with DBL_EPSILON.
 template <int M>
-__device__ inline double norm_off_diag( double const *E )
+__device__ inline bool norm_above_tol_exact( double const *E )
 {
 #pragma clang fp contract( off )
     double norm = 0.0;
-#pragma unroll
     for ( int i = 0; i < M; i++ )
-#pragma unroll
         for ( int j = 0; j < M; j++ )
             if ( i != j )
                 norm += E[i * M + j] * E[i * M + j];
-    return __builtin_sqrt( norm );
+    return __builtin_sqrt( norm ) > DBL_EPSILON;
 }
 
+// The neighbour-independent geometry of one neighbour: basis row and the
+// distance used by the radius and the weights (Impl.hpp:121-125, 162-165,
+// 184-187), recomputed bit-identically wherever it is needed.
 template <int M>
-__device__ inline void argmax_off_diag( double const *E, int &p, int &q )
+__device__ inline double neighbour_row( int geometry, double4 const &sp,
+                                        double4 const &tp, double *P )
 {
-    p = -1;
-    q = -1;
-    double mx = -1;
-#pragma unroll
-    for ( int i = 0; i < M; i++ )
-#pragma unroll
-        for ( int j = 0; j < M; j++ )
-            if ( i != j )
-            {
-                double v = fabs( E[i * M + j] );
-                if ( v > mx )
-                {
-                    p = i;
-                    q = j;
-                    mx = v;
-                }
-            }
+#pragma clang fp contract( off )
+    double const xr[3] = {sp.x - tp.x, sp.y - tp.y, sp.z - tp.z};
+    dev::basis_row<M>( geometry, xr, P );
+    return dev::rel_distance( geometry, xr );
 }
 
 template <int M, int G>
@@ -82,80 +87,153 @@ __global__ void __launch_bounds__( kWave )
 {
 #pragma clang fp contract( off )
     constexpr int S = kWave / G;
+    constexpr int MM = M * M;
+    // contiguous row-major chunk of E scanned by each lane
+    constexpr int CS = ( MM + G - 1 ) / G;
     extern __shared__ double lds[];
     int const sys = threadIdx.x / G, lane = threadIdx.x % G;
+    int const gbase = threadIdx.x - lane; // first lane of the group
     int64_t const slot = blockIdx.x * (int64_t)S + sys;
     bool const valid = slot < n_tgt;
     int64_t const t = valid ? ( order ? order[slot] : slot ) : 0;
+    int32_t const *nbr = indices + t * kk;
+    double4 const tp = valid ? tgt[t] : double4{0, 0, 0, 0};
 
-    int const per_sys = 2 * M * M + 2 * M + kk * ( M + 1 );
-    double *E = lds + sys * per_sys;
-    double *U = E + M * M;
-    double *V0 = U + M * M;
-    double *A0 = V0 + M;
-    double *P = A0 + M;
-    double *PHI = P + kk * M;
+    double *E = lds + sys * per_system( M );
+    double *W = E + MM;
+    double *U = W, *V0 = W + MM; // Jacobi view of W
+    constexpr int kChunk = chunk_of( M );
+    double *P = W, *PHI = W + kChunk * M; // moments view of W
+    double *A0 = W + region_w( M );
 
-    // neighbours: relative coordinates -> basis rows and distances
+    // radius (Impl.hpp:118-134): max is exact, so the lane split is free
     double dmax = 10. * DBL_EPSILON;
     if ( valid )
-    {
-        double4 const tp = tgt[t];
         for ( int l = lane; l < kk; l += G )
         {
-            int32_t const s = indices[t * kk + l];
-            double4 const sp = src[s];
-            double const xr[3] = {sp.x - tp.x, sp.y - tp.y, sp.z - tp.z};
-            dev::basis_row<M>( geometry, xr, P + l * M );
-            double const d = dev::rel_distance( geometry, xr );
-            PHI[l] = d;
+            double Pl[M];
+            double const d = neighbour_row<M>( geometry, src[nbr[l]], tp, Pl );
             if ( d > dmax )
                 dmax = d;
         }
-    }
 #pragma unroll
     for ( int off = G / 2; off > 0; off >>= 1 )
         dmax = fmax( dmax, __shfl_xor( dmax, off, G ) );
     double const radius = 1.1 * dmax;
-    __syncthreads();
-    if ( valid )
-        for ( int l = lane; l < kk; l += G )
-            PHI[l] = dev::rbf_eval( rbf, PHI[l] / radius );
-    __syncthreads();
 
-    // moments, identity U, V e_0
+    // moments A = P^T Phi P, each entry summed over l ascending
+    // (Impl.hpp:242-253), neighbours staged kChunk at a time; E holds the
+    // running sums between chunks, so the order of additions is unchanged
+    if ( valid )
+        for ( int e = lane; e < MM; e += G )
+            E[e] = 0.0;
+    for ( int l0 = 0; l0 < kk; l0 += kChunk )
+    {
+        int const nl = kk - l0 < kChunk ? kk - l0 : kChunk;
+        if ( valid )
+            for ( int l = lane; l < nl; l += G )
+            {
+                double const d = neighbour_row<M>( geometry, src[nbr[l0 + l]],
+                                                   tp, P + l * M );
+                PHI[l] = dev::rbf_eval( rbf, d / radius );
+            }
+        __syncthreads();
+        if ( valid )
+            for ( int e = lane; e < MM; e += G )
+            {
+                int const r = e / M, c = e % M;
+                double tmp = E[e];
+                for ( int l = 0; l < nl; ++l )
+                    tmp += P[l * M + r] * PHI[l] * P[l * M + c];
+                E[e] = tmp;
+            }
+        __syncthreads();
+    }
     if ( valid )
     {
-        for ( int e = lane; e < M * M; e += G )
-        {
-            int const r = e / M, c = e % M;
-            double tmp = 0.;
-            for ( int l = 0; l < kk; ++l )
-                tmp += P[l * M + r] * PHI[l] * P[l * M + c];
-            E[e] = tmp;
-            U[e] = ( r == c ) ? 1.0 : 0.0;
-        }
+        for ( int e = lane; e < MM; e += G )
+            U[e] = ( e / M == e % M ) ? 1.0 : 0.0;
         for ( int j = lane; j < M; j += G )
             V0[j] = ( j == 0 ) ? 1.0 : 0.0;
     }
     __syncthreads();
 
-    // classical Jacobi
-    double norm = valid ? norm_off_diag<M>( E ) : 0.0;
-    bool active = valid && norm > DBL_EPSILON;
+    // classical Jacobi (SVD.hpp:236-274)
+    bool active = valid;
     long rot = 0;
-    while ( __syncthreads_or( active ) )
+    bool unconverged = false;
+    while ( true )
     {
-        int p = 0, q = 0;
+        // fused pass over this lane's row-major chunk: partial off-diagonal
+        // norm and the first strict maximum of |E(i,j)|, i != j
+        double part = 0.0, mx = -1.0;
+        int mi = INT32_MAX;
+        if ( active )
+        {
+            int const e0 = lane * CS, e1 = e0 + CS < MM ? e0 + CS : MM;
+            for ( int e = e0; e < e1; ++e )
+            {
+                if ( e / M == e % M )
+                    continue;
+                double const v = E[e];
+                part += v * v;
+                double const av = fabs( v );
+                if ( av > mx )
+                {
+                    mx = av;
+                    mi = e;
+                }
+            }
+        }
+        // group reductions: the sum in a fixed tree order; the maximum with
+        // ties to the lower row-major index, which is the reference's first
+        // strict maximum
+#pragma unroll
+        for ( int off = 1; off < G; off <<= 1 )
+        {
+            double const op = __shfl_xor( part, off, G );
+            double const om = __shfl_xor( mx, off, G );
+            int const oi = __shfl_xor( mi, off, G );
+            part = ( lane & off ) ? op + part : part + op;
+            if ( om > mx || ( om == mx && oi < mi ) )
+            {
+                mx = om;
+                mi = oi;
+            }
+        }
+        if ( active )
+        {
+            // norm > DBL_EPSILON decided on the tree sum unless it lies within
+            // a relative 1e-12 of DBL_EPSILON^2 (the two sums of the same
+            // non-negative terms differ by less than 2e-14 relative); then the
+            // reference's sequential sum decides
+            double const tol2 = DBL_EPSILON * DBL_EPSILON;
+            bool above;
+            if ( part > tol2 * ( 1.0 + 1e-12 ) )
+                above = true;
+            else if ( part < tol2 * ( 1.0 - 1e-12 ) )
+                above = false;
+            else
+                above = norm_above_tol_exact<M>( E );
+            if ( !above )
+                active = false;
+            else if ( rot >= max_rot )
+            {
+                active = false;
+                unconverged = true;
+            }
+        }
+        if ( !__syncthreads_or( active ) )
+            break;
+        int p = 0, q = 1;
         double cl = 1, sl = 0, cr = 1, sr = 0;
         if ( active )
         {
-            argmax_off_diag<M>( E, p, q );
-            if ( p < 0 || q < 0 )
+            p = mi / M;
+            q = mi % M;
+            if ( mi >= MM || mi < 0 )
             {
-                // unreachable (the loop runs only for a finite norm above
-                // DBL_EPSILON); keeps every LDS index in range regardless
-                p = 0;
+                p = 0; // unreachable: an active system has a finite maximum
                 q = M - 1;
             }
             if ( p > q )
@@ -164,10 +242,35 @@ __global__ void __launch_bounds__( kWave )
                 p = q;
                 q = tt;
             }
-            dev::svd_2x2_rot( E[p * M + p], E[p * M + q], E[q * M + p],
-                              E[q * M + q], cl, sl, cr, sr );
+            double const a00 = E[p * M + p], a01 = E[p * M + q];
+            double const a10 = E[q * M + p], a11 = E[q * M + q];
+            // the two angle chains of svd_2x2 run on lanes 0 and 1 of the
+            // group (identical code on different inputs)
+            double y, x;
+            dev::svd_2x2_angle_args( a00, a01, a10, a11, ( lane & 1 ) != 0, y,
+                                     x );
+            double const ang = 0.5 * crm::atan2( y, x );
+            double sa, ca;
+            crm::sincos( ang, &sa, &ca );
+            double cphi, sphi, ctheta, stheta;
+            if ( G > 1 )
+            {
+                cphi = __shfl( ca, gbase, kWave );
+                sphi = __shfl( sa, gbase, kWave );
+                ctheta = __shfl( ca, gbase + 1, kWave );
+                stheta = __shfl( sa, gbase + 1, kWave );
+            }
+            else
+            {
+                cphi = ca;
+                sphi = sa;
+                double y2, x2;
+                dev::svd_2x2_angle_args( a00, a01, a10, a11, true, y2, x2 );
+                crm::sincos( 0.5 * crm::atan2( y2, x2 ), &stheta, &ctheta );
+            }
+            dev::svd_2x2_from_angles( a00, a01, a10, a11, cphi, sphi, ctheta,
+                                      stheta, cl, sl, cr, sr );
         }
-        __syncthreads();
         if ( active )
             for ( int j = lane; j < M; j += G )
             {
@@ -193,17 +296,12 @@ __global__ void __launch_bounds__( kWave )
                 V0[p] = cr * a - sr * b;
                 V0[q] = sr * a + cr * b;
             }
+            ++rot;
         }
         __syncthreads();
-        if ( active )
-        {
-            norm = norm_off_diag<M>( E );
-            ++rot;
-            active = norm > DBL_EPSILON && rot < max_rot;
-        }
     }
 
-    // pseudo-inverse row 0 and coefficients
+    // pseudo-inverse row 0 (SVD.hpp:281-297)
     bool under = false;
     if ( valid )
     {
@@ -235,22 +333,36 @@ __global__ void __launch_bounds__( kWave )
     __syncthreads();
     if ( valid )
     {
+        // coefficients (Impl.hpp:326-333), the basis row recomputed
+        double a0[M];
+#pragma unroll
+        for ( int j = 0; j < M; ++j )
+            a0[j] = A0[j];
         for ( int l = lane; l < kk; l += G )
         {
+            double Pl[M];
+            double const d = neighbour_row<M>( geometry, src[nbr[l]], tp, Pl );
+            double const phi = dev::rbf_eval( rbf, d / radius );
             double c = 0.;
+#pragma unroll
             for ( int j = 0; j < M; j++ )
-                c += A0[j] * P[l * M + j] * PHI[l];
+                c += a0[j] * Pl[j] * phi;
             coeffs[t * kk + l] = c;
         }
         if ( lane == 0 && counters )
         {
             if ( under )
                 atomicAdd( counters + 0, 1ull );
-            if ( norm > DBL_EPSILON )
+            if ( unconverged )
                 atomicAdd( counters + 1, 1ull );
             atomicAdd( counters + 2, (unsigned long long)rot );
         }
     }
+}
+
+std::size_t lds_bytes( int m, int g )
+{
+    return sizeof( double ) * ( kWave / g ) * per_system( m );
 }
 
 template <int M, int G>
@@ -260,8 +372,7 @@ void launch( double4 const *src, double4 const *tgt, int64_t n_tgt,
              unsigned long long *counters, hipStream_t s )
 {
     constexpr int S = kWave / G;
-    std::size_t const lds =
-        sizeof( double ) * S * ( 2 * M * M + 2 * M + p.kk * ( M + 1 ) );
+    std::size_t const lds = lds_bytes( M, G );
     DTKX_REQUIRE( lds <= 64 * 1024, DTKX_ERR_UNSUPPORTED );
     setup_classical_kernel<M, G><<<ceil_div( n_tgt, S ), kWave, lds, s>>>(
         src, tgt, n_tgt, indices, order, p.kk, p.rbf, p.geometry, p.truncation,
@@ -269,25 +380,14 @@ void launch( double4 const *src, double4 const *tgt, int64_t n_tgt,
     DTKX_HIP( hipGetLastError() );
 }
 
-std::size_t lds_bytes( int m, int kk, int g )
-{
-    return sizeof( double ) * ( kWave / g ) *
-           ( 2 * m * m + 2 * m + kk * ( m + 1 ) );
-}
-
 } // namespace
 
 int setup_classical_group( int m, int kk )
 {
+    (void)kk; // the staged chunk bounds the LDS independently of kk
     if ( m == 1 )
         return 1;
-    // smallest group (most systems per wave) whose LDS fits 32 KB, so that
-    // several waves share a CU
-    int const opts[] = {m == 4 ? 4 : 8, 16};
-    for ( int g : opts )
-        if ( lds_bytes( m, kk, g ) <= 32 * 1024 )
-            return g;
-    return 16;
+    return m == 4 ? 4 : 8;
 }
 
 int setup_classical( double4 const *src, double4 const *tgt, int64_t n_tgt,
@@ -321,6 +421,12 @@ int setup_classical( double4 const *src, double4 const *tgt, int64_t n_tgt,
                       s );
         return 4;
     case 10:
+        if ( g <= 2 )
+        {
+            launch<10, 2>( src, tgt, n_tgt, indices, order, p, coeffs, a0,
+                           counters, s );
+            return 2;
+        }
         if ( g <= 4 )
         {
             launch<10, 4>( src, tgt, n_tgt, indices, order, p, coeffs, a0,

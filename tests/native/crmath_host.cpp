@@ -21,6 +21,11 @@ void crm_cos( const double *x, double *out, int64_t n )
     for ( int64_t i = 0; i < n; ++i )
         out[i] = dtkx::crm::cos( x[i] );
 }
+void crm_sincos( const double *x, double *s, double *c, int64_t n )
+{
+    for ( int64_t i = 0; i < n; ++i )
+        dtkx::crm::sincos( x[i], s + i, c + i );
+}
 void libm_atan2( const double *y, const double *x, double *out, int64_t n )
 {
     for ( int64_t i = 0; i < n; ++i )

@@ -54,6 +54,18 @@ def test_sin_cos_match_quadmath(crm, oracle_lib, fn):
     assert bad.sum() == 0, (x[bad][:5], got[bad][:5], ref[bad][:5])
 
 
+def test_sincos_equals_sin_and_cos(crm):
+    rng = np.random.default_rng(4)
+    x = np.concatenate([rng.uniform(-np.pi / 2, np.pi / 2, 200_000), rng.uniform(-40, 40, 20_000),
+                        [0.0, -0.0, 1e-300, -1e-9, np.pi / 2, -np.pi / 2]])
+    s = np.empty_like(x)
+    c = np.empty_like(x)
+    crm.crm_sincos(x.ctypes.data_as(_dp), s.ctypes.data_as(_dp), c.ctypes.data_as(_dp),
+                   ctypes.c_int64(x.size))
+    assert np.array_equal(s.view(np.uint64), _call(crm, "crm_sin", x).view(np.uint64))
+    assert np.array_equal(c.view(np.uint64), _call(crm, "crm_cos", x).view(np.uint64))
+
+
 def test_atan2_matches_quadmath(crm, oracle_lib):
     rng = np.random.default_rng(2)
     n = 1_000_000
