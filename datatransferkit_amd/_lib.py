@@ -10,7 +10,9 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libdtkx.so")
+# DTKX_LIBRARY selects another build of the same library (e.g. one built for
+# a different --offload-arch); by default the in-tree build is loaded
+LIB_PATH = os.environ.get("DTKX_LIBRARY") or os.path.join(HERE, "libdtkx.so")
 
 # status codes
 SUCCESS = 0

@@ -573,13 +573,17 @@ __global__ void __launch_bounds__( kQueryBlock )
             node = next;
         }
     }
+    // compile-time indices keep the k-best list in registers (a loop bounded
+    // by the runtime kk would index it dynamically and move it to scratch)
     int64_t row = (int64_t)t * kk;
-    for ( int j = 0; j < kk; ++j )
-    {
-        out_idx[row + j] = best.index( j );
-        if ( out_dist )
-            out_dist[row + j] = best.distance( j );
-    }
+#pragma unroll
+    for ( int j = 0; j < K; ++j )
+        if ( j < kk )
+        {
+            out_idx[row + j] = best.index( j );
+            if ( out_dist )
+                out_dist[row + j] = best.distance( j );
+        }
 }
 
 template <typename T>

@@ -170,18 +170,27 @@ __global__ void __launch_bounds__( kWave )
         int mi = INT32_MAX;
         if ( active )
         {
-            int const e0 = lane * CS, e1 = e0 + CS < MM ? e0 + CS : MM;
-            for ( int e = e0; e < e1; ++e )
+            // compile-time trip count: all LDS loads of the chunk issue
+            // back to back instead of one load latency per entry
+            int const e0 = lane * CS;
+            double v[CS];
+#pragma unroll
+            for ( int i = 0; i < CS; ++i )
+                v[i] = ( e0 + i < MM ) ? E[e0 + i] : 0.0;
+#pragma unroll
+            for ( int i = 0; i < CS; ++i )
             {
-                if ( e / M == e % M )
-                    continue;
-                double const v = E[e];
-                part += v * v;
-                double const av = fabs( v );
-                if ( av > mx )
+                int const e = e0 + i;
+                // diagonal entries are e = r (M + 1)
+                if ( e < MM && e % ( M + 1 ) != 0 )
                 {
-                    mx = av;
-                    mi = e;
+                    part += v[i] * v[i];
+                    double const av = fabs( v[i] );
+                    if ( av > mx )
+                    {
+                        mx = av;
+                        mi = e;
+                    }
                 }
             }
         }
@@ -271,24 +280,35 @@ __global__ void __launch_bounds__( kWave )
             dev::svd_2x2_from_angles( a00, a01, a10, a11, cphi, sphi, ctheta,
                                       stheta, cl, sl, cr, sr );
         }
+        constexpr int JR = ( M + G - 1 ) / G; // rows/columns per lane
         if ( active )
-            for ( int j = lane; j < M; j += G )
+#pragma unroll
+            for ( int jj = 0; jj < JR; ++jj )
             {
-                double const a = E[p * M + j], b = E[q * M + j];
-                E[p * M + j] = cl * a - sl * b;
-                E[q * M + j] = sl * a + cl * b;
+                int const j = lane + jj * G;
+                if ( j < M )
+                {
+                    double const a = E[p * M + j], b = E[q * M + j];
+                    E[p * M + j] = cl * a - sl * b;
+                    E[q * M + j] = sl * a + cl * b;
+                }
             }
         __syncthreads();
         if ( active )
         {
-            for ( int j = lane; j < M; j += G )
+#pragma unroll
+            for ( int jj = 0; jj < JR; ++jj )
             {
-                double const a = E[j * M + p], b = E[j * M + q];
-                E[j * M + p] = cr * a - sr * b;
-                E[j * M + q] = sr * a + cr * b;
-                double const u = U[j * M + p], v = U[j * M + q];
-                U[j * M + p] = cl * u - sl * v;
-                U[j * M + q] = sl * u + cl * v;
+                int const j = lane + jj * G;
+                if ( j < M )
+                {
+                    double const a = E[j * M + p], b = E[j * M + q];
+                    E[j * M + p] = cr * a - sr * b;
+                    E[j * M + q] = sr * a + cr * b;
+                    double const u = U[j * M + p], v = U[j * M + q];
+                    U[j * M + p] = cl * u - sl * v;
+                    U[j * M + q] = sl * u + cl * v;
+                }
             }
             if ( lane == 0 )
             {
@@ -387,7 +407,7 @@ int setup_classical_group( int m, int kk )
     (void)kk; // the staged chunk bounds the LDS independently of kk
     if ( m == 1 )
         return 1;
-    return m == 4 ? 4 : 8;
+    return 4;
 }
 
 int setup_classical( double4 const *src, double4 const *tgt, int64_t n_tgt,
