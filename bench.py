@@ -120,8 +120,10 @@ def main():
     src, tgt, vals = make_inputs(n, n, args.fields)
     d_src = torch.from_numpy(src).to(dev)
     d_tgt = torch.from_numpy(tgt).to(dev)
-    d_vals = torch.from_numpy(vals).to(dev)
-    d_out = torch.zeros((args.fields, n), dtype=torch.float64, device=dev)
+    # the FP32-bulk form takes float values (the halved-traffic apply)
+    vdt = torch.float32 if args.precision == "fp32_bulk" else torch.float64
+    d_vals = torch.from_numpy(vals).to(dev).to(vdt)
+    d_out = torch.zeros((args.fields, n), dtype=vdt, device=dev)
     stream = torch.cuda.current_stream(dev).cuda_stream
     kw = dict(basis="quadratic", rbf=args.rbf, k=args.k, solver=args.solver,
               precision=args.precision, stream=stream, group_size=args.group)

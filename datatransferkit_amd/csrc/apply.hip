@@ -10,6 +10,42 @@ namespace
 
 constexpr int kBlock = 256;
 
+// Accumulators: plain (FP64 path, the reference's sequential row sum) or
+// compensated (FP32 path: the error of each float addition is carried in a
+// second float, so a long row sums to float accuracy, not k times worse).
+template <typename T>
+struct PlainSum
+{
+    T s = 0;
+    __device__ void add( T v ) { s += v; }
+    __device__ T value() const { return s; }
+};
+
+struct KahanSum
+{
+    float s = 0.f, c = 0.f;
+    __device__ void add( float v )
+    {
+#pragma clang fp contract( off )
+        float const y = v - c;
+        float const t = s + y;
+        c = ( t - s ) - y;
+        s = t;
+    }
+    __device__ float value() const { return s; }
+};
+
+template <typename Acc>
+struct SumOf
+{
+    using type = PlainSum<Acc>;
+};
+template <>
+struct SumOf<float>
+{
+    using type = KahanSum;
+};
+
 // One lane per target row; every field of the row reuses the row's
 // coefficients and indices from registers. Sequential sum per row.
 template <typename C, typename X, typename Acc, int NF>
@@ -17,39 +53,41 @@ __global__ void __launch_bounds__( kBlock )
     crs_apply_kernel( int32_t const *__restrict__ idx,
                       C const *__restrict__ coeffs, int64_t n_tgt, int kk,
                       X const *__restrict__ x, int64_t ld_src,
-                      X *__restrict__ y, int64_t ld_tgt, int n_fields )
+                      X *__restrict__ y, int64_t ld_tgt, int n_fields,
+                      int32_t const *__restrict__ order )
 {
 #pragma clang fp contract( off )
-    int64_t const i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-    if ( i >= n_tgt )
+    using Sum = typename SumOf<Acc>::type;
+    int64_t const slot = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( slot >= n_tgt )
         return;
+    // rows in the spatial (Morton) order of the targets when known: a
+    // wave's gathers then touch overlapping source sets, which stay in L2
+    int64_t const i = order ? order[slot] : slot;
     int64_t const row = i * kk;
     if ( NF > 0 )
     {
-        Acc acc[NF > 0 ? NF : 1];
-#pragma unroll
-        for ( int f = 0; f < NF; ++f )
-            acc[f] = 0;
+        Sum acc[NF > 0 ? NF : 1];
         for ( int j = 0; j < kk; ++j )
         {
             int32_t const s = idx[row + j];
             Acc const c = (Acc)coeffs[row + j];
 #pragma unroll
             for ( int f = 0; f < NF; ++f )
-                acc[f] += c * (Acc)x[f * ld_src + s];
+                acc[f].add( c * (Acc)x[f * ld_src + s] );
         }
 #pragma unroll
         for ( int f = 0; f < NF; ++f )
-            y[f * ld_tgt + i] = (X)acc[f];
+            y[f * ld_tgt + i] = (X)acc[f].value();
     }
     else
     {
         for ( int f = 0; f < n_fields; ++f )
         {
-            Acc acc = 0;
+            Sum acc;
             for ( int j = 0; j < kk; ++j )
-                acc += (Acc)coeffs[row + j] * (Acc)x[f * ld_src + idx[row + j]];
-            y[f * ld_tgt + i] = (X)acc;
+                acc.add( (Acc)coeffs[row + j] * (Acc)x[f * ld_src + idx[row + j]] );
+            y[f * ld_tgt + i] = (X)acc.value();
         }
     }
 }
@@ -57,7 +95,7 @@ __global__ void __launch_bounds__( kBlock )
 template <typename C, typename X, typename Acc>
 void launch_apply( int32_t const *idx, C const *coeffs, int64_t n_tgt, int kk,
                    X const *x, int64_t ld_src, X *y, int64_t ld_tgt,
-                   int n_fields, hipStream_t s )
+                   int n_fields, int32_t const *order, hipStream_t s )
 {
     if ( n_tgt == 0 || n_fields == 0 )
         return;
@@ -66,23 +104,23 @@ void launch_apply( int32_t const *idx, C const *coeffs, int64_t n_tgt, int kk,
     {
     case 1:
         crs_apply_kernel<C, X, Acc, 1><<<grid, block, 0, s>>>(
-            idx, coeffs, n_tgt, kk, x, ld_src, y, ld_tgt, n_fields );
+            idx, coeffs, n_tgt, kk, x, ld_src, y, ld_tgt, n_fields, order );
         break;
     case 2:
         crs_apply_kernel<C, X, Acc, 2><<<grid, block, 0, s>>>(
-            idx, coeffs, n_tgt, kk, x, ld_src, y, ld_tgt, n_fields );
+            idx, coeffs, n_tgt, kk, x, ld_src, y, ld_tgt, n_fields, order );
         break;
     case 4:
         crs_apply_kernel<C, X, Acc, 4><<<grid, block, 0, s>>>(
-            idx, coeffs, n_tgt, kk, x, ld_src, y, ld_tgt, n_fields );
+            idx, coeffs, n_tgt, kk, x, ld_src, y, ld_tgt, n_fields, order );
         break;
     case 8:
         crs_apply_kernel<C, X, Acc, 8><<<grid, block, 0, s>>>(
-            idx, coeffs, n_tgt, kk, x, ld_src, y, ld_tgt, n_fields );
+            idx, coeffs, n_tgt, kk, x, ld_src, y, ld_tgt, n_fields, order );
         break;
     default:
         crs_apply_kernel<C, X, Acc, 0><<<grid, block, 0, s>>>(
-            idx, coeffs, n_tgt, kk, x, ld_src, y, ld_tgt, n_fields );
+            idx, coeffs, n_tgt, kk, x, ld_src, y, ld_tgt, n_fields, order );
     }
     DTKX_HIP( hipGetLastError() );
 }
@@ -138,55 +176,30 @@ __global__ void __launch_bounds__( kBlock )
 
 void crs_apply_f64( int32_t const *idx, double const *coeffs, int64_t n_tgt,
                     int kk, double const *x, int64_t ld_src, double *y,
-                    int64_t ld_tgt, int n_fields, hipStream_t s )
+                    int64_t ld_tgt, int n_fields, int32_t const *order,
+                    hipStream_t s )
 {
     launch_apply<double, double, double>( idx, coeffs, n_tgt, kk, x, ld_src,
-                                          y, ld_tgt, n_fields, s );
+                                          y, ld_tgt, n_fields, order, s );
 }
 
 void crs_apply_f32( int32_t const *idx, float const *coeffs, int64_t n_tgt,
                     int kk, float const *x, int64_t ld_src, float *y,
-                    int64_t ld_tgt, int n_fields, hipStream_t s )
+                    int64_t ld_tgt, int n_fields, int32_t const *order,
+                    hipStream_t s )
 {
     launch_apply<float, float, float>( idx, coeffs, n_tgt, kk, x, ld_src, y,
-                                       ld_tgt, n_fields, s );
+                                       ld_tgt, n_fields, order, s );
 }
-
-namespace
-{
-template <int NF>
-__global__ void __launch_bounds__( kBlock )
-    apply_f32_dvals_kernel( int32_t const *__restrict__ idx,
-                            float const *__restrict__ coeffs, int64_t n_tgt,
-                            int kk, double const *__restrict__ x,
-                            int64_t ld_src, double *__restrict__ y,
-                            int64_t ld_tgt, int n_fields )
-{
-    int64_t const i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-    if ( i >= n_tgt )
-        return;
-    int64_t const row = i * kk;
-    for ( int f = 0; f < n_fields; ++f )
-    {
-        float acc = 0.f;
-        for ( int j = 0; j < kk; ++j )
-            acc = fmaf( coeffs[row + j], (float)x[f * ld_src + idx[row + j]],
-                        acc );
-        y[f * ld_tgt + i] = (double)acc;
-    }
-}
-} // namespace
 
 void crs_apply_f32_dvals( int32_t const *idx, float const *coeffs,
                           int64_t n_tgt, int kk, double const *x,
                           int64_t ld_src, double *y, int64_t ld_tgt,
-                          int n_fields, hipStream_t s )
+                          int n_fields, int32_t const *order, hipStream_t s )
 {
-    if ( n_tgt == 0 || n_fields == 0 )
-        return;
-    apply_f32_dvals_kernel<0><<<ceil_div( n_tgt, kBlock ), kBlock, 0, s>>>(
-        idx, coeffs, n_tgt, kk, x, ld_src, y, ld_tgt, n_fields );
-    DTKX_HIP( hipGetLastError() );
+    // values are rounded to float on load, products and sums in float
+    launch_apply<float, double, float>( idx, coeffs, n_tgt, kk, x, ld_src, y,
+                                        ld_tgt, n_fields, order, s );
 }
 
 void build_csc( int32_t const *idx, int64_t nnz, int64_t n_src,

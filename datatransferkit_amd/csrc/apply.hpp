@@ -16,20 +16,24 @@
 namespace dtkx
 {
 
+// order (nullable): the rows are processed in this order (the targets'
+// Morton order from the kNN), which only changes which rows run together.
 void crs_apply_f64( int32_t const *idx, double const *coeffs, int64_t n_tgt,
                     int kk, double const *x, int64_t ld_src, double *y,
-                    int64_t ld_tgt, int n_fields, hipStream_t s );
+                    int64_t ld_tgt, int n_fields, int32_t const *order,
+                    hipStream_t s );
 
 void crs_apply_f32( int32_t const *idx, float const *coeffs, int64_t n_tgt,
                     int kk, float const *x, int64_t ld_src, float *y,
-                    int64_t ld_tgt, int n_fields, hipStream_t s );
+                    int64_t ld_tgt, int n_fields, int32_t const *order,
+                    hipStream_t s );
 
 // FP32 coefficients applied to double values at the boundary: values are
 // rounded to float on load, accumulated in float, widened on store.
 void crs_apply_f32_dvals( int32_t const *idx, float const *coeffs,
                           int64_t n_tgt, int kk, double const *x,
                           int64_t ld_src, double *y, int64_t ld_tgt,
-                          int n_fields, hipStream_t s );
+                          int n_fields, int32_t const *order, hipStream_t s );
 
 struct CscIndex
 {

@@ -204,13 +204,17 @@ void run_setup( dtkx_mls *h, unsigned long long *counters_d )
     p.group_size = h->opts.group_size;
     if ( h->opts.store_pinv )
         h->a0.resize( h->n_tgt * h->m, h->stream );
-    if ( h->opts.solver == DTKX_SOLVER_CYCLIC_JACOBI )
+    if ( h->opts.precision == DTKX_PRECISION_FP32_BULK ||
+         h->opts.solver == DTKX_SOLVER_CYCLIC_JACOBI )
     {
+        // the FP32-bulk form always uses the register-resident cyclic solver
         p.max_rot = h->opts.max_rotations; // sweeps; 0 = default
+        bool const f32 = h->opts.precision == DTKX_PRECISION_FP32_BULK;
         dtkx::setup_cyclic( h->src_d.get(), h->tgt_d.get(), h->n_tgt,
                             h->indices.get(), h->order.get(), p,
-                            h->coeffs.get(), h->a0.get(), counters_d,
-                            h->stream );
+                            f32 ? nullptr : h->coeffs.get(),
+                            f32 ? h->coeffs32.get() : nullptr, h->a0.get(),
+                            counters_d, h->stream );
         h->stats.group_size_used = h->m == 10 ? 2 : 1;
         return;
     }
@@ -230,13 +234,6 @@ void finish_create( dtkx_mls *h, unsigned long long *counters_d )
     offsets_kernel<<<dtkx::ceil_div( h->n_tgt + 1, 256 ), 256, 0,
                      h->stream>>>( h->offsets.get(), h->n_tgt, h->kk );
     DTKX_HIP( hipGetLastError() );
-    if ( h->opts.precision == DTKX_PRECISION_FP32_BULK && nnz > 0 )
-    {
-        h->coeffs32.resize( nnz, h->stream );
-        to_f32_kernel<<<dtkx::ceil_div( nnz, 256 ), 256, 0, h->stream>>>(
-            h->coeffs.get(), h->coeffs32.get(), nnz );
-        DTKX_HIP( hipGetLastError() );
-    }
     unsigned long long c[3] = {0, 0, 0};
     DTKX_HIP( hipMemcpyAsync( c, counters_d, sizeof( c ),
                               hipMemcpyDeviceToHost, h->stream ) );
@@ -249,6 +246,21 @@ void finish_create( dtkx_mls *h, unsigned long long *counters_d )
     h->stats.n_underdetermined = (int64_t)c[0];
     h->stats.n_unconverged = (int64_t)c[1];
     h->stats.total_rotations = (int64_t)c[2];
+}
+
+// coefficient storage of the handle's precision only
+void alloc_coeffs( dtkx_mls *h, int64_t nnz )
+{
+    if ( h->opts.precision == DTKX_PRECISION_FP32_BULK )
+    {
+        h->coeffs.release();
+        h->coeffs32.resize( std::max<int64_t>( nnz, 1 ), h->stream );
+    }
+    else
+    {
+        h->coeffs32.release();
+        h->coeffs.resize( std::max<int64_t>( nnz, 1 ), h->stream );
+    }
 }
 
 std::unique_ptr<dtkx_mls> make_handle( double const *src, int64_t n_src,
@@ -280,7 +292,7 @@ std::unique_ptr<dtkx_mls> make_handle( double const *src, int64_t n_src,
     dtkx::points_to_double4( s_tgt.ptr, n_tgt, opts->layout, h->tgt_d.get(),
                              h->stream );
     h->indices.resize( std::max<int64_t>( n_tgt * h->kk, 1 ), h->stream );
-    h->coeffs.resize( std::max<int64_t>( n_tgt * h->kk, 1 ), h->stream );
+    alloc_coeffs( h.get(), n_tgt * h->kk );
     h->order.resize( std::max<int64_t>( n_tgt, 1 ), h->stream );
     // the staging buffers are freed in stream order at scope exit
     return h;
@@ -332,6 +344,9 @@ int dtkx_mls_create( const double *src_xyz, int64_t n_src,
             EventTimer tk( h->stream );
             dtkx::KnnInfo info;
             int const k = opts->k > 0 ? opts->k : h->m;
+            dtkx::KnnOutput kout;
+            kout.indices = h->indices.get();
+            kout.query_order = h->order.get();
             if ( opts->geometry == DTKX_GEOMETRY_FLOAT )
             {
                 // float points rounded from the caller's doubles
@@ -342,13 +357,12 @@ int dtkx_mls_create( const double *src_xyz, int64_t n_src,
                 dtkx::double4_to_float4( h->tgt_d.get(), n_tgt, tf.get(),
                                          h->stream );
                 dtkx::knn_search( opts->geometry, sf.get(), n_src, tf.get(),
-                                  n_tgt, k, h->indices.get(), nullptr,
-                                  h->order.get(), &info, h->stream );
+                                  n_tgt, k, kout, &info, h->stream );
             }
             else
                 dtkx::knn_search( opts->geometry, h->src_d.get(), n_src,
-                                  h->tgt_d.get(), n_tgt, k, h->indices.get(),
-                                  nullptr, h->order.get(), &info, h->stream );
+                                  h->tgt_d.get(), n_tgt, k, kout, &info,
+                                  h->stream );
             h->stats.ms_knn = tk.stop_ms();
             h->stats.tree_height = info.tree_height;
             EventTimer ts( h->stream );
@@ -378,7 +392,7 @@ int dtkx_mls_create_with_neighbors( const double *src_xyz, int64_t n_src,
         h->opts.k = k_row;
         h->kk = k_row;
         h->indices.resize( std::max<int64_t>( n_tgt * k_row, 1 ), h->stream );
-        h->coeffs.resize( std::max<int64_t>( n_tgt * k_row, 1 ), h->stream );
+        alloc_coeffs( h.get(), n_tgt * k_row );
         h->order.release(); // natural target order
         if ( n_tgt > 0 )
         {
@@ -430,11 +444,12 @@ int dtkx_mls_apply( const dtkx_mls *h, const double *src_vals,
         if ( h->opts.precision == DTKX_PRECISION_FP32_BULK )
             dtkx::crs_apply_f32_dvals( h->indices.get(), h->coeffs32.get(),
                                        h->n_tgt, h->kk, x.ptr, ld_src, y.ptr,
-                                       ld_tgt, n_fields, h->stream );
+                                       ld_tgt, n_fields, nullptr,
+                                       h->stream );
         else
             dtkx::crs_apply_f64( h->indices.get(), h->coeffs.get(), h->n_tgt,
                                  h->kk, x.ptr, ld_src, y.ptr, ld_tgt, n_fields,
-                                 h->stream );
+                                 nullptr, h->stream );
         y.finish( h->stream );
         DTKX_HIP( hipStreamSynchronize( h->stream ) );
         return DTKX_SUCCESS;
@@ -462,7 +477,7 @@ int dtkx_mls_apply_fp32( const dtkx_mls *h, const float *src_vals,
         StagedOut<float> y( tgt_vals, nout, h->stream );
         dtkx::crs_apply_f32( h->indices.get(), h->coeffs32.get(), h->n_tgt,
                              h->kk, x.ptr, ld_src, y.ptr, ld_tgt, n_fields,
-                             h->stream );
+                             nullptr, h->stream );
         y.finish( h->stream );
         DTKX_HIP( hipStreamSynchronize( h->stream ) );
         return DTKX_SUCCESS;
@@ -577,13 +592,16 @@ int dtkx_knn( const double *src_xyz, int64_t n_src, const double *tgt_xyz,
         Staged<double> tx( tgt_xyz, 3 * n_tgt, s );
         StagedOut<int32_t> oi( indices, (std::size_t)n_tgt * kk, s );
         StagedOut<double> od( dist, dist ? (std::size_t)n_tgt * kk : 0, s );
+        dtkx::KnnOutput kout;
+        kout.indices = oi.ptr;
+        kout.dist = od.ptr;
         if ( geometry == DTKX_GEOMETRY_FLOAT )
         {
             dtkx::DeviceBuffer<float4> sf( n_src, s ), tf( n_tgt, s );
             dtkx::points_to_float4( sx.ptr, n_src, layout, sf.get(), s );
             dtkx::points_to_float4( tx.ptr, n_tgt, layout, tf.get(), s );
             dtkx::knn_search( geometry, sf.get(), n_src, tf.get(), n_tgt, k,
-                              oi.ptr, od.ptr, nullptr, nullptr, s );
+                              kout, nullptr, s );
         }
         else
         {
@@ -591,7 +609,7 @@ int dtkx_knn( const double *src_xyz, int64_t n_src, const double *tgt_xyz,
             dtkx::points_to_double4( sx.ptr, n_src, layout, sd.get(), s );
             dtkx::points_to_double4( tx.ptr, n_tgt, layout, td.get(), s );
             dtkx::knn_search( geometry, sd.get(), n_src, td.get(), n_tgt, k,
-                              oi.ptr, od.ptr, nullptr, nullptr, s );
+                              kout, nullptr, s );
         }
         oi.finish( s );
         od.finish( s );

@@ -574,13 +574,15 @@ __global__ void __launch_bounds__( kQueryBlock )
         }
     }
     // compile-time indices keep the k-best list in registers (a loop bounded
-    // by the runtime kk would index it dynamically and move it to scratch)
-    int64_t row = (int64_t)t * kk;
+    // by the runtime kk would index it dynamically and move it to scratch).
+    // Morton layout (src_pos given): row qi, indices as source positions.
+    int64_t const row = ( src_pos ? qi : (int64_t)t ) * kk;
 #pragma unroll
     for ( int j = 0; j < K; ++j )
         if ( j < kk )
         {
-            out_idx[row + j] = best.index( j );
+            int32_t const id = best.index( j );
+            out_idx[row + j] = src_pos ? src_pos[id] : id;
             if ( out_dist )
                 out_dist[row + j] = best.distance( j );
         }
@@ -594,32 +596,33 @@ struct QueryLauncher
     static void run( int stack, Node<T> const *nodes, V const *src,
                      V const *tgt,
                      int32_t const *order, int64_t n_tgt, int64_t n_src,
-                     int kk, int32_t *idx, double *dist, int2 *gstack,
-                     int depth, hipStream_t s )
+                     int kk, int32_t *idx, double *dist, int32_t const *src_pos,
+                     int2 *gstack, int depth, hipStream_t s )
     {
         dim3 grid( ceil_div( n_tgt, kQueryBlock ) ), block( kQueryBlock );
         if ( stack == 32 )
             knn_query_kernel<T, K, 32><<<grid, block, 0, s>>>(
-                nodes, src, tgt, order, n_tgt, n_src, kk, idx, dist, gstack, depth );
+                nodes, src, tgt, order, n_tgt, n_src, kk, idx, dist, src_pos, gstack, depth );
         else if ( stack == 64 )
             knn_query_kernel<T, K, 64><<<grid, block, 0, s>>>(
-                nodes, src, tgt, order, n_tgt, n_src, kk, idx, dist, gstack, depth );
+                nodes, src, tgt, order, n_tgt, n_src, kk, idx, dist, src_pos, gstack, depth );
         else
             knn_query_kernel<T, K, 0><<<grid, block, 0, s>>>(
-                nodes, src, tgt, order, n_tgt, n_src, kk, idx, dist, gstack, depth );
+                nodes, src, tgt, order, n_tgt, n_src, kk, idx, dist, src_pos, gstack, depth );
         DTKX_HIP( hipGetLastError() );
     }
     static void dispatch( int k, int stack, Node<T> const *nodes,
                           V const *src, V const *tgt,
                           int32_t const *order, int64_t n_tgt, int64_t n_src,
-                          int kk, int32_t *idx, double *dist, int2 *gstack,
-                          int depth, hipStream_t s )
+                          int kk, int32_t *idx, double *dist,
+                          int32_t const *src_pos, int2 *gstack, int depth,
+                          hipStream_t s )
     {
 #define DTKX_KNN_CASE( KV )                                                    \
     if ( k <= KV )                                                             \
     {                                                                          \
         run<KV>( stack, nodes, src, tgt, order, n_tgt, n_src, kk, idx, dist,        \
-                 gstack, depth, s );                                           \
+                 src_pos, gstack, depth, s );                                  \
         return;                                                                \
     }
         DTKX_KNN_CASE( 1 )
@@ -637,9 +640,18 @@ struct QueryLauncher
 };
 
 template <typename T>
+__global__ void __launch_bounds__( kBlock )
+    inverse_perm_kernel( int32_t const *perm, int64_t n, int32_t *pos )
+{
+    int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( i < n )
+        pos[perm[i]] = (int32_t)i;
+}
+
+template <typename T>
 void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
-               int64_t n_tgt, int k, int32_t *indices, double *dist,
-               int32_t *query_order, KnnInfo *info, hipStream_t s )
+               int64_t n_tgt, int k, KnnOutput const &out, KnnInfo *info,
+               hipStream_t s )
 {
     using V = typename Vec4<T>::type;
     V const *src = static_cast<V const *>( src_v );
@@ -656,7 +668,13 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
 
     // Morton codes and sort: sources
     DeviceBuffer<uint64_t> codes( n_src, s ), codes_sorted( n_src, s );
-    DeviceBuffer<int32_t> ids( n_src, s ), ids_sorted( n_src, s );
+    DeviceBuffer<int32_t> ids( n_src, s ), ids_sorted_buf;
+    int32_t *ids_sorted = out.src_perm;
+    if ( !ids_sorted )
+    {
+        ids_sorted_buf.resize( n_src, s );
+        ids_sorted = ids_sorted_buf.get();
+    }
     morton_kernel<T, V><<<ceil_div( n_src, kBlock ), kBlock, 0, s>>>(
         src, n_src, box.get(), codes.get(), ids.get() );
     DTKX_HIP( hipGetLastError() );
@@ -664,11 +682,24 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     int64_t const nmax = std::max( n_src, n_tgt );
     DTKX_HIP( hipcub::DeviceRadixSort::SortPairs(
         nullptr, temp_bytes, codes.get(), codes_sorted.get(), ids.get(),
-        ids_sorted.get(), static_cast<int>( nmax ), 0, 63, s ) );
+        ids_sorted, static_cast<int>( nmax ), 0, 63, s ) );
     DeviceBuffer<char> temp( temp_bytes, s );
     DTKX_HIP( hipcub::DeviceRadixSort::SortPairs(
         temp.get(), temp_bytes, codes.get(), codes_sorted.get(), ids.get(),
-        ids_sorted.get(), static_cast<int>( n_src ), 0, 63, s ) );
+        ids_sorted, static_cast<int>( n_src ), 0, 63, s ) );
+    DeviceBuffer<int32_t> pos_buf;
+    int32_t *src_pos = out.src_pos;
+    if ( out.morton_layout && !src_pos )
+    {
+        pos_buf.resize( n_src, s );
+        src_pos = pos_buf.get();
+    }
+    if ( src_pos )
+    {
+        inverse_perm_kernel<<<ceil_div( n_src, kBlock ), kBlock, 0, s>>>(
+            ids_sorted, n_src, src_pos );
+        DTKX_HIP( hipGetLastError() );
+    }
 
     // hierarchy + refit
     int64_t const n_int = n_src - 1;
@@ -680,7 +711,7 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     {
         DTKX_HIP( hipMemsetAsync( height.get(), 0, sizeof( int32_t ), s ) );
         karras_kernel<T, V><<<ceil_div( n_int, kBlock ), kBlock, 0, s>>>(
-            codes_sorted.get(), ids_sorted.get(), src, n_src, box.get(),
+            codes_sorted.get(), ids_sorted, src, n_src, box.get(),
             nodes.get(), par_int.get(), par_leaf.get() );
         leaf_depth_kernel<<<ceil_div( n_src, kBlock ), kBlock, 0, s>>>(
             par_int.get(), par_leaf.get(), n_src, height.get() );
@@ -698,7 +729,7 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
 
     // Morton order of the targets (same scene box)
     DeviceBuffer<int32_t> order_buf;
-    int32_t *order = query_order;
+    int32_t *order = out.query_order;
     if ( !order )
     {
         order_buf.resize( n_tgt, s );
@@ -722,8 +753,9 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     if ( stack == 0 )
         gstack.resize( static_cast<std::size_t>( need + 1 ) * n_tgt, s );
     QueryLauncher<T>::dispatch( k, stack, nodes.get(), src, tgt, order, n_tgt,
-                                n_src, kk, indices, dist, gstack.get(),
-                                need + 1, s );
+                                n_src, kk, out.indices, out.dist,
+                                out.morton_layout ? src_pos : nullptr,
+                                gstack.get(), need + 1, s );
     if ( info )
     {
         info->tree_height = h_tree;
@@ -735,20 +767,17 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
 } // namespace
 
 void knn_search( int geometry, void const *src, int64_t n_src,
-                 void const *tgt, int64_t n_tgt, int k, int32_t *indices,
-                 double *dist, int32_t *query_order, KnnInfo *info,
-                 hipStream_t s )
+                 void const *tgt, int64_t n_tgt, int k, KnnOutput const &out,
+                 KnnInfo *info, hipStream_t s )
 {
     DTKX_REQUIRE( n_src > 0, DTKX_ERR_EMPTY_SOURCE );
     DTKX_REQUIRE( k >= 1 && k <= knn_max_k, DTKX_ERR_UNSUPPORTED );
     DTKX_REQUIRE( n_src < INT32_MAX && n_tgt < INT32_MAX,
                   DTKX_ERR_INVALID_ARGUMENT );
     if ( geometry == DTKX_GEOMETRY_FLOAT )
-        knn_impl<float>( src, n_src, tgt, n_tgt, k, indices, dist,
-                         query_order, info, s );
+        knn_impl<float>( src, n_src, tgt, n_tgt, k, out, info, s );
     else
-        knn_impl<double>( src, n_src, tgt, n_tgt, k, indices, dist,
-                          query_order, info, s );
+        knn_impl<double>( src, n_src, tgt, n_tgt, k, out, info, s );
 }
 
 } // namespace dtkx

@@ -59,6 +59,34 @@ __device__ inline double rbf_eval( int rbf, double x )
     }
 }
 
+// The same functions in float arithmetic for the FP32-bulk form.
+__device__ inline float rbf_eval_f( int rbf, float x )
+{
+    float const a = 1.0f - x;
+    float const a2 = a * a, a4 = a2 * a2;
+    switch ( rbf )
+    {
+    case DTKX_RBF_WENDLAND0:
+        return a2;
+    case DTKX_RBF_WENDLAND2:
+        return a4 * ( 4.0f * x + 1.0f );
+    case DTKX_RBF_WENDLAND4:
+        return a4 * a2 * ( ( 35.0f * x + 18.0f ) * x + 3.0f );
+    case DTKX_RBF_WENDLAND6:
+        return a4 * a4 * ( ( ( 32.0f * x + 25.0f ) * x + 8.0f ) * x + 1.0f );
+    case DTKX_RBF_WU2:
+        return a4 * ( ( ( 3.0f * x + 12.0f ) * x + 16.0f ) * x + 4.0f );
+    case DTKX_RBF_WU4:
+        return a4 * a2 *
+               ( ( ( ( ( 5.0f * x + 30.0f ) * x + 72.0f ) * x + 82.0f ) * x +
+                   36.0f ) *
+                     x +
+                 6.0f );
+    default:
+        return (float)rbf_eval( rbf, (double)x );
+    }
+}
+
 // Basis row P(l, 0..M-1) of a relative coordinate (DTK_MultivariatePolynomial
 // Basis.hpp:249-276). Float geometry: the coordinate is rounded to float and
 // products are float products, stored as double.

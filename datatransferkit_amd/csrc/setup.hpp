@@ -40,11 +40,13 @@ int setup_classical( double4 const *src, double4 const *tgt, int64_t n_tgt,
 int setup_classical_group( int m, int kk );
 
 // Cyclic symmetric Jacobi eigen-solver, registers only (fast path);
-// p.max_rot caps the sweeps (0 = 30).
+// p.max_rot caps the sweeps (0 = 30). With coeffs32 != nullptr the FP32-bulk
+// form runs: float geometry, weights, Vandermonde and coefficients (written
+// to coeffs32), FP64 moments, Jacobi and truncation.
 void setup_cyclic( double4 const *src, double4 const *tgt, int64_t n_tgt,
                    int32_t const *indices, int32_t const *order,
-                   SetupParams const &p, double *coeffs, double *a0,
-                   unsigned long long *counters, hipStream_t s );
+                   SetupParams const &p, double *coeffs, float *coeffs32,
+                   double *a0, unsigned long long *counters, hipStream_t s );
 
 // Batched pseudo-inverse of general n x n matrices (n <= 64), one matrix
 // per workgroup (tstSVD path).

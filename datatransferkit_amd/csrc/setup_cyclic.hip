@@ -68,6 +68,49 @@ __device__ inline void basis_any( int geometry, double const *xr, double *P )
         basis_fast<M, double>( xr[0], xr[1], xr[2], P );
 }
 
+// Geometry of one neighbour: the distance (radius pass) and, given the
+// radius, the basis row and weight. FP64 form: the reference's operations
+// (float or double geometry). FP32 form: float relative coordinates,
+// distance, monomials and weight; the basis row is returned widened so the
+// moments are formed in FP64.
+template <int M, bool F32>
+__device__ inline double nbr_distance( int geometry, double4 const &sp,
+                                       double4 const &tp )
+{
+    if constexpr ( F32 )
+    {
+        float const x = (float)( sp.x - tp.x ), y = (float)( sp.y - tp.y ),
+                    z = (float)( sp.z - tp.z );
+        return (double)__builtin_sqrtf( x * x + y * y + z * z );
+    }
+    else
+    {
+        double const xr[3] = {sp.x - tp.x, sp.y - tp.y, sp.z - tp.z};
+        return dev::rel_distance( geometry, xr );
+    }
+}
+
+template <int M, bool F32>
+__device__ inline double nbr_row( int geometry, int rbf, double radius,
+                                  double4 const &sp, double4 const &tp,
+                                  double *P )
+{
+    if constexpr ( F32 )
+    {
+        float const x = (float)( sp.x - tp.x ), y = (float)( sp.y - tp.y ),
+                    z = (float)( sp.z - tp.z );
+        basis_fast<M, float>( x, y, z, P );
+        float const d = __builtin_sqrtf( x * x + y * y + z * z );
+        return (double)dev::rbf_eval_f( rbf, d / (float)radius );
+    }
+    else
+    {
+        double const xr[3] = {sp.x - tp.x, sp.y - tp.y, sp.z - tp.z};
+        basis_any<M>( geometry, xr, P );
+        return dev::rbf_eval( rbf, dev::rel_distance( geometry, xr ) / radius );
+    }
+}
+
 // V holds R rows of the eigenvector matrix (all of it when R == M)
 template <int M, int R, int P_, int Q_>
 __device__ inline void rotate( double *A, double *V )
@@ -152,7 +195,7 @@ __device__ inline bool converged( double const *A )
 // 256 VGPRs): every lane of the pair keeps the whole moment matrix and runs
 // the same rotations on it (identical values, so both agree), and owns
 // R = M / H rows of V.
-template <int M, int H>
+template <int M, int H, bool F32>
 __global__ void __launch_bounds__( kBlock )
     setup_cyclic_kernel( double4 const *__restrict__ src,
                          double4 const *__restrict__ tgt, int64_t n_tgt,
@@ -160,6 +203,7 @@ __global__ void __launch_bounds__( kBlock )
                          int32_t const *__restrict__ order, int kk, int rbf,
                          int geometry, int truncation, double rcond,
                          int max_sweeps, double *__restrict__ coeffs,
+                         float *__restrict__ coeffs32,
                          double *__restrict__ a0_out,
                          unsigned long long *__restrict__ counters )
 {
@@ -178,12 +222,10 @@ __global__ void __launch_bounds__( kBlock )
     if ( valid )
         for ( int l = 0; l < kk; ++l )
         {
-            double4 const sp = src[nbr[l]];
-            double const xr[3] = {sp.x - tp.x, sp.y - tp.y, sp.z - tp.z};
-            double const d = dev::rel_distance( geometry, xr );
+            double const d = nbr_distance<M, F32>( geometry, src[nbr[l]], tp );
             dmax = d > dmax ? d : dmax;
         }
-    double const radius = 1.1 * dmax;
+    double const radius = F32 ? (double)( 1.1f * (float)dmax ) : 1.1 * dmax;
 
     // moments, upper triangle
     double A[S::size];
@@ -193,12 +235,9 @@ __global__ void __launch_bounds__( kBlock )
     if ( valid )
         for ( int l = 0; l < kk; ++l )
         {
-            double4 const sp = src[nbr[l]];
-            double const xr[3] = {sp.x - tp.x, sp.y - tp.y, sp.z - tp.z};
             double P[M];
-            basis_any<M>( geometry, xr, P );
             double const phi =
-                dev::rbf_eval( rbf, dev::rel_distance( geometry, xr ) / radius );
+                nbr_row<M, F32>( geometry, rbf, radius, src[nbr[l]], tp, P );
 #pragma unroll
             for ( int r = 0; r < M; ++r )
             {
@@ -282,17 +321,28 @@ __global__ void __launch_bounds__( kBlock )
     // coefficients c_l = phi_l (P_l . x), neighbours split over the H lanes
     for ( int l = h; l < kk; l += H )
     {
-        double4 const sp = src[nbr[l]];
-        double const xr[3] = {sp.x - tp.x, sp.y - tp.y, sp.z - tp.z};
         double P[M];
-        basis_any<M>( geometry, xr, P );
         double const phi =
-            dev::rbf_eval( rbf, dev::rel_distance( geometry, xr ) / radius );
-        double acc = 0.0;
+            nbr_row<M, F32>( geometry, rbf, radius, src[nbr[l]], tp, P );
+        if constexpr ( F32 )
+        {
+            // the float basis row and weight are exact in double: the short
+            // dot product with x (large cancelling terms, the monomials span
+            // several scales) is formed in double and rounded once
+            double acc = 0.0;
 #pragma unroll
-        for ( int j = 0; j < M; ++j )
-            acc = fma( x[j], P[j], acc );
-        coeffs[t * kk + l] = acc * phi;
+            for ( int j = 0; j < M; ++j )
+                acc = fma( x[j], P[j], acc );
+            coeffs32[t * kk + l] = (float)( acc * phi );
+        }
+        else
+        {
+            double acc = 0.0;
+#pragma unroll
+            for ( int j = 0; j < M; ++j )
+                acc = fma( x[j], P[j], acc );
+            coeffs[t * kk + l] = acc * phi;
+        }
     }
     if ( counters && h == 0 )
     {
@@ -307,13 +357,19 @@ __global__ void __launch_bounds__( kBlock )
 template <int M, int H>
 void launch_cyclic( double4 const *src, double4 const *tgt, int64_t n_tgt,
                     int32_t const *indices, int32_t const *order,
-                    SetupParams const &p, double *coeffs, double *a0,
-                    unsigned long long *counters, hipStream_t s )
+                    SetupParams const &p, double *coeffs, float *coeffs32,
+                    double *a0, unsigned long long *counters, hipStream_t s )
 {
     int const sweeps = p.max_rot > 0 ? static_cast<int>( p.max_rot ) : 30;
-    setup_cyclic_kernel<M, H><<<ceil_div( n_tgt, kBlock / H ), kBlock, 0, s>>>(
-        src, tgt, n_tgt, indices, order, p.kk, p.rbf, p.geometry, p.truncation,
-        p.rcond, sweeps, coeffs, a0, counters );
+    dim3 const grid( ceil_div( n_tgt, kBlock / H ) ), block( kBlock );
+    if ( coeffs32 )
+        setup_cyclic_kernel<M, H, true><<<grid, block, 0, s>>>(
+            src, tgt, n_tgt, indices, order, p.kk, p.rbf, p.geometry,
+            p.truncation, p.rcond, sweeps, coeffs, coeffs32, a0, counters );
+    else
+        setup_cyclic_kernel<M, H, false><<<grid, block, 0, s>>>(
+            src, tgt, n_tgt, indices, order, p.kk, p.rbf, p.geometry,
+            p.truncation, p.rcond, sweeps, coeffs, coeffs32, a0, counters );
     DTKX_HIP( hipGetLastError() );
 }
 
@@ -321,24 +377,24 @@ void launch_cyclic( double4 const *src, double4 const *tgt, int64_t n_tgt,
 
 void setup_cyclic( double4 const *src, double4 const *tgt, int64_t n_tgt,
                    int32_t const *indices, int32_t const *order,
-                   SetupParams const &p, double *coeffs, double *a0,
-                   unsigned long long *counters, hipStream_t s )
+                   SetupParams const &p, double *coeffs, float *coeffs32,
+                   double *a0, unsigned long long *counters, hipStream_t s )
 {
     if ( n_tgt == 0 )
         return;
     switch ( p.m )
     {
     case 1:
-        launch_cyclic<1, 1>( src, tgt, n_tgt, indices, order, p, coeffs, a0,
-                          counters, s );
+        launch_cyclic<1, 1>( src, tgt, n_tgt, indices, order, p, coeffs,
+                                 coeffs32, a0, counters, s );
         return;
     case 4:
-        launch_cyclic<4, 1>( src, tgt, n_tgt, indices, order, p, coeffs, a0,
-                          counters, s );
+        launch_cyclic<4, 1>( src, tgt, n_tgt, indices, order, p, coeffs,
+                                 coeffs32, a0, counters, s );
         return;
     case 10:
-        launch_cyclic<10, 2>( src, tgt, n_tgt, indices, order, p, coeffs, a0,
-                           counters, s );
+        launch_cyclic<10, 2>( src, tgt, n_tgt, indices, order, p, coeffs,
+                                 coeffs32, a0, counters, s );
         return;
     }
     DTKX_REQUIRE( p.m == 1 || p.m == 4 || p.m == 10, DTKX_ERR_INVALID_ARGUMENT );
