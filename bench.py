@@ -4,12 +4,22 @@ A step is one pass of the hot path over one batch: build the operator for
 1,000,000 sources uniform in [0,1]^3 and 1,000,000 targets uniform in
 [0.05,0.95]^3 (kNN + fused setup: weights, Vandermonde, moments, Jacobi
 pseudo-inverse, coefficients), then apply it to 4 scalar fields (smooth
-trig polynomials + 1e-6 Gaussian noise). Inputs are resident in HBM before
-the timed region. The headline value is target points per second,
-N_t / (t_knn + t_setup + t_apply(4 fields)), for the plain FP64 form with
-the reference's quadratic basis and k = basis size (10), Wendland C2.
+trig polynomials + 1e-6 Gaussian noise). Quadratic basis, Wendland C2,
+k = 32 neighbours by default. Inputs are resident in HBM before the timed
+region. Throughput is target points per second over whole steps.
 
-Prints one JSON line (rank 0). See DESIGN.md for the roofline definitions.
+Three forms of the operator run on the same inputs:
+  fp32_bulk    float geometry, weights, Vandermonde, coefficients and apply
+               (compensated float sums); FP64 moments, Jacobi eigen-solver and
+               truncation. This is the headline `value`.
+  fp64         the plain FP64 drop-in: the reference's classical two-sided
+               Jacobi, bit-exact with the CPU oracle.
+  fp64_cyclic  FP64 throughout with the register-resident cyclic Jacobi.
+Each form's error against the FP64 CPU oracle is measured on the oracle's
+sample of targets, and the FP32 form's error against the fp64 GPU form on
+every target.
+
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -24,6 +34,16 @@ sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FORMS = {
+    "fp32_bulk": dict(precision="fp32_bulk", solver="cyclic"),
+    "fp64": dict(precision="fp64", solver="classical"),
+    "fp64_cyclic": dict(precision="fp64", solver="cyclic"),
+}
+SETUP_KERNEL = {
+    "fp32_bulk": "setup_cyclic_kernel<10,2,true> (fused MLS setup, FP32 bulk + FP64 Jacobi)",
+    "fp64": "setup_classical_kernel<10,G> (fused MLS setup, FP64 classical Jacobi)",
+    "fp64_cyclic": "setup_cyclic_kernel<10,2,false> (fused MLS setup, FP64 cyclic Jacobi)",
+}
 
 
 def f_setup(m, k):
@@ -64,43 +84,108 @@ def make_inputs(n_src, n_tgt, fields, seed=1):
     return src, tgt, vals
 
 
-def cpu_baseline(src, tgt, vals, m, k, rbf, sample):
-    """The oracle (C++ restatement of the reference, OpenMP) on the first
-    `sample` targets against all sources, same stages: kNN, setup, apply."""
+def rel_inf(y, ref):
+    """Per-field relative error in the infinity norm."""
     import numpy as np
 
+    return (np.abs(y - ref).max(axis=1) / np.abs(ref).max(axis=1)).tolist()
+
+
+def cpu_baseline(src, tgt, vals, m, k, rbf, sample):
+    """The oracle (C++ restatement of the reference, OpenMP) on the first
+    `sample` targets against all sources, same stages: kNN, setup, apply.
+    Returns the record and the oracle's target values."""
     import oracle
 
     threads = min(os.cpu_count() or 1, int(os.environ.get("DTKX_CPU_THREADS", "16")))
     oracle.set_num_threads(threads)
-    t0 = time.perf_counter()
-    idx, coeffs, _ = oracle.mls_create(src, tgt[:sample], m, k=k, rbf_name=rbf)
-    oracle.apply(idx, coeffs, vals)
-    dt = time.perf_counter() - t0
-    return {
+    # elementary functions correctly rounded, as the GPU kernels evaluate them
+    with oracle.libm_mode(oracle.LIBM_CORRECTLY_ROUNDED):
+        t0 = time.perf_counter()
+        idx, coeffs, _ = oracle.mls_create(src, tgt[:sample], m, k=k, rbf_name=rbf)
+        ref = oracle.apply(idx, coeffs, vals)
+        dt = time.perf_counter() - t0
+    rec = {
         "value": sample / dt,
         "unit": "target pts/s",
         "cores": threads,
         "kind": "port",
         "sample": f"first {sample} of {len(tgt)} targets, all {len(src)} sources, "
-                  f"k-d tree kNN + setup + apply of {vals.shape[0]} fields ({dt:.1f} s)",
+                  f"k-d tree kNN + classical-Jacobi setup (correctly rounded libm) + "
+                  f"apply of {vals.shape[0]} fields ({dt:.1f} s)",
     }
+    return rec, ref
+
+
+def load_traffic(form, k):
+    """HBM bytes per launch of the setup kernel from rocprofv3 --pmc
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel), if measured."""
+    path = os.path.join(REPO, "profiles", "setup_traffic.json")
+    try:
+        with open(path) as fh:
+            return json.load(fh).get(f"{form}_k{k}")
+    except (OSError, ValueError):
+        return None
+
+
+def run_form(dtk, torch, name, d_src, d_tgt, d_vals, n_fields, args, stream):
+    """Warm up, then time `steps` whole steps (create + apply) of one form.
+    Returns timings, the last step's output and the last step's stats."""
+    import numpy as np
+
+    form = FORMS[name]
+    dev = d_src.device
+    vdt = torch.float32 if form["precision"] == "fp32_bulk" else torch.float64
+    x = d_vals.to(vdt)
+    y = torch.zeros((n_fields, d_tgt.shape[0]), dtype=vdt, device=dev)
+    kw = dict(basis="quadratic", rbf=args.rbf, k=args.k, stream=stream, **form)
+
+    def step():
+        op = dtk.MovingLeastSquaresOperator(d_src, d_tgt, **kw)
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        op.apply(x, y)
+        b.record()
+        b.synchronize()
+        st = op.stats()
+        st["ms_apply"] = a.elapsed_time(b)
+        del op
+        return st
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        stats.append(step())
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    n = d_tgt.shape[0]
+    res = {
+        "value": n * args.steps / dt,
+        "ms_per_step": 1000 * dt / args.steps,
+        "ms_knn": float(np.mean([s["ms_knn"] for s in stats])),
+        "ms_setup": float(np.mean([s["ms_setup"] for s in stats])),
+        "ms_apply": float(np.mean([s["ms_apply"] for s in stats])),
+    }
+    return res, y.double().cpu().numpy(), stats[-1]
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--k", type=int, default=10)
+    ap.add_argument("--k", type=int, default=32)
     ap.add_argument("--fields", type=int, default=4)
     ap.add_argument("--rbf", default="wendland2")
-    ap.add_argument("--solver", default="classical")
-    ap.add_argument("--precision", default="fp64")
-    ap.add_argument("--group", type=int, default=0,
-                    help="lanes per system for the classical solver (0 = default)")
-    ap.add_argument("--cpu-sample", type=int, default=20_000)
+    ap.add_argument("--form", default="fp32_bulk", choices=sorted(FORMS))
+    ap.add_argument("--compare", default="fp64,fp64_cyclic",
+                    help="comma-separated forms timed beside the headline form ('' for none)")
+    ap.add_argument("--cpu-sample", type=int, default=250_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
 
@@ -120,95 +205,81 @@ def main():
     src, tgt, vals = make_inputs(n, n, args.fields)
     d_src = torch.from_numpy(src).to(dev)
     d_tgt = torch.from_numpy(tgt).to(dev)
-    # the FP32-bulk form takes float values (the halved-traffic apply)
-    vdt = torch.float32 if args.precision == "fp32_bulk" else torch.float64
-    d_vals = torch.from_numpy(vals).to(dev).to(vdt)
-    d_out = torch.zeros((args.fields, n), dtype=vdt, device=dev)
+    d_vals = torch.from_numpy(vals).to(dev)
     stream = torch.cuda.current_stream(dev).cuda_stream
-    kw = dict(basis="quadratic", rbf=args.rbf, k=args.k, solver=args.solver,
-              precision=args.precision, stream=stream, group_size=args.group)
 
-    def step():
-        op = dtk.MovingLeastSquaresOperator(d_src, d_tgt, **kw)
-        a = torch.cuda.Event(enable_timing=True)
-        b = torch.cuda.Event(enable_timing=True)
-        a.record()
-        op.apply(d_vals, d_out)
-        b.record()
-        b.synchronize()
-        st = op.stats()
-        st["ms_apply"] = a.elapsed_time(b)
-        del op
-        return st
+    others = [f for f in args.compare.split(",") if f and f != args.form]
+    results, outputs, last_stats = {}, {}, {}
+    for name in [args.form] + others:
+        results[name], outputs[name], last_stats[name] = run_form(
+            dtk, torch, name, d_src, d_tgt, d_vals, args.fields, args, stream)
+        st = last_stats[name]
+        results[name].update({
+            "rotations_or_sweeps_per_target": st["total_rotations"] / max(n, 1),
+            "n_underdetermined": st["n_underdetermined"],
+            "n_unconverged": st["n_unconverged"],
+        })
+    head = results[args.form]
+    st = last_stats[args.form]
+    kk, nnz = st["k"], st["nnz"]
+    fp32 = FORMS[args.form]["precision"] == "fp32_bulk"
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    stats = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        stats.append(step())
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    ms_step = 1000 * dt / args.steps
-    value = n * args.steps / dt
+    for name, r in results.items():
+        r["setup_TFLOPs_algorithmic"] = f_setup(m, kk) * n / (r["ms_setup"] * 1e-3) / 1e12
+        r["apply_GBps_algorithmic"] = b_apply(
+            nnz, n, n, args.fields, FORMS[name]["precision"] == "fp32_bulk") / (r["ms_apply"] * 1e-3) / 1e9
+        r["knn_GBps_algorithmic"] = b_knn(n, n, kk) / (r["ms_knn"] * 1e-3) / 1e9
+    if "fp64" in outputs and args.form != "fp64":
+        head["rel_err_vs_fp64_gpu_all_targets"] = rel_inf(outputs[args.form], outputs["fp64"])
 
-    ms_knn = float(np.mean([s["ms_knn"] for s in stats]))
-    ms_setup = float(np.mean([s["ms_setup"] for s in stats]))
-    ms_apply = float(np.mean([s["ms_apply"] for s in stats]))
-    kk = stats[0]["k"]
-    nnz = stats[0]["nnz"]
-    setup_tflops = f_setup(m, kk) * n / (ms_setup * 1e-3) / 1e12
-    apply_gbs = b_apply(nnz, n, n, args.fields, args.precision == "fp32_bulk") / (ms_apply * 1e-3) / 1e9
-    knn_gbs = b_knn(n, n, kk) / (ms_knn * 1e-3) / 1e9
-
+    setup_tflops = head["setup_TFLOPs_algorithmic"]
     roofline = {
         "bound": "mfma",
-        "kernel": "setup_classical_kernel (fused MLS setup, FP64 VALU)",
+        "kernel": SETUP_KERNEL[args.form],
         "achieved": round(setup_tflops, 4),
         "peak": FP64_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(setup_tflops / FP64_PEAK_TFLOPS, 5),
-        "traffic": None,
-        "algorithmic_per_unit": f"F_setup(m={m},k={kk}) = {f_setup(m, kk)} flop per target",
+        "traffic": load_traffic(args.form, kk),
+        "algorithmic_per_unit": f"F_setup(m={m},k={kk}) = {f_setup(m, kk)} flop per target, "
+                                f"{n} targets per launch",
+        "note": "FP64 VALU work (the k x m systems are too small for MFMA); "
+                "peak = MI355X FP64 78.6 TFLOP/s",
     }
     out = {
         "metric": "MLS setup+apply target pts/s (1M pts, quad basis): plain FP64 vs batched/FP32",
-        "value": value,
+        "value": head["value"],
         "unit": "target pts/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_step,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "dtype": "f32+f64" if fp32 else "f64",
         "data": "synthetic",
         "config": {
             "workload": f"MLS transfer {n} sources U[0,1]^3 -> {n} targets U[0.05,0.95]^3, "
                         f"quadratic basis, k={kk}, {args.rbf}, {args.fields} fields, "
-                        f"{args.precision}, {args.solver} Jacobi",
+                        f"form {args.form}",
             "n_src": n, "n_tgt": n, "basis": "quadratic", "k": kk, "rbf": args.rbf,
-            "fields": args.fields, "precision": args.precision, "solver": args.solver,
+            "fields": args.fields, "form": args.form,
         },
-        "stages": {
-            "ms_knn": ms_knn, "ms_setup": ms_setup, "ms_apply": ms_apply,
-            "knn_GBps_algorithmic": knn_gbs,
-            "setup_TFLOPs_algorithmic": setup_tflops,
-            "apply_GBps_algorithmic": apply_gbs,
-            "apply_frac_hbm": apply_gbs / HBM_PEAK_GBS,
-            "rotations_per_target": stats[0]["total_rotations"] / n,
-            "n_underdetermined": stats[0]["n_underdetermined"],
-            "n_unconverged": stats[0]["n_unconverged"],
-            "tree_height": stats[0]["tree_height"],
-            "group_size": stats[0]["group_size_used"],
-        },
+        "forms": results,
+        "plain_fp64_value": results.get("fp64", {}).get("value"),
+        "speedup_vs_plain_fp64": (head["value"] / results["fp64"]["value"]
+                                  if "fp64" in results else None),
         "roofline": roofline,
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(src, tgt, vals, m, args.k, args.rbf, args.cpu_sample)
+        s = min(args.cpu_sample, n)
+        rec, ref = cpu_baseline(src, tgt, vals, m, kk, args.rbf, s)
+        out["cpu_baseline"] = rec
+        for name, y in outputs.items():
+            results[name]["rel_err_vs_oracle_sample"] = rel_inf(y[:, :s], ref)
+            results[name]["bit_exact_vs_oracle_sample"] = bool(np.array_equal(y[:, :s], ref))
     print(json.dumps(out), flush=True)
 
 

@@ -6,6 +6,8 @@ Jacobi pseudo-inverse, CRS apply and transpose) behind a C ABI
 (``include/dtkx/mls.h``), with this Python mirror of the operator surface.
 """
 from ._lib import DtkxError  # noqa: F401
-from .operators import MovingLeastSquaresOperator, knn, svd_pseudo_inverse  # noqa: F401
+from .operators import (  # noqa: F401
+    MovingLeastSquaresOperator, knn, release_cached_memory, svd_pseudo_inverse)
 
-__all__ = ["MovingLeastSquaresOperator", "knn", "svd_pseudo_inverse", "DtkxError"]
+__all__ = ["MovingLeastSquaresOperator", "knn", "svd_pseudo_inverse",
+           "release_cached_memory", "DtkxError"]

@@ -38,7 +38,8 @@ LAYOUT = {"aos": 0, "soa": 1}
 
 # every symbol include/dtkx/mls.h declares
 EXPORTED = [
-    "dtkx_last_error", "dtkx_version", "dtkx_mls_options_init",
+    "dtkx_last_error", "dtkx_version", "dtkx_release_cached_memory",
+    "dtkx_mls_options_init",
     "dtkx_mls_create", "dtkx_mls_create_with_neighbors", "dtkx_mls_apply",
     "dtkx_mls_apply_fp32", "dtkx_mls_apply_transpose",
     "dtkx_mls_target_gradient", "dtkx_mls_get_graph", "dtkx_mls_get_stats",
@@ -109,6 +110,7 @@ def load():
     L = ctypes.CDLL(LIB_PATH)
     L.dtkx_last_error.restype = ctypes.c_char_p
     L.dtkx_version.restype = ctypes.c_char_p
+    L.dtkx_release_cached_memory.argtypes = []
     L.dtkx_mls_options_init.argtypes = [ctypes.POINTER(Options)]
     L.dtkx_mls_create.argtypes = [_vp, _i64, _vp, _i64, ctypes.POINTER(Options),
                                   ctypes.POINTER(_vp)]

@@ -19,6 +19,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdtkx.so")
 
 SOURCES = [
+    "memory.hip",
     "points.hip",
     "knn.hip",
     "setup_classical.hip",

@@ -46,203 +46,376 @@ struct SumOf<float>
     using type = KahanSum;
 };
 
-// One lane per target row; every field of the row reuses the row's
-// coefficients and indices from registers. Sequential sum per row.
-template <typename C, typename X, typename Acc, int NF>
+// The NF field values of one source position (or one row), contiguous.
+template <typename T, int N>
+struct alignas( sizeof( T ) * N > 16 ? 16 : sizeof( T ) * N ) Pack
+{
+    T v[N];
+};
+
+// xs[p] = (x[f][perm[p]] for f < nf, zero-padded to NF), converted to XS.
+// Also used for the rows of the transpose (perm = order).
+template <typename X, typename XS, int NF>
 __global__ void __launch_bounds__( kBlock )
-    crs_apply_kernel( int32_t const *__restrict__ idx,
-                      C const *__restrict__ coeffs, int64_t n_tgt, int kk,
-                      X const *__restrict__ x, int64_t ld_src,
-                      X *__restrict__ y, int64_t ld_tgt, int n_fields,
-                      int32_t const *__restrict__ order )
+    gather_fields_kernel( X const *__restrict__ x, int64_t ld, int nf,
+                          int32_t const *__restrict__ perm, int64_t n,
+                          Pack<XS, NF> *__restrict__ xs )
+{
+    int64_t const p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( p >= n )
+        return;
+    int64_t const s = perm ? perm[p] : p;
+    Pack<XS, NF> v;
+#pragma unroll
+    for ( int f = 0; f < NF; ++f )
+        v.v[f] = f < nf ? (XS)x[f * ld + s] : XS( 0 );
+    xs[p] = v;
+}
+
+// One lane per stored row (consecutive lanes: spatially adjacent targets);
+// every field of the row reuses the row's coefficients and indices.
+// Sequential sum per row, j ascending. Each wave owns 64 consecutive rows
+// and stages their indices and coefficients through LDS kChunk columns at
+// a time, so the HBM reads are runs of whole rows rather than one strided
+// element per lane.
+constexpr int kChunk = 8;
+constexpr int kPitch = kChunk + 1; // odd row pitch: no LDS bank conflicts
+
+template <typename C, typename XS, typename X, typename Acc, int NF>
+__global__ void __launch_bounds__( kBlock )
+    crs_rows_kernel( int32_t const *__restrict__ idx,
+                     C const *__restrict__ coeffs, int64_t n_tgt, int kk,
+                     Pack<XS, NF> const *__restrict__ xs,
+                     int32_t const *__restrict__ order, X *__restrict__ y,
+                     int64_t ld_tgt, int nf )
 {
 #pragma clang fp contract( off )
     using Sum = typename SumOf<Acc>::type;
-    int64_t const slot = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-    if ( slot >= n_tgt )
+    constexpr int kWaves = kBlock / 64;
+    __shared__ int32_t s_idx[kWaves][64 * kPitch];
+    __shared__ C s_c[kWaves][64 * kPitch];
+    int const wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    int64_t const r0 = blockIdx.x * (int64_t)kBlock + wave * 64;
+    if ( r0 >= n_tgt )
+        return; // whole wave: no block-level barrier below
+    int const nrows = n_tgt - r0 < 64 ? (int)( n_tgt - r0 ) : 64;
+    int32_t *const si = s_idx[wave];
+    C *const sc = s_c[wave];
+    Sum acc[NF];
+    for ( int j0 = 0; j0 < kk; j0 += kChunk )
+    {
+        int const ch = kk - j0 < kChunk ? kk - j0 : kChunk;
+        // runs of ch contiguous entries per row
+        for ( int e = lane; e < nrows * ch; e += 64 )
+        {
+            int const rr = e / ch, jj = e - rr * ch;
+            int64_t const g = ( r0 + rr ) * kk + j0 + jj;
+            si[rr * kPitch + jj] = idx[g];
+            sc[rr * kPitch + jj] = coeffs[g];
+        }
+        __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
+        if ( lane < nrows )
+            for ( int jj = 0; jj < ch; ++jj )
+            {
+                int32_t const p = si[lane * kPitch + jj];
+                Acc const c = (Acc)sc[lane * kPitch + jj];
+                Pack<XS, NF> const v = xs[p];
+#pragma unroll
+                for ( int f = 0; f < NF; ++f )
+                    acc[f].add( c * (Acc)v.v[f] );
+            }
+        __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
+    }
+    if ( lane >= nrows )
         return;
-    // rows in the spatial (Morton) order of the targets when known: a
-    // wave's gathers then touch overlapping source sets, which stay in L2
-    int64_t const i = order ? order[slot] : slot;
-    int64_t const row = i * kk;
-    if ( NF > 0 )
-    {
-        Sum acc[NF > 0 ? NF : 1];
-        for ( int j = 0; j < kk; ++j )
-        {
-            int32_t const s = idx[row + j];
-            Acc const c = (Acc)coeffs[row + j];
+    int64_t const r = r0 + lane;
+    int64_t const t = order ? order[r] : r;
 #pragma unroll
-            for ( int f = 0; f < NF; ++f )
-                acc[f].add( c * (Acc)x[f * ld_src + s] );
-        }
-#pragma unroll
-        for ( int f = 0; f < NF; ++f )
-            y[f * ld_tgt + i] = (X)acc[f].value();
-    }
-    else
-    {
-        for ( int f = 0; f < n_fields; ++f )
-        {
-            Sum acc;
-            for ( int j = 0; j < kk; ++j )
-                acc.add( (Acc)coeffs[row + j] * (Acc)x[f * ld_src + idx[row + j]] );
-            y[f * ld_tgt + i] = (X)acc.value();
-        }
-    }
+    for ( int f = 0; f < NF; ++f )
+        if ( f < nf )
+            y[f * ld_tgt + t] = (X)acc[f].value();
 }
 
-template <typename C, typename X, typename Acc>
-void launch_apply( int32_t const *idx, C const *coeffs, int64_t n_tgt, int kk,
-                   X const *x, int64_t ld_src, X *y, int64_t ld_tgt,
-                   int n_fields, int32_t const *order, hipStream_t s )
+// fields processed per pass: 1, 2, 4 or 8 (the last pass zero-padded)
+inline int pass_width( int remaining )
 {
-    if ( n_tgt == 0 || n_fields == 0 )
-        return;
-    dim3 grid( ceil_div( n_tgt, kBlock ) ), block( kBlock );
-    switch ( n_fields )
-    {
-    case 1:
-        crs_apply_kernel<C, X, Acc, 1><<<grid, block, 0, s>>>(
-            idx, coeffs, n_tgt, kk, x, ld_src, y, ld_tgt, n_fields, order );
-        break;
-    case 2:
-        crs_apply_kernel<C, X, Acc, 2><<<grid, block, 0, s>>>(
-            idx, coeffs, n_tgt, kk, x, ld_src, y, ld_tgt, n_fields, order );
-        break;
-    case 4:
-        crs_apply_kernel<C, X, Acc, 4><<<grid, block, 0, s>>>(
-            idx, coeffs, n_tgt, kk, x, ld_src, y, ld_tgt, n_fields, order );
-        break;
-    case 8:
-        crs_apply_kernel<C, X, Acc, 8><<<grid, block, 0, s>>>(
-            idx, coeffs, n_tgt, kk, x, ld_src, y, ld_tgt, n_fields, order );
-        break;
-    default:
-        crs_apply_kernel<C, X, Acc, 0><<<grid, block, 0, s>>>(
-            idx, coeffs, n_tgt, kk, x, ld_src, y, ld_tgt, n_fields, order );
-    }
+    return remaining >= 5 ? 8 : ( remaining >= 3 ? 4 : remaining );
+}
+
+template <typename C, typename X, typename XS, typename Acc, int NF>
+void apply_pass( RowLayout const &g, C const *coeffs, X const *x,
+                 int64_t ld_src, X *y, int64_t ld_tgt, int nf, XS *scratch,
+                 hipStream_t s )
+{
+    auto *xs = reinterpret_cast<Pack<XS, NF> *>( scratch );
+    gather_fields_kernel<X, XS, NF>
+        <<<ceil_div( g.n_src, kBlock ), kBlock, 0, s>>>( x, ld_src, nf,
+                                                         g.perm, g.n_src, xs );
+    crs_rows_kernel<C, XS, X, Acc, NF>
+        <<<ceil_div( g.n_tgt, kBlock ), kBlock, 0, s>>>(
+            g.idx, coeffs, g.n_tgt, g.kk, xs, g.order, y, ld_tgt, nf );
     DTKX_HIP( hipGetLastError() );
 }
 
-// ------------------------------------------------------------ transpose
-__global__ void __launch_bounds__( kBlock )
-    iota_kernel( int32_t *v, int64_t n )
+template <typename C, typename X, typename XS, typename Acc>
+void launch_apply( RowLayout const &g, C const *coeffs, X const *x,
+                   int64_t ld_src, X *y, int64_t ld_tgt, int n_fields,
+                   hipStream_t s )
 {
-    int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-    if ( i < n )
-        v[i] = (int32_t)i;
+    if ( g.n_tgt == 0 || n_fields == 0 )
+        return;
+    DeviceBuffer<XS> scratch(
+        (std::size_t)g.n_src * pass_width( n_fields ), s );
+    for ( int f0 = 0; f0 < n_fields; )
+    {
+        int const w = pass_width( n_fields - f0 );
+        int const nf = std::min( w, n_fields - f0 );
+        X const *xf = x + f0 * ld_src;
+        X *yf = y + f0 * ld_tgt;
+        switch ( w )
+        {
+        case 1:
+            apply_pass<C, X, XS, Acc, 1>( g, coeffs, xf, ld_src, yf, ld_tgt,
+                                          nf, scratch.get(), s );
+            break;
+        case 2:
+            apply_pass<C, X, XS, Acc, 2>( g, coeffs, xf, ld_src, yf, ld_tgt,
+                                          nf, scratch.get(), s );
+            break;
+        case 4:
+            apply_pass<C, X, XS, Acc, 4>( g, coeffs, xf, ld_src, yf, ld_tgt,
+                                          nf, scratch.get(), s );
+            break;
+        default:
+            apply_pass<C, X, XS, Acc, 8>( g, coeffs, xf, ld_src, yf, ld_tgt,
+                                          nf, scratch.get(), s );
+        }
+        f0 += nf;
+    }
 }
 
-// col_offsets[s] = first position p in the sorted keys with key[p] >= s
+// ------------------------------------------------------------ transpose
+// Entries enumerated in canonical order e = t * kk + j (target t, rank j):
+// key = the entry's column (source position), value = its storage slot.
+__global__ void __launch_bounds__( kBlock )
+    canonical_keys_kernel( int32_t const *__restrict__ idx,
+                           int32_t const *__restrict__ trow, int64_t nnz,
+                           int kk, int32_t *__restrict__ keys,
+                           int32_t *__restrict__ slots )
+{
+    int64_t const e = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( e >= nnz )
+        return;
+    int64_t const t = e / kk, j = e - t * kk;
+    int64_t const l = ( trow ? trow[t] : t ) * kk + j;
+    keys[e] = idx[l];
+    slots[e] = (int32_t)l;
+}
+
+// col_offsets[p] = first index q in the sorted keys with key[q] >= p
 __global__ void __launch_bounds__( kBlock )
     col_offsets_kernel( int32_t const *keys, int64_t nnz, int64_t n_src,
                         int32_t *off )
 {
-    int64_t p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-    if ( p > nnz )
+    int64_t q = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( q > nnz )
         return;
-    int64_t const prev = p == 0 ? -1 : keys[p - 1];
-    int64_t const cur = p == nnz ? n_src : keys[p];
-    for ( int64_t s = prev + 1; s <= cur; ++s )
-        off[s] = (int32_t)p;
+    int64_t const prev = q == 0 ? -1 : keys[q - 1];
+    int64_t const cur = q == nnz ? n_src : keys[q];
+    for ( int64_t p = prev + 1; p <= cur; ++p )
+        off[p] = (int32_t)q;
 }
 
+// One lane per source position: its column's entries in canonical order,
+// each times the row's gathered ybar values.
+template <int NF>
 __global__ void __launch_bounds__( kBlock )
     transpose_kernel( int32_t const *__restrict__ off,
                       int32_t const *__restrict__ ent,
                       double const *__restrict__ coeffs, int kk, int64_t n_src,
-                      double const *__restrict__ ybar, int64_t ld_tgt,
-                      double *__restrict__ xbar, int64_t ld_src, int n_fields )
+                      Pack<double, NF> const *__restrict__ ys,
+                      int32_t const *__restrict__ perm,
+                      double *__restrict__ xbar, int64_t ld_src, int nf )
 {
 #pragma clang fp contract( off )
-    int64_t const s = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-    if ( s >= n_src )
+    int64_t const p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( p >= n_src )
         return;
-    int32_t const b = off[s], e = off[s + 1];
-    for ( int f = 0; f < n_fields; ++f )
+    int32_t const b = off[p], e = off[p + 1];
+    double acc[NF];
+#pragma unroll
+    for ( int f = 0; f < NF; ++f )
+        acc[f] = 0.;
+    for ( int32_t q = b; q < e; ++q )
     {
-        double acc = 0.;
-        for ( int32_t p = b; p < e; ++p )
-        {
-            int32_t const en = ent[p];
-            acc += coeffs[en] * ybar[f * ld_tgt + en / kk];
-        }
-        xbar[f * ld_src + s] = acc;
+        int32_t const l = ent[q];
+        double const c = coeffs[l];
+        Pack<double, NF> const v = ys[l / kk];
+#pragma unroll
+        for ( int f = 0; f < NF; ++f )
+            acc[f] += c * v.v[f];
     }
+    int64_t const s = perm ? perm[p] : p;
+#pragma unroll
+    for ( int f = 0; f < NF; ++f )
+        if ( f < nf )
+            xbar[f * ld_src + s] = acc[f];
+}
+
+template <int NF>
+void transpose_pass( RowLayout const &g, CscIndex const &csc,
+                     double const *coeffs, double const *ybar, int64_t ld_tgt,
+                     double *xbar, int64_t ld_src, int nf, double *scratch,
+                     hipStream_t s )
+{
+    auto *ys = reinterpret_cast<Pack<double, NF> *>( scratch );
+    gather_fields_kernel<double, double, NF>
+        <<<ceil_div( g.n_tgt, kBlock ), kBlock, 0, s>>>(
+            ybar, ld_tgt, nf, g.order, g.n_tgt, ys );
+    transpose_kernel<NF><<<ceil_div( g.n_src, kBlock ), kBlock, 0, s>>>(
+        csc.col_offsets.get(), csc.entries.get(), coeffs, g.kk, g.n_src, ys,
+        g.perm, xbar, ld_src, nf );
+    DTKX_HIP( hipGetLastError() );
+}
+
+template <typename C>
+__global__ void __launch_bounds__( kBlock )
+    canonical_graph_kernel( int32_t const *__restrict__ idx,
+                            int32_t const *__restrict__ trow,
+                            int32_t const *__restrict__ perm,
+                            C const *__restrict__ coeffs, int64_t nnz, int kk,
+                            int32_t *__restrict__ idx_out,
+                            C *__restrict__ coeffs_out )
+{
+    int64_t const e = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( e >= nnz )
+        return;
+    int64_t const t = e / kk, j = e - t * kk;
+    int64_t const l = ( trow ? trow[t] : t ) * kk + j;
+    int32_t const p = idx[l];
+    idx_out[e] = perm ? perm[p] : p;
+    coeffs_out[e] = coeffs[l];
 }
 
 } // namespace
 
-void crs_apply_f64( int32_t const *idx, double const *coeffs, int64_t n_tgt,
-                    int kk, double const *x, int64_t ld_src, double *y,
-                    int64_t ld_tgt, int n_fields, int32_t const *order,
+void crs_apply_f64( RowLayout const &g, double const *coeffs, double const *x,
+                    int64_t ld_src, double *y, int64_t ld_tgt, int n_fields,
                     hipStream_t s )
 {
-    launch_apply<double, double, double>( idx, coeffs, n_tgt, kk, x, ld_src,
-                                          y, ld_tgt, n_fields, order, s );
+    launch_apply<double, double, double, double>( g, coeffs, x, ld_src, y,
+                                                  ld_tgt, n_fields, s );
 }
 
-void crs_apply_f32( int32_t const *idx, float const *coeffs, int64_t n_tgt,
-                    int kk, float const *x, int64_t ld_src, float *y,
-                    int64_t ld_tgt, int n_fields, int32_t const *order,
+void crs_apply_f32( RowLayout const &g, float const *coeffs, float const *x,
+                    int64_t ld_src, float *y, int64_t ld_tgt, int n_fields,
                     hipStream_t s )
 {
-    launch_apply<float, float, float>( idx, coeffs, n_tgt, kk, x, ld_src, y,
-                                       ld_tgt, n_fields, order, s );
+    launch_apply<float, float, float, float>( g, coeffs, x, ld_src, y, ld_tgt,
+                                              n_fields, s );
 }
 
-void crs_apply_f32_dvals( int32_t const *idx, float const *coeffs,
-                          int64_t n_tgt, int kk, double const *x,
-                          int64_t ld_src, double *y, int64_t ld_tgt,
-                          int n_fields, int32_t const *order, hipStream_t s )
+void crs_apply_f32_dvals( RowLayout const &g, float const *coeffs,
+                          double const *x, int64_t ld_src, double *y,
+                          int64_t ld_tgt, int n_fields, hipStream_t s )
 {
-    // values are rounded to float on load, products and sums in float
-    launch_apply<float, double, float>( idx, coeffs, n_tgt, kk, x, ld_src, y,
-                                        ld_tgt, n_fields, order, s );
+    // values are rounded to float by the gather, products and sums in float
+    launch_apply<float, double, float, float>( g, coeffs, x, ld_src, y,
+                                               ld_tgt, n_fields, s );
 }
 
-void build_csc( int32_t const *idx, int64_t nnz, int64_t n_src,
-                CscIndex &csc, hipStream_t s )
+void build_csc( RowLayout const &g, CscIndex &csc, hipStream_t s )
 {
+    int64_t const nnz = g.n_tgt * g.kk;
     DTKX_REQUIRE( nnz < INT32_MAX, DTKX_ERR_UNSUPPORTED );
-    csc.col_offsets.resize( n_src + 1, s );
+    csc.col_offsets.resize( g.n_src + 1, s );
     csc.entries.resize( std::max<int64_t>( nnz, 1 ), s );
     if ( nnz == 0 )
     {
         DTKX_HIP( hipMemsetAsync( csc.col_offsets.get(), 0,
-                                  sizeof( int32_t ) * ( n_src + 1 ), s ) );
+                                  sizeof( int32_t ) * ( g.n_src + 1 ), s ) );
         return;
     }
-    DeviceBuffer<int32_t> ids( nnz, s ), keys_sorted( nnz, s );
-    iota_kernel<<<ceil_div( nnz, kBlock ), kBlock, 0, s>>>( ids.get(), nnz );
+    DeviceBuffer<int32_t> keys( nnz, s ), slots( nnz, s ),
+        keys_sorted( nnz, s );
+    canonical_keys_kernel<<<ceil_div( nnz, kBlock ), kBlock, 0, s>>>(
+        g.idx, g.trow, nnz, g.kk, keys.get(), slots.get() );
+    DTKX_HIP( hipGetLastError() );
     int end_bit = 1;
-    while ( ( int64_t( 1 ) << end_bit ) < n_src )
+    while ( ( int64_t( 1 ) << end_bit ) < g.n_src )
         ++end_bit;
+    // the radix sort is stable: each column keeps the canonical order
     std::size_t temp_bytes = 0;
     DTKX_HIP( hipcub::DeviceRadixSort::SortPairs(
-        nullptr, temp_bytes, idx, keys_sorted.get(), ids.get(),
+        nullptr, temp_bytes, keys.get(), keys_sorted.get(), slots.get(),
         csc.entries.get(), static_cast<int>( nnz ), 0, end_bit, s ) );
     DeviceBuffer<char> temp( temp_bytes, s );
     DTKX_HIP( hipcub::DeviceRadixSort::SortPairs(
-        temp.get(), temp_bytes, idx, keys_sorted.get(), ids.get(),
+        temp.get(), temp_bytes, keys.get(), keys_sorted.get(), slots.get(),
         csc.entries.get(), static_cast<int>( nnz ), 0, end_bit, s ) );
     col_offsets_kernel<<<ceil_div( nnz + 1, kBlock ), kBlock, 0, s>>>(
-        keys_sorted.get(), nnz, n_src, csc.col_offsets.get() );
+        keys_sorted.get(), nnz, g.n_src, csc.col_offsets.get() );
     DTKX_HIP( hipGetLastError() );
 }
 
-void crs_apply_transpose_f64( CscIndex const &csc, double const *coeffs,
-                              int kk, int64_t n_src, double const *ybar,
+void crs_apply_transpose_f64( RowLayout const &g, CscIndex const &csc,
+                              double const *coeffs, double const *ybar,
                               int64_t ld_tgt, double *xbar, int64_t ld_src,
                               int n_fields, hipStream_t s )
 {
-    if ( n_src == 0 || n_fields == 0 )
+    if ( g.n_src == 0 || n_fields == 0 )
         return;
-    transpose_kernel<<<ceil_div( n_src, kBlock ), kBlock, 0, s>>>(
-        csc.col_offsets.get(), csc.entries.get(), coeffs, kk, n_src, ybar,
-        ld_tgt, xbar, ld_src, n_fields );
+    DeviceBuffer<double> scratch(
+        (std::size_t)std::max<int64_t>( g.n_tgt, 1 ) *
+            pass_width( n_fields ),
+        s );
+    for ( int f0 = 0; f0 < n_fields; )
+    {
+        int const w = pass_width( n_fields - f0 );
+        int const nf = std::min( w, n_fields - f0 );
+        double const *yf = ybar + f0 * ld_tgt;
+        double *xf = xbar + f0 * ld_src;
+        switch ( w )
+        {
+        case 1:
+            transpose_pass<1>( g, csc, coeffs, yf, ld_tgt, xf, ld_src, nf,
+                               scratch.get(), s );
+            break;
+        case 2:
+            transpose_pass<2>( g, csc, coeffs, yf, ld_tgt, xf, ld_src, nf,
+                               scratch.get(), s );
+            break;
+        case 4:
+            transpose_pass<4>( g, csc, coeffs, yf, ld_tgt, xf, ld_src, nf,
+                               scratch.get(), s );
+            break;
+        default:
+            transpose_pass<8>( g, csc, coeffs, yf, ld_tgt, xf, ld_src, nf,
+                               scratch.get(), s );
+        }
+        f0 += nf;
+    }
+}
+
+template <typename C>
+void canonical_graph( RowLayout const &g, C const *coeffs, int32_t *idx_out,
+                      C *coeffs_out, hipStream_t s )
+{
+    int64_t const nnz = g.n_tgt * g.kk;
+    if ( nnz == 0 )
+        return;
+    canonical_graph_kernel<C><<<ceil_div( nnz, kBlock ), kBlock, 0, s>>>(
+        g.idx, g.trow, g.perm, coeffs, nnz, g.kk, idx_out, coeffs_out );
     DTKX_HIP( hipGetLastError() );
 }
+
+template void canonical_graph<double>( RowLayout const &, double const *,
+                                       int32_t *, double *, hipStream_t );
+template void canonical_graph<float>( RowLayout const &, float const *,
+                                      int32_t *, float *, hipStream_t );
 
 } // namespace dtkx

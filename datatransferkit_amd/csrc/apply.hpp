@@ -6,8 +6,19 @@
 //            in-kernel gather that replaces the reference's fetch()
 //            (DTK_DetailsNearestNeighborOperatorImpl.hpp:152-192) on one rank.
 // transpose: xbar[f][s] = sum over entries e with idx(e) = s, in ascending
-//            entry order, of c(e) ybar[f][row(e)]; gathered through a
-//            column index (CSC) built once, so no atomics and a fixed order.
+//            canonical entry order (target, then neighbour rank), of
+//            c(e) ybar[f][row(e)]; gathered through a column index (CSC)
+//            built once, so no atomics and a fixed order.
+//
+// Row layout. Rows are stored in the spatial (Morton) order of the targets
+// and column indices name source *positions* in the Morton order of the
+// sources: row r belongs to target order[r], position p is source perm[p].
+// Neighbouring rows then gather from neighbouring positions, so one pass
+// over the rows reads the values nearly sequentially. An apply first
+// gathers the caller's fields into a position-ordered, field-interleaved
+// copy (one random read per source and field instead of one per entry),
+// runs the row sums over it and scatters each row's results to its target.
+// Null order / perm mean the identity (rows and columns in caller order).
 #ifndef DTKX_APPLY_HPP
 #define DTKX_APPLY_HPP
 
@@ -16,38 +27,47 @@
 namespace dtkx
 {
 
-// order (nullable): the rows are processed in this order (the targets'
-// Morton order from the kNN), which only changes which rows run together.
-void crs_apply_f64( int32_t const *idx, double const *coeffs, int64_t n_tgt,
-                    int kk, double const *x, int64_t ld_src, double *y,
-                    int64_t ld_tgt, int n_fields, int32_t const *order,
+struct RowLayout
+{
+    int32_t const *idx = nullptr;   // [n_tgt * kk] source positions
+    int32_t const *order = nullptr; // row -> target (nullable)
+    int32_t const *perm = nullptr;  // position -> source (nullable)
+    int32_t const *trow = nullptr;  // target -> row (nullable, inverse of order)
+    int64_t n_src = 0, n_tgt = 0;
+    int kk = 0;
+};
+
+void crs_apply_f64( RowLayout const &g, double const *coeffs, double const *x,
+                    int64_t ld_src, double *y, int64_t ld_tgt, int n_fields,
                     hipStream_t s );
 
-void crs_apply_f32( int32_t const *idx, float const *coeffs, int64_t n_tgt,
-                    int kk, float const *x, int64_t ld_src, float *y,
-                    int64_t ld_tgt, int n_fields, int32_t const *order,
+void crs_apply_f32( RowLayout const &g, float const *coeffs, float const *x,
+                    int64_t ld_src, float *y, int64_t ld_tgt, int n_fields,
                     hipStream_t s );
 
 // FP32 coefficients applied to double values at the boundary: values are
 // rounded to float on load, accumulated in float, widened on store.
-void crs_apply_f32_dvals( int32_t const *idx, float const *coeffs,
-                          int64_t n_tgt, int kk, double const *x,
-                          int64_t ld_src, double *y, int64_t ld_tgt,
-                          int n_fields, int32_t const *order, hipStream_t s );
+void crs_apply_f32_dvals( RowLayout const &g, float const *coeffs,
+                          double const *x, int64_t ld_src, double *y,
+                          int64_t ld_tgt, int n_fields, hipStream_t s );
 
 struct CscIndex
 {
-    DeviceBuffer<int32_t> col_offsets; // [n_src + 1]
-    DeviceBuffer<int32_t> entries;     // [nnz], entry ids sorted by column
+    DeviceBuffer<int32_t> col_offsets; // [n_src + 1], by source position
+    DeviceBuffer<int32_t> entries;     // [nnz] storage slots, canonical order
 };
 
-void build_csc( int32_t const *idx, int64_t nnz, int64_t n_src,
-                CscIndex &csc, hipStream_t s );
+void build_csc( RowLayout const &g, CscIndex &csc, hipStream_t s );
 
-void crs_apply_transpose_f64( CscIndex const &csc, double const *coeffs,
-                              int kk, int64_t n_src, double const *ybar,
+void crs_apply_transpose_f64( RowLayout const &g, CscIndex const &csc,
+                              double const *coeffs, double const *ybar,
                               int64_t ld_tgt, double *xbar, int64_t ld_src,
                               int n_fields, hipStream_t s );
+
+// The graph in caller order: row t = target t, indices = source indices.
+template <typename C>
+void canonical_graph( RowLayout const &g, C const *coeffs, int32_t *idx_out,
+                      C *coeffs_out, hipStream_t s );
 
 } // namespace dtkx
 

@@ -20,13 +20,35 @@ struct dtkx_mls
     int64_t n_src = 0, n_tgt = 0;
     int m = 0, kk = 0;
     hipStream_t stream = nullptr;
-    dtkx::DeviceBuffer<int32_t> offsets, indices, order;
+    // CRS rows in the Morton order of the targets (row r: target order[r],
+    // trow the inverse), columns as Morton positions of the sources
+    // (position p: source perm[p]); empty buffers mean caller order
+    dtkx::DeviceBuffer<int32_t> offsets, indices, order, trow, perm;
     dtkx::DeviceBuffer<double> coeffs, a0;
     dtkx::DeviceBuffer<float> coeffs32;
+    // src_d in position order, tgt_d in caller order
     dtkx::DeviceBuffer<double4> src_d, tgt_d;
     mutable dtkx::CscIndex csc;
     mutable bool csc_built = false;
+    // the graph in caller order, built on the first dtkx_mls_get_graph
+    mutable dtkx::DeviceBuffer<int32_t> graph_idx;
+    mutable dtkx::DeviceBuffer<double> graph_c64;
+    mutable dtkx::DeviceBuffer<float> graph_c32;
+    mutable bool graph_built = false;
     dtkx_mls_stats stats{};
+
+    dtkx::RowLayout layout() const
+    {
+        dtkx::RowLayout g;
+        g.idx = indices.get();
+        g.order = order.get();
+        g.perm = perm.get();
+        g.trow = trow.get();
+        g.n_src = n_src;
+        g.n_tgt = n_tgt;
+        g.kk = kk;
+        return g;
+    }
 };
 
 namespace
@@ -163,31 +185,38 @@ __global__ void to_f32_kernel( double const *a, float *b, int64_t n )
         b[i] = (float)a[i];
 }
 
-struct EventTimer
+// Events around the create stages, read after the create's final
+// synchronisation so timing adds no host-device round trip.
+struct StageEvents
 {
-    hipEvent_t a = nullptr, b = nullptr;
+    hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+    bool marked[3] = {false, false, false};
     hipStream_t s;
-    explicit EventTimer( hipStream_t st )
+    explicit StageEvents( hipStream_t st )
         : s( st )
     {
-        DTKX_HIP( hipEventCreate( &a ) );
-        DTKX_HIP( hipEventCreate( &b ) );
-        DTKX_HIP( hipEventRecord( a, s ) );
+        for ( auto &x : e )
+            DTKX_HIP( hipEventCreate( &x ) );
     }
-    double stop_ms()
+    void mark( int i )
     {
-        DTKX_HIP( hipEventRecord( b, s ) );
-        DTKX_HIP( hipEventSynchronize( b ) );
-        float ms = 0;
-        DTKX_HIP( hipEventElapsedTime( &ms, a, b ) );
-        return ms;
+        DTKX_HIP( hipEventRecord( e[i], s ) );
+        marked[i] = true;
     }
-    ~EventTimer()
+    double ms( int i, int j )
     {
-        if ( a )
-            (void)hipEventDestroy( a );
-        if ( b )
-            (void)hipEventDestroy( b );
+        if ( !marked[i] || !marked[j] )
+            return 0.0;
+        DTKX_HIP( hipEventSynchronize( e[j] ) );
+        float t = 0;
+        DTKX_HIP( hipEventElapsedTime( &t, e[i], e[j] ) );
+        return t;
+    }
+    ~StageEvents()
+    {
+        for ( auto x : e )
+            if ( x )
+                (void)hipEventDestroy( x );
     }
 };
 
@@ -306,6 +335,14 @@ const char *dtkx_last_error( void ) { return g_last_error.c_str(); }
 
 const char *dtkx_version( void ) { return "dtkx 0.1 (gfx950)"; }
 
+int dtkx_release_cached_memory( void )
+{
+    return guarded( [&]() {
+        dtkx::cache_release();
+        return DTKX_SUCCESS;
+    } );
+}
+
 int dtkx_mls_options_init( dtkx_mls_options *o )
 {
     if ( !o )
@@ -336,17 +373,21 @@ int dtkx_mls_create( const double *src_xyz, int64_t n_src,
         REQ( out != nullptr, DTKX_ERR_INVALID_ARGUMENT );
         *out = nullptr;
         auto h = make_handle( src_xyz, n_src, tgt_xyz, n_tgt, opts );
+        StageEvents ev( h->stream );
         dtkx::DeviceBuffer<unsigned long long> counters( 3, h->stream );
         DTKX_HIP( hipMemsetAsync( counters.get(), 0, 3 * sizeof( uint64_t ),
                                   h->stream ) );
         if ( n_tgt > 0 )
         {
-            EventTimer tk( h->stream );
+            ev.mark( 0 );
             dtkx::KnnInfo info;
             int const k = opts->k > 0 ? opts->k : h->m;
             dtkx::KnnOutput kout;
+            h->perm.resize( n_src, h->stream );
             kout.indices = h->indices.get();
             kout.query_order = h->order.get();
+            kout.src_perm = h->perm.get();
+            kout.morton_layout = true;
             if ( opts->geometry == DTKX_GEOMETRY_FLOAT )
             {
                 // float points rounded from the caller's doubles
@@ -363,13 +404,24 @@ int dtkx_mls_create( const double *src_xyz, int64_t n_src,
                 dtkx::knn_search( opts->geometry, h->src_d.get(), n_src,
                                   h->tgt_d.get(), n_tgt, k, kout, &info,
                                   h->stream );
-            h->stats.ms_knn = tk.stop_ms();
+            // sources in position order for the setup's gathers
+            {
+                dtkx::DeviceBuffer<double4> sorted( n_src, h->stream );
+                dtkx::gather_points( h->src_d.get(), h->perm.get(), n_src,
+                                     sorted.get(), h->stream );
+                h->src_d = std::move( sorted );
+            }
+            h->trow.resize( n_tgt, h->stream );
+            dtkx::invert_permutation( h->order.get(), n_tgt, h->trow.get(),
+                                      h->stream );
+            ev.mark( 1 );
             h->stats.tree_height = info.tree_height;
-            EventTimer ts( h->stream );
             run_setup( h.get(), counters.get() );
-            h->stats.ms_setup = ts.stop_ms();
+            ev.mark( 2 );
         }
         finish_create( h.get(), counters.get() );
+        h->stats.ms_knn = ev.ms( 0, 1 );
+        h->stats.ms_setup = ev.ms( 1, 2 );
         *out = h.release();
         return DTKX_SUCCESS;
     } );
@@ -409,16 +461,18 @@ int dtkx_mls_create_with_neighbors( const double *src_xyz, int64_t n_src,
                                       hipMemcpyHostToDevice, h->stream ) );
             DTKX_HIP( hipStreamSynchronize( h->stream ) );
         }
+        StageEvents ev( h->stream );
         dtkx::DeviceBuffer<unsigned long long> counters( 3, h->stream );
         DTKX_HIP( hipMemsetAsync( counters.get(), 0, 3 * sizeof( uint64_t ),
                                   h->stream ) );
         if ( n_tgt > 0 )
         {
-            EventTimer ts( h->stream );
+            ev.mark( 1 );
             run_setup( h.get(), counters.get() );
-            h->stats.ms_setup = ts.stop_ms();
+            ev.mark( 2 );
         }
         finish_create( h.get(), counters.get() );
+        h->stats.ms_setup = ev.ms( 1, 2 );
         *out = h.release();
         return DTKX_SUCCESS;
     } );
@@ -442,14 +496,12 @@ int dtkx_mls_apply( const dtkx_mls *h, const double *src_vals,
         Staged<double> x( src_vals, nin, h->stream );
         StagedOut<double> y( tgt_vals, nout, h->stream );
         if ( h->opts.precision == DTKX_PRECISION_FP32_BULK )
-            dtkx::crs_apply_f32_dvals( h->indices.get(), h->coeffs32.get(),
-                                       h->n_tgt, h->kk, x.ptr, ld_src, y.ptr,
-                                       ld_tgt, n_fields, nullptr,
+            dtkx::crs_apply_f32_dvals( h->layout(), h->coeffs32.get(), x.ptr,
+                                       ld_src, y.ptr, ld_tgt, n_fields,
                                        h->stream );
         else
-            dtkx::crs_apply_f64( h->indices.get(), h->coeffs.get(), h->n_tgt,
-                                 h->kk, x.ptr, ld_src, y.ptr, ld_tgt, n_fields,
-                                 nullptr, h->stream );
+            dtkx::crs_apply_f64( h->layout(), h->coeffs.get(), x.ptr, ld_src,
+                                 y.ptr, ld_tgt, n_fields, h->stream );
         y.finish( h->stream );
         DTKX_HIP( hipStreamSynchronize( h->stream ) );
         return DTKX_SUCCESS;
@@ -475,9 +527,8 @@ int dtkx_mls_apply_fp32( const dtkx_mls *h, const float *src_vals,
         std::size_t const nout = (std::size_t)( n_fields - 1 ) * ld_tgt + h->n_tgt;
         Staged<float> x( src_vals, nin, h->stream );
         StagedOut<float> y( tgt_vals, nout, h->stream );
-        dtkx::crs_apply_f32( h->indices.get(), h->coeffs32.get(), h->n_tgt,
-                             h->kk, x.ptr, ld_src, y.ptr, ld_tgt, n_fields,
-                             nullptr, h->stream );
+        dtkx::crs_apply_f32( h->layout(), h->coeffs32.get(), x.ptr, ld_src,
+                             y.ptr, ld_tgt, n_fields, h->stream );
         y.finish( h->stream );
         DTKX_HIP( hipStreamSynchronize( h->stream ) );
         return DTKX_SUCCESS;
@@ -498,17 +549,16 @@ int dtkx_mls_apply_transpose( const dtkx_mls *h, const double *tgt_bar,
         REQ( h->opts.precision == DTKX_PRECISION_FP64, DTKX_ERR_UNSUPPORTED );
         if ( !h->csc_built )
         {
-            dtkx::build_csc( h->indices.get(), h->n_tgt * h->kk, h->n_src,
-                             h->csc, h->stream );
+            dtkx::build_csc( h->layout(), h->csc, h->stream );
             h->csc_built = true;
         }
         std::size_t const nin = (std::size_t)( n_fields - 1 ) * ld_tgt + h->n_tgt;
         std::size_t const nout = (std::size_t)( n_fields - 1 ) * ld_src + h->n_src;
         Staged<double> yb( tgt_bar, nin, h->stream );
         StagedOut<double> xb( src_bar, nout, h->stream );
-        dtkx::crs_apply_transpose_f64( h->csc, h->coeffs.get(), h->kk,
-                                       h->n_src, yb.ptr, ld_tgt, xb.ptr,
-                                       ld_src, n_fields, h->stream );
+        dtkx::crs_apply_transpose_f64( h->layout(), h->csc, h->coeffs.get(),
+                                       yb.ptr, ld_tgt, xb.ptr, ld_src,
+                                       n_fields, h->stream );
         xb.finish( h->stream );
         DTKX_HIP( hipStreamSynchronize( h->stream ) );
         return DTKX_SUCCESS;
@@ -538,16 +588,43 @@ int dtkx_mls_get_graph( const dtkx_mls *h, const int32_t **offsets,
 {
     return guarded( [&]() {
         REQ( h != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        bool const f32 = h->opts.precision == DTKX_PRECISION_FP32_BULK;
+        bool const permuted = h->order.get() || h->perm.get();
+        int64_t const n_entries = h->n_tgt * h->kk;
+        if ( permuted && !h->graph_built )
+        {
+            std::size_t const len = std::max<int64_t>( n_entries, 1 );
+            h->graph_idx.resize( len, h->stream );
+            if ( f32 )
+            {
+                h->graph_c32.resize( len, h->stream );
+                dtkx::canonical_graph( h->layout(), h->coeffs32.get(),
+                                       h->graph_idx.get(), h->graph_c32.get(),
+                                       h->stream );
+            }
+            else
+            {
+                h->graph_c64.resize( len, h->stream );
+                dtkx::canonical_graph( h->layout(), h->coeffs.get(),
+                                       h->graph_idx.get(), h->graph_c64.get(),
+                                       h->stream );
+            }
+            DTKX_HIP( hipStreamSynchronize( h->stream ) );
+            h->graph_built = true;
+        }
         if ( offsets )
             *offsets = h->offsets.get();
         if ( indices )
-            *indices = h->indices.get();
+            *indices = permuted ? h->graph_idx.get() : h->indices.get();
         if ( coeffs )
-            *coeffs = h->opts.precision == DTKX_PRECISION_FP32_BULK
-                          ? static_cast<void const *>( h->coeffs32.get() )
-                          : static_cast<void const *>( h->coeffs.get() );
+        {
+            if ( f32 )
+                *coeffs = permuted ? h->graph_c32.get() : h->coeffs32.get();
+            else
+                *coeffs = permuted ? h->graph_c64.get() : h->coeffs.get();
+        }
         if ( nnz )
-            *nnz = h->n_tgt * h->kk;
+            *nnz = n_entries;
         return DTKX_SUCCESS;
     } );
 }
@@ -587,7 +664,7 @@ int dtkx_knn( const double *src_xyz, int64_t n_src, const double *tgt_xyz,
              DTKX_ERR_INVALID_ARGUMENT );
         if ( n_tgt == 0 )
             return DTKX_SUCCESS;
-        int const kk = k < n_src ? k : static_cast<int>( n_src );
+            int const kk = k < n_src ? k : static_cast<int>( n_src );
         Staged<double> sx( src_xyz, 3 * n_src, s );
         Staged<double> tx( tgt_xyz, 3 * n_tgt, s );
         StagedOut<int32_t> oi( indices, (std::size_t)n_tgt * kk, s );

@@ -47,17 +47,28 @@ struct Error : std::runtime_error
                            __LINE__ );                                         \
     } while ( 0 )
 
-// Owning device buffer, move-only. With a stream it uses the stream-ordered
-// allocator (hipMallocAsync / hipFreeAsync on that stream), which neither
-// synchronises the device nor pays hipMalloc's cost on every call; without
-// one it falls back to hipMalloc / hipFree.
+// Device block cache (memory.hip). Blocks freed on a stream return to that
+// stream's free list and are handed out again only to allocations on the
+// same stream, whose later work is ordered after every earlier use, so no
+// event or synchronisation is needed; a miss allocates with hipMalloc. The
+// cache keeps blocks until dtkx_release_cached_memory() (or process end):
+// building an operator then costs no allocator calls once warm, where the
+// runtime's own stream-ordered allocator spends milliseconds per operator
+// in hipFreeAsync at a million targets.
+void *cache_alloc( std::size_t bytes, hipStream_t s );
+void cache_free( void *p, std::size_t bytes, hipStream_t s );
+// Returns the cached blocks of the current device to the runtime.
+void cache_release();
+
+// Owning device buffer, move-only. With a stream it draws from the block
+// cache of that stream; without one it uses hipMalloc / hipFree.
 template <typename T>
 struct DeviceBuffer
 {
     T *ptr = nullptr;
     std::size_t n = 0;
     hipStream_t stream = nullptr;
-    bool async = false;
+    bool cached = false;
     DeviceBuffer() = default;
     explicit DeviceBuffer( std::size_t count ) { resize( count ); }
     DeviceBuffer( std::size_t count, hipStream_t s ) { resize( count, s ); }
@@ -88,29 +99,26 @@ struct DeviceBuffer
             }
         }
         n = count;
-        async = false;
+        cached = false;
     }
     void resize( std::size_t count, hipStream_t s )
     {
         release();
         if ( count > 0 )
-        {
-            hipError_t e = hipMallocAsync( (void **)&ptr, count * sizeof( T ), s );
-            if ( e != hipSuccess )
-            {
-                ptr = nullptr;
-                ::dtkx::raise( DTKX_ERR_OUT_OF_MEMORY, "hipMallocAsync",
-                               __FILE__, __LINE__ );
-            }
-        }
+            ptr = static_cast<T *>( cache_alloc( count * sizeof( T ), s ) );
         n = count;
         stream = s;
-        async = true;
+        cached = true;
     }
     void release()
     {
         if ( ptr )
-            (void)( async ? hipFreeAsync( ptr, stream ) : hipFree( ptr ) );
+        {
+            if ( cached )
+                cache_free( ptr, n * sizeof( T ), stream );
+            else
+                (void)hipFree( ptr );
+        }
         ptr = nullptr;
         n = 0;
     }
@@ -122,7 +130,7 @@ struct DeviceBuffer
         ptr = o.ptr;
         n = o.n;
         stream = o.stream;
-        async = o.async;
+        cached = o.cached;
         o.ptr = nullptr;
         o.n = 0;
     }
@@ -140,6 +148,14 @@ void points_to_double4( double const *xyz, int64_t n, int layout,
 
 void double4_to_float4( double4 const *in, int64_t n, float4 *out,
                         hipStream_t s );
+
+// out[i] = in[perm[i]]
+void gather_points( double4 const *in, int32_t const *perm, int64_t n,
+                    double4 *out, hipStream_t s );
+
+// inv[perm[i]] = i
+void invert_permutation( int32_t const *perm, int64_t n, int32_t *inv,
+                         hipStream_t s );
 
 inline unsigned ceil_div( int64_t a, int64_t b )
 {

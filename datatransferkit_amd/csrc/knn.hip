@@ -473,7 +473,7 @@ __global__ void __launch_bounds__( kQueryBlock )
     knn_query_kernel( Node<T> const *nodes, typename Vec4<T>::type const *src,
                       typename Vec4<T>::type const *tgt, int32_t const *order, int64_t n_tgt, int64_t n_src,
                       int kk, int32_t *out_idx, double *out_dist,
-                      int2 *gstack, int depth )
+                      int32_t const *src_pos, int2 *gstack, int depth )
 {
     constexpr int LS = STACK > 0 ? STACK : 1;
     __shared__ int32_t st_node[LS * kQueryBlock];
@@ -640,15 +640,6 @@ struct QueryLauncher
 };
 
 template <typename T>
-__global__ void __launch_bounds__( kBlock )
-    inverse_perm_kernel( int32_t const *perm, int64_t n, int32_t *pos )
-{
-    int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-    if ( i < n )
-        pos[perm[i]] = (int32_t)i;
-}
-
-template <typename T>
 void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
                int64_t n_tgt, int k, KnnOutput const &out, KnnInfo *info,
                hipStream_t s )
@@ -696,9 +687,7 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     }
     if ( src_pos )
     {
-        inverse_perm_kernel<<<ceil_div( n_src, kBlock ), kBlock, 0, s>>>(
-            ids_sorted, n_src, src_pos );
-        DTKX_HIP( hipGetLastError() );
+        invert_permutation( ids_sorted, n_src, src_pos, s );
     }
 
     // hierarchy + refit

@@ -1,0 +1,135 @@
+// Device block cache behind DeviceBuffer (see common.hpp).
+//
+// Free lists per (device, stream). A block freed on stream s is reused only
+// by a later allocation on s: work on s that uses the new owner is queued
+// after all work that used the old owner, so the reuse needs no event.
+// Sizes are rounded (powers of two up to 1 MiB, 2 MiB multiples above) so
+// the same sequence of allocations, as when operators of the same shape are
+// built again, finds exact fits.
+#include "common.hpp"
+
+#include <map>
+#include <mutex>
+#include <unordered_map>
+#include <utility>
+
+namespace dtkx
+{
+namespace
+{
+
+struct BlockCache
+{
+    std::mutex mu;
+    std::map<std::pair<int, hipStream_t>, std::multimap<std::size_t, void *>>
+        free_lists;
+    std::unordered_map<void *, std::pair<int, std::size_t>> live; // size, device
+};
+
+// Never destroyed: blocks must not be freed after the HIP runtime has shut
+// down at process exit.
+BlockCache &cache()
+{
+    static BlockCache *c = new BlockCache;
+    return *c;
+}
+
+std::size_t round_size( std::size_t b )
+{
+    constexpr std::size_t kMiB = std::size_t( 1 ) << 20;
+    if ( b > kMiB )
+        return ( b + 2 * kMiB - 1 ) / ( 2 * kMiB ) * ( 2 * kMiB );
+    std::size_t r = 512;
+    while ( r < b )
+        r <<= 1;
+    return r;
+}
+
+// Frees every cached block of `dev` (caller holds the lock); pending work
+// may still use them, so the device is synchronised first.
+void release_locked( BlockCache &c, int dev )
+{
+    bool synced = false;
+    for ( auto it = c.free_lists.begin(); it != c.free_lists.end(); )
+    {
+        if ( it->first.first != dev )
+        {
+            ++it;
+            continue;
+        }
+        if ( !synced )
+        {
+            (void)hipDeviceSynchronize();
+            synced = true;
+        }
+        for ( auto &b : it->second )
+            (void)hipFree( b.second );
+        it = c.free_lists.erase( it );
+    }
+}
+
+} // namespace
+
+void *cache_alloc( std::size_t bytes, hipStream_t s )
+{
+    std::size_t const r = round_size( bytes );
+    int dev = 0;
+    DTKX_HIP( hipGetDevice( &dev ) );
+    BlockCache &c = cache();
+    std::lock_guard<std::mutex> lock( c.mu );
+    auto fl = c.free_lists.find( {dev, s} );
+    if ( fl != c.free_lists.end() )
+    {
+        auto b = fl->second.lower_bound( r );
+        // accept up to 1/8 slack
+        if ( b != fl->second.end() && b->first <= r + r / 8 )
+        {
+            void *p = b->second;
+            c.live[p] = {dev, b->first};
+            fl->second.erase( b );
+            return p;
+        }
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc( &p, r );
+    if ( e == hipErrorOutOfMemory )
+    {
+        (void)hipGetLastError();
+        release_locked( c, dev );
+        e = hipMalloc( &p, r );
+    }
+    if ( e != hipSuccess || !p )
+    {
+        (void)hipGetLastError();
+        raise( DTKX_ERR_OUT_OF_MEMORY, "device allocation", __FILE__,
+               __LINE__ );
+    }
+    c.live[p] = {dev, r};
+    return p;
+}
+
+void cache_free( void *p, std::size_t bytes, hipStream_t s )
+{
+    (void)bytes;
+    if ( !p )
+        return;
+    BlockCache &c = cache();
+    std::lock_guard<std::mutex> lock( c.mu );
+    auto it = c.live.find( p );
+    if ( it == c.live.end() )
+        return;
+    auto const [dev, size] = it->second;
+    c.live.erase( it );
+    c.free_lists[{dev, s}].emplace( size, p );
+}
+
+void cache_release()
+{
+    int dev = 0;
+    DTKX_HIP( hipGetDevice( &dev ) );
+    BlockCache &c = cache();
+    std::lock_guard<std::mutex> lock( c.mu );
+    release_locked( c, dev );
+}
+
+} // namespace dtkx

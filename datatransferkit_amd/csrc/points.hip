@@ -79,7 +79,43 @@ __global__ void __launch_bounds__( kBlock )
     out[i] = float4{(float)v.x, (float)v.y, (float)v.z, 0.f};
 }
 
+__global__ void __launch_bounds__( kBlock )
+    gather_d4_kernel( double4 const *in, int32_t const *perm, int64_t n,
+                      double4 *out )
+{
+    int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( i < n )
+        out[i] = in[perm[i]];
+}
+
+__global__ void __launch_bounds__( kBlock )
+    invert_kernel( int32_t const *perm, int64_t n, int32_t *inv )
+{
+    int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( i < n )
+        inv[perm[i]] = (int32_t)i;
+}
+
 } // namespace
+
+void gather_points( double4 const *in, int32_t const *perm, int64_t n,
+                    double4 *out, hipStream_t s )
+{
+    if ( n == 0 )
+        return;
+    gather_d4_kernel<<<ceil_div( n, kBlock ), kBlock, 0, s>>>( in, perm, n,
+                                                               out );
+    DTKX_HIP( hipGetLastError() );
+}
+
+void invert_permutation( int32_t const *perm, int64_t n, int32_t *inv,
+                         hipStream_t s )
+{
+    if ( n == 0 )
+        return;
+    invert_kernel<<<ceil_div( n, kBlock ), kBlock, 0, s>>>( perm, n, inv );
+    DTKX_HIP( hipGetLastError() );
+}
 
 void double4_to_float4( double4 const *in, int64_t n, float4 *out,
                         hipStream_t s )

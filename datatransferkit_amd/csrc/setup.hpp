@@ -27,8 +27,10 @@ struct SetupParams
 
 // counters[0] = n_underdetermined, [1] = n_unconverged, [2] = rotations
 // (classical) or sweeps (cyclic); accumulated atomically.
-// order (nullable): targets in processing order (a permutation).
-// a0 (nullable): receives row 0 of the pseudo-inverse per target [n][m].
+// Row r of indices, coeffs and a0 belongs to target order[r] (identity
+// when order is null); indices address src, which may be the sources in a
+// permuted (Morton) order.
+// a0 (nullable): receives row 0 of the pseudo-inverse per row [n][m].
 
 // Classical max-pivot two-sided Jacobi, reference operation order.
 int setup_classical( double4 const *src, double4 const *tgt, int64_t n_tgt,

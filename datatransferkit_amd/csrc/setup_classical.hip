@@ -96,7 +96,7 @@ __global__ void __launch_bounds__( kWave )
     int64_t const slot = blockIdx.x * (int64_t)S + sys;
     bool const valid = slot < n_tgt;
     int64_t const t = valid ? ( order ? order[slot] : slot ) : 0;
-    int32_t const *nbr = indices + t * kk;
+    int32_t const *nbr = indices + slot * kk;
     double4 const tp = valid ? tgt[t] : double4{0, 0, 0, 0};
 
     double *E = lds + sys * per_system( M );
@@ -347,7 +347,7 @@ __global__ void __launch_bounds__( kWave )
             }
             A0[j] = value;
             if ( a0_out )
-                a0_out[t * M + j] = value;
+                a0_out[slot * M + j] = value;
         }
     }
     __syncthreads();
@@ -367,7 +367,7 @@ __global__ void __launch_bounds__( kWave )
 #pragma unroll
             for ( int j = 0; j < M; j++ )
                 c += a0[j] * Pl[j] * phi;
-            coeffs[t * kk + l] = c;
+            coeffs[slot * kk + l] = c;
         }
         if ( lane == 0 && counters )
         {

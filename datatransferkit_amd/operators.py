@@ -234,3 +234,8 @@ def svd_pseudo_inverse(matrices, max_rotations=0, stream=None):
         n, nm, ap, op, ctypes.byref(under), ctypes.byref(unconv),
         int(max_rotations), ctypes.c_void_p(stream) if stream else None))
     return out, under.value, unconv.value
+
+
+def release_cached_memory():
+    """Return the library's cached device blocks (current device) to HIP."""
+    check(_lib.load().dtkx_release_cached_memory())

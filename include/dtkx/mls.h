@@ -153,6 +153,11 @@ typedef struct dtkx_mls dtkx_mls; /* opaque operator handle */
 /* ------------------------------------------------------------ generic */
 const char *dtkx_last_error(void);
 const char *dtkx_version(void);
+/* Device memory freed by the library stays cached per (device, stream) for
+ * reuse by later allocations on the same stream. This returns the current
+ * device's cached blocks to the HIP runtime (after synchronising the
+ * device). No counterpart in the reference (Kokkos manages its own pool). */
+int dtkx_release_cached_memory(void);
 /* Fill *opts with the defaults (reference semantics: Linear basis,
  * Wendland<0>, k = basis size, FP64, float geometry, classical Jacobi,
  * absolute DBL_EPSILON truncation, AoS points). */
