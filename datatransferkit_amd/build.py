@@ -25,6 +25,7 @@ SOURCES = [
     "setup_classical.hip",
     "setup_cyclic.hip",
     "apply.hip",
+    "gradient.hip",
     "capi.hip",
 ]
 

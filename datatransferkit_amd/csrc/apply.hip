@@ -282,6 +282,19 @@ void transpose_pass( RowLayout const &g, CscIndex const &csc,
     DTKX_HIP( hipGetLastError() );
 }
 
+__global__ void __launch_bounds__( kBlock )
+    gather_pitched_kernel( double const *__restrict__ x, int64_t ld, int nf,
+                           int32_t const *__restrict__ perm, int64_t n,
+                           double *__restrict__ out, int pitch )
+{
+    int64_t const p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( p >= n )
+        return;
+    int64_t const s = perm ? perm[p] : p;
+    for ( int f = 0; f < nf; ++f )
+        out[p * pitch + f] = x[f * ld + s];
+}
+
 template <typename C>
 __global__ void __launch_bounds__( kBlock )
     canonical_graph_kernel( int32_t const *__restrict__ idx,
@@ -399,6 +412,17 @@ void crs_apply_transpose_f64( RowLayout const &g, CscIndex const &csc,
         }
         f0 += nf;
     }
+}
+
+void gather_fields_f64( double const *x, int64_t ld, int nf,
+                        int32_t const *perm, int64_t n, double *out, int pitch,
+                        hipStream_t s )
+{
+    if ( n == 0 || nf == 0 )
+        return;
+    gather_pitched_kernel<<<ceil_div( n, kBlock ), kBlock, 0, s>>>(
+        x, ld, nf, perm, n, out, pitch );
+    DTKX_HIP( hipGetLastError() );
 }
 
 template <typename C>

@@ -64,6 +64,11 @@ void crs_apply_transpose_f64( RowLayout const &g, CscIndex const &csc,
                               int64_t ld_tgt, double *xbar, int64_t ld_src,
                               int n_fields, hipStream_t s );
 
+// out[p * pitch + f] = x[f][perm[p]] for f < nf (pitch >= nf), doubles.
+void gather_fields_f64( double const *x, int64_t ld, int nf,
+                        int32_t const *perm, int64_t n, double *out, int pitch,
+                        hipStream_t s );
+
 // The graph in caller order: row t = target t, indices = source indices.
 template <typename C>
 void canonical_graph( RowLayout const &g, C const *coeffs, int32_t *idx_out,

@@ -7,6 +7,7 @@
 // :129-130) are always checked here and reported as status codes.
 #include "apply.hpp"
 #include "common.hpp"
+#include "gradient.hpp"
 #include "knn.hpp"
 #include "setup.hpp"
 
@@ -24,7 +25,9 @@ struct dtkx_mls
     // trow the inverse), columns as Morton positions of the sources
     // (position p: source perm[p]); empty buffers mean caller order
     dtkx::DeviceBuffer<int32_t> offsets, indices, order, trow, perm;
-    dtkx::DeviceBuffer<double> coeffs, a0;
+    dtkx::DeviceBuffer<double> coeffs;
+    // per-row pseudo-inverse, packed upper triangle (store_pinv)
+    dtkx::DeviceBuffer<double> pinv;
     dtkx::DeviceBuffer<float> coeffs32;
     // src_d in position order, tgt_d in caller order
     dtkx::DeviceBuffer<double4> src_d, tgt_d;
@@ -112,6 +115,10 @@ void validate_options( dtkx_mls_options const &o )
          DTKX_ERR_INVALID_ARGUMENT );
     REQ( o.group_size >= 0 && o.group_size <= 16, DTKX_ERR_INVALID_ARGUMENT );
     REQ( o.max_rotations >= 0, DTKX_ERR_INVALID_ARGUMENT );
+    // the stored pseudo-inverse comes from the cyclic eigen-solver
+    REQ( !o.store_pinv || o.solver == DTKX_SOLVER_CYCLIC_JACOBI ||
+             o.precision == DTKX_PRECISION_FP32_BULK,
+         DTKX_ERR_UNSUPPORTED );
 }
 
 // Device view of a caller array: used in place if it is device memory,
@@ -232,7 +239,7 @@ void run_setup( dtkx_mls *h, unsigned long long *counters_d )
     p.rcond = h->opts.rcond;
     p.group_size = h->opts.group_size;
     if ( h->opts.store_pinv )
-        h->a0.resize( h->n_tgt * h->m, h->stream );
+        h->pinv.resize( h->n_tgt * ( h->m * ( h->m + 1 ) / 2 ), h->stream );
     if ( h->opts.precision == DTKX_PRECISION_FP32_BULK ||
          h->opts.solver == DTKX_SOLVER_CYCLIC_JACOBI )
     {
@@ -242,7 +249,7 @@ void run_setup( dtkx_mls *h, unsigned long long *counters_d )
         dtkx::setup_cyclic( h->src_d.get(), h->tgt_d.get(), h->n_tgt,
                             h->indices.get(), h->order.get(), p,
                             f32 ? nullptr : h->coeffs.get(),
-                            f32 ? h->coeffs32.get() : nullptr, h->a0.get(),
+                            f32 ? h->coeffs32.get() : nullptr, h->pinv.get(),
                             counters_d, h->stream );
         h->stats.group_size_used = h->m == 10 ? 2 : 1;
         return;
@@ -251,7 +258,7 @@ void run_setup( dtkx_mls *h, unsigned long long *counters_d )
                                           : default_rotation_cap( h->m );
     int g = dtkx::setup_classical(
         h->src_d.get(), h->tgt_d.get(), h->n_tgt, h->indices.get(),
-        h->order.get(), p, h->coeffs.get(), h->a0.get(), counters_d,
+        h->order.get(), p, h->coeffs.get(), counters_d,
         h->stream );
     h->stats.group_size_used = g;
 }
@@ -569,16 +576,47 @@ int dtkx_mls_target_gradient( const dtkx_mls *h, const double *src_vals,
                               const double *tgt_bar, int32_t n_fields,
                               int64_t ld_src, int64_t ld_tgt, double *d_tgt )
 {
-    (void)src_vals;
-    (void)tgt_bar;
-    (void)n_fields;
-    (void)ld_src;
-    (void)ld_tgt;
-    (void)d_tgt;
-    return guarded( [&]() -> int {
+    return guarded( [&]() {
         REQ( h != nullptr, DTKX_ERR_INVALID_ARGUMENT );
-        dtkx::raise( DTKX_ERR_UNSUPPORTED, "target gradient not built",
-                     __FILE__, __LINE__ );
+        REQ( n_fields >= 1, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( ld_src >= h->n_src, DTKX_ERR_SIZE_MISMATCH );
+        REQ( ld_tgt >= h->n_tgt, DTKX_ERR_SIZE_MISMATCH );
+        REQ( src_vals != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( h->n_tgt == 0 || ( tgt_bar != nullptr && d_tgt != nullptr ),
+             DTKX_ERR_INVALID_ARGUMENT );
+        // float-rounded coordinates make the operator piecewise constant
+        REQ( h->opts.geometry == DTKX_GEOMETRY_DOUBLE, DTKX_ERR_UNSUPPORTED );
+        REQ( h->opts.precision == DTKX_PRECISION_FP64, DTKX_ERR_UNSUPPORTED );
+        if ( h->n_tgt == 0 )
+            return DTKX_SUCCESS;
+        std::size_t const nin = (std::size_t)( n_fields - 1 ) * ld_src + h->n_src;
+        std::size_t const nbar = (std::size_t)( n_fields - 1 ) * ld_tgt + h->n_tgt;
+        Staged<double> x( src_vals, nin, h->stream );
+        Staged<double> yb( tgt_bar, nbar, h->stream );
+        StagedOut<double> g( d_tgt, (std::size_t)h->n_tgt * 3, h->stream );
+        dtkx::GradientArgs a;
+        a.src = h->src_d.get();
+        a.tgt = h->tgt_d.get();
+        a.indices = h->indices.get();
+        a.order = h->order.get();
+        a.perm = h->perm.get();
+        a.pinv = h->pinv.get();
+        a.n_src = h->n_src;
+        a.n_tgt = h->n_tgt;
+        a.m = h->m;
+        a.kk = h->kk;
+        a.rbf = h->opts.rbf;
+        a.truncation = h->opts.truncation;
+        a.rcond = h->opts.rcond;
+        a.max_sweeps = h->opts.solver == DTKX_SOLVER_CYCLIC_JACOBI
+                           ? h->opts.max_rotations
+                           : 0;
+        a.ld_tgt = ld_tgt;
+        a.d_tgt = g.ptr;
+        dtkx::target_gradient( a, x.ptr, ld_src, n_fields, yb.ptr, h->stream );
+        g.finish( h->stream );
+        DTKX_HIP( hipStreamSynchronize( h->stream ) );
+        return DTKX_SUCCESS;
     } );
 }
 

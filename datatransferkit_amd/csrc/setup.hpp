@@ -27,15 +27,14 @@ struct SetupParams
 
 // counters[0] = n_underdetermined, [1] = n_unconverged, [2] = rotations
 // (classical) or sweeps (cyclic); accumulated atomically.
-// Row r of indices, coeffs and a0 belongs to target order[r] (identity
-// when order is null); indices address src, which may be the sources in a
+// Row r of indices and coeffs belongs to target order[r] (identity when
+// order is null); indices address src, which may be the sources in a
 // permuted (Morton) order.
-// a0 (nullable): receives row 0 of the pseudo-inverse per row [n][m].
 
 // Classical max-pivot two-sided Jacobi, reference operation order.
 int setup_classical( double4 const *src, double4 const *tgt, int64_t n_tgt,
                      int32_t const *indices, int32_t const *order,
-                     SetupParams const &p, double *coeffs, double *a0,
+                     SetupParams const &p, double *coeffs,
                      unsigned long long *counters, hipStream_t s );
 
 // Lane group size the classical solver uses for (m, kk) when not forced.
@@ -44,11 +43,13 @@ int setup_classical_group( int m, int kk );
 // Cyclic symmetric Jacobi eigen-solver, registers only (fast path);
 // p.max_rot caps the sweeps (0 = 30). With coeffs32 != nullptr the FP32-bulk
 // form runs: float geometry, weights, Vandermonde and coefficients (written
-// to coeffs32), FP64 moments, Jacobi and truncation.
+// to coeffs32), FP64 moments, Jacobi and truncation. pinv (nullable)
+// receives each row's pseudo-inverse A^+ as a packed upper triangle
+// [n][m(m+1)/2] (entry (i, j), i <= j, at i*m - i(i-1)/2 + j - i).
 void setup_cyclic( double4 const *src, double4 const *tgt, int64_t n_tgt,
                    int32_t const *indices, int32_t const *order,
                    SetupParams const &p, double *coeffs, float *coeffs32,
-                   double *a0, unsigned long long *counters, hipStream_t s );
+                   double *pinv, unsigned long long *counters, hipStream_t s );
 
 // Batched pseudo-inverse of general n x n matrices (n <= 64), one matrix
 // per workgroup (tstSVD path).

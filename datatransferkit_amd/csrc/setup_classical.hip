@@ -82,7 +82,6 @@ __global__ void __launch_bounds__( kWave )
                             int32_t const *__restrict__ order, int kk, int rbf,
                             int geometry, int truncation, double rcond,
                             long max_rot, double *__restrict__ coeffs,
-                            double *__restrict__ a0_out,
                             unsigned long long *__restrict__ counters )
 {
 #pragma clang fp contract( off )
@@ -346,8 +345,6 @@ __global__ void __launch_bounds__( kWave )
                     value += V0[k] * U[j * M + k] / ekk;
             }
             A0[j] = value;
-            if ( a0_out )
-                a0_out[slot * M + j] = value;
         }
     }
     __syncthreads();
@@ -388,7 +385,7 @@ std::size_t lds_bytes( int m, int g )
 template <int M, int G>
 void launch( double4 const *src, double4 const *tgt, int64_t n_tgt,
              int32_t const *indices, int32_t const *order,
-             SetupParams const &p, double *coeffs, double *a0,
+             SetupParams const &p, double *coeffs,
              unsigned long long *counters, hipStream_t s )
 {
     constexpr int S = kWave / G;
@@ -396,7 +393,7 @@ void launch( double4 const *src, double4 const *tgt, int64_t n_tgt,
     DTKX_REQUIRE( lds <= 64 * 1024, DTKX_ERR_UNSUPPORTED );
     setup_classical_kernel<M, G><<<ceil_div( n_tgt, S ), kWave, lds, s>>>(
         src, tgt, n_tgt, indices, order, p.kk, p.rbf, p.geometry, p.truncation,
-        p.rcond, p.max_rot, coeffs, a0, counters );
+        p.rcond, p.max_rot, coeffs, counters );
     DTKX_HIP( hipGetLastError() );
 }
 
@@ -412,7 +409,7 @@ int setup_classical_group( int m, int kk )
 
 int setup_classical( double4 const *src, double4 const *tgt, int64_t n_tgt,
                      int32_t const *indices, int32_t const *order,
-                     SetupParams const &p, double *coeffs, double *a0,
+                     SetupParams const &p, double *coeffs,
                      unsigned long long *counters, hipStream_t s )
 {
     if ( n_tgt == 0 )
@@ -421,45 +418,45 @@ int setup_classical( double4 const *src, double4 const *tgt, int64_t n_tgt,
     switch ( p.m )
     {
     case 1:
-        launch<1, 1>( src, tgt, n_tgt, indices, order, p, coeffs, a0, counters,
+        launch<1, 1>( src, tgt, n_tgt, indices, order, p, coeffs, counters,
                       s );
         return 1;
     case 4:
         if ( g <= 1 )
         {
-            launch<4, 1>( src, tgt, n_tgt, indices, order, p, coeffs, a0,
+            launch<4, 1>( src, tgt, n_tgt, indices, order, p, coeffs,
                           counters, s );
             return 1;
         }
         if ( g == 2 )
         {
-            launch<4, 2>( src, tgt, n_tgt, indices, order, p, coeffs, a0,
+            launch<4, 2>( src, tgt, n_tgt, indices, order, p, coeffs,
                           counters, s );
             return 2;
         }
-        launch<4, 4>( src, tgt, n_tgt, indices, order, p, coeffs, a0, counters,
+        launch<4, 4>( src, tgt, n_tgt, indices, order, p, coeffs, counters,
                       s );
         return 4;
     case 10:
         if ( g <= 2 )
         {
-            launch<10, 2>( src, tgt, n_tgt, indices, order, p, coeffs, a0,
+            launch<10, 2>( src, tgt, n_tgt, indices, order, p, coeffs,
                            counters, s );
             return 2;
         }
         if ( g <= 4 )
         {
-            launch<10, 4>( src, tgt, n_tgt, indices, order, p, coeffs, a0,
+            launch<10, 4>( src, tgt, n_tgt, indices, order, p, coeffs,
                            counters, s );
             return 4;
         }
         if ( g == 8 )
         {
-            launch<10, 8>( src, tgt, n_tgt, indices, order, p, coeffs, a0,
+            launch<10, 8>( src, tgt, n_tgt, indices, order, p, coeffs,
                            counters, s );
             return 8;
         }
-        launch<10, 16>( src, tgt, n_tgt, indices, order, p, coeffs, a0,
+        launch<10, 16>( src, tgt, n_tgt, indices, order, p, coeffs,
                         counters, s );
         return 16;
     }

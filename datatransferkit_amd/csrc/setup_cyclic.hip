@@ -19,6 +19,8 @@
 #include "cyclic_device.hpp"
 #include "setup.hpp"
 
+#include <type_traits>
+
 namespace dtkx
 {
 namespace
@@ -31,7 +33,7 @@ constexpr int kBlock = 64;
 // 256 VGPRs): every lane of the pair keeps the whole moment matrix and runs
 // the same rotations on it (identical values, so both agree), and owns
 // R = M / H rows of V.
-template <int M, int H, bool F32>
+template <int M, int H, bool F32, bool STORE>
 __global__ void __launch_bounds__( kBlock )
     setup_cyclic_kernel( double4 const *__restrict__ src,
                          double4 const *__restrict__ tgt, int64_t n_tgt,
@@ -40,7 +42,7 @@ __global__ void __launch_bounds__( kBlock )
                          int geometry, int truncation, double rcond,
                          int max_sweeps, double *__restrict__ coeffs,
                          float *__restrict__ coeffs32,
-                         double *__restrict__ a0_out,
+                         double *__restrict__ pinv_out,
                          unsigned long long *__restrict__ counters )
 {
     using S = SymIdx<M>;
@@ -148,12 +150,48 @@ __global__ void __launch_bounds__( kBlock )
         }
     }
 
+    if constexpr ( STORE )
+    {
+        // the whole pseudo-inverse V diag(w) V^T, packed upper triangle;
+        // each lane writes the rows it owns, with the partner's V rows
+        // fetched by shuffles (H == 2)
+        double wk[M];
+#pragma unroll
+        for ( int k = 0; k < M; ++k )
+        {
+            double const lk = A[S::at( k, k )];
+            wk[k] = fabs( lk ) >= thr ? 1.0 / lk : 0.0;
+        }
+#pragma unroll
+        for ( int jr = 0; jr < M; ++jr )
+        {
+            // row jr of V, from the lane that owns it
+            double vj[M];
+#pragma unroll
+            for ( int k = 0; k < M; ++k )
+            {
+                double const mine = V[( jr % R ) * M + k];
+                double const other =
+                    H > 1 ? __shfl_xor( mine, 1, kBlock ) : mine;
+                vj[k] = ( jr / R == h ) ? mine : other;
+            }
+#pragma unroll
+            for ( int r = 0; r < R; ++r )
+            {
+                int const i = h * R + r;
+                if ( valid && i <= jr )
+                {
+                    double acc = 0.0;
+#pragma unroll
+                    for ( int k = 0; k < M; ++k )
+                        acc = fma( V[r * M + k] * wk[k], vj[k], acc );
+                    pinv_out[slot * S::size + S::at( i, jr )] = acc;
+                }
+            }
+        }
+    }
     if ( !valid )
         return;
-    if ( a0_out && h == 0 )
-#pragma unroll
-        for ( int j = 0; j < M; ++j )
-            a0_out[slot * M + j] = x[j];
     // coefficients c_l = phi_l (P_l . x), neighbours split over the H lanes
     for ( int l = h; l < kk; l += H )
     {
@@ -194,18 +232,22 @@ template <int M, int H>
 void launch_cyclic( double4 const *src, double4 const *tgt, int64_t n_tgt,
                     int32_t const *indices, int32_t const *order,
                     SetupParams const &p, double *coeffs, float *coeffs32,
-                    double *a0, unsigned long long *counters, hipStream_t s )
+                    double *pinv, unsigned long long *counters, hipStream_t s )
 {
     int const sweeps = p.max_rot > 0 ? static_cast<int>( p.max_rot ) : 30;
     dim3 const grid( ceil_div( n_tgt, kBlock / H ) ), block( kBlock );
+    auto run = [&]( auto f32, auto store ) {
+        setup_cyclic_kernel<M, H, decltype( f32 )::value, decltype( store )::value>
+            <<<grid, block, 0, s>>>( src, tgt, n_tgt, indices, order, p.kk,
+                                     p.rbf, p.geometry, p.truncation, p.rcond,
+                                     sweeps, coeffs, coeffs32, pinv, counters );
+    };
+    using T = std::true_type;
+    using F = std::false_type;
     if ( coeffs32 )
-        setup_cyclic_kernel<M, H, true><<<grid, block, 0, s>>>(
-            src, tgt, n_tgt, indices, order, p.kk, p.rbf, p.geometry,
-            p.truncation, p.rcond, sweeps, coeffs, coeffs32, a0, counters );
+        pinv ? run( T{}, T{} ) : run( T{}, F{} );
     else
-        setup_cyclic_kernel<M, H, false><<<grid, block, 0, s>>>(
-            src, tgt, n_tgt, indices, order, p.kk, p.rbf, p.geometry,
-            p.truncation, p.rcond, sweeps, coeffs, coeffs32, a0, counters );
+        pinv ? run( F{}, T{} ) : run( F{}, F{} );
     DTKX_HIP( hipGetLastError() );
 }
 
@@ -214,7 +256,7 @@ void launch_cyclic( double4 const *src, double4 const *tgt, int64_t n_tgt,
 void setup_cyclic( double4 const *src, double4 const *tgt, int64_t n_tgt,
                    int32_t const *indices, int32_t const *order,
                    SetupParams const &p, double *coeffs, float *coeffs32,
-                   double *a0, unsigned long long *counters, hipStream_t s )
+                   double *pinv, unsigned long long *counters, hipStream_t s )
 {
     if ( n_tgt == 0 )
         return;
@@ -222,15 +264,15 @@ void setup_cyclic( double4 const *src, double4 const *tgt, int64_t n_tgt,
     {
     case 1:
         launch_cyclic<1, 1>( src, tgt, n_tgt, indices, order, p, coeffs,
-                                 coeffs32, a0, counters, s );
+                                 coeffs32, pinv, counters, s );
         return;
     case 4:
         launch_cyclic<4, 1>( src, tgt, n_tgt, indices, order, p, coeffs,
-                                 coeffs32, a0, counters, s );
+                                 coeffs32, pinv, counters, s );
         return;
     case 10:
         launch_cyclic<10, 2>( src, tgt, n_tgt, indices, order, p, coeffs,
-                                 coeffs32, a0, counters, s );
+                                 coeffs32, pinv, counters, s );
         return;
     }
     DTKX_REQUIRE( p.m == 1 || p.m == 4 || p.m == 10, DTKX_ERR_INVALID_ARGUMENT );

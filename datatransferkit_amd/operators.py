@@ -148,6 +148,24 @@ class MovingLeastSquaresOperator:
         check(_lib.load().dtkx_mls_apply_transpose(self._h, xp, yp, nf, ldt, lds))
         return source_bar
 
+    def target_gradient(self, source_values, target_bar, out=None):
+        """dL/d(target coordinates) [n_tgt, 3] for L = sum_f <target_bar_f,
+        C(t) source_values_f>, neighbour sets frozen (double geometry)."""
+        import torch
+
+        nf, lds = self._fields(source_values, self._n_src, "source_values")
+        nf2, ldt = self._fields(target_bar, self._n_tgt, "target_bar")
+        if nf != nf2:
+            raise DtkxError(_lib.ERR_SIZE_MISMATCH, "field counts differ")
+        if out is None:
+            out = torch.empty((self._n_tgt, 3), dtype=torch.float64,
+                              device=getattr(target_bar, "device", "cuda"))
+        xp, ka = _ptr(source_values)
+        yp, kb = _ptr(target_bar)
+        gp, kc = _ptr(out)
+        check(_lib.load().dtkx_mls_target_gradient(self._h, xp, yp, nf, lds, ldt, gp))
+        return out
+
     # ------------------------------------------------------------ inspection
     def stats(self) -> dict:
         s = _lib.Stats()

@@ -126,8 +126,10 @@ typedef struct dtkx_mls_options {
     int32_t layout;     /* dtkx_layout of the point arrays */
     int32_t group_size; /* lanes per system for the classical solver
                            (1,2,4,8,16); 0 => autotuned */
-    int32_t store_pinv; /* keep x = A^+ e1 per target for the adjoint
-                           (1) or recompute it (0) */
+    int32_t store_pinv; /* keep each target's pseudo-inverse A^+ (packed
+                           upper triangle, m(m+1)/2 doubles) for
+                           dtkx_mls_target_gradient (1), or let the adjoint
+                           recompute it (0); needs the cyclic solver */
     void *stream;       /* hipStream_t; NULL = default stream */
 } dtkx_mls_options;
 
@@ -203,8 +205,13 @@ int dtkx_mls_apply_transpose(const dtkx_mls *h, const double *tgt_bar,
                              int64_t ld_tgt, int64_t ld_src);
 
 /* Gradient of L = sum_f <tgt_bar_f, C(t) src_vals_f> with respect to the
- * target coordinates, neighbour sets frozen; needs DTKX_GEOMETRY_DOUBLE.
- * d_tgt is [n_tgt][3] (AoS), overwritten. */
+ * target coordinates t, neighbour sets frozen (derivatives of the weights,
+ * the radius and the basis rows through the moment matrix). src_vals is
+ * [n_fields][ld_src], tgt_bar [n_fields][ld_tgt]; d_tgt is [n_tgt][3] (AoS),
+ * overwritten. Needs DTKX_GEOMETRY_DOUBLE and FP64. The pseudo-inverse is
+ * read back when the handle stored it (store_pinv), else recomputed per
+ * target with the cyclic eigen-solver. No counterpart in the reference,
+ * whose MLS operator has no adjoint. */
 int dtkx_mls_target_gradient(const dtkx_mls *h, const double *src_vals,
                              const double *tgt_bar, int32_t n_fields,
                              int64_t ld_src, int64_t ld_tgt, double *d_tgt);
