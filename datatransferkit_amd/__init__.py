@@ -7,7 +7,8 @@ Jacobi pseudo-inverse, CRS apply and transpose) behind a C ABI
 """
 from ._lib import DtkxError  # noqa: F401
 from .operators import (  # noqa: F401
-    MovingLeastSquaresOperator, knn, release_cached_memory, svd_pseudo_inverse)
+    MovingLeastSquaresBatch, MovingLeastSquaresOperator, knn,
+    release_cached_memory, svd_pseudo_inverse)
 
-__all__ = ["MovingLeastSquaresOperator", "knn", "svd_pseudo_inverse",
-           "release_cached_memory", "DtkxError"]
+__all__ = ["MovingLeastSquaresOperator", "MovingLeastSquaresBatch", "knn",
+           "svd_pseudo_inverse", "release_cached_memory", "DtkxError"]

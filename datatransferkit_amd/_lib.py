@@ -130,7 +130,7 @@ def load():
         _i32, _i64, _vp, _vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), _i32, _vp]
     L.dtkx_mls_batch_create.argtypes = [_i32, ctypes.POINTER(Problem),
                                         ctypes.POINTER(Options), ctypes.POINTER(_vp)]
-    L.dtkx_mls_batch_apply.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp)]
+    L.dtkx_mls_batch_apply.argtypes = [_vp, _vp, _vp]
     L.dtkx_mls_batch_get_stats.argtypes = [_vp, ctypes.POINTER(Stats)]
     L.dtkx_mls_batch_destroy.argtypes = [_vp]
     for name in EXPORTED:

@@ -314,7 +314,80 @@ __global__ void __launch_bounds__( kBlock )
     coeffs_out[e] = coeffs[l];
 }
 
+// ------------------------------------------------------------- batched
+template <typename XS>
+__global__ void __launch_bounds__( kBlock )
+    batch_gather_kernel( double const *const *__restrict__ x,
+                         int32_t const *__restrict__ perm,
+                         int32_t const *__restrict__ src_prob,
+                         int32_t const *__restrict__ src_off, int64_t n,
+                         XS *__restrict__ xs )
+{
+    int64_t const p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( p >= n )
+        return;
+    int32_t const s = perm[p];
+    int32_t const q = src_prob[s];
+    xs[p] = (XS)x[q][s - src_off[q]];
+}
+
+// One lane per row, the row's entries in order (sequential sum).
+template <typename C, typename XS, typename Acc>
+__global__ void __launch_bounds__( kBlock )
+    batch_rows_kernel( int32_t const *__restrict__ idx,
+                       C const *__restrict__ coeffs,
+                       int64_t const *__restrict__ row_off, int64_t n_rows,
+                       XS const *__restrict__ xs,
+                       int32_t const *__restrict__ order,
+                       int32_t const *__restrict__ tgt_prob,
+                       int32_t const *__restrict__ tgt_off,
+                       double *const *__restrict__ y )
+{
+#pragma clang fp contract( off )
+    using Sum = typename SumOf<Acc>::type;
+    int64_t const r = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( r >= n_rows )
+        return;
+    int64_t const b = row_off[r], e = row_off[r + 1];
+    Sum acc;
+    for ( int64_t j = b; j < e; ++j )
+        acc.add( (Acc)coeffs[j] * (Acc)xs[idx[j]] );
+    int32_t const t = order[r];
+    int32_t const q = tgt_prob[t];
+    y[q][t - tgt_off[q]] = (double)acc.value();
+}
+
+template <typename C, typename XS, typename Acc>
+void launch_batch( BatchLayout const &g, C const *coeffs,
+                   double const *const *x, double *const *y, XS *xs,
+                   hipStream_t s )
+{
+    if ( g.n_tgt == 0 )
+        return;
+    batch_gather_kernel<XS><<<ceil_div( g.n_src, kBlock ), kBlock, 0, s>>>(
+        x, g.perm, g.src_prob, g.src_off, g.n_src, xs );
+    batch_rows_kernel<C, XS, Acc>
+        <<<ceil_div( g.n_tgt, kBlock ), kBlock, 0, s>>>(
+            g.idx, coeffs, g.row_off, g.n_tgt, xs, g.order, g.tgt_prob,
+            g.tgt_off, y );
+    DTKX_HIP( hipGetLastError() );
+}
+
 } // namespace
+
+void batch_apply_f64( BatchLayout const &g, double const *coeffs,
+                      double const *const *x, double *const *y, double *xs,
+                      hipStream_t s )
+{
+    launch_batch<double, double, double>( g, coeffs, x, y, xs, s );
+}
+
+void batch_apply_f32( BatchLayout const &g, float const *coeffs,
+                      double const *const *x, double *const *y, float *xs,
+                      hipStream_t s )
+{
+    launch_batch<float, float, float>( g, coeffs, x, y, xs, s );
+}
 
 void crs_apply_f64( RowLayout const &g, double const *coeffs, double const *x,
                     int64_t ld_src, double *y, int64_t ld_tgt, int n_fields,

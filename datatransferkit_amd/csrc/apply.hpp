@@ -69,6 +69,29 @@ void gather_fields_f64( double const *x, int64_t ld, int nf,
                         int32_t const *perm, int64_t n, double *out, int pitch,
                         hipStream_t s );
 
+// Batched operators, one field each: rows in CSR form (row_off), columns
+// as global positions of the concatenated problem-major sources. Problem q
+// reads its values from x[q] (local source index = perm[p] - src_off[q])
+// and writes y[q] (local target index = order[r] - tgt_off[q]); x and y are
+// device arrays of device pointers, read when the kernels run.
+struct BatchLayout
+{
+    int32_t const *idx = nullptr;
+    int64_t const *row_off = nullptr;
+    int32_t const *order = nullptr, *perm = nullptr;
+    int32_t const *src_off = nullptr, *tgt_off = nullptr;
+    int32_t const *src_prob = nullptr, *tgt_prob = nullptr;
+    int64_t n_src = 0, n_tgt = 0;
+};
+
+// xs: scratch of n_src doubles (or floats for float coefficients)
+void batch_apply_f64( BatchLayout const &g, double const *coeffs,
+                      double const *const *x, double *const *y, double *xs,
+                      hipStream_t s );
+void batch_apply_f32( BatchLayout const &g, float const *coeffs,
+                      double const *const *x, double *const *y, float *xs,
+                      hipStream_t s );
+
 // The graph in caller order: row t = target t, indices = source indices.
 template <typename C>
 void canonical_graph( RowLayout const &g, C const *coeffs, int32_t *idx_out,

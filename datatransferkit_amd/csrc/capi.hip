@@ -11,6 +11,9 @@
 #include "knn.hpp"
 #include "setup.hpp"
 
+#include <algorithm>
+#include <vector>
+#include <type_traits>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -768,44 +771,445 @@ int dtkx_svd_pseudo_inverse( int32_t n, int64_t n_matrices, const double *a,
     } );
 }
 
+} // extern "C"
+
 // ------------------------------------------------------------- batched
 struct dtkx_mls_batch
 {
-    int unused;
+    dtkx_mls_options opts{};
+    int n_prob = 0;
+    hipStream_t stream = nullptr;
+    int64_t n_src = 0, n_tgt = 0, nnz = 0;
+    std::vector<int64_t> tgt_count; // per problem
+    // concatenated problems: offsets, problem ids, Morton layout, CSR rows
+    dtkx::DeviceBuffer<int32_t> src_off, tgt_off, src_prob, tgt_prob;
+    dtkx::DeviceBuffer<int32_t> order, perm, pos, indices;
+    dtkx::DeviceBuffer<int64_t> row_off;
+    dtkx::DeviceBuffer<double> coeffs;
+    dtkx::DeviceBuffer<float> coeffs32;
+    // apply: gathered values and the per-problem pointer arrays the
+    // captured graph reads
+    mutable dtkx::DeviceBuffer<double> xs64;
+    mutable dtkx::DeviceBuffer<float> xs32;
+    mutable dtkx::DeviceBuffer<double const *> xptr;
+    mutable dtkx::DeviceBuffer<double *> yptr;
+    mutable hipGraph_t graph = nullptr;
+    mutable hipGraphExec_t exec = nullptr;
+    dtkx_mls_stats stats{};
+
+    dtkx::BatchLayout layout() const
+    {
+        dtkx::BatchLayout g;
+        g.idx = indices.get();
+        g.row_off = row_off.get();
+        g.order = order.get();
+        g.perm = perm.get();
+        g.src_off = src_off.get();
+        g.tgt_off = tgt_off.get();
+        g.src_prob = src_prob.get();
+        g.tgt_prob = tgt_prob.get();
+        g.n_src = n_src;
+        g.n_tgt = n_tgt;
+        return g;
+    }
+    void launch_apply() const
+    {
+        if ( opts.precision == DTKX_PRECISION_FP32_BULK )
+            dtkx::batch_apply_f32( layout(), coeffs32.get(), xptr.get(),
+                                   yptr.get(), xs32.get(), stream );
+        else
+            dtkx::batch_apply_f64( layout(), coeffs.get(), xptr.get(),
+                                   yptr.get(), xs64.get(), stream );
+    }
+    ~dtkx_mls_batch()
+    {
+        if ( exec )
+            (void)hipGraphExecDestroy( exec );
+        if ( graph )
+            (void)hipGraphDestroy( graph );
+    }
 };
+
+namespace
+{
+
+// the K of the query kernel that holds up to k results
+int query_bucket( int k )
+{
+    int const ks[] = {1, 4, 8, 10, 16, 24, 32, 48, 64};
+    for ( int v : ks )
+        if ( k <= v )
+            return v;
+    return 64;
+}
+
+// Lanes per system of the classical solver for rows [r0, r1): the fastest
+// of the candidates on a sample of the rows (every candidate gives the same
+// bits, so the sample's outputs are simply overwritten afterwards).
+int autotune_group( dtkx_mls_batch *b, dtkx::SetupParams p, double4 const *src,
+                    double4 const *tgt, int64_t r0, int64_t r1 )
+{
+    std::vector<int> cand;
+    if ( p.m == 1 )
+        cand = {1};
+    else if ( p.m == 4 )
+        cand = {1, 2, 4};
+    else
+        cand = {2, 4, 8, 16};
+    if ( cand.size() == 1 )
+        return cand[0];
+    int64_t const end = std::min<int64_t>( r1, r0 + 65536 );
+    dtkx::DeviceBuffer<unsigned long long> scratch( 3, b->stream );
+    p.row0 = r0;
+    int best = cand[0];
+    double best_ms = 1e300;
+    for ( int g : cand )
+    {
+        p.group_size = g;
+        StageEvents ev( b->stream );
+        dtkx::setup_classical( src, tgt, end, b->indices.get(),
+                               b->order.get(), p, b->coeffs.get(),
+                               scratch.get(), b->stream ); // warm
+        ev.mark( 0 );
+        dtkx::setup_classical( src, tgt, end, b->indices.get(),
+                               b->order.get(), p, b->coeffs.get(),
+                               scratch.get(), b->stream );
+        ev.mark( 1 );
+        double const ms = ev.ms( 0, 1 );
+        if ( ms < best_ms )
+        {
+            best_ms = ms;
+            best = g;
+        }
+    }
+    return best;
+}
+
+} // namespace
+
+extern "C" {
 
 int dtkx_mls_batch_create( int32_t n_problems, const dtkx_mls_problem *problems,
                            const dtkx_mls_options *opts, dtkx_mls_batch **out )
 {
-    (void)n_problems;
-    (void)problems;
-    (void)opts;
-    if ( out )
+    return guarded( [&]() {
+        REQ( out != nullptr, DTKX_ERR_INVALID_ARGUMENT );
         *out = nullptr;
-    return fail( DTKX_ERR_UNSUPPORTED, "dtkx: batched operators not built" );
+        REQ( n_problems >= 1 && problems != nullptr && opts != nullptr,
+             DTKX_ERR_INVALID_ARGUMENT );
+        validate_options( *opts );
+        REQ( !opts->store_pinv, DTKX_ERR_UNSUPPORTED );
+        int const P = n_problems;
+        auto b = std::make_unique<dtkx_mls_batch>();
+        b->opts = *opts;
+        b->n_prob = P;
+        b->stream = static_cast<hipStream_t>( opts->stream );
+        if ( !b->stream )
+        {
+            // graph capture needs a stream other than the null stream; the
+            // null stream's earlier work is waited for first
+            DTKX_HIP( hipDeviceSynchronize() );
+            b->stream = dtkx::library_stream();
+        }
+        hipStream_t const s = b->stream;
+
+        // host bookkeeping: sizes, basis, neighbours, row order
+        std::vector<int32_t> so( P + 1, 0 ), to( P + 1, 0 ), m( P ), kk( P ),
+            key( P ), rstart( P );
+        std::vector<int64_t> rbase( P );
+        std::vector<double const *> sp( P ), tp( P );
+        int64_t ns = 0, nt = 0;
+        for ( int p = 0; p < P; ++p )
+        {
+            dtkx_mls_problem const &q = problems[p];
+            REQ( q.n_src >= 1, DTKX_ERR_EMPTY_SOURCE );
+            REQ( q.n_tgt >= 0, DTKX_ERR_INVALID_ARGUMENT );
+            REQ( q.basis == DTKX_BASIS_CONSTANT || q.basis == DTKX_BASIS_LINEAR ||
+                     q.basis == DTKX_BASIS_QUADRATIC,
+                 DTKX_ERR_INVALID_ARGUMENT );
+            REQ( q.k >= 0 && q.k <= dtkx::knn_max_k, DTKX_ERR_UNSUPPORTED );
+            REQ( q.src_xyz != nullptr && dtkx::is_device_pointer( q.src_xyz ),
+                 DTKX_ERR_INVALID_ARGUMENT );
+            REQ( q.n_tgt == 0 || ( q.tgt_xyz != nullptr &&
+                                   dtkx::is_device_pointer( q.tgt_xyz ) ),
+                 DTKX_ERR_INVALID_ARGUMENT );
+            m[p] = q.basis;
+            int const k = q.k > 0 ? q.k : q.basis;
+            kk[p] = static_cast<int32_t>( std::min<int64_t>( k, q.n_src ) );
+            ns += q.n_src;
+            nt += q.n_tgt;
+            REQ( ns < INT32_MAX && nt < INT32_MAX, DTKX_ERR_UNSUPPORTED );
+            so[p + 1] = static_cast<int32_t>( ns );
+            to[p + 1] = static_cast<int32_t>( nt );
+            sp[p] = q.src_xyz;
+            tp[p] = q.tgt_xyz;
+        }
+        b->n_src = ns;
+        b->n_tgt = nt;
+        b->tgt_count.resize( P );
+        for ( int p = 0; p < P; ++p )
+            b->tgt_count[p] = problems[p].n_tgt;
+        // rows grouped by (basis, query kernel, problem): one setup launch
+        // per basis, one query launch per kernel
+        std::vector<int> rank( P );
+        for ( int p = 0; p < P; ++p )
+            rank[p] = p;
+        std::stable_sort( rank.begin(), rank.end(), [&]( int a, int c ) {
+            if ( m[a] != m[c] )
+                return m[a] < m[c];
+            return query_bucket( kk[a] ) < query_bucket( kk[c] );
+        } );
+        dtkx::KnnBatch kb;
+        struct Group
+        {
+            int m;
+            int64_t r0, r1;
+        };
+        std::vector<Group> groups;
+        int64_t row = 0, nnz = 0;
+        for ( int i = 0; i < P; ++i )
+        {
+            int const p = rank[i];
+            key[p] = i;
+            rstart[p] = static_cast<int32_t>( row );
+            rbase[p] = nnz;
+            int64_t const n = problems[p].n_tgt;
+            int const kb_p = query_bucket( kk[p] );
+            if ( n > 0 )
+            {
+                if ( !kb.launches.empty() && kb.launches.back().end == row &&
+                     query_bucket( kb.launches.back().kmax ) == kb_p )
+                {
+                    kb.launches.back().end = row + n;
+                    kb.launches.back().kmax =
+                        std::max( kb.launches.back().kmax, (int)kk[p] );
+                }
+                else
+                    kb.launches.push_back( {row, row + n, kk[p]} );
+                if ( !groups.empty() && groups.back().m == m[p] &&
+                     groups.back().r1 == row )
+                    groups.back().r1 = row + n;
+                else
+                    groups.push_back( {m[p], row, row + n} );
+            }
+            row += n;
+            nnz += n * kk[p];
+        }
+        b->nnz = nnz;
+
+        // device copies of the bookkeeping
+        auto upload = [&]( auto &buf, auto const &vec ) {
+            using T = typename std::decay_t<decltype( vec )>::value_type;
+            buf.resize( vec.size(), s );
+            DTKX_HIP( hipMemcpyAsync( buf.get(), vec.data(),
+                                      vec.size() * sizeof( T ),
+                                      hipMemcpyHostToDevice, s ) );
+        };
+        dtkx::DeviceBuffer<int32_t> d_kk, d_key, d_rstart;
+        dtkx::DeviceBuffer<int64_t> d_rbase;
+        dtkx::DeviceBuffer<double const *> d_sp, d_tp;
+        upload( b->src_off, so );
+        upload( b->tgt_off, to );
+        upload( d_kk, kk );
+        upload( d_key, key );
+        upload( d_rstart, rstart );
+        upload( d_rbase, rbase );
+        upload( d_sp, sp );
+        upload( d_tp, tp );
+        b->src_prob.resize( ns, s );
+        b->tgt_prob.resize( std::max<int64_t>( nt, 1 ), s );
+        dtkx::problem_ids( b->src_off.get(), P, ns, b->src_prob.get(), s );
+        dtkx::problem_ids( b->tgt_off.get(), P, nt, b->tgt_prob.get(), s );
+        dtkx::DeviceBuffer<double4> src_d( ns, s ),
+            tgt_d( std::max<int64_t>( nt, 1 ), s );
+        dtkx::concat_points( d_sp.get(), b->src_off.get(), b->src_prob.get(),
+                             ns, src_d.get(), s );
+        dtkx::concat_points( d_tp.get(), b->tgt_off.get(), b->tgt_prob.get(),
+                             nt, tgt_d.get(), s );
+        // the host vectors above must outlive the uploads
+        DTKX_HIP( hipStreamSynchronize( s ) );
+
+        dtkx::DeviceBuffer<unsigned long long> counters( 3, s );
+        DTKX_HIP( hipMemsetAsync( counters.get(), 0, 3 * sizeof( uint64_t ), s ) );
+        b->order.resize( std::max<int64_t>( nt, 1 ), s );
+        b->perm.resize( ns, s );
+        b->pos.resize( ns, s );
+        b->row_off.resize( nt + 1, s );
+        b->indices.resize( std::max<int64_t>( nnz, 1 ), s );
+        if ( opts->precision == DTKX_PRECISION_FP32_BULK )
+            b->coeffs32.resize( std::max<int64_t>( nnz, 1 ), s );
+        else
+            b->coeffs.resize( std::max<int64_t>( nnz, 1 ), s );
+
+        StageEvents ev( s );
+        ev.mark( 0 );
+        kb.n_problems = P;
+        kb.src_off = b->src_off.get();
+        kb.src_prob = b->src_prob.get();
+        kb.tgt_prob = b->tgt_prob.get();
+        kb.row_key = d_key.get();
+        kb.key_bits = 1;
+        while ( ( int64_t( 1 ) << kb.key_bits ) < P )
+            ++kb.key_bits;
+        kb.kk_prob = d_kk.get();
+        kb.row_base = d_rbase.get();
+        kb.row_start = d_rstart.get();
+        kb.nnz = nnz;
+        dtkx::KnnBatchOutput ko;
+        ko.order = b->order.get();
+        ko.src_perm = b->perm.get();
+        ko.src_pos = b->pos.get();
+        ko.row_off = b->row_off.get();
+        ko.indices = b->indices.get();
+        dtkx::KnnInfo info;
+        if ( opts->geometry == DTKX_GEOMETRY_FLOAT )
+        {
+            dtkx::DeviceBuffer<float4> sf( ns, s ), tf( std::max<int64_t>( nt, 1 ), s );
+            dtkx::double4_to_float4( src_d.get(), ns, sf.get(), s );
+            dtkx::double4_to_float4( tgt_d.get(), nt, tf.get(), s );
+            dtkx::knn_search_batched( opts->geometry, sf.get(), ns, tf.get(),
+                                      nt, kb, ko, &info, s );
+        }
+        else
+            dtkx::knn_search_batched( opts->geometry, src_d.get(), ns,
+                                      tgt_d.get(), nt, kb, ko, &info, s );
+        {
+            dtkx::DeviceBuffer<double4> sorted( ns, s );
+            dtkx::gather_points( src_d.get(), b->perm.get(), ns, sorted.get(),
+                                 s );
+            src_d = std::move( sorted );
+        }
+        ev.mark( 1 );
+
+        // setup: one launch per basis over its rows
+        bool const cyclic = opts->precision == DTKX_PRECISION_FP32_BULK ||
+                            opts->solver == DTKX_SOLVER_CYCLIC_JACOBI;
+        int group_used = 0;
+        for ( auto const &g : groups )
+        {
+            dtkx::SetupParams p;
+            p.m = g.m;
+            p.kk = 0;
+            p.rbf = opts->rbf;
+            p.geometry = opts->geometry;
+            p.truncation = opts->truncation;
+            p.rcond = opts->rcond;
+            p.row0 = g.r0;
+            p.row_off = b->row_off.get();
+            if ( cyclic )
+            {
+                p.max_rot = opts->max_rotations;
+                bool const f32 = opts->precision == DTKX_PRECISION_FP32_BULK;
+                dtkx::setup_cyclic( src_d.get(), tgt_d.get(), g.r1,
+                                    b->indices.get(), b->order.get(), p,
+                                    f32 ? nullptr : b->coeffs.get(),
+                                    f32 ? b->coeffs32.get() : nullptr, nullptr,
+                                    counters.get(), s );
+                group_used = g.m == 10 ? 2 : 1;
+            }
+            else
+            {
+                p.max_rot = opts->max_rotations > 0 ? opts->max_rotations
+                                                    : default_rotation_cap( g.m );
+                p.group_size = opts->group_size > 0
+                                   ? opts->group_size
+                                   : autotune_group( b.get(), p, src_d.get(),
+                                                     tgt_d.get(), g.r0, g.r1 );
+                group_used = dtkx::setup_classical(
+                    src_d.get(), tgt_d.get(), g.r1, b->indices.get(),
+                    b->order.get(), p, b->coeffs.get(), counters.get(), s );
+            }
+        }
+        ev.mark( 2 );
+        unsigned long long c[3] = {0, 0, 0};
+        DTKX_HIP( hipMemcpyAsync( c, counters.get(), sizeof( c ),
+                                  hipMemcpyDeviceToHost, s ) );
+        DTKX_HIP( hipStreamSynchronize( s ) );
+        b->stats.n_source = ns;
+        b->stats.n_target = nt;
+        b->stats.nnz = nnz;
+        b->stats.basis_size = *std::max_element( m.begin(), m.end() );
+        b->stats.k = *std::max_element( kk.begin(), kk.end() );
+        b->stats.n_underdetermined = (int64_t)c[0];
+        b->stats.n_unconverged = (int64_t)c[1];
+        b->stats.total_rotations = (int64_t)c[2];
+        b->stats.ms_knn = ev.ms( 0, 1 );
+        b->stats.ms_setup = ev.ms( 1, 2 );
+        b->stats.group_size_used = group_used;
+        b->stats.tree_height = info.tree_height;
+
+        // apply resources, fixed for the captured graph
+        if ( opts->precision == DTKX_PRECISION_FP32_BULK )
+            b->xs32.resize( ns, s );
+        else
+            b->xs64.resize( ns, s );
+        b->xptr.resize( P, s );
+        b->yptr.resize( P, s );
+        *out = b.release();
+        return DTKX_SUCCESS;
+    } );
 }
 
 int dtkx_mls_batch_apply( const dtkx_mls_batch *b,
                           const double *const *src_vals,
                           double *const *tgt_vals )
 {
-    (void)b;
-    (void)src_vals;
-    (void)tgt_vals;
-    return fail( DTKX_ERR_UNSUPPORTED, "dtkx: batched operators not built" );
+    return guarded( [&]() {
+        REQ( b != nullptr && src_vals != nullptr && tgt_vals != nullptr,
+             DTKX_ERR_INVALID_ARGUMENT );
+        for ( int p = 0; p < b->n_prob; ++p )
+            REQ( src_vals[p] != nullptr &&
+                     ( tgt_vals[p] != nullptr || b->tgt_count[p] == 0 ),
+                 DTKX_ERR_INVALID_ARGUMENT );
+        hipStream_t const s = b->stream;
+        DTKX_HIP( hipMemcpyAsync( b->xptr.get(), src_vals,
+                                  sizeof( double * ) * b->n_prob,
+                                  hipMemcpyHostToDevice, s ) );
+        DTKX_HIP( hipMemcpyAsync( b->yptr.get(), tgt_vals,
+                                  sizeof( double * ) * b->n_prob,
+                                  hipMemcpyHostToDevice, s ) );
+        if ( !b->exec )
+        {
+            // gather + row sums of every problem, captured once and
+            // replayed by each apply
+            DTKX_HIP( hipStreamBeginCapture( s, hipStreamCaptureModeThreadLocal ) );
+            try
+            {
+                b->launch_apply();
+            }
+            catch ( ... )
+            {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture( s, &g );
+                if ( g )
+                    (void)hipGraphDestroy( g );
+                throw;
+            }
+            DTKX_HIP( hipStreamEndCapture( s, &b->graph ) );
+            DTKX_HIP( hipGraphInstantiate( &b->exec, b->graph, nullptr,
+                                           nullptr, 0 ) );
+        }
+        DTKX_HIP( hipGraphLaunch( b->exec, s ) );
+        DTKX_HIP( hipStreamSynchronize( s ) );
+        return DTKX_SUCCESS;
+    } );
 }
 
 int dtkx_mls_batch_get_stats( const dtkx_mls_batch *b, dtkx_mls_stats *stats )
 {
-    (void)b;
-    (void)stats;
-    return fail( DTKX_ERR_UNSUPPORTED, "dtkx: batched operators not built" );
+    return guarded( [&]() {
+        REQ( b != nullptr && stats != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        *stats = b->stats;
+        return DTKX_SUCCESS;
+    } );
 }
 
 int dtkx_mls_batch_destroy( dtkx_mls_batch *b )
 {
-    delete b;
-    return DTKX_SUCCESS;
+    return guarded( [&]() {
+        delete b;
+        return DTKX_SUCCESS;
+    } );
 }
 
 } // extern "C"

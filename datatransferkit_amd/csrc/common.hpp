@@ -59,6 +59,10 @@ void *cache_alloc( std::size_t bytes, hipStream_t s );
 void cache_free( void *p, std::size_t bytes, hipStream_t s );
 // Returns the cached blocks of the current device to the runtime.
 void cache_release();
+// A stream of the library's own on the current device (default priority),
+// created on first use and kept: for work that cannot run on the null
+// stream, such as graph capture.
+hipStream_t library_stream();
 
 // Owning device buffer, move-only. With a stream it draws from the block
 // cache of that stream; without one it uses hipMalloc / hipFree.
@@ -152,6 +156,15 @@ void double4_to_float4( double4 const *in, int64_t n, float4 *out,
 // out[i] = in[perm[i]]
 void gather_points( double4 const *in, int32_t const *perm, int64_t n,
                     double4 *out, hipStream_t s );
+
+// Batches: prob[i] = the problem p with off[p] <= i < off[p + 1]
+void problem_ids( int32_t const *off, int n_prob, int64_t n, int32_t *prob,
+                  hipStream_t s );
+// out[i] = point i - off[p] of problem p = prob[i], read from the AoS
+// device array ptrs[p]
+void concat_points( double const *const *ptrs, int32_t const *off,
+                    int32_t const *prob, int64_t n, double4 *out,
+                    hipStream_t s );
 
 // inv[perm[i]] = i
 void invert_permutation( int32_t const *perm, int64_t n, int32_t *inv,

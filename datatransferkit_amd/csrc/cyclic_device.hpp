@@ -98,9 +98,14 @@ __device__ inline double nbr_row( int geometry, int rbf, double radius,
     }
 }
 
-// V holds R rows of the eigenvector matrix (all of it when R == M)
+// V holds R rows of the eigenvector matrix (all of it when R == M).
+// skip: a system that has converged while its wave goes on. t = 0 makes
+// every update of V and of A's diagonal and other entries add an exact
+// zero, so the eigenvalues and vectors keep their bits without a branch
+// around the straight-line sweep; only A(p, q), already negligible and read
+// by nothing after convergence, is cleared as usual.
 template <int M, int R, int P_, int Q_>
-__device__ inline void rotate( double *A, double *V )
+__device__ inline void rotate( double *A, double *V, bool skip )
 {
     using S = SymIdx<M>;
     double const apq = A[S::at( P_, Q_ )];
@@ -112,7 +117,7 @@ __device__ inline void rotate( double *A, double *V )
     t = theta < 0 ? -t : t;
     // huge |theta|: theta^2 overflows, t -> 1/(2 theta); apq == 0: no-op
     t = at > 1e150 ? 0.5 / theta : t;
-    t = ( apq == 0.0 ) ? 0.0 : t;
+    t = ( apq == 0.0 || skip ) ? 0.0 : t;
     double const c = 1.0 / __builtin_sqrt( fma( t, t, 1.0 ) );
     double const s = t * c;
     double const tau = s / ( 1.0 + c );
@@ -144,21 +149,21 @@ __device__ inline void rotate( double *A, double *V )
 template <int M, int R, int P_, int Q_>
 struct Sweep
 {
-    __device__ static inline void run( double *A, double *V )
+    __device__ static inline void run( double *A, double *V, bool skip )
     {
-        rotate<M, R, P_, Q_>( A, V );
+        rotate<M, R, P_, Q_>( A, V, skip );
         if constexpr ( Q_ + 1 < M )
-            Sweep<M, R, P_, Q_ + 1>::run( A, V );
+            Sweep<M, R, P_, Q_ + 1>::run( A, V, skip );
         else if constexpr ( P_ + 2 < M )
-            Sweep<M, R, P_ + 1, P_ + 2>::run( A, V );
+            Sweep<M, R, P_ + 1, P_ + 2>::run( A, V, skip );
     }
 };
 
 template <int M, int R>
-__device__ inline void sweep_all( double *A, double *V )
+__device__ inline void sweep_all( double *A, double *V, bool skip )
 {
     if constexpr ( M >= 2 )
-        Sweep<M, R, 0, 1>::run( A, V );
+        Sweep<M, R, 0, 1>::run( A, V, skip );
 }
 
 template <int M>

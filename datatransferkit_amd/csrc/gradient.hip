@@ -238,12 +238,14 @@ __global__ void __launch_bounds__( kBlock )
 #pragma unroll
         for ( int e = 0; e < R * M; ++e )
             V[e] = ( h * R + e / M == e % M ) ? 1.0 : 0.0;
+        // each system stops at its own convergence, so its result does not
+        // depend on which systems share its wave
         int sweeps = 0;
         bool done = !valid || converged<M>( A );
-        while ( !__all( done ) && sweeps < max_sweeps )
+        for ( int pass = 0; !__all( done ) && pass < max_sweeps; ++pass )
         {
-            sweep_all<M, R>( A, V );
-            ++sweeps;
+            sweep_all<M, R>( A, V, done );
+            sweeps += done ? 0 : 1;
             done = done || converged<M>( A );
         }
         double thr = DBL_EPSILON;

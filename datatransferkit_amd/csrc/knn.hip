@@ -138,14 +138,16 @@ __device__ inline uint64_t expand21( uint32_t v )
     return x;
 }
 
+// prob (nullable): problem of each point, whose scene box is box + 6 prob
 template <typename T, typename V>
 __global__ void __launch_bounds__( kBlock )
-    morton_kernel( V const *pts, int64_t n, T const *box, uint64_t *codes,
-                   int32_t *ids )
+    morton_kernel( V const *pts, int64_t n, T const *boxes, uint64_t *codes,
+                   int32_t *ids, int32_t const *prob )
 {
     int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
     if ( i >= n )
         return;
+    T const *box = prob ? boxes + 6 * prob[i] : boxes;
     V p = pts[i];
     double c[3] = {(double)p.x, (double)p.y, (double)p.z};
     uint32_t q[3];
@@ -234,15 +236,52 @@ __device__ inline void cell_box( uint64_t code, int delta, T const *scene,
 // exact box, a larger one the Morton cell of its common prefix.
 constexpr int kTight = 8;
 
+// Problem owning internal node i of a batch: the largest p with
+// src_off[p] - p <= i (problem p's internal nodes are
+// [src_off[p] - p, src_off[p + 1] - p - 1), empty for one source).
+__device__ inline int node_problem( int32_t const *src_off, int n_prob,
+                                    int64_t i )
+{
+    int lo = 0, hi = n_prob - 1;
+    while ( lo < hi )
+    {
+        int const mid = ( lo + hi + 1 ) >> 1;
+        if ( src_off[mid] - mid <= i )
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// One problem (src_off null): n points, internal nodes [0, n - 1).
+// Batch: n_int internal nodes in all; node i belongs to problem p, whose
+// sorted points are [src_off[p], src_off[p + 1]) with scene box scene + 6 p;
+// the hierarchy of each problem is built over its own range (child node
+// indices, parents and leaf codes are global).
 template <typename T, typename V>
 __global__ void __launch_bounds__( kBlock )
-    karras_kernel( uint64_t const *codes, int32_t const *sorted_ids,
-                   V const *pts, int64_t n, T const *scene, Node<T> *nodes,
+    karras_kernel( uint64_t const *codes_all, int32_t const *sorted_all,
+                   V const *pts, int64_t n_int, T const *scenes,
+                   int32_t const *src_off, int n_prob, Node<T> *nodes,
                    int32_t *parent_internal, int32_t *parent_leaf )
 {
-    int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-    if ( i >= n - 1 )
+    int64_t const gi = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( gi >= n_int )
         return;
+    int64_t so = 0, io = 0, n = n_int + 1;
+    T const *scene = scenes;
+    if ( src_off )
+    {
+        int const pb = node_problem( src_off, n_prob, gi );
+        so = src_off[pb];
+        n = src_off[pb + 1] - so;
+        io = so - pb;
+        scene = scenes + 6 * pb;
+    }
+    uint64_t const *codes = codes_all + so;
+    int32_t const *sorted_ids = sorted_all + so;
+    int64_t const i = gi - io;
     int d = ( prefix_delta( codes, n, i, i + 1 ) -
               prefix_delta( codes, n, i, i - 1 ) ) >= 0
                 ? 1
@@ -280,12 +319,12 @@ __global__ void __launch_bounds__( kBlock )
             nd.lo[c][1] = nd.hi[c][1] = p.y;
             nd.lo[c][2] = nd.hi[c][2] = p.z;
             nd.child[c] = ~orig;
-            parent_leaf[b] = (int32_t)( ( i << 1 ) | c );
+            parent_leaf[so + b] = (int32_t)( ( gi << 1 ) | c );
             continue;
         }
-        int64_t const child = c == 0 ? gamma : gamma + 1;
+        int64_t const child = io + ( c == 0 ? gamma : gamma + 1 );
         nd.child[c] = (int32_t)child;
-        parent_internal[child] = (int32_t)( ( i << 1 ) | c );
+        parent_internal[child] = (int32_t)( ( gi << 1 ) | c );
         if ( e - b + 1 <= kTight )
         {
             T lo[3] = {INFINITY, INFINITY, INFINITY};
@@ -311,21 +350,34 @@ __global__ void __launch_bounds__( kBlock )
                          nd.lo[c], nd.hi[c] );
     }
     nd.height[0] = nd.height[1] = 0;
-    nodes[i] = nd;
+    nodes[gi] = nd;
 }
 
 // Depth of every leaf by walking its parents (reads only); the maximum is
 // the tree height that sizes the traversal stack.
+// Batch: src_prob gives the problem of each sorted position (problem-major,
+// the same ranges as the input order); a problem's root is its first
+// internal node and a one-source problem has no internal node.
 __global__ void __launch_bounds__( kBlock )
     leaf_depth_kernel( int32_t const *parent_internal,
-                       int32_t const *parent_leaf, int64_t n, int32_t *height )
+                       int32_t const *parent_leaf, int64_t n,
+                       int32_t const *src_off, int32_t const *src_prob,
+                       int32_t *height )
 {
     int64_t p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
     if ( p >= n )
         return;
+    int32_t root = 0;
+    if ( src_off )
+    {
+        int const pb = src_prob[p];
+        if ( src_off[pb + 1] - src_off[pb] < 2 )
+            return;
+        root = src_off[pb] - pb;
+    }
     int32_t node = parent_leaf[p] >> 1;
     int32_t depth = 1;
-    while ( node != 0 && depth < 4096 )
+    while ( node != root && depth < 4096 )
     {
         node = parent_internal[node] >> 1;
         ++depth;
@@ -465,43 +517,77 @@ __device__ inline double sqrt_rn( double s ) { return __builtin_sqrt( s ); }
 
 constexpr int kQueryBlock = 64;
 
+// Arguments of a query launch. One problem: rows [row0, row0 + n_rows) of
+// the Morton-ordered targets against n_src sources, kk results per row.
+// Batch (src_off set): target t belongs to problem tgt_prob[t], whose
+// sources are [src_off[p], src_off[p + 1]), whose hierarchy is rooted at
+// internal node src_off[p] - p and whose rows hold kk_prob[p] results at
+// CSR offset row_off[r].
+template <typename T>
+struct QueryArgs
+{
+    Node<T> const *nodes = nullptr;
+    typename Vec4<T>::type const *src = nullptr, *tgt = nullptr;
+    int32_t const *order = nullptr; // row -> target
+    int64_t row0 = 0, n_rows = 0;
+    int64_t n_src = 0;
+    int kk = 0;
+    int32_t const *src_off = nullptr, *tgt_prob = nullptr,
+                  *kk_prob = nullptr;
+    int64_t const *row_off = nullptr;
+    int32_t *out_idx = nullptr;
+    double *out_dist = nullptr;
+    // Morton layout: row r written at row r with indices as positions
+    int32_t const *src_pos = nullptr;
+    int2 *gstack = nullptr;
+};
+
 // One lane per query. STACK > 0: LDS stack of that many entries per lane;
-// STACK == 0: per-lane stack in global memory (gstack, depth entries,
-// laid out [level][lane]).
+// STACK == 0: per-lane stack in global memory (gstack, laid out
+// [level][lane]).
 template <typename T, int K, int STACK>
 __global__ void __launch_bounds__( kQueryBlock )
-    knn_query_kernel( Node<T> const *nodes, typename Vec4<T>::type const *src,
-                      typename Vec4<T>::type const *tgt, int32_t const *order, int64_t n_tgt, int64_t n_src,
-                      int kk, int32_t *out_idx, double *out_dist,
-                      int32_t const *src_pos, int2 *gstack, int depth )
+    knn_query_kernel( QueryArgs<T> const a )
 {
     constexpr int LS = STACK > 0 ? STACK : 1;
     __shared__ int32_t st_node[LS * kQueryBlock];
     __shared__ T st_d[LS * kQueryBlock];
-    int64_t qi = blockIdx.x * (int64_t)kQueryBlock + threadIdx.x;
-    if ( qi >= n_tgt )
+    int64_t const qi = blockIdx.x * (int64_t)kQueryBlock + threadIdx.x;
+    if ( qi >= a.n_rows )
         return;
-    int32_t t = order[qi];
-    typename Vec4<T>::type qp = tgt[t];
+    int64_t const r = a.row0 + qi;
+    int32_t const t = a.order[r];
+    int64_t so = 0, np = a.n_src;
+    int32_t root = 0;
+    int kk = a.kk;
+    if ( a.src_off )
+    {
+        int const pb = a.tgt_prob[t];
+        so = a.src_off[pb];
+        np = a.src_off[pb + 1] - so;
+        root = (int32_t)( so - pb );
+        kk = a.kk_prob[pb];
+    }
+    typename Vec4<T>::type qp = a.tgt[t];
     T const q[3] = {qp.x, qp.y, qp.z};
     typename KBest<T, K>::type best;
     best.init();
     T thr = INFINITY;
 
-    if ( n_src == 1 )
+    if ( np == 1 )
     {
         // single source: the tree has no internal node
-        typename Vec4<T>::type sp0 = src[0];
+        typename Vec4<T>::type sp0 = a.src[so];
         T const b[3] = {sp0.x, sp0.y, sp0.z};
-        best.insert( sqrt_rn( box_sq( q, b, b ) ), 0 );
+        best.insert( sqrt_rn( box_sq( q, b, b ) ), (int32_t)so );
     }
     else
     {
         int sp = 0;
-        int32_t node = 0;
+        int32_t node = root;
         while ( node >= 0 )
         {
-            Node<T> const nd = nodes[node];
+            Node<T> const nd = a.nodes[node];
             T s0 = box_sq( q, nd.lo[0], nd.hi[0] );
             T s1 = box_sq( q, nd.lo[1], nd.hi[1] );
             int32_t c0 = nd.child[0], c1 = nd.child[1];
@@ -534,7 +620,7 @@ __global__ void __launch_bounds__( kQueryBlock )
                     int2 e;
                     e.x = far;
                     e.y = __float_as_int( (float)fs );
-                    gstack[(int64_t)sp * n_tgt + qi] = e;
+                    a.gstack[(int64_t)sp * a.n_rows + qi] = e;
                 }
                 ++sp;
             }
@@ -556,7 +642,7 @@ __global__ void __launch_bounds__( kQueryBlock )
                     }
                     else
                     {
-                        int2 e = gstack[(int64_t)sp * n_tgt + qi];
+                        int2 e = a.gstack[(int64_t)sp * a.n_rows + qi];
                         cand = e.x;
                         // the float copy of a double bound is rounded; a
                         // rounded-down copy could prune wrongly, so it is
@@ -574,55 +660,45 @@ __global__ void __launch_bounds__( kQueryBlock )
         }
     }
     // compile-time indices keep the k-best list in registers (a loop bounded
-    // by the runtime kk would index it dynamically and move it to scratch).
-    // Morton layout (src_pos given): row qi, indices as source positions.
-    int64_t const row = ( src_pos ? qi : (int64_t)t ) * kk;
+    // by the runtime kk would index it dynamically and move it to scratch)
+    int64_t const base =
+        a.row_off ? a.row_off[r] : ( a.src_pos ? r : (int64_t)t ) * kk;
 #pragma unroll
     for ( int j = 0; j < K; ++j )
         if ( j < kk )
         {
             int32_t const id = best.index( j );
-            out_idx[row + j] = src_pos ? src_pos[id] : id;
-            if ( out_dist )
-                out_dist[row + j] = best.distance( j );
+            a.out_idx[base + j] = a.src_pos ? a.src_pos[id] : id;
+            if ( a.out_dist )
+                a.out_dist[base + j] = best.distance( j );
         }
 }
 
 template <typename T>
 struct QueryLauncher
 {
-    using V = typename Vec4<T>::type;
     template <int K>
-    static void run( int stack, Node<T> const *nodes, V const *src,
-                     V const *tgt,
-                     int32_t const *order, int64_t n_tgt, int64_t n_src,
-                     int kk, int32_t *idx, double *dist, int32_t const *src_pos,
-                     int2 *gstack, int depth, hipStream_t s )
+    static void run( int stack, QueryArgs<T> const &a, hipStream_t s )
     {
-        dim3 grid( ceil_div( n_tgt, kQueryBlock ) ), block( kQueryBlock );
+        dim3 grid( ceil_div( a.n_rows, kQueryBlock ) ), block( kQueryBlock );
         if ( stack == 32 )
-            knn_query_kernel<T, K, 32><<<grid, block, 0, s>>>(
-                nodes, src, tgt, order, n_tgt, n_src, kk, idx, dist, src_pos, gstack, depth );
+            knn_query_kernel<T, K, 32><<<grid, block, 0, s>>>( a );
         else if ( stack == 64 )
-            knn_query_kernel<T, K, 64><<<grid, block, 0, s>>>(
-                nodes, src, tgt, order, n_tgt, n_src, kk, idx, dist, src_pos, gstack, depth );
+            knn_query_kernel<T, K, 64><<<grid, block, 0, s>>>( a );
         else
-            knn_query_kernel<T, K, 0><<<grid, block, 0, s>>>(
-                nodes, src, tgt, order, n_tgt, n_src, kk, idx, dist, src_pos, gstack, depth );
+            knn_query_kernel<T, K, 0><<<grid, block, 0, s>>>( a );
         DTKX_HIP( hipGetLastError() );
     }
-    static void dispatch( int k, int stack, Node<T> const *nodes,
-                          V const *src, V const *tgt,
-                          int32_t const *order, int64_t n_tgt, int64_t n_src,
-                          int kk, int32_t *idx, double *dist,
-                          int32_t const *src_pos, int2 *gstack, int depth,
+    // k: the largest number of results of a row in this launch
+    static void dispatch( int k, int stack, QueryArgs<T> const &a,
                           hipStream_t s )
     {
+        if ( a.n_rows == 0 )
+            return;
 #define DTKX_KNN_CASE( KV )                                                    \
     if ( k <= KV )                                                             \
     {                                                                          \
-        run<KV>( stack, nodes, src, tgt, order, n_tgt, n_src, kk, idx, dist,        \
-                 src_pos, gstack, depth, s );                                  \
+        run<KV>( stack, a, s );                                                \
         return;                                                                \
     }
         DTKX_KNN_CASE( 1 )
@@ -638,6 +714,22 @@ struct QueryLauncher
         DTKX_REQUIRE( k <= knn_max_k, DTKX_ERR_UNSUPPORTED );
     }
 };
+
+// the K of the query kernel instantiated for up to k results
+inline int query_bucket( int k )
+{
+    int const ks[] = {1, 4, 8, 10, 16, 24, 32, 48, 64};
+    for ( int v : ks )
+        if ( k <= v )
+            return v;
+    return 64;
+}
+
+// traversal stack: LDS for up to 64 levels, else global memory
+inline int stack_kind( int height )
+{
+    return height <= 32 ? 32 : ( height <= 64 ? 64 : 0 );
+}
 
 template <typename T>
 void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
@@ -667,7 +759,7 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
         ids_sorted = ids_sorted_buf.get();
     }
     morton_kernel<T, V><<<ceil_div( n_src, kBlock ), kBlock, 0, s>>>(
-        src, n_src, box.get(), codes.get(), ids.get() );
+        src, n_src, box.get(), codes.get(), ids.get(), nullptr );
     DTKX_HIP( hipGetLastError() );
     std::size_t temp_bytes = 0;
     int64_t const nmax = std::max( n_src, n_tgt );
@@ -700,10 +792,11 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     {
         DTKX_HIP( hipMemsetAsync( height.get(), 0, sizeof( int32_t ), s ) );
         karras_kernel<T, V><<<ceil_div( n_int, kBlock ), kBlock, 0, s>>>(
-            codes_sorted.get(), ids_sorted, src, n_src, box.get(),
+            codes_sorted.get(), ids_sorted, src, n_int, box.get(), nullptr, 1,
             nodes.get(), par_int.get(), par_leaf.get() );
         leaf_depth_kernel<<<ceil_div( n_src, kBlock ), kBlock, 0, s>>>(
-            par_int.get(), par_leaf.get(), n_src, height.get() );
+            par_int.get(), par_leaf.get(), n_src, nullptr, nullptr,
+            height.get() );
         DTKX_HIP( hipGetLastError() );
         DTKX_HIP( hipMemcpyAsync( &h_tree, height.get(), sizeof( int32_t ),
                                   hipMemcpyDeviceToHost, s ) );
@@ -728,7 +821,7 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
         DeviceBuffer<uint64_t> tcodes( n_tgt, s ), tcodes_sorted( n_tgt, s );
         DeviceBuffer<int32_t> tids( n_tgt, s );
         morton_kernel<T, V><<<ceil_div( n_tgt, kBlock ), kBlock, 0, s>>>(
-            tgt, n_tgt, box.get(), tcodes.get(), tids.get() );
+            tgt, n_tgt, box.get(), tcodes.get(), tids.get(), nullptr );
         DTKX_HIP( hipGetLastError() );
         DTKX_HIP( hipcub::DeviceRadixSort::SortPairs(
             temp.get(), temp_bytes, tcodes.get(), tcodes_sorted.get(),
@@ -737,14 +830,24 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
 
     // a descent pushes at most one node per internal level below the root
     int const need = h_tree;
-    int stack = need <= 32 ? 32 : ( need <= 64 ? 64 : 0 );
+    int const stack = stack_kind( need );
     DeviceBuffer<int2> gstack;
     if ( stack == 0 )
         gstack.resize( static_cast<std::size_t>( need + 1 ) * n_tgt, s );
-    QueryLauncher<T>::dispatch( k, stack, nodes.get(), src, tgt, order, n_tgt,
-                                n_src, kk, out.indices, out.dist,
-                                out.morton_layout ? src_pos : nullptr,
-                                gstack.get(), need + 1, s );
+    QueryArgs<T> qa;
+    qa.nodes = nodes.get();
+    qa.src = src;
+    qa.tgt = tgt;
+    qa.order = order;
+    qa.row0 = 0;
+    qa.n_rows = n_tgt;
+    qa.n_src = n_src;
+    qa.kk = kk;
+    qa.out_idx = out.indices;
+    qa.out_dist = out.dist;
+    qa.src_pos = out.morton_layout ? src_pos : nullptr;
+    qa.gstack = gstack.get();
+    QueryLauncher<T>::dispatch( k, stack, qa, s );
     if ( info )
     {
         info->tree_height = h_tree;
@@ -753,7 +856,234 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     // buffers are released at scope exit, ordered on the stream
 }
 
+
+// ------------------------------------------------------------- batched
+template <typename T, typename V>
+__global__ void __launch_bounds__( kBlock )
+    bbox_segments( V const *pts, int32_t const *src_off, T *boxes )
+{
+    int const pb = blockIdx.x;
+    int64_t const b = src_off[pb], e = src_off[pb + 1];
+    T lo[3] = {INFINITY, INFINITY, INFINITY};
+    T hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for ( int64_t i = b + threadIdx.x; i < e; i += kBlock )
+    {
+        V p = pts[i];
+        lo[0] = fmin( lo[0], p.x );
+        lo[1] = fmin( lo[1], p.y );
+        lo[2] = fmin( lo[2], p.z );
+        hi[0] = fmax( hi[0], p.x );
+        hi[1] = fmax( hi[1], p.y );
+        hi[2] = fmax( hi[2], p.z );
+    }
+    __shared__ T sh[6][kBlock];
+    for ( int d = 0; d < 3; ++d )
+    {
+        sh[d][threadIdx.x] = lo[d];
+        sh[3 + d][threadIdx.x] = hi[d];
+    }
+    __syncthreads();
+    for ( int off = kBlock / 2; off > 0; off >>= 1 )
+    {
+        if ( threadIdx.x < off )
+            for ( int d = 0; d < 3; ++d )
+            {
+                sh[d][threadIdx.x] =
+                    fmin( sh[d][threadIdx.x], sh[d][threadIdx.x + off] );
+                sh[3 + d][threadIdx.x] = fmax(
+                    sh[3 + d][threadIdx.x], sh[3 + d][threadIdx.x + off] );
+            }
+        __syncthreads();
+    }
+    if ( threadIdx.x == 0 )
+        for ( int d = 0; d < 6; ++d )
+            boxes[pb * 6 + d] = sh[d][0];
+}
+
+// keys[i] = table[prob[ids[i]]] (table null: prob[ids[i]])
+__global__ void __launch_bounds__( kBlock )
+    group_key_kernel( int32_t const *ids, int32_t const *prob,
+                      int32_t const *table, int64_t n, int32_t *keys )
+{
+    int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( i >= n )
+        return;
+    int32_t const pb = prob[ids[i]];
+    keys[i] = table ? table[pb] : pb;
+}
+
+__global__ void __launch_bounds__( kBlock )
+    gather_codes_kernel( uint64_t const *codes, int32_t const *ids, int64_t n,
+                         uint64_t *out )
+{
+    int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( i < n )
+        out[i] = codes[ids[i]];
+}
+
+__global__ void __launch_bounds__( kBlock )
+    row_offsets_kernel( int32_t const *order, int32_t const *tgt_prob,
+                        int64_t const *row_base, int32_t const *row_start,
+                        int32_t const *kk_prob, int64_t n, int64_t nnz,
+                        int64_t *row_off )
+{
+    int64_t r = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( r > n )
+        return;
+    if ( r == n )
+    {
+        row_off[n] = nnz;
+        return;
+    }
+    int32_t const pb = tgt_prob[order[r]];
+    row_off[r] = row_base[pb] + ( r - row_start[pb] ) * (int64_t)kk_prob[pb];
+}
+
+// Sort (codes, identity) by code, then stably by group key: the result is
+// ordered by (key, code). ids_out receives the permutation.
+template <typename T, typename V>
+void sort_by_group( V const *pts, int64_t n, T const *boxes,
+                    int32_t const *prob, int32_t const *table, int key_bits,
+                    DeviceBuffer<uint64_t> &codes, int32_t *ids_out,
+                    hipStream_t s )
+{
+    codes.resize( n, s );
+    DeviceBuffer<uint64_t> codes_sorted( n, s );
+    DeviceBuffer<int32_t> ids( n, s ), ids1( n, s ), keys( n, s ),
+        keys_sorted( n, s );
+    morton_kernel<T, V><<<ceil_div( n, kBlock ), kBlock, 0, s>>>(
+        pts, n, boxes, codes.get(), ids.get(), prob );
+    DTKX_HIP( hipGetLastError() );
+    std::size_t tb1 = 0, tb2 = 0;
+    DTKX_HIP( hipcub::DeviceRadixSort::SortPairs(
+        nullptr, tb1, codes.get(), codes_sorted.get(), ids.get(), ids1.get(),
+        static_cast<int>( n ), 0, 63, s ) );
+    DTKX_HIP( hipcub::DeviceRadixSort::SortPairs(
+        nullptr, tb2, keys.get(), keys_sorted.get(), ids1.get(), ids_out,
+        static_cast<int>( n ), 0, key_bits, s ) );
+    DeviceBuffer<char> temp( std::max( tb1, tb2 ), s );
+    DTKX_HIP( hipcub::DeviceRadixSort::SortPairs(
+        temp.get(), tb1, codes.get(), codes_sorted.get(), ids.get(),
+        ids1.get(), static_cast<int>( n ), 0, 63, s ) );
+    group_key_kernel<<<ceil_div( n, kBlock ), kBlock, 0, s>>>(
+        ids1.get(), prob, table, n, keys.get() );
+    DTKX_HIP( hipGetLastError() );
+    DTKX_HIP( hipcub::DeviceRadixSort::SortPairs(
+        temp.get(), tb2, keys.get(), keys_sorted.get(), ids1.get(), ids_out,
+        static_cast<int>( n ), 0, key_bits, s ) );
+}
+
+template <typename T>
+void knn_batched_impl( void const *src_v, int64_t n_src, void const *tgt_v,
+                       int64_t n_tgt, KnnBatch const &b,
+                       KnnBatchOutput const &out, KnnInfo *info,
+                       hipStream_t s )
+{
+    using V = typename Vec4<T>::type;
+    V const *src = static_cast<V const *>( src_v );
+    V const *tgt = static_cast<V const *>( tgt_v );
+    int const P = b.n_problems;
+    int prob_bits = 1;
+    while ( ( int64_t( 1 ) << prob_bits ) < P )
+        ++prob_bits;
+
+    DeviceBuffer<T> boxes( 6 * (std::size_t)P, s );
+    bbox_segments<T, V><<<P, kBlock, 0, s>>>( src, b.src_off, boxes.get() );
+    DTKX_HIP( hipGetLastError() );
+
+    // sources: problem-major, Morton order inside each problem
+    DeviceBuffer<uint64_t> codes;
+    sort_by_group<T, V>( src, n_src, boxes.get(), b.src_prob, nullptr,
+                         prob_bits, codes, out.src_perm, s );
+    DeviceBuffer<uint64_t> codes_sorted( n_src, s );
+    gather_codes_kernel<<<ceil_div( n_src, kBlock ), kBlock, 0, s>>>(
+        codes.get(), out.src_perm, n_src, codes_sorted.get() );
+    DTKX_HIP( hipGetLastError() );
+    codes.release();
+    invert_permutation( out.src_perm, n_src, out.src_pos, s );
+
+    // one hierarchy per problem, all in one launch
+    int64_t const n_int = n_src - P;
+    DeviceBuffer<Node<T>> nodes( std::max<int64_t>( n_int, 1 ), s );
+    DeviceBuffer<int32_t> par_int( std::max<int64_t>( n_int, 1 ), s ),
+        par_leaf( n_src, s ), height( 1, s );
+    int h_tree = 0;
+    if ( n_int > 0 )
+    {
+        DTKX_HIP( hipMemsetAsync( height.get(), 0, sizeof( int32_t ), s ) );
+        karras_kernel<T, V><<<ceil_div( n_int, kBlock ), kBlock, 0, s>>>(
+            codes_sorted.get(), out.src_perm, src, n_int, boxes.get(),
+            b.src_off, P, nodes.get(), par_int.get(), par_leaf.get() );
+        leaf_depth_kernel<<<ceil_div( n_src, kBlock ), kBlock, 0, s>>>(
+            par_int.get(), par_leaf.get(), n_src, b.src_off, b.src_prob,
+            height.get() );
+        DTKX_HIP( hipGetLastError() );
+        DTKX_HIP( hipMemcpyAsync( &h_tree, height.get(), sizeof( int32_t ),
+                                  hipMemcpyDeviceToHost, s ) );
+        DTKX_HIP( hipStreamSynchronize( s ) );
+    }
+    codes_sorted.release();
+    if ( info )
+        info->tree_height = h_tree;
+    if ( n_tgt == 0 )
+        return;
+
+    // targets: rows ordered by (row key of the problem, Morton code)
+    {
+        DeviceBuffer<uint64_t> tcodes;
+        sort_by_group<T, V>( tgt, n_tgt, boxes.get(), b.tgt_prob, b.row_key,
+                             b.key_bits, tcodes, out.order, s );
+    }
+    row_offsets_kernel<<<ceil_div( n_tgt + 1, kBlock ), kBlock, 0, s>>>(
+        out.order, b.tgt_prob, b.row_base, b.row_start, b.kk_prob, n_tgt,
+        b.nnz, out.row_off );
+    DTKX_HIP( hipGetLastError() );
+
+    int const stack = stack_kind( h_tree );
+    int64_t max_rows = 0;
+    for ( auto const &l : b.launches )
+        max_rows = std::max( max_rows, l.end - l.begin );
+    DeviceBuffer<int2> gstack;
+    if ( stack == 0 )
+        gstack.resize( static_cast<std::size_t>( h_tree + 1 ) * max_rows, s );
+    for ( auto const &l : b.launches )
+    {
+        QueryArgs<T> qa;
+        qa.nodes = nodes.get();
+        qa.src = src;
+        qa.tgt = tgt;
+        qa.order = out.order;
+        qa.row0 = l.begin;
+        qa.n_rows = l.end - l.begin;
+        qa.src_off = b.src_off;
+        qa.tgt_prob = b.tgt_prob;
+        qa.kk_prob = b.kk_prob;
+        qa.row_off = out.row_off;
+        qa.out_idx = out.indices;
+        qa.src_pos = out.src_pos;
+        qa.gstack = gstack.get();
+        QueryLauncher<T>::dispatch( l.kmax, stack, qa, s );
+    }
+    if ( info )
+        info->stack_used = stack;
+}
+
 } // namespace
+
+void knn_search_batched( int geometry, void const *src, int64_t n_src,
+                         void const *tgt, int64_t n_tgt, KnnBatch const &b,
+                         KnnBatchOutput const &out, KnnInfo *info,
+                         hipStream_t s )
+{
+    DTKX_REQUIRE( b.n_problems >= 1, DTKX_ERR_INVALID_ARGUMENT );
+    DTKX_REQUIRE( n_src >= b.n_problems, DTKX_ERR_EMPTY_SOURCE );
+    DTKX_REQUIRE( n_src < INT32_MAX && n_tgt < INT32_MAX,
+                  DTKX_ERR_UNSUPPORTED );
+    if ( geometry == DTKX_GEOMETRY_FLOAT )
+        knn_batched_impl<float>( src, n_src, tgt, n_tgt, b, out, info, s );
+    else
+        knn_batched_impl<double>( src, n_src, tgt, n_tgt, b, out, info, s );
+}
 
 void knn_search( int geometry, void const *src, int64_t n_src,
                  void const *tgt, int64_t n_tgt, int k, KnnOutput const &out,

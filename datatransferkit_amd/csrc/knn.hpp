@@ -12,6 +12,8 @@
 
 #include "common.hpp"
 
+#include <vector>
+
 namespace dtkx
 {
 
@@ -44,6 +46,52 @@ struct KnnOutput
 void knn_search( int geometry, void const *src, int64_t n_src,
                  void const *tgt, int64_t n_tgt, int k, KnnOutput const &out,
                  KnnInfo *info, hipStream_t s );
+
+// Many independent problems in one build and one query pass per launch
+// group. Problem p owns sources [src_off[p], src_off[p + 1]) and targets
+// [tgt_off[p], tgt_off[p + 1]) of the concatenated arrays (every problem has
+// at least one source). Each problem's hierarchy is built over its own
+// sorted range inside one set of kernels, and each query starts at its
+// problem's root. Output rows are ordered by (row_key[problem], Morton code
+// of the target in its problem's box); row r holds kk_prob[p] neighbours at
+// CSR offset row_off[r] = row_base[p] + (r - row_start[p]) kk_prob[p], as
+// global source positions (the sorted order is problem-major, so problem p's
+// positions are [src_off[p], src_off[p + 1]) as well). Ties are broken by
+// the source index, so each row equals the one-problem search.
+struct KnnBatch
+{
+    int n_problems = 0;
+    int32_t const *src_off = nullptr;   // device [P + 1]
+    int32_t const *src_prob = nullptr;  // device [n_src]
+    int32_t const *tgt_prob = nullptr;  // device [n_tgt]
+    int32_t const *row_key = nullptr;   // device [P]
+    int key_bits = 1;                   // row_key values < 2^key_bits
+    int32_t const *kk_prob = nullptr;   // device [P], min(k_p, n_src_p)
+    int64_t const *row_base = nullptr;  // device [P]
+    int32_t const *row_start = nullptr; // device [P]
+    int64_t nnz = 0;                    // row_off[n_tgt]
+    // query launches over row ranges: {row_begin, row_end, max k in range}
+    struct Launch
+    {
+        int64_t begin, end;
+        int kmax;
+    };
+    std::vector<Launch> launches;
+};
+
+struct KnnBatchOutput
+{
+    int32_t *order = nullptr;    // [n_tgt] row -> target
+    int32_t *src_perm = nullptr; // [n_src] position -> source
+    int32_t *src_pos = nullptr;  // [n_src] source -> position
+    int64_t *row_off = nullptr;  // [n_tgt + 1]
+    int32_t *indices = nullptr;  // [nnz]
+};
+
+void knn_search_batched( int geometry, void const *src, int64_t n_src,
+                         void const *tgt, int64_t n_tgt, KnnBatch const &b,
+                         KnnBatchOutput const &out, KnnInfo *info,
+                         hipStream_t s );
 
 // Largest k the query kernels are instantiated for.
 constexpr int knn_max_k = 64;

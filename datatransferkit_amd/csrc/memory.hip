@@ -123,6 +123,22 @@ void cache_free( void *p, std::size_t bytes, hipStream_t s )
     c.free_lists[{dev, s}].emplace( size, p );
 }
 
+hipStream_t library_stream()
+{
+    static std::mutex mu;
+    static std::map<int, hipStream_t> streams;
+    int dev = 0;
+    DTKX_HIP( hipGetDevice( &dev ) );
+    std::lock_guard<std::mutex> lock( mu );
+    auto it = streams.find( dev );
+    if ( it != streams.end() )
+        return it->second;
+    hipStream_t st = nullptr;
+    DTKX_HIP( hipStreamCreate( &st ) );
+    streams[dev] = st;
+    return st;
+}
+
 void cache_release()
 {
     int dev = 0;

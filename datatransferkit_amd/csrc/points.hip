@@ -96,7 +96,59 @@ __global__ void __launch_bounds__( kBlock )
         inv[perm[i]] = (int32_t)i;
 }
 
+__global__ void __launch_bounds__( kBlock )
+    problem_ids_kernel( int32_t const *off, int n_prob, int64_t n,
+                        int32_t *prob )
+{
+    int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( i >= n )
+        return;
+    int lo = 0, hi = n_prob - 1; // largest p with off[p] <= i
+    while ( lo < hi )
+    {
+        int const mid = ( lo + hi + 1 ) >> 1;
+        if ( off[mid] <= i )
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    prob[i] = lo;
+}
+
+__global__ void __launch_bounds__( kBlock )
+    concat_points_kernel( double const *const *ptrs, int32_t const *off,
+                          int32_t const *prob, int64_t n, double4 *out )
+{
+    int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( i >= n )
+        return;
+    int32_t const q = prob[i];
+    double const *p = ptrs[q] + 3 * ( i - off[q] );
+    out[i] = double4{p[0], p[1], p[2], 0.0};
+}
+
 } // namespace
+
+void problem_ids( int32_t const *off, int n_prob, int64_t n, int32_t *prob,
+                  hipStream_t s )
+{
+    if ( n == 0 )
+        return;
+    problem_ids_kernel<<<ceil_div( n, kBlock ), kBlock, 0, s>>>( off, n_prob,
+                                                                 n, prob );
+    DTKX_HIP( hipGetLastError() );
+}
+
+void concat_points( double const *const *ptrs, int32_t const *off,
+                    int32_t const *prob, int64_t n, double4 *out,
+                    hipStream_t s )
+{
+    if ( n == 0 )
+        return;
+    concat_points_kernel<<<ceil_div( n, kBlock ), kBlock, 0, s>>>(
+        ptrs, off, prob, n, out );
+    DTKX_HIP( hipGetLastError() );
+}
 
 void gather_points( double4 const *in, int32_t const *perm, int64_t n,
                     double4 *out, hipStream_t s )

@@ -23,6 +23,10 @@ struct SetupParams
     double rcond = 0.0;
     long max_rot = 0;
     int group_size = 0; // classical solver: lanes per system, 0 = choose
+    // rows [row0, n_tgt) are set up; row_off (nullable) gives CSR offsets
+    // of the rows' neighbour lists (kk per row when null)
+    int64_t row0 = 0;
+    int64_t const *row_off = nullptr;
 };
 
 // counters[0] = n_underdetermined, [1] = n_unconverged, [2] = rotations

@@ -79,7 +79,9 @@ __global__ void __launch_bounds__( kWave )
     setup_classical_kernel( double4 const *__restrict__ src,
                             double4 const *__restrict__ tgt, int64_t n_tgt,
                             int32_t const *__restrict__ indices,
-                            int32_t const *__restrict__ order, int kk, int rbf,
+                            int32_t const *__restrict__ order,
+                            int64_t row0, int64_t const *__restrict__ row_off,
+                            int kk_uniform, int rbf,
                             int geometry, int truncation, double rcond,
                             long max_rot, double *__restrict__ coeffs,
                             unsigned long long *__restrict__ counters )
@@ -92,11 +94,26 @@ __global__ void __launch_bounds__( kWave )
     extern __shared__ double lds[];
     int const sys = threadIdx.x / G, lane = threadIdx.x % G;
     int const gbase = threadIdx.x - lane; // first lane of the group
-    int64_t const slot = blockIdx.x * (int64_t)S + sys;
+    int64_t const slot = row0 + blockIdx.x * (int64_t)S + sys;
     bool const valid = slot < n_tgt;
     int64_t const t = valid ? ( order ? order[slot] : slot ) : 0;
-    int32_t const *nbr = indices + slot * kk;
+    int64_t const rbeg =
+        valid ? ( row_off ? row_off[slot] : slot * kk_uniform ) : 0;
+    int const kk =
+        valid ? ( row_off ? (int)( row_off[slot + 1] - rbeg ) : kk_uniform )
+              : 0;
+    int32_t const *nbr = indices + rbeg;
     double4 const tp = valid ? tgt[t] : double4{0, 0, 0, 0};
+    // the staged moment loop has barriers: its trip count is the block's
+    // largest row length (rows of one launch may differ in a batch)
+    __shared__ int kk_block;
+    if ( threadIdx.x == 0 )
+        kk_block = 0;
+    __syncthreads();
+    if ( lane == 0 && valid )
+        atomicMax( &kk_block, kk );
+    __syncthreads();
+    int const kk_max = kk_block;
 
     double *E = lds + sys * per_system( M );
     double *W = E + MM;
@@ -126,9 +143,10 @@ __global__ void __launch_bounds__( kWave )
     if ( valid )
         for ( int e = lane; e < MM; e += G )
             E[e] = 0.0;
-    for ( int l0 = 0; l0 < kk; l0 += kChunk )
+    for ( int l0 = 0; l0 < kk_max; l0 += kChunk )
     {
-        int const nl = kk - l0 < kChunk ? kk - l0 : kChunk;
+        int const nl =
+            kk - l0 < kChunk ? ( kk - l0 > 0 ? kk - l0 : 0 ) : kChunk;
         if ( valid )
             for ( int l = lane; l < nl; l += G )
             {
@@ -364,7 +382,7 @@ __global__ void __launch_bounds__( kWave )
 #pragma unroll
             for ( int j = 0; j < M; j++ )
                 c += a0[j] * Pl[j] * phi;
-            coeffs[slot * kk + l] = c;
+            coeffs[rbeg + l] = c;
         }
         if ( lane == 0 && counters )
         {
@@ -391,9 +409,10 @@ void launch( double4 const *src, double4 const *tgt, int64_t n_tgt,
     constexpr int S = kWave / G;
     std::size_t const lds = lds_bytes( M, G );
     DTKX_REQUIRE( lds <= 64 * 1024, DTKX_ERR_UNSUPPORTED );
-    setup_classical_kernel<M, G><<<ceil_div( n_tgt, S ), kWave, lds, s>>>(
-        src, tgt, n_tgt, indices, order, p.kk, p.rbf, p.geometry, p.truncation,
-        p.rcond, p.max_rot, coeffs, counters );
+    setup_classical_kernel<M, G>
+        <<<ceil_div( n_tgt - p.row0, S ), kWave, lds, s>>>(
+            src, tgt, n_tgt, indices, order, p.row0, p.row_off, p.kk, p.rbf,
+            p.geometry, p.truncation, p.rcond, p.max_rot, coeffs, counters );
     DTKX_HIP( hipGetLastError() );
 }
 
@@ -412,7 +431,7 @@ int setup_classical( double4 const *src, double4 const *tgt, int64_t n_tgt,
                      SetupParams const &p, double *coeffs,
                      unsigned long long *counters, hipStream_t s )
 {
-    if ( n_tgt == 0 )
+    if ( n_tgt <= p.row0 )
         return 0;
     int g = p.group_size > 0 ? p.group_size : setup_classical_group( p.m, p.kk );
     switch ( p.m )

@@ -10,8 +10,9 @@
 // V stay in registers: no LDS, no barriers, no divergence except the
 // per-wave sweep count. Rotations use the Rutishauser form (t = sgn(theta)
 // / (|theta| + sqrt(theta^2 + 1)), c = 1/sqrt(1 + t^2), tau = s/(1 + c)).
-// Sweeps stop when every lane's off-diagonal mass is below
-// (DBL_EPSILON/8)^2 of its diagonal mass, or at the sweep cap.
+// Each system sweeps until its off-diagonal mass is below (DBL_EPSILON/8)^2
+// of its diagonal mass, or to the sweep cap (a converged system idles while
+// the rest of its wave finishes, so its bits do not depend on its wave).
 // Truncation keeps the reference's semantics: eigenvalues with |l| >=
 // DBL_EPSILON (or rcond * max|l|) contribute.
 // Neighbour data is re-gathered from L2 in three passes (radius, moments,
@@ -38,7 +39,9 @@ __global__ void __launch_bounds__( kBlock )
     setup_cyclic_kernel( double4 const *__restrict__ src,
                          double4 const *__restrict__ tgt, int64_t n_tgt,
                          int32_t const *__restrict__ indices,
-                         int32_t const *__restrict__ order, int kk, int rbf,
+                         int32_t const *__restrict__ order, int64_t row0,
+                         int64_t const *__restrict__ row_off, int kk_uniform,
+                         int rbf,
                          int geometry, int truncation, double rcond,
                          int max_sweeps, double *__restrict__ coeffs,
                          float *__restrict__ coeffs32,
@@ -49,10 +52,15 @@ __global__ void __launch_bounds__( kBlock )
     constexpr int R = M / H;
     constexpr int SPB = kBlock / H; // systems per block
     int const h = threadIdx.x % H;
-    int64_t const slot = blockIdx.x * (int64_t)SPB + threadIdx.x / H;
+    int64_t const slot = row0 + blockIdx.x * (int64_t)SPB + threadIdx.x / H;
     bool const valid = slot < n_tgt;
     int64_t const t = valid ? ( order ? order[slot] : slot ) : 0;
-    int32_t const *nbr = indices + slot * kk;
+    int64_t const rbeg =
+        valid ? ( row_off ? row_off[slot] : slot * kk_uniform ) : 0;
+    int const kk =
+        valid ? ( row_off ? (int)( row_off[slot + 1] - rbeg ) : kk_uniform )
+              : 0;
+    int32_t const *nbr = indices + rbeg;
     double4 const tp = valid ? tgt[t] : double4{0, 0, 0, 0};
 
     // radius (Impl.hpp:118-134): the largest neighbour distance
@@ -91,12 +99,14 @@ __global__ void __launch_bounds__( kBlock )
 #pragma unroll
     for ( int e = 0; e < R * M; ++e )
         V[e] = ( h * R + e / M == e % M ) ? 1.0 : 0.0;
+    // each system stops at its own convergence, so its result does not
+    // depend on which systems share its wave
     int sweeps = 0;
     bool done = !valid || converged<M>( A );
-    while ( !__all( done ) && sweeps < max_sweeps )
+    for ( int pass = 0; !__all( done ) && pass < max_sweeps; ++pass )
     {
-        sweep_all<M, R>( A, V );
-        ++sweeps;
+        sweep_all<M, R>( A, V, done );
+        sweeps += done ? 0 : 1;
         done = done || converged<M>( A );
     }
 
@@ -207,7 +217,7 @@ __global__ void __launch_bounds__( kBlock )
 #pragma unroll
             for ( int j = 0; j < M; ++j )
                 acc = fma( x[j], P[j], acc );
-            coeffs32[slot * kk + l] = (float)( acc * phi );
+            coeffs32[rbeg + l] = (float)( acc * phi );
         }
         else
         {
@@ -215,7 +225,7 @@ __global__ void __launch_bounds__( kBlock )
 #pragma unroll
             for ( int j = 0; j < M; ++j )
                 acc = fma( x[j], P[j], acc );
-            coeffs[slot * kk + l] = acc * phi;
+            coeffs[rbeg + l] = acc * phi;
         }
     }
     if ( counters && h == 0 )
@@ -235,10 +245,11 @@ void launch_cyclic( double4 const *src, double4 const *tgt, int64_t n_tgt,
                     double *pinv, unsigned long long *counters, hipStream_t s )
 {
     int const sweeps = p.max_rot > 0 ? static_cast<int>( p.max_rot ) : 30;
-    dim3 const grid( ceil_div( n_tgt, kBlock / H ) ), block( kBlock );
+    dim3 const grid( ceil_div( n_tgt - p.row0, kBlock / H ) ), block( kBlock );
     auto run = [&]( auto f32, auto store ) {
         setup_cyclic_kernel<M, H, decltype( f32 )::value, decltype( store )::value>
-            <<<grid, block, 0, s>>>( src, tgt, n_tgt, indices, order, p.kk,
+            <<<grid, block, 0, s>>>( src, tgt, n_tgt, indices, order, p.row0,
+                                     p.row_off, p.kk,
                                      p.rbf, p.geometry, p.truncation, p.rcond,
                                      sweeps, coeffs, coeffs32, pinv, counters );
     };
@@ -258,7 +269,7 @@ void setup_cyclic( double4 const *src, double4 const *tgt, int64_t n_tgt,
                    SetupParams const &p, double *coeffs, float *coeffs32,
                    double *pinv, unsigned long long *counters, hipStream_t s )
 {
-    if ( n_tgt == 0 )
+    if ( n_tgt <= p.row0 )
         return;
     switch ( p.m )
     {

@@ -200,6 +200,83 @@ class MovingLeastSquaresOperator:
         return offsets, indices, coeffs
 
 
+class MovingLeastSquaresBatch:
+    """Many independent MLS operators built together: one segmented LBVH
+    build and one query launch per kernel size for all problems, one setup
+    launch per basis, and an apply replayed from a captured HIP graph.
+
+    problems: sequence of (source_points [n_s, 3], target_points [n_t, 3],
+    basis, k) with float64 CUDA tensors; options as for
+    MovingLeastSquaresOperator (group_size 0 lets the classical solver pick
+    its lanes per system by timing)."""
+
+    def __init__(self, problems, *, rbf="wendland0", precision="fp64",
+                 geometry="float", solver="classical", truncation="absolute",
+                 rcond=1e-12, max_rotations=0, group_size=0, stream=None):
+        o = _lib.default_options()
+        o.basis = _lib.BASIS["linear"]
+        o.rbf = _lib.RBF[rbf]
+        o.precision = _lib.PRECISION[precision]
+        o.geometry = _lib.GEOMETRY[geometry]
+        o.solver = _lib.SOLVER[solver]
+        o.truncation = _lib.TRUNCATION[truncation]
+        o.rcond = float(rcond)
+        o.max_rotations = int(max_rotations)
+        o.group_size = int(group_size)
+        o.stream = ctypes.c_void_p(stream) if stream else None
+        n = len(problems)
+        arr = (_lib.Problem * n)()
+        self._keep = []
+        self._sizes = []
+        for i, (src, tgt, basis, k) in enumerate(problems):
+            ns, _ = _check_points(src, "source_points")
+            nt, _ = _check_points(tgt, "target_points")
+            sp, ka = _ptr(src)
+            tp, kb = _ptr(tgt)
+            self._keep += [src, tgt]
+            arr[i].src_xyz, arr[i].n_src = sp.value, ns
+            arr[i].tgt_xyz, arr[i].n_tgt = tp.value, nt
+            arr[i].basis, arr[i].k = _lib.BASIS[basis], int(k)
+            self._sizes.append((ns, nt))
+        self.precision = precision
+        h = ctypes.c_void_p()
+        check(_lib.load().dtkx_mls_batch_create(n, arr, ctypes.byref(o), ctypes.byref(h)))
+        self._h = h
+        self._keep = []  # points are copied at create
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.load().dtkx_mls_batch_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def apply(self, source_values, target_values):
+        """One field per problem: target_values[p] = C_p source_values[p]
+        (float64 CUDA tensors of the problems' sizes)."""
+        n = len(self._sizes)
+        if len(source_values) != n or len(target_values) != n:
+            raise DtkxError(_lib.ERR_SIZE_MISMATCH, "one array per problem")
+        xs = (ctypes.c_void_p * n)()
+        ys = (ctypes.c_void_p * n)()
+        for i, ((ns, nt), x, y) in enumerate(zip(self._sizes, source_values, target_values)):
+            if x.shape[-1] != ns or y.shape[-1] != nt:
+                raise DtkxError(_lib.ERR_SIZE_MISMATCH, f"problem {i}: value sizes")
+            if _dtype_name(x) != "float64" or _dtype_name(y) != "float64":
+                raise DtkxError(_lib.ERR_INVALID_ARGUMENT, "values must be float64")
+            xs[i] = x.data_ptr()
+            ys[i] = y.data_ptr()
+        check(_lib.load().dtkx_mls_batch_apply(self._h, xs, ys))
+        return target_values
+
+    def stats(self) -> dict:
+        s = _lib.Stats()
+        check(_lib.load().dtkx_mls_batch_get_stats(self._h, ctypes.byref(s)))
+        return s.as_dict()
+
+
 def _memcpy_d2d(dst_tensor, src_ptr, nbytes):
     import torch
 

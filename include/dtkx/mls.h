@@ -257,11 +257,22 @@ typedef struct dtkx_mls_problem {
 
 typedef struct dtkx_mls_batch dtkx_mls_batch;
 
-/* Build many independent operators (rbf/precision/solver from opts). kNN
- * runs per problem; setup of all problems is one persistent launch. */
+/* Build many independent operators, each with its own basis and k (other
+ * options from opts; store_pinv is not supported). Every problem needs at
+ * least one source; points are device pointers and are copied. All trees
+ * are built by one set of kernels over the concatenated, problem-sorted
+ * sources; queries run in one launch per neighbour-list size and the setup
+ * in one launch per basis. With the classical solver and group_size 0 the
+ * lanes per system are chosen by timing a sample of each basis' rows.
+ * Each problem's result is bit-identical to a dtkx_mls_create of that
+ * problem with the same options. No counterpart in the reference (one
+ * operator per MovingLeastSquaresOperator construction). */
 int dtkx_mls_batch_create(int32_t n_problems, const dtkx_mls_problem *problems,
                           const dtkx_mls_options *opts, dtkx_mls_batch **out);
-/* src_vals[p] / tgt_vals[p]: device pointers per problem (one field). */
+/* One field per problem: src_vals[p] (n_src_p doubles) and tgt_vals[p]
+ * (n_tgt_p doubles) are device pointers in host arrays of n_problems
+ * entries. The gather and row sums of all problems are captured in a HIP
+ * graph on the first call and replayed; synchronous at return. */
 int dtkx_mls_batch_apply(const dtkx_mls_batch *b,
                          const double *const *src_vals,
                          double *const *tgt_vals);
