@@ -273,6 +273,156 @@ def bench_batched(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_adjoint(args):
+    """Forward (create + apply of 8 fields) and backward (transpose apply and
+    target-coordinate gradient for an upstream N(0,1) gradient) of the MLS
+    transfer, with the per-target pseudo-inverse recomputed in the backward
+    pass or stored by the forward one. 1M sources, 500k targets, quadratic,
+    k = 16, Wendland C2, FP64, double geometry."""
+    import numpy as np
+    import torch
+
+    import datatransferkit_amd as dtk
+
+    n_src, n_tgt, nf = args.n, args.n // 2, 8
+    rng = np.random.default_rng(11)
+    src = rng.random((n_src, 3))
+    tgt = 0.05 + 0.9 * rng.random((n_tgt, 3))
+    x0, y0, z0 = src[:, 0], src[:, 1], src[:, 2]
+    vals = np.stack([np.sin(2 * np.pi * (f + 1) * x0 / 4) * np.cos(3 * y0) + z0 * f
+                     for f in range(nf)])
+    d_src, d_tgt = torch.from_numpy(src).cuda(), torch.from_numpy(tgt).cuda()
+    x = torch.from_numpy(vals).cuda()
+    ybar = torch.from_numpy(rng.standard_normal((nf, n_tgt))).cuda()
+    y = torch.zeros((nf, n_tgt), dtype=torch.float64, device="cuda")
+    xbar = torch.zeros_like(x)
+    stream = torch.cuda.current_stream().cuda_stream
+    kw = dict(basis="quadratic", rbf="wendland2", k=16, solver="cyclic",
+              geometry="double", stream=stream)
+    results, grads = {}, {}
+    for mode in ("recompute", "store"):
+        def step():
+            op = dtk.MovingLeastSquaresOperator(d_src, d_tgt, store_pinv=(mode == "store"), **kw)
+            op.apply(x, y)
+            op.apply_transpose(ybar, xbar)
+            g = op.target_gradient(x, ybar)
+            del op
+            return g
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            g = step()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        results[mode] = {"value": n_tgt / dt, "ms_per_step": 1000 * dt}
+        grads[mode] = g.cpu().numpy()
+    out = {
+        "metric": "MLS forward+backward target pts/s (adjoint, pinv recompute vs store)",
+        "value": results["recompute"]["value"],
+        "unit": "target pts/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": results["recompute"]["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"MLS adjoint {n_src} sources -> {n_tgt} targets, quadratic, "
+                               f"k=16, wendland2, {nf} fields, FP64 double geometry, "
+                               f"cyclic Jacobi",
+                   "n_src": n_src, "n_tgt": n_tgt, "fields": nf},
+        "modes": results,
+        "store_vs_recompute_rel": float(np.abs(grads["store"] - grads["recompute"]).max()
+                                        / np.abs(grads["recompute"]).max()),
+        "fd_check": "central differences at h = 1e-6 in extended precision on a "
+                    "sample at this density: tests/test_gpu_adjoint.py",
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_shell(args):
+    """Mixed precision on an ill-conditioned cloud: sources on a thin
+    spherical shell (radius 1, thickness 1e-3), targets on the unit sphere
+    jittered +-2e-4 radially; quadratic, k = 32, Wendland C4, relative
+    truncation at rcond = 1e-12. The FP32-bulk form against the all-FP64
+    form (classical Jacobi) of the same GPU, per field."""
+    import numpy as np
+    import torch
+
+    import datatransferkit_amd as dtk
+
+    n = args.n
+    rng = np.random.default_rng(21)
+
+    def sphere(m):
+        v = rng.standard_normal((m, 3))
+        return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+    src = sphere(n) * (1 + rng.uniform(-5e-4, 5e-4, (n, 1)))
+    tgt = sphere(n) * (1 + rng.uniform(-2e-4, 2e-4, (n, 1)))
+    x, y, z = src[:, 0], src[:, 1], src[:, 2]
+    vals = np.stack([x, y * z, x * x - y * y, np.sin(3 * x) * np.cos(2 * y) + z])
+    d_src, d_tgt = torch.from_numpy(src).cuda(), torch.from_numpy(tgt).cuda()
+    stream = torch.cuda.current_stream().cuda_stream
+    out_forms = {}
+    outs = {}
+    for name, form in (("fp32_bulk", FORMS["fp32_bulk"]), ("fp64", FORMS["fp64"])):
+        vdt = torch.float32 if form["precision"] == "fp32_bulk" else torch.float64
+        xv = torch.from_numpy(vals).cuda().to(vdt)
+        yv = torch.zeros((4, n), dtype=vdt, device="cuda")
+        kw = dict(basis="quadratic", rbf="wendland4", k=32, truncation="relative",
+                  rcond=1e-12, stream=stream, **form)
+
+        def step():
+            op = dtk.MovingLeastSquaresOperator(d_src, d_tgt, **kw)
+            op.apply(xv, yv)
+            st = op.stats()
+            del op
+            return st
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            st = step()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        out_forms[name] = {"value": n / dt, "ms_per_step": 1000 * dt,
+                           "ms_knn": st["ms_knn"], "ms_setup": st["ms_setup"],
+                           "n_underdetermined": st["n_underdetermined"]}
+        outs[name] = yv.double().cpu().numpy()
+    err = rel_inf(outs["fp32_bulk"], outs["fp64"])
+    out = {
+        "metric": "MLS setup+apply target pts/s, thin shell, FP32 bulk vs all-FP64",
+        "value": out_forms["fp32_bulk"]["value"],
+        "unit": "target pts/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": out_forms["fp32_bulk"]["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic",
+        "config": {"workload": f"MLS transfer {n} sources on a shell r=1+-5e-4 -> {n} targets "
+                               f"r=1+-2e-4, quadratic, k=32, wendland4, rcond=1e-12, 4 fields",
+                   "n_src": n, "n_tgt": n},
+        "forms": out_forms,
+        "rel_err_fp32_vs_fp64_per_field": err,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_copy(args):
+    """Calibration for HBM counters: a plain device-to-device copy of
+    1 GiB (reads 1 GiB, writes 1 GiB per launch), repeated."""
+    import torch
+
+    n = (1 << 30) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(args.steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n,
+                      "launches": args.steps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -285,11 +435,14 @@ def main():
     ap.add_argument("--form", default="fp32_bulk", choices=sorted(FORMS))
     ap.add_argument("--compare", default="fp64,fp64_cyclic",
                     help="comma-separated forms timed beside the headline form ('' for none)")
-    ap.add_argument("--cpu-sample", type=int, default=250_000)
+    ap.add_argument("--cpu-sample", type=int, default=0,
+                    help="targets timed on the CPU oracle (0: all)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="transfer", choices=["transfer", "batched"],
+    ap.add_argument("--workload", default="transfer",
+                    choices=["transfer", "batched", "adjoint", "shell", "copy"],
                     help="transfer: one 1M x 1M operator (default); batched: many "
-                         "uneven operators")
+                         "uneven operators; adjoint: forward + backward; shell: "
+                         "FP32 bulk on a thin shell")
     ap.add_argument("--problems", type=int, default=4096)
     ap.add_argument("--max-src", type=int, default=65536)
     args = ap.parse_args()
@@ -300,6 +453,17 @@ def main():
         sys.exit(2)
     if args.workload == "batched":
         bench_batched(args)
+        return
+    if args.workload == "adjoint":
+        bench_adjoint(args)
+        return
+    if args.workload == "copy":
+        bench_copy(args)
+        return
+    if args.workload == "shell":
+        if args.n == 1_000_000:
+            args.n = 2_000_000
+        bench_shell(args)
         return
 
     import numpy as np
@@ -382,7 +546,7 @@ def main():
         "cpu_baseline": None,
     }
     if not args.no_cpu_baseline:
-        s = min(args.cpu_sample, n)
+        s = min(args.cpu_sample, n) if args.cpu_sample > 0 else n
         rec, ref = cpu_baseline(src, tgt, vals, m, kk, args.rbf, s)
         out["cpu_baseline"] = rec
         for name, y in outputs.items():

@@ -95,7 +95,7 @@ __global__ void __launch_bounds__( kBlock )
     __shared__ int32_t s_idx[kWaves][64 * kPitch];
     __shared__ C s_c[kWaves][64 * kPitch];
     int const wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-    int64_t const r0 = blockIdx.x * (int64_t)kBlock + wave * 64;
+    int64_t const r0 = xcd_tile() * kBlock + wave * 64;
     if ( r0 >= n_tgt )
         return; // whole wave: no block-level barrier below
     int const nrows = n_tgt - r0 < 64 ? (int)( n_tgt - r0 ) : 64;
@@ -242,7 +242,7 @@ __global__ void __launch_bounds__( kBlock )
                       double *__restrict__ xbar, int64_t ld_src, int nf )
 {
 #pragma clang fp contract( off )
-    int64_t const p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    int64_t const p = xcd_tile() * kBlock + threadIdx.x;
     if ( p >= n_src )
         return;
     int32_t const b = off[p], e = off[p + 1];
@@ -345,7 +345,7 @@ __global__ void __launch_bounds__( kBlock )
 {
 #pragma clang fp contract( off )
     using Sum = typename SumOf<Acc>::type;
-    int64_t const r = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    int64_t const r = xcd_tile() * kBlock + threadIdx.x;
     if ( r >= n_rows )
         return;
     int64_t const b = row_off[r], e = row_off[r + 1];

@@ -170,6 +170,20 @@ void concat_points( double const *const *ptrs, int32_t const *off,
 void invert_permutation( int32_t const *perm, int64_t n, int32_t *inv,
                          hipStream_t s );
 
+// Work tile of the current block under the MI355X dispatch order, which
+// hands consecutive workgroups to the 8 XCDs in turn (each XCD with its own
+// L2): the blocks an XCD receives take one contiguous range of tiles, so
+// spatially adjacent work (Morton-ordered rows) shares an L2 instead of
+// being spread over all eight. A bijection of [0, gridDim.x).
+__device__ inline int64_t xcd_tile()
+{
+    constexpr int64_t kXcds = 8;
+    int64_t const nb = gridDim.x, b = blockIdx.x;
+    int64_t const per = nb / kXcds, rem = nb % kXcds;
+    int64_t const x = b % kXcds, i = b / kXcds;
+    return x * per + ( x < rem ? x : rem ) + i;
+}
+
 inline unsigned ceil_div( int64_t a, int64_t b )
 {
     return static_cast<unsigned>( ( a + b - 1 ) / b );

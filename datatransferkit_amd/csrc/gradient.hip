@@ -125,7 +125,7 @@ __global__ void __launch_bounds__( kBlock )
     constexpr int R = M / H;
     constexpr int SPB = kBlock / H;
     int const h = threadIdx.x % H;
-    int64_t const slot = blockIdx.x * (int64_t)SPB + threadIdx.x / H;
+    int64_t const slot = xcd_tile() * SPB + threadIdx.x / H;
     bool const valid = slot < n_tgt;
     int64_t const t = valid ? ( order ? order[slot] : slot ) : 0;
     int32_t const *nbr = indices + slot * kk;

@@ -552,7 +552,7 @@ __global__ void __launch_bounds__( kQueryBlock )
     constexpr int LS = STACK > 0 ? STACK : 1;
     __shared__ int32_t st_node[LS * kQueryBlock];
     __shared__ T st_d[LS * kQueryBlock];
-    int64_t const qi = blockIdx.x * (int64_t)kQueryBlock + threadIdx.x;
+    int64_t const qi = xcd_tile() * kQueryBlock + threadIdx.x;
     if ( qi >= a.n_rows )
         return;
     int64_t const r = a.row0 + qi;

@@ -94,7 +94,7 @@ __global__ void __launch_bounds__( kWave )
     extern __shared__ double lds[];
     int const sys = threadIdx.x / G, lane = threadIdx.x % G;
     int const gbase = threadIdx.x - lane; // first lane of the group
-    int64_t const slot = row0 + blockIdx.x * (int64_t)S + sys;
+    int64_t const slot = row0 + xcd_tile() * S + sys;
     bool const valid = slot < n_tgt;
     int64_t const t = valid ? ( order ? order[slot] : slot ) : 0;
     int64_t const rbeg =

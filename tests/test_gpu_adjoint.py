@@ -193,3 +193,32 @@ def test_adjoint_preconditions(dtk):
         dtk.MovingLeastSquaresOperator(cuda(src), cuda(tgt), basis="linear",
                                        store_pinv=True)
     assert e.value.status == 7
+
+
+def test_target_gradient_at_scale_against_extended_differences(dtk):
+    """1M sources (the adjoint workload's density): the moment matrices of
+    monomials of relative coordinates are ill-conditioned enough that a
+    double-precision difference quotient at h = 1e-6 carries ~1e-5 noise, so
+    the central differences are taken in extended precision (the oracle's
+    restatement of the operator) on a sample of targets."""
+    from oracle import extended
+
+    rng = np.random.default_rng(31)
+    src = rng.random((1_000_000, 3))
+    tgt = 0.05 + 0.9 * rng.random((2000, 3))
+    nf = 8
+    x = fields(src, nf, seed=2)
+    ybar = rng.standard_normal((nf, len(tgt)))
+    op = dtk.MovingLeastSquaresOperator(cuda(src), cuda(tgt), basis="quadratic",
+                                        rbf="wendland2", k=16, solver="cyclic",
+                                        geometry="double")
+    g = op.target_gradient(cuda(x), cuda(ybar)).cpu().numpy()
+    _, nbr, _ = op.graph()
+    nbr = nbr.cpu().numpy()
+    h = 1e-6
+    gfd = extended.target_gradient_fd(src, tgt, nbr, x, ybar, 10, "wendland2", h=h)
+    smooth = radius_gap(src, tgt, nbr) > 10 * h
+    err = np.abs(g - gfd)[smooth].max() / np.abs(gfd)[smooth].max()
+    print(f"1M sources: |g - g_fd|_inf / |g_fd|_inf = {err:.3e} over {smooth.sum()} targets")
+    assert smooth.mean() > 0.95
+    assert err < 1e-6

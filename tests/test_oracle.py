@@ -375,3 +375,42 @@ def test_correctly_rounded_mode_matches_mpmath(oracle_lib):
         assert s[i] == float(mp.sin(mp.mpf(x[i])))
         assert co[i] == float(mp.cos(mp.mpf(x[i])))
         assert at[i] == float(mp.atan2(mp.mpf(y[i]), mp.mpf(x[i])))
+
+
+# ------------------------------------------------ extended-precision values
+def test_extended_precision_is_extended():
+    assert np.finfo(np.longdouble).eps < 1e-18
+
+
+def test_extended_values_match_double_oracle(oracle_lib):
+    from oracle import extended
+
+    rng = np.random.default_rng(4)
+    src = rng.random((400, 3))
+    tgt = 0.2 + 0.6 * rng.random((50, 3))
+    x = np.stack([np.sin(3 * src[:, 0]) + src[:, 1], src[:, 2] ** 2])
+    idx, c, _ = oracle_lib.mls_create(src, tgt, 10, k=24, rbf_name="wendland2",
+                                      geometry=oracle_lib.GEOMETRY_DOUBLE)
+    ref = oracle_lib.apply(idx, c, x)
+    y = extended.target_values(src, tgt, idx, x, 10, "wendland2").astype(np.float64)
+    np.testing.assert_allclose(y, ref, rtol=1e-11, atol=1e-12)
+
+
+def test_extended_reproduces_polynomials_and_their_gradient():
+    from oracle import extended
+
+    rng = np.random.default_rng(5)
+    src = rng.random((300, 3))
+    tgt = 0.3 + 0.4 * rng.random((20, 3))
+    nbr = np.argsort(((src[None] - tgt[:, None]) ** 2).sum(-1), axis=1)[:, :20]
+    a = np.array([0.7, -1.3, 2.1])
+    lin = src @ a + 0.25
+    quad = src[:, 0] * src[:, 1] - 2 * src[:, 2] ** 2
+    y = extended.target_values(src, tgt, nbr, np.stack([lin, quad]), 10, "wendland4")
+    np.testing.assert_allclose(y[0].astype(float), tgt @ a + 0.25, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(y[1].astype(float),
+                               tgt[:, 0] * tgt[:, 1] - 2 * tgt[:, 2] ** 2, atol=1e-14)
+    # d/dt of a reproduced linear field is its gradient, whatever the weights
+    g = extended.target_gradient_fd(src, tgt, nbr, lin[None], np.ones((1, 20)), 4,
+                                    "wendland2")
+    np.testing.assert_allclose(g, np.tile(a, (20, 1)), rtol=1e-9)
