@@ -8,13 +8,16 @@ trig polynomials + 1e-6 Gaussian noise). Quadratic basis, Wendland C2,
 k = 32 neighbours by default. Inputs are resident in HBM before the timed
 region. Throughput is target points per second over whole steps.
 
-Three forms of the operator run on the same inputs:
-  fp32_bulk    float geometry, weights, Vandermonde, coefficients and apply
-               (compensated float sums); FP64 moments, Jacobi eigen-solver and
-               truncation. This is the headline `value`.
-  fp64         the plain FP64 drop-in: the reference's classical two-sided
-               Jacobi, bit-exact with the CPU oracle.
-  fp64_cyclic  FP64 throughout with the register-resident cyclic Jacobi.
+Forms of the operator run on the same inputs:
+  fp32_bulk         float geometry, weights, Vandermonde, coefficients and
+                    apply (compensated float sums); FP64 moments, certified
+                    Cholesky solve (cyclic Jacobi for uncertified systems) and
+                    truncation. This is the headline `value`.
+  fp32_bulk_cyclic  the same with the cyclic Jacobi for every system.
+  fp64              the plain FP64 drop-in: the reference's classical
+                    two-sided Jacobi, bit-exact with the CPU oracle.
+  fp64_cholesky     FP64 throughout, certified Cholesky.
+  fp64_cyclic       FP64 throughout, register-resident cyclic Jacobi.
 Each form's error against the FP64 CPU oracle is measured on the oracle's
 sample of targets, and the FP32 form's error against the fp64 GPU form on
 every target.
@@ -35,14 +38,22 @@ sys.path.insert(0, REPO)
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 FORMS = {
-    "fp32_bulk": dict(precision="fp32_bulk", solver="cyclic"),
+    "fp32_bulk": dict(precision="fp32_bulk", solver="cholesky"),
+    "fp32_bulk_cyclic": dict(precision="fp32_bulk", solver="cyclic"),
     "fp64": dict(precision="fp64", solver="classical"),
     "fp64_cyclic": dict(precision="fp64", solver="cyclic"),
+    "fp64_cholesky": dict(precision="fp64", solver="cholesky"),
 }
 SETUP_KERNEL = {
-    "fp32_bulk": "setup_cyclic_kernel<10,2,true> (fused MLS setup, FP32 bulk + FP64 Jacobi)",
+    "fp32_bulk": "setup_cholesky_kernel<10,true,false> (fused MLS setup, FP32 bulk + "
+                 "certified FP64 Cholesky)",
+    "fp32_bulk_cyclic": "setup_cyclic_kernel<10,2,true,false> (fused MLS setup, FP32 bulk "
+                        "+ FP64 cyclic Jacobi)",
     "fp64": "setup_classical_kernel<10,G> (fused MLS setup, FP64 classical Jacobi)",
-    "fp64_cyclic": "setup_cyclic_kernel<10,2,false> (fused MLS setup, FP64 cyclic Jacobi)",
+    "fp64_cyclic": "setup_cyclic_kernel<10,2,false,false> (fused MLS setup, FP64 cyclic "
+                   "Jacobi)",
+    "fp64_cholesky": "setup_cholesky_kernel<10,false,false> (fused MLS setup, certified "
+                     "FP64 Cholesky)",
 }
 
 
@@ -433,7 +444,7 @@ def main():
     ap.add_argument("--fields", type=int, default=4)
     ap.add_argument("--rbf", default="wendland2")
     ap.add_argument("--form", default="fp32_bulk", choices=sorted(FORMS))
-    ap.add_argument("--compare", default="fp64,fp64_cyclic",
+    ap.add_argument("--compare", default="fp64,fp64_cholesky,fp64_cyclic,fp32_bulk_cyclic",
                     help="comma-separated forms timed beside the headline form ('' for none)")
     ap.add_argument("--cpu-sample", type=int, default=0,
                     help="targets timed on the CPU oracle (0: all)")

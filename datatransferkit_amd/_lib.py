@@ -32,7 +32,7 @@ RBF = {
 }
 PRECISION = {"fp64": 0, "fp32_bulk": 1}
 GEOMETRY = {"float": 0, "double": 1}
-SOLVER = {"classical": 0, "cyclic": 1}
+SOLVER = {"classical": 0, "cyclic": 1, "cholesky": 2}
 TRUNCATION = {"absolute": 0, "relative": 1}
 LAYOUT = {"aos": 0, "soa": 1}
 

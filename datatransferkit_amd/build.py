@@ -24,6 +24,7 @@ SOURCES = [
     "knn.hip",
     "setup_classical.hip",
     "setup_cyclic.hip",
+    "setup_fast.hip",
     "apply.hip",
     "gradient.hip",
     "capi.hip",

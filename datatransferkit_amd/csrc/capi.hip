@@ -109,7 +109,8 @@ void validate_options( dtkx_mls_options const &o )
              o.geometry == DTKX_GEOMETRY_DOUBLE,
          DTKX_ERR_INVALID_ARGUMENT );
     REQ( o.solver == DTKX_SOLVER_CLASSICAL_JACOBI ||
-             o.solver == DTKX_SOLVER_CYCLIC_JACOBI,
+             o.solver == DTKX_SOLVER_CYCLIC_JACOBI ||
+             o.solver == DTKX_SOLVER_CHOLESKY,
          DTKX_ERR_INVALID_ARGUMENT );
     REQ( o.truncation == DTKX_TRUNCATION_ABSOLUTE ||
              o.truncation == DTKX_TRUNCATION_RELATIVE,
@@ -118,8 +119,8 @@ void validate_options( dtkx_mls_options const &o )
          DTKX_ERR_INVALID_ARGUMENT );
     REQ( o.group_size >= 0 && o.group_size <= 16, DTKX_ERR_INVALID_ARGUMENT );
     REQ( o.max_rotations >= 0, DTKX_ERR_INVALID_ARGUMENT );
-    // the stored pseudo-inverse comes from the cyclic eigen-solver
-    REQ( !o.store_pinv || o.solver == DTKX_SOLVER_CYCLIC_JACOBI ||
+    // the stored pseudo-inverse comes from the cyclic or Cholesky solvers
+    REQ( !o.store_pinv || o.solver != DTKX_SOLVER_CLASSICAL_JACOBI ||
              o.precision == DTKX_PRECISION_FP32_BULK,
          DTKX_ERR_UNSUPPORTED );
 }
@@ -244,16 +245,18 @@ void run_setup( dtkx_mls *h, unsigned long long *counters_d )
     if ( h->opts.store_pinv )
         h->pinv.resize( h->n_tgt * ( h->m * ( h->m + 1 ) / 2 ), h->stream );
     if ( h->opts.precision == DTKX_PRECISION_FP32_BULK ||
-         h->opts.solver == DTKX_SOLVER_CYCLIC_JACOBI )
+         h->opts.solver != DTKX_SOLVER_CLASSICAL_JACOBI )
     {
-        // the FP32-bulk form always uses the register-resident cyclic solver
+        // the FP32-bulk form uses the cyclic solver unless Cholesky is asked
         p.max_rot = h->opts.max_rotations; // sweeps; 0 = default
         bool const f32 = h->opts.precision == DTKX_PRECISION_FP32_BULK;
-        dtkx::setup_cyclic( h->src_d.get(), h->tgt_d.get(), h->n_tgt,
-                            h->indices.get(), h->order.get(), p,
-                            f32 ? nullptr : h->coeffs.get(),
-                            f32 ? h->coeffs32.get() : nullptr, h->pinv.get(),
-                            counters_d, h->stream );
+        auto *setup = h->opts.solver == DTKX_SOLVER_CHOLESKY
+                          ? &dtkx::setup_fast
+                          : &dtkx::setup_cyclic;
+        setup( h->src_d.get(), h->tgt_d.get(), h->n_tgt, h->indices.get(),
+               h->order.get(), p, f32 ? nullptr : h->coeffs.get(),
+               f32 ? h->coeffs32.get() : nullptr, h->pinv.get(), counters_d,
+               h->stream );
         h->stats.group_size_used = h->m == 10 ? 2 : 1;
         return;
     }
@@ -1083,7 +1086,7 @@ int dtkx_mls_batch_create( int32_t n_problems, const dtkx_mls_problem *problems,
 
         // setup: one launch per basis over its rows
         bool const cyclic = opts->precision == DTKX_PRECISION_FP32_BULK ||
-                            opts->solver == DTKX_SOLVER_CYCLIC_JACOBI;
+                            opts->solver != DTKX_SOLVER_CLASSICAL_JACOBI;
         int group_used = 0;
         for ( auto const &g : groups )
         {
@@ -1100,11 +1103,13 @@ int dtkx_mls_batch_create( int32_t n_problems, const dtkx_mls_problem *problems,
             {
                 p.max_rot = opts->max_rotations;
                 bool const f32 = opts->precision == DTKX_PRECISION_FP32_BULK;
-                dtkx::setup_cyclic( src_d.get(), tgt_d.get(), g.r1,
-                                    b->indices.get(), b->order.get(), p,
-                                    f32 ? nullptr : b->coeffs.get(),
-                                    f32 ? b->coeffs32.get() : nullptr, nullptr,
-                                    counters.get(), s );
+                auto *setup = opts->solver == DTKX_SOLVER_CHOLESKY
+                                  ? &dtkx::setup_fast
+                                  : &dtkx::setup_cyclic;
+                setup( src_d.get(), tgt_d.get(), g.r1, b->indices.get(),
+                       b->order.get(), p, f32 ? nullptr : b->coeffs.get(),
+                       f32 ? b->coeffs32.get() : nullptr, nullptr,
+                       counters.get(), s );
                 group_used = g.m == 10 ? 2 : 1;
             }
             else

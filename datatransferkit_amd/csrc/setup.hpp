@@ -27,6 +27,10 @@ struct SetupParams
     // of the rows' neighbour lists (kk per row when null)
     int64_t row0 = 0;
     int64_t const *row_off = nullptr;
+    // row list (nullable, cyclic solver): set up rows rows[i] for
+    // i < *n_rows (device count, at most n_tgt - row0) instead of a range
+    int32_t const *rows = nullptr;
+    int32_t const *n_rows = nullptr;
 };
 
 // counters[0] = n_underdetermined, [1] = n_unconverged, [2] = rotations
@@ -54,6 +58,17 @@ void setup_cyclic( double4 const *src, double4 const *tgt, int64_t n_tgt,
                    int32_t const *indices, int32_t const *order,
                    SetupParams const &p, double *coeffs, float *coeffs32,
                    double *pinv, unsigned long long *counters, hipStream_t s );
+
+// Certified Cholesky fast path (DTKX_SOLVER_CHOLESKY): each system is
+// factored A = U^T U in FP64; when the bound lambda_min(A) >= 1 / |U^-1|_F^2
+// proves that no eigenvalue reaches the truncation threshold (with a margin
+// for the eigen-solver's rounding), A^+ = A^-1 and a0 = U^-1 U^-T e_0 is
+// used. The other systems are listed and set up by the cyclic eigen-solver.
+// Same outputs and counters as setup_cyclic.
+void setup_fast( double4 const *src, double4 const *tgt, int64_t n_tgt,
+                 int32_t const *indices, int32_t const *order,
+                 SetupParams const &p, double *coeffs, float *coeffs32,
+                 double *pinv, unsigned long long *counters, hipStream_t s );
 
 // Batched pseudo-inverse of general n x n matrices (n <= 64), one matrix
 // per workgroup (tstSVD path).

@@ -40,7 +40,9 @@ __global__ void __launch_bounds__( kBlock )
                          double4 const *__restrict__ tgt, int64_t n_tgt,
                          int32_t const *__restrict__ indices,
                          int32_t const *__restrict__ order, int64_t row0,
-                         int64_t const *__restrict__ row_off, int kk_uniform,
+                         int64_t const *__restrict__ row_off,
+                         int32_t const *__restrict__ rows,
+                         int32_t const *__restrict__ n_rows, int kk_uniform,
                          int rbf,
                          int geometry, int truncation, double rcond,
                          int max_sweeps, double *__restrict__ coeffs,
@@ -52,8 +54,19 @@ __global__ void __launch_bounds__( kBlock )
     constexpr int R = M / H;
     constexpr int SPB = kBlock / H; // systems per block
     int const h = threadIdx.x % H;
-    int64_t const slot = row0 + xcd_tile() * SPB + threadIdx.x / H;
-    bool const valid = slot < n_tgt;
+    int64_t const li = xcd_tile() * SPB + threadIdx.x / H;
+    bool valid;
+    int64_t slot;
+    if ( rows )
+    {
+        valid = li < *n_rows;
+        slot = valid ? rows[li] : 0;
+    }
+    else
+    {
+        slot = row0 + li;
+        valid = slot < n_tgt;
+    }
     int64_t const t = valid ? ( order ? order[slot] : slot ) : 0;
     int64_t const rbeg =
         valid ? ( row_off ? row_off[slot] : slot * kk_uniform ) : 0;
@@ -249,7 +262,7 @@ void launch_cyclic( double4 const *src, double4 const *tgt, int64_t n_tgt,
     auto run = [&]( auto f32, auto store ) {
         setup_cyclic_kernel<M, H, decltype( f32 )::value, decltype( store )::value>
             <<<grid, block, 0, s>>>( src, tgt, n_tgt, indices, order, p.row0,
-                                     p.row_off, p.kk,
+                                     p.row_off, p.rows, p.n_rows, p.kk,
                                      p.rbf, p.geometry, p.truncation, p.rcond,
                                      sweeps, coeffs, coeffs32, pinv, counters );
     };

@@ -1,0 +1,244 @@
+// Fused MLS setup with a certified Cholesky solve (DTKX_SOLVER_CHOLESKY).
+//
+// The reference inverts the moment matrix A = P^T Phi P through a Jacobi
+// SVD and drops singular values below DBL_EPSILON (DTK_DetailsSVDImpl.hpp
+// :281-297). When every eigenvalue of A is provably above that threshold,
+// the pseudo-inverse is the inverse, and a Cholesky factorisation gives
+// a0 = A^-1 e_0 with m^3/3 flops instead of several Jacobi sweeps. The
+// proof is a bound: with A = U^T U (U upper triangular) and V = U^-1,
+//   lambda_min(A) = sigma_min(U)^2 = 1 / |V|_2^2 >= 1 / |V|_F^2,
+// compared against the truncation threshold (absolute DBL_EPSILON, or
+// rcond * trace(A) >= rcond * lambda_max) times 4 plus 64 m eps trace(A),
+// a margin for the rounding of the eigenvalues an eigen-solver would
+// compute. A system that fails the test, or whose factorisation meets a
+// non-positive pivot, is appended to a list that the cyclic eigen-solver
+// then sets up; it is counted there, so the counters mean what they mean
+// for the cyclic solver.
+//
+// One lane per system, everything in registers: the packed triangle of A
+// is overwritten by U, then by V (all indices compile-time constants).
+#include "cyclic_device.hpp"
+#include "setup.hpp"
+
+#include <type_traits>
+
+namespace dtkx
+{
+namespace
+{
+
+using namespace cyc;
+constexpr int kBlock = 64;
+
+// two waves per SIMD: the compiler otherwise budgets for one
+template <int M, bool F32, bool STORE>
+__global__ void __launch_bounds__( kBlock, 2 )
+    setup_cholesky_kernel( double4 const *__restrict__ src,
+                           double4 const *__restrict__ tgt, int64_t n_tgt,
+                           int32_t const *__restrict__ indices,
+                           int32_t const *__restrict__ order, int64_t row0,
+                           int64_t const *__restrict__ row_off, int kk_uniform,
+                           int rbf, int geometry, int truncation, double rcond,
+                           double *__restrict__ coeffs,
+                           float *__restrict__ coeffs32,
+                           double *__restrict__ pinv_out,
+                           int32_t *__restrict__ fallback,
+                           int32_t *__restrict__ n_fallback )
+{
+    using S = SymIdx<M>;
+    int64_t const slot = row0 + xcd_tile() * kBlock + threadIdx.x;
+    if ( slot >= n_tgt )
+        return;
+    int64_t const t = order ? order[slot] : slot;
+    int64_t const rbeg = row_off ? row_off[slot] : slot * kk_uniform;
+    int const kk =
+        row_off ? (int)( row_off[slot + 1] - rbeg ) : kk_uniform;
+    int32_t const *nbr = indices + rbeg;
+    double4 const tp = tgt[t];
+
+    // radius (Impl.hpp:118-134)
+    double dmax = 10. * DBL_EPSILON;
+    for ( int l = 0; l < kk; ++l )
+    {
+        double const d = nbr_distance<M, F32>( geometry, src[nbr[l]], tp );
+        dmax = d > dmax ? d : dmax;
+    }
+    double const radius = F32 ? (double)( 1.1f * (float)dmax ) : 1.1 * dmax;
+
+    // moments, upper triangle
+    double A[S::size];
+#pragma unroll
+    for ( int e = 0; e < S::size; ++e )
+        A[e] = 0.0;
+    for ( int l = 0; l < kk; ++l )
+    {
+        double P[M];
+        double const phi =
+            nbr_row<M, F32>( geometry, rbf, radius, src[nbr[l]], tp, P );
+#pragma unroll
+        for ( int r = 0; r < M; ++r )
+        {
+            double const pr = P[r] * phi;
+#pragma unroll
+            for ( int c = r; c < M; ++c )
+                A[S::at( r, c )] = fma( pr, P[c], A[S::at( r, c )] );
+        }
+    }
+    double trace = 0.0;
+#pragma unroll
+    for ( int i = 0; i < M; ++i )
+        trace += A[S::at( i, i )];
+
+    // A = U^T U in place (left-looking, column by column)
+    bool ok = true;
+#pragma unroll
+    for ( int j = 0; j < M; ++j )
+    {
+#pragma unroll
+        for ( int i = 0; i <= j; ++i )
+        {
+            double s = A[S::at( i, j )];
+#pragma unroll
+            for ( int k = 0; k < i; ++k )
+                s = fma( -A[S::at( k, i )], A[S::at( k, j )], s );
+            if ( i < j )
+                A[S::at( i, j )] = s / A[S::at( i, i )];
+            else
+            {
+                ok = ok && s > 0.0;
+                A[S::at( j, j )] = __builtin_sqrt( s > 0.0 ? s : 1.0 );
+            }
+        }
+    }
+    // V = U^-1 in place: column j from the columns before it
+#pragma unroll
+    for ( int j = 0; j < M; ++j )
+    {
+        double const vjj = 1.0 / A[S::at( j, j )];
+#pragma unroll
+        for ( int i = 0; i < j; ++i )
+        {
+            double s = 0.0;
+#pragma unroll
+            for ( int k = i; k < j; ++k )
+                s = fma( A[S::at( i, k )], A[S::at( k, j )], s );
+            A[S::at( i, j )] = -s * vjj;
+        }
+        A[S::at( j, j )] = vjj;
+    }
+    double fro = 0.0;
+#pragma unroll
+    for ( int e = 0; e < S::size; ++e )
+        fro = fma( A[e], A[e], fro );
+    double const lambda_lb = 1.0 / fro;
+    double const thr = truncation == DTKX_TRUNCATION_RELATIVE
+                           ? rcond * trace
+                           : DBL_EPSILON;
+    ok = ok && lambda_lb > 4.0 * thr + 64.0 * M * DBL_EPSILON * trace &&
+         lambda_lb == lambda_lb;
+    if ( !ok )
+    {
+        fallback[atomicAdd( n_fallback, 1 )] = (int32_t)slot;
+        return;
+    }
+
+    // a0 = V V^T e_0: a0_i = sum_{k >= i} V(i, k) V(0, k)
+    double x[M];
+#pragma unroll
+    for ( int i = 0; i < M; ++i )
+    {
+        double s = 0.0;
+#pragma unroll
+        for ( int k = i; k < M; ++k )
+            s = fma( A[S::at( i, k )], A[S::at( 0, k )], s );
+        x[i] = s;
+    }
+    if constexpr ( STORE )
+    {
+        // A^-1 = V V^T, packed upper triangle
+#pragma unroll
+        for ( int i = 0; i < M; ++i )
+#pragma unroll
+            for ( int j = i; j < M; ++j )
+            {
+                double s = 0.0;
+#pragma unroll
+                for ( int k = j; k < M; ++k )
+                    s = fma( A[S::at( i, k )], A[S::at( j, k )], s );
+                pinv_out[slot * S::size + S::at( i, j )] = s;
+            }
+    }
+
+    // coefficients c_l = phi_l (P_l . x)
+    for ( int l = 0; l < kk; ++l )
+    {
+        double P[M];
+        double const phi =
+            nbr_row<M, F32>( geometry, rbf, radius, src[nbr[l]], tp, P );
+        double acc = 0.0;
+#pragma unroll
+        for ( int j = 0; j < M; ++j )
+            acc = fma( x[j], P[j], acc );
+        if constexpr ( F32 )
+            coeffs32[rbeg + l] = (float)( acc * phi );
+        else
+            coeffs[rbeg + l] = acc * phi;
+    }
+}
+
+template <int M>
+void launch_fast( double4 const *src, double4 const *tgt, int64_t n_tgt,
+                  int32_t const *indices, int32_t const *order,
+                  SetupParams const &p, double *coeffs, float *coeffs32,
+                  double *pinv, int32_t *list, int32_t *count, hipStream_t s )
+{
+    dim3 const grid( ceil_div( n_tgt - p.row0, kBlock ) ), block( kBlock );
+    auto run = [&]( auto f32, auto store ) {
+        setup_cholesky_kernel<M, decltype( f32 )::value,
+                              decltype( store )::value>
+            <<<grid, block, 0, s>>>( src, tgt, n_tgt, indices, order, p.row0,
+                                     p.row_off, p.kk, p.rbf, p.geometry,
+                                     p.truncation, p.rcond, coeffs, coeffs32,
+                                     pinv, list, count );
+    };
+    using T = std::true_type;
+    using F = std::false_type;
+    if ( coeffs32 )
+        pinv ? run( T{}, T{} ) : run( T{}, F{} );
+    else
+        pinv ? run( F{}, T{} ) : run( F{}, F{} );
+    DTKX_HIP( hipGetLastError() );
+}
+
+} // namespace
+
+void setup_fast( double4 const *src, double4 const *tgt, int64_t n_tgt,
+                 int32_t const *indices, int32_t const *order,
+                 SetupParams const &p, double *coeffs, float *coeffs32,
+                 double *pinv, unsigned long long *counters, hipStream_t s )
+{
+    if ( n_tgt <= p.row0 )
+        return;
+    DTKX_REQUIRE( p.m == 1 || p.m == 4 || p.m == 10,
+                  DTKX_ERR_INVALID_ARGUMENT );
+    DeviceBuffer<int32_t> list( n_tgt - p.row0, s ), count( 1, s );
+    DTKX_HIP( hipMemsetAsync( count.get(), 0, sizeof( int32_t ), s ) );
+    if ( p.m == 1 )
+        launch_fast<1>( src, tgt, n_tgt, indices, order, p, coeffs, coeffs32,
+                        pinv, list.get(), count.get(), s );
+    else if ( p.m == 4 )
+        launch_fast<4>( src, tgt, n_tgt, indices, order, p, coeffs, coeffs32,
+                        pinv, list.get(), count.get(), s );
+    else
+        launch_fast<10>( src, tgt, n_tgt, indices, order, p, coeffs, coeffs32,
+                         pinv, list.get(), count.get(), s );
+    // the listed systems through the eigen-solver (a grid for every row;
+    // blocks past the device-side count return at once)
+    SetupParams q = p;
+    q.rows = list.get();
+    q.n_rows = count.get();
+    setup_cyclic( src, tgt, n_tgt, indices, order, q, coeffs, coeffs32, pinv,
+                  counters, s );
+}
+
+} // namespace dtkx

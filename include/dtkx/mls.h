@@ -95,7 +95,12 @@ typedef enum dtkx_solver {
     DTKX_SOLVER_CLASSICAL_JACOBI = 0,
     /* Cyclic symmetric Jacobi eigen-solver, one system per lane, registers
      * only. Same pseudo-inverse up to rounding; much faster. */
-    DTKX_SOLVER_CYCLIC_JACOBI = 1
+    DTKX_SOLVER_CYCLIC_JACOBI = 1,
+    /* Cholesky solve for every system whose smallest eigenvalue is proved
+     * (lambda_min >= 1/|U^-1|_F^2) to lie above the truncation threshold,
+     * where the pseudo-inverse is the inverse; the cyclic eigen-solver for
+     * the others. Same pseudo-inverse up to rounding; fastest. */
+    DTKX_SOLVER_CHOLESKY = 2
 } dtkx_solver;
 
 typedef enum dtkx_truncation {

@@ -103,12 +103,13 @@ def test_target_gradient_matches_central_differences(dtk, basis, rbf, k):
     assert err < 1e-6
 
 
-def test_store_and_recompute_agree(dtk):
+@pytest.mark.parametrize("solver", ["cyclic", "cholesky"])
+def test_store_and_recompute_agree(dtk, solver):
     src, tgt, _ = c1_inputs(20_000, seed=5)
     tgt = tgt[:4000] * 0.9 + 0.05
     x = cuda(fields(src, 3, seed=1))
     ybar = cuda(np.random.default_rng(2).standard_normal((3, len(tgt))))
-    kw = dict(basis="quadratic", rbf="wendland2", k=16, solver="cyclic", geometry="double")
+    kw = dict(basis="quadratic", rbf="wendland2", k=16, solver=solver, geometry="double")
     g1 = dtk.MovingLeastSquaresOperator(cuda(src), cuda(tgt), **kw).target_gradient(x, ybar)
     op2 = dtk.MovingLeastSquaresOperator(cuda(src), cuda(tgt), store_pinv=True, **kw)
     g2 = op2.target_gradient(x, ybar)

@@ -56,7 +56,9 @@ def make_problems(n, seed, max_src=3000):
 FORMS = [
     dict(solver="classical", precision="fp64"),
     dict(solver="cyclic", precision="fp64"),
+    dict(solver="cholesky", precision="fp64"),
     dict(solver="cyclic", precision="fp32_bulk"),
+    dict(solver="cholesky", precision="fp32_bulk"),
 ]
 
 

@@ -24,18 +24,19 @@ def cuda(a, dtype=torch.float64):
     return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).cuda()
 
 
+@pytest.mark.parametrize("solver", ["cyclic", "cholesky"])
 @pytest.mark.parametrize("m,basis,k,rbf", [
     (10, "quadratic", 32, "wendland2"),
     (10, "quadratic", 16, "wendland2"),
     (4, "linear", 16, "wendland4"),
     (10, "quadratic", 64, "wendland4"),
 ])
-def test_fp32_bulk_against_fp64_oracle(dtk, oracle_lib, m, basis, k, rbf):
+def test_fp32_bulk_against_fp64_oracle(dtk, oracle_lib, m, basis, k, rbf, solver):
     src, tgt, f = c1_inputs(20_000, seed=80 + k)
     tgt = tgt[:5000] * 0.9 + 0.05
     fields = np.stack([f, np.cos(3 * src[:, 0]) * src[:, 1], src[:, 2] - 0.5])
     op = dtk.MovingLeastSquaresOperator(cuda(src), cuda(tgt), basis=basis, k=k, rbf=rbf,
-                                        precision="fp32_bulk")
+                                        precision="fp32_bulk", solver=solver)
     _, idx, coeffs = op.graph()
     assert coeffs.dtype == torch.float32
     gi, gc, _ = oracle_lib.mls_create(src, tgt, m, k=k, rbf_name=rbf)
@@ -49,7 +50,7 @@ def test_fp32_bulk_against_fp64_oracle(dtk, oracle_lib, m, basis, k, rbf):
     op.apply(cuda(fields), y64)
     for y in (y32.double().cpu().numpy(), y64.cpu().numpy()):
         rel = np.abs(y - ref).max(axis=1) / np.abs(ref).max(axis=1)
-        print(f"fp32 m={m} k={k}: per-field rel_inf {rel}")
+        print(f"fp32 {solver} m={m} k={k}: per-field rel_inf {rel}")
         assert np.all(rel < 1e-6)
 
 
