@@ -178,6 +178,7 @@ def run_form(dtk, torch, name, d_src, d_tgt, d_vals, n_fields, args, stream):
         "value": n * args.steps / dt,
         "ms_per_step": 1000 * dt / args.steps,
         "ms_knn": float(np.mean([s["ms_knn"] for s in stats])),
+        "ms_knn_query": float(np.mean([s["ms_knn_query"] for s in stats])),
         "ms_setup": float(np.mean([s["ms_setup"] for s in stats])),
         "ms_apply": float(np.mean([s["ms_apply"] for s in stats])),
     }
@@ -501,6 +502,7 @@ def main():
             "rotations_or_sweeps_per_target": st["total_rotations"] / max(n, 1),
             "n_underdetermined": st["n_underdetermined"],
             "n_unconverged": st["n_unconverged"],
+            "n_eigen_fallback": st["n_eigen_fallback"],
         })
     head = results[args.form]
     st = last_stats[args.form]

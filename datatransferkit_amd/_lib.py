@@ -69,6 +69,7 @@ class Stats(ctypes.Structure):
         ("n_unconverged", ctypes.c_int64), ("total_rotations", ctypes.c_int64),
         ("ms_knn", ctypes.c_double), ("ms_setup", ctypes.c_double),
         ("group_size_used", ctypes.c_int32), ("tree_height", ctypes.c_int32),
+        ("ms_knn_query", ctypes.c_double), ("n_eigen_fallback", ctypes.c_int64),
     ]
 
     def as_dict(self):

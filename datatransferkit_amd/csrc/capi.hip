@@ -200,8 +200,8 @@ __global__ void to_f32_kernel( double const *a, float *b, int64_t n )
 // synchronisation so timing adds no host-device round trip.
 struct StageEvents
 {
-    hipEvent_t e[3] = {nullptr, nullptr, nullptr};
-    bool marked[3] = {false, false, false};
+    hipEvent_t e[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    bool marked[5] = {false, false, false, false, false};
     hipStream_t s;
     explicit StageEvents( hipStream_t st )
         : s( st )
@@ -213,6 +213,13 @@ struct StageEvents
     {
         DTKX_HIP( hipEventRecord( e[i], s ) );
         marked[i] = true;
+    }
+    // events 3 and 4 bracket the kNN query kernels (recorded by the search)
+    void pass_query_events( dtkx::KnnInfo &info )
+    {
+        info.query_begin = e[3];
+        info.query_end = e[4];
+        marked[3] = marked[4] = true;
     }
     double ms( int i, int j )
     {
@@ -276,7 +283,7 @@ void finish_create( dtkx_mls *h, unsigned long long *counters_d )
     offsets_kernel<<<dtkx::ceil_div( h->n_tgt + 1, 256 ), 256, 0,
                      h->stream>>>( h->offsets.get(), h->n_tgt, h->kk );
     DTKX_HIP( hipGetLastError() );
-    unsigned long long c[3] = {0, 0, 0};
+    unsigned long long c[4] = {0, 0, 0, 0};
     DTKX_HIP( hipMemcpyAsync( c, counters_d, sizeof( c ),
                               hipMemcpyDeviceToHost, h->stream ) );
     DTKX_HIP( hipStreamSynchronize( h->stream ) );
@@ -288,6 +295,7 @@ void finish_create( dtkx_mls *h, unsigned long long *counters_d )
     h->stats.n_underdetermined = (int64_t)c[0];
     h->stats.n_unconverged = (int64_t)c[1];
     h->stats.total_rotations = (int64_t)c[2];
+    h->stats.n_eigen_fallback = (int64_t)c[3];
 }
 
 // coefficient storage of the handle's precision only
@@ -387,13 +395,14 @@ int dtkx_mls_create( const double *src_xyz, int64_t n_src,
         *out = nullptr;
         auto h = make_handle( src_xyz, n_src, tgt_xyz, n_tgt, opts );
         StageEvents ev( h->stream );
-        dtkx::DeviceBuffer<unsigned long long> counters( 3, h->stream );
-        DTKX_HIP( hipMemsetAsync( counters.get(), 0, 3 * sizeof( uint64_t ),
+        dtkx::DeviceBuffer<unsigned long long> counters( 4, h->stream );
+        DTKX_HIP( hipMemsetAsync( counters.get(), 0, 4 * sizeof( uint64_t ),
                                   h->stream ) );
         if ( n_tgt > 0 )
         {
             ev.mark( 0 );
             dtkx::KnnInfo info;
+            ev.pass_query_events( info );
             int const k = opts->k > 0 ? opts->k : h->m;
             dtkx::KnnOutput kout;
             h->perm.resize( n_src, h->stream );
@@ -435,6 +444,7 @@ int dtkx_mls_create( const double *src_xyz, int64_t n_src,
         finish_create( h.get(), counters.get() );
         h->stats.ms_knn = ev.ms( 0, 1 );
         h->stats.ms_setup = ev.ms( 1, 2 );
+        h->stats.ms_knn_query = n_tgt > 0 ? ev.ms( 3, 4 ) : 0.0;
         *out = h.release();
         return DTKX_SUCCESS;
     } );
@@ -475,8 +485,8 @@ int dtkx_mls_create_with_neighbors( const double *src_xyz, int64_t n_src,
             DTKX_HIP( hipStreamSynchronize( h->stream ) );
         }
         StageEvents ev( h->stream );
-        dtkx::DeviceBuffer<unsigned long long> counters( 3, h->stream );
-        DTKX_HIP( hipMemsetAsync( counters.get(), 0, 3 * sizeof( uint64_t ),
+        dtkx::DeviceBuffer<unsigned long long> counters( 4, h->stream );
+        DTKX_HIP( hipMemsetAsync( counters.get(), 0, 4 * sizeof( uint64_t ),
                                   h->stream ) );
         if ( n_tgt > 0 )
         {
@@ -1032,8 +1042,8 @@ int dtkx_mls_batch_create( int32_t n_problems, const dtkx_mls_problem *problems,
         // the host vectors above must outlive the uploads
         DTKX_HIP( hipStreamSynchronize( s ) );
 
-        dtkx::DeviceBuffer<unsigned long long> counters( 3, s );
-        DTKX_HIP( hipMemsetAsync( counters.get(), 0, 3 * sizeof( uint64_t ), s ) );
+        dtkx::DeviceBuffer<unsigned long long> counters( 4, s );
+        DTKX_HIP( hipMemsetAsync( counters.get(), 0, 4 * sizeof( uint64_t ), s ) );
         b->order.resize( std::max<int64_t>( nt, 1 ), s );
         b->perm.resize( ns, s );
         b->pos.resize( ns, s );
@@ -1065,6 +1075,8 @@ int dtkx_mls_batch_create( int32_t n_problems, const dtkx_mls_problem *problems,
         ko.row_off = b->row_off.get();
         ko.indices = b->indices.get();
         dtkx::KnnInfo info;
+        if ( nt > 0 )
+            ev.pass_query_events( info );
         if ( opts->geometry == DTKX_GEOMETRY_FLOAT )
         {
             dtkx::DeviceBuffer<float4> sf( ns, s ), tf( std::max<int64_t>( nt, 1 ), s );
@@ -1126,7 +1138,7 @@ int dtkx_mls_batch_create( int32_t n_problems, const dtkx_mls_problem *problems,
             }
         }
         ev.mark( 2 );
-        unsigned long long c[3] = {0, 0, 0};
+        unsigned long long c[4] = {0, 0, 0, 0};
         DTKX_HIP( hipMemcpyAsync( c, counters.get(), sizeof( c ),
                                   hipMemcpyDeviceToHost, s ) );
         DTKX_HIP( hipStreamSynchronize( s ) );
@@ -1138,8 +1150,10 @@ int dtkx_mls_batch_create( int32_t n_problems, const dtkx_mls_problem *problems,
         b->stats.n_underdetermined = (int64_t)c[0];
         b->stats.n_unconverged = (int64_t)c[1];
         b->stats.total_rotations = (int64_t)c[2];
+        b->stats.n_eigen_fallback = (int64_t)c[3];
         b->stats.ms_knn = ev.ms( 0, 1 );
         b->stats.ms_setup = ev.ms( 1, 2 );
+        b->stats.ms_knn_query = nt > 0 ? ev.ms( 3, 4 ) : 0.0;
         b->stats.group_size_used = group_used;
         b->stats.tree_height = info.tree_height;
 

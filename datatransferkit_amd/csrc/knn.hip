@@ -847,7 +847,11 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     qa.out_dist = out.dist;
     qa.src_pos = out.morton_layout ? src_pos : nullptr;
     qa.gstack = gstack.get();
+    if ( info && info->query_begin )
+        DTKX_HIP( hipEventRecord( info->query_begin, s ) );
     QueryLauncher<T>::dispatch( k, stack, qa, s );
+    if ( info && info->query_end )
+        DTKX_HIP( hipEventRecord( info->query_end, s ) );
     if ( info )
     {
         info->tree_height = h_tree;
@@ -1046,6 +1050,8 @@ void knn_batched_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     DeviceBuffer<int2> gstack;
     if ( stack == 0 )
         gstack.resize( static_cast<std::size_t>( h_tree + 1 ) * max_rows, s );
+    if ( info && info->query_begin )
+        DTKX_HIP( hipEventRecord( info->query_begin, s ) );
     for ( auto const &l : b.launches )
     {
         QueryArgs<T> qa;
@@ -1064,6 +1070,8 @@ void knn_batched_impl( void const *src_v, int64_t n_src, void const *tgt_v,
         qa.gstack = gstack.get();
         QueryLauncher<T>::dispatch( l.kmax, stack, qa, s );
     }
+    if ( info && info->query_end )
+        DTKX_HIP( hipEventRecord( info->query_end, s ) );
     if ( info )
         info->stack_used = stack;
 }

@@ -21,6 +21,8 @@ struct KnnInfo
 {
     int tree_height = 0; // internal levels on the longest root-leaf path
     int stack_used = 0;  // traversal stack capacity chosen
+    // (optional) recorded on the stream around the query kernels
+    hipEvent_t query_begin = nullptr, query_end = nullptr;
 };
 
 // Results of a search. indices: [n_tgt][kk], kk = min(k, n_src); dist

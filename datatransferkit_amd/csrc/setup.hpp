@@ -34,7 +34,8 @@ struct SetupParams
 };
 
 // counters[0] = n_underdetermined, [1] = n_unconverged, [2] = rotations
-// (classical) or sweeps (cyclic); accumulated atomically.
+// (classical) or sweeps (cyclic), [3] = systems the Cholesky path left to
+// the eigen-solver; accumulated atomically.
 // Row r of indices and coeffs belongs to target order[r] (identity when
 // order is null); indices address src, which may be the sources in a
 // permuted (Morton) order.

@@ -30,6 +30,19 @@ namespace
 using namespace cyc;
 constexpr int kBlock = 64;
 
+// The coefficients of a wave's rows are written with the lanes walking the
+// wave's entries in memory order (consecutive rows hold consecutive
+// entries), the rows' radius, target and a0 read from LDS: contiguous index
+// loads and coefficient stores instead of one strided row per lane.
+template <int M>
+struct RowShare
+{
+    double x[kBlock][M];
+    double geo[kBlock][4]; // radius, target x, y, z
+    int64_t beg[kBlock];
+    int kk[kBlock];
+};
+
 // two waves per SIMD: the compiler otherwise budgets for one
 template <int M, bool F32, bool STORE>
 __global__ void __launch_bounds__( kBlock, 2 )
@@ -43,18 +56,22 @@ __global__ void __launch_bounds__( kBlock, 2 )
                            float *__restrict__ coeffs32,
                            double *__restrict__ pinv_out,
                            int32_t *__restrict__ fallback,
-                           int32_t *__restrict__ n_fallback )
+                           int32_t *__restrict__ n_fallback,
+                           unsigned long long *__restrict__ counters )
 {
     using S = SymIdx<M>;
-    int64_t const slot = row0 + xcd_tile() * kBlock + threadIdx.x;
-    if ( slot >= n_tgt )
-        return;
-    int64_t const t = order ? order[slot] : slot;
-    int64_t const rbeg = row_off ? row_off[slot] : slot * kk_uniform;
+    __shared__ RowShare<M> sh;
+    int64_t const slot0 = row0 + xcd_tile() * kBlock;
+    int64_t const slot = slot0 + threadIdx.x;
+    bool const valid = slot < n_tgt;
+    int64_t const t = valid ? ( order ? order[slot] : slot ) : 0;
+    int64_t const rbeg =
+        valid ? ( row_off ? row_off[slot] : slot * kk_uniform ) : 0;
     int const kk =
-        row_off ? (int)( row_off[slot + 1] - rbeg ) : kk_uniform;
+        valid ? ( row_off ? (int)( row_off[slot + 1] - rbeg ) : kk_uniform )
+              : 0;
     int32_t const *nbr = indices + rbeg;
-    double4 const tp = tgt[t];
+    double4 const tp = valid ? tgt[t] : double4{0, 0, 0, 0};
 
     // radius (Impl.hpp:118-134)
     double dmax = 10. * DBL_EPSILON;
@@ -88,9 +105,18 @@ __global__ void __launch_bounds__( kBlock, 2 )
 #pragma unroll
     for ( int i = 0; i < M; ++i )
         trace += A[S::at( i, i )];
+    if ( !valid )
+    {
+        // an identity keeps the factorisation below finite
+#pragma unroll
+        for ( int i = 0; i < M; ++i )
+            A[S::at( i, i )] = 1.0;
+    }
 
-    // A = U^T U in place (left-looking, column by column)
+    // A = U^T U in place (left-looking, column by column); each pivot's
+    // reciprocal is kept so a column divides once
     bool ok = true;
+    double rinv[M];
 #pragma unroll
     for ( int j = 0; j < M; ++j )
     {
@@ -102,11 +128,13 @@ __global__ void __launch_bounds__( kBlock, 2 )
             for ( int k = 0; k < i; ++k )
                 s = fma( -A[S::at( k, i )], A[S::at( k, j )], s );
             if ( i < j )
-                A[S::at( i, j )] = s / A[S::at( i, i )];
+                A[S::at( i, j )] = s * rinv[i];
             else
             {
                 ok = ok && s > 0.0;
-                A[S::at( j, j )] = __builtin_sqrt( s > 0.0 ? s : 1.0 );
+                double const u = __builtin_sqrt( s > 0.0 ? s : 1.0 );
+                A[S::at( j, j )] = u;
+                rinv[j] = 1.0 / u;
             }
         }
     }
@@ -114,7 +142,6 @@ __global__ void __launch_bounds__( kBlock, 2 )
 #pragma unroll
     for ( int j = 0; j < M; ++j )
     {
-        double const vjj = 1.0 / A[S::at( j, j )];
 #pragma unroll
         for ( int i = 0; i < j; ++i )
         {
@@ -122,9 +149,9 @@ __global__ void __launch_bounds__( kBlock, 2 )
 #pragma unroll
             for ( int k = i; k < j; ++k )
                 s = fma( A[S::at( i, k )], A[S::at( k, j )], s );
-            A[S::at( i, j )] = -s * vjj;
+            A[S::at( i, j )] = -s * rinv[j];
         }
-        A[S::at( j, j )] = vjj;
+        A[S::at( j, j )] = rinv[j];
     }
     double fro = 0.0;
 #pragma unroll
@@ -136,14 +163,16 @@ __global__ void __launch_bounds__( kBlock, 2 )
                            : DBL_EPSILON;
     ok = ok && lambda_lb > 4.0 * thr + 64.0 * M * DBL_EPSILON * trace &&
          lambda_lb == lambda_lb;
-    if ( !ok )
+    if ( valid && !ok )
     {
+        // set up by the eigen-solver afterwards (it also rewrites this
+        // row's coefficients, which the shared loop below fills regardless)
         fallback[atomicAdd( n_fallback, 1 )] = (int32_t)slot;
-        return;
+        if ( counters )
+            atomicAdd( counters + 3, 1ull );
     }
 
     // a0 = V V^T e_0: a0_i = sum_{k >= i} V(i, k) V(0, k)
-    double x[M];
 #pragma unroll
     for ( int i = 0; i < M; ++i )
     {
@@ -151,38 +180,56 @@ __global__ void __launch_bounds__( kBlock, 2 )
 #pragma unroll
         for ( int k = i; k < M; ++k )
             s = fma( A[S::at( i, k )], A[S::at( 0, k )], s );
-        x[i] = s;
+        sh.x[threadIdx.x][i] = s;
     }
     if constexpr ( STORE )
     {
         // A^-1 = V V^T, packed upper triangle
+        if ( valid && ok )
 #pragma unroll
-        for ( int i = 0; i < M; ++i )
+            for ( int i = 0; i < M; ++i )
 #pragma unroll
-            for ( int j = i; j < M; ++j )
-            {
-                double s = 0.0;
+                for ( int j = i; j < M; ++j )
+                {
+                    double s = 0.0;
 #pragma unroll
-                for ( int k = j; k < M; ++k )
-                    s = fma( A[S::at( i, k )], A[S::at( j, k )], s );
-                pinv_out[slot * S::size + S::at( i, j )] = s;
-            }
+                    for ( int k = j; k < M; ++k )
+                        s = fma( A[S::at( i, k )], A[S::at( j, k )], s );
+                    pinv_out[slot * S::size + S::at( i, j )] = s;
+                }
     }
+    sh.geo[threadIdx.x][0] = radius;
+    sh.geo[threadIdx.x][1] = tp.x;
+    sh.geo[threadIdx.x][2] = tp.y;
+    sh.geo[threadIdx.x][3] = tp.z;
+    sh.beg[threadIdx.x] = rbeg;
+    sh.kk[threadIdx.x] = kk;
+    __syncthreads();
 
-    // coefficients c_l = phi_l (P_l . x)
-    for ( int l = 0; l < kk; ++l )
+    // coefficients c_l = phi_l (P_l . x), entries of the wave in order
+    int const nrows = n_tgt - slot0 < kBlock ? (int)( n_tgt - slot0 ) : kBlock;
+    int64_t const wbeg = sh.beg[0];
+    int64_t const wend = sh.beg[nrows - 1] + sh.kk[nrows - 1];
+    int row = 0;
+    for ( int64_t g = wbeg + threadIdx.x; g < wend; g += kBlock )
     {
+        // the row holding entry g: the last row beginning at or before g
+        while ( row + 1 < nrows && sh.beg[row + 1] <= g )
+            ++row;
+        double const *xr = sh.x[row];
+        double const rr = sh.geo[row][0];
+        double4 const tr = {sh.geo[row][1], sh.geo[row][2], sh.geo[row][3], 0.0};
         double P[M];
-        double const phi =
-            nbr_row<M, F32>( geometry, rbf, radius, src[nbr[l]], tp, P );
+        double const phi = nbr_row<M, F32>( geometry, rbf, rr, src[indices[g]],
+                                            tr, P );
         double acc = 0.0;
 #pragma unroll
         for ( int j = 0; j < M; ++j )
-            acc = fma( x[j], P[j], acc );
+            acc = fma( xr[j], P[j], acc );
         if constexpr ( F32 )
-            coeffs32[rbeg + l] = (float)( acc * phi );
+            coeffs32[g] = (float)( acc * phi );
         else
-            coeffs[rbeg + l] = acc * phi;
+            coeffs[g] = acc * phi;
     }
 }
 
@@ -190,7 +237,8 @@ template <int M>
 void launch_fast( double4 const *src, double4 const *tgt, int64_t n_tgt,
                   int32_t const *indices, int32_t const *order,
                   SetupParams const &p, double *coeffs, float *coeffs32,
-                  double *pinv, int32_t *list, int32_t *count, hipStream_t s )
+                  double *pinv, int32_t *list, int32_t *count,
+                  unsigned long long *counters, hipStream_t s )
 {
     dim3 const grid( ceil_div( n_tgt - p.row0, kBlock ) ), block( kBlock );
     auto run = [&]( auto f32, auto store ) {
@@ -199,7 +247,7 @@ void launch_fast( double4 const *src, double4 const *tgt, int64_t n_tgt,
             <<<grid, block, 0, s>>>( src, tgt, n_tgt, indices, order, p.row0,
                                      p.row_off, p.kk, p.rbf, p.geometry,
                                      p.truncation, p.rcond, coeffs, coeffs32,
-                                     pinv, list, count );
+                                     pinv, list, count, counters );
     };
     using T = std::true_type;
     using F = std::false_type;
@@ -225,13 +273,13 @@ void setup_fast( double4 const *src, double4 const *tgt, int64_t n_tgt,
     DTKX_HIP( hipMemsetAsync( count.get(), 0, sizeof( int32_t ), s ) );
     if ( p.m == 1 )
         launch_fast<1>( src, tgt, n_tgt, indices, order, p, coeffs, coeffs32,
-                        pinv, list.get(), count.get(), s );
+                        pinv, list.get(), count.get(), counters, s );
     else if ( p.m == 4 )
         launch_fast<4>( src, tgt, n_tgt, indices, order, p, coeffs, coeffs32,
-                        pinv, list.get(), count.get(), s );
+                        pinv, list.get(), count.get(), counters, s );
     else
         launch_fast<10>( src, tgt, n_tgt, indices, order, p, coeffs, coeffs32,
-                         pinv, list.get(), count.get(), s );
+                         pinv, list.get(), count.get(), counters, s );
     // the listed systems through the eigen-solver (a grid for every row;
     // blocks past the device-side count return at once)
     SetupParams q = p;

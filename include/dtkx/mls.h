@@ -153,6 +153,9 @@ typedef struct dtkx_mls_stats {
     double ms_setup;            /* with HIP events on the handle's stream   */
     int32_t group_size_used;
     int32_t tree_height;
+    double ms_knn_query;        /* the kNN query kernels alone */
+    int64_t n_eigen_fallback;   /* Cholesky solver: systems not certified,
+                                   set up by the cyclic eigen-solver */
 } dtkx_mls_stats;
 
 typedef struct dtkx_mls dtkx_mls; /* opaque operator handle */
