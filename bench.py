@@ -65,6 +65,13 @@ def f_setup(m, k):
     return 25 * k + k * m * (m + 1) + 6 * (m * (m - 1) // 2) * (18 * m + 30) + 2 * m * m + k * (2 * m + 1)
 
 
+def f_cholesky(m, k):
+    """Algorithmic FP64 flops per target of the certified Cholesky setup:
+    geometry 25k, symmetric moments k m(m+1), Cholesky m^3/3, triangular
+    inverse m^3/3, a0 m(m+1), coefficients k(2m+1)."""
+    return 25 * k + k * m * (m + 1) + 2 * (m ** 3 // 3) + m * (m + 1) + k * (2 * m + 1)
+
+
 def b_apply(nnz, n_tgt, n_src, fields, fp32=False):
     """Algorithmic HBM bytes of one apply of `fields` fields."""
     if fp32:
@@ -128,15 +135,58 @@ def cpu_baseline(src, tgt, vals, m, k, rbf, sample):
     return rec, ref
 
 
-def load_traffic(form, k):
-    """HBM bytes per launch of the setup kernel from rocprofv3 --pmc
-    (FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel), if measured."""
-    path = os.path.join(REPO, "profiles", "setup_traffic.json")
+def load_traffic(key):
+    """HBM bytes per launch of a kernel at this workload, from two
+    rocprofv3 --pmc runs (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB, the
+    factors a 1 GiB copy gives), if measured."""
+    path = os.path.join(REPO, "profiles", "kernel_traffic.json")
     try:
         with open(path) as fh:
-            return json.load(fh).get(f"{form}_k{k}")
+            return json.load(fh).get(key)
     except (OSError, ValueError):
         return None
+
+
+def kernel_rooflines(form, r, n, m, kk, nnz, fields):
+    """The three stages' kernels against their rooflines: the kNN query and
+    the apply against HBM, the setup against FP64."""
+    fp32 = FORMS[form]["precision"] == "fp32_bulk"
+    solver = FORMS[form]["solver"]
+    flops = f_cholesky(m, kk) if solver == "cholesky" else f_setup(m, kk)
+    setup = flops * n / (r["ms_setup"] * 1e-3) / 1e12
+    query = b_knn(n, n, kk) / (r["ms_knn_query"] * 1e-3) / 1e9
+    apply = b_apply(nnz, n, n, fields, fp32) / (r["ms_apply"] * 1e-3) / 1e9
+    geo = "float"
+    out = {
+        "knn_query": {
+            "bound": "hbm", "kernel": f"knn_query_kernel<{geo},{kk},32>",
+            "ms": r["ms_knn_query"], "achieved": round(query, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(query / HBM_PEAK_GBS, 5),
+            "traffic": load_traffic(f"knn_query_kernel<{geo},{kk},32>@{n}x{n}"),
+            "algorithmic_per_unit": f"B_knn = 24 N_s + N_t (24 + 4k) = {b_knn(n, n, kk)} bytes "
+                                    f"per launch ({n} queries)",
+        },
+        "setup": {
+            "bound": "mfma", "kernel": SETUP_KERNEL[form],
+            "ms": r["ms_setup"], "achieved": round(setup, 3), "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(setup / FP64_PEAK_TFLOPS, 5),
+            "traffic": load_traffic(f"{SETUP_KERNEL[form].split(' ')[0]}@{n}x{n},k{kk}"),
+            "algorithmic_per_unit": (f"F_cholesky(m={m},k={kk}) = {f_cholesky(m, kk)}"
+                                     if solver == "cholesky" else
+                                     f"F_setup(m={m},k={kk}) = {f_setup(m, kk)}")
+                                    + f" flop per target, {n} targets per launch",
+            "note": "FP64 VALU work (the k x m systems are too small for MFMA); "
+                    "peak = MI355X FP64 78.6 TFLOP/s",
+        },
+        "apply": {
+            "bound": "hbm", "kernel": "gather_fields_kernel + crs_rows_kernel",
+            "ms": r["ms_apply"], "achieved": round(apply, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(apply / HBM_PEAK_GBS, 5), "traffic": None,
+            "algorithmic_per_unit": f"B_apply = {b_apply(nnz, n, n, fields, fp32)} bytes "
+                                    f"per apply of {fields} fields",
+        },
+    }
+    return out
 
 
 def run_form(dtk, torch, name, d_src, d_tgt, d_vals, n_fields, args, stream):
@@ -510,27 +560,15 @@ def main():
     fp32 = FORMS[args.form]["precision"] == "fp32_bulk"
 
     for name, r in results.items():
-        r["setup_TFLOPs_algorithmic"] = f_setup(m, kk) * n / (r["ms_setup"] * 1e-3) / 1e12
-        r["apply_GBps_algorithmic"] = b_apply(
-            nnz, n, n, args.fields, FORMS[name]["precision"] == "fp32_bulk") / (r["ms_apply"] * 1e-3) / 1e9
-        r["knn_GBps_algorithmic"] = b_knn(n, n, kk) / (r["ms_knn"] * 1e-3) / 1e9
+        r["rooflines"] = kernel_rooflines(name, r, n, m, kk, nnz, args.fields)
     if "fp64" in outputs and args.form != "fp64":
         head["rel_err_vs_fp64_gpu_all_targets"] = rel_inf(outputs[args.form], outputs["fp64"])
 
-    setup_tflops = head["setup_TFLOPs_algorithmic"]
-    roofline = {
-        "bound": "mfma",
-        "kernel": SETUP_KERNEL[args.form],
-        "achieved": round(setup_tflops, 4),
-        "peak": FP64_PEAK_TFLOPS,
-        "unit": "TFLOP/s",
-        "frac": round(setup_tflops / FP64_PEAK_TFLOPS, 5),
-        "traffic": load_traffic(args.form, kk),
-        "algorithmic_per_unit": f"F_setup(m={m},k={kk}) = {f_setup(m, kk)} flop per target, "
-                                f"{n} targets per launch",
-        "note": "FP64 VALU work (the k x m systems are too small for MFMA); "
-                "peak = MI355X FP64 78.6 TFLOP/s",
-    }
+    # the dominant kernel of the headline form: the longest of the three
+    roofs = head["rooflines"]
+    dominant = max(roofs, key=lambda key: roofs[key]["ms"])
+    roofline = dict(roofs[dominant])
+    roofline["stage"] = dominant
     out = {
         "metric": "MLS setup+apply target pts/s (1M pts, quad basis): plain FP64 vs batched/FP32",
         "value": head["value"],

@@ -68,7 +68,7 @@ class Stats(ctypes.Structure):
         ("k", ctypes.c_int32), ("n_underdetermined", ctypes.c_int64),
         ("n_unconverged", ctypes.c_int64), ("total_rotations", ctypes.c_int64),
         ("ms_knn", ctypes.c_double), ("ms_setup", ctypes.c_double),
-        ("group_size_used", ctypes.c_int32), ("tree_height", ctypes.c_int32),
+        ("group_size_used", ctypes.c_int32), ("n_stack_overflow", ctypes.c_int32),
         ("ms_knn_query", ctypes.c_double), ("n_eigen_fallback", ctypes.c_int64),
     ]
 

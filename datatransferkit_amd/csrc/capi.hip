@@ -437,7 +437,7 @@ int dtkx_mls_create( const double *src_xyz, int64_t n_src,
             dtkx::invert_permutation( h->order.get(), n_tgt, h->trow.get(),
                                       h->stream );
             ev.mark( 1 );
-            h->stats.tree_height = info.tree_height;
+            h->stats.n_stack_overflow = info.n_overflow;
             run_setup( h.get(), counters.get() );
             ev.mark( 2 );
         }
@@ -1155,7 +1155,7 @@ int dtkx_mls_batch_create( int32_t n_problems, const dtkx_mls_problem *problems,
         b->stats.ms_setup = ev.ms( 1, 2 );
         b->stats.ms_knn_query = nt > 0 ? ev.ms( 3, 4 ) : 0.0;
         b->stats.group_size_used = group_used;
-        b->stats.tree_height = info.tree_height;
+        b->stats.n_stack_overflow = info.n_overflow;
 
         // apply resources, fixed for the captured graph
         if ( opts->precision == DTKX_PRECISION_FP32_BULK )

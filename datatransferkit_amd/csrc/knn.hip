@@ -47,7 +47,6 @@ struct alignas( 16 ) Node
     T lo[2][3];
     T hi[2][3];
     int32_t child[2];
-    int32_t height[2];
 };
 
 constexpr int kBlock = 256;
@@ -349,40 +348,7 @@ __global__ void __launch_bounds__( kBlock )
             cell_box<T>( codes[b], prefix_delta( codes, n, b, e ), scene,
                          nd.lo[c], nd.hi[c] );
     }
-    nd.height[0] = nd.height[1] = 0;
     nodes[gi] = nd;
-}
-
-// Depth of every leaf by walking its parents (reads only); the maximum is
-// the tree height that sizes the traversal stack.
-// Batch: src_prob gives the problem of each sorted position (problem-major,
-// the same ranges as the input order); a problem's root is its first
-// internal node and a one-source problem has no internal node.
-__global__ void __launch_bounds__( kBlock )
-    leaf_depth_kernel( int32_t const *parent_internal,
-                       int32_t const *parent_leaf, int64_t n,
-                       int32_t const *src_off, int32_t const *src_prob,
-                       int32_t *height )
-{
-    int64_t p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-    if ( p >= n )
-        return;
-    int32_t root = 0;
-    if ( src_off )
-    {
-        int const pb = src_prob[p];
-        if ( src_off[pb + 1] - src_off[pb] < 2 )
-            return;
-        root = src_off[pb] - pb;
-    }
-    int32_t node = parent_leaf[p] >> 1;
-    int32_t depth = 1;
-    while ( node != root && depth < 4096 )
-    {
-        node = parent_internal[node] >> 1;
-        ++depth;
-    }
-    atomicMax( height, depth );
 }
 
 // --------------------------------------------------------- k-best lists
@@ -539,12 +505,22 @@ struct QueryArgs
     double *out_dist = nullptr;
     // Morton layout: row r written at row r with indices as positions
     int32_t const *src_pos = nullptr;
+    // LDS-stack launch: rows whose descent would push past the LDS stack
+    // are listed here (and written by the re-run) instead of finished
+    int32_t *overflow_rows = nullptr;
+    int32_t *n_overflow = nullptr;
+    // global-stack re-run: rows rows[i] for i < *n_rows_dev, stack depth
+    // gdepth per lane laid out [level][lane]
+    int32_t const *rows = nullptr;
+    int32_t const *n_rows_dev = nullptr;
     int2 *gstack = nullptr;
+    int64_t gstride = 0;
 };
 
-// One lane per query. STACK > 0: LDS stack of that many entries per lane;
-// STACK == 0: per-lane stack in global memory (gstack, laid out
-// [level][lane]).
+// One lane per query. STACK > 0: LDS stack of that many entries per lane
+// (a query that would need more is listed for the re-run); STACK == 0:
+// per-lane stack in global memory (gstack, laid out [level][lane]) deep
+// enough for any tree.
 template <typename T, int K, int STACK>
 __global__ void __launch_bounds__( kQueryBlock )
     knn_query_kernel( QueryArgs<T> const a )
@@ -553,9 +529,19 @@ __global__ void __launch_bounds__( kQueryBlock )
     __shared__ int32_t st_node[LS * kQueryBlock];
     __shared__ T st_d[LS * kQueryBlock];
     int64_t const qi = xcd_tile() * kQueryBlock + threadIdx.x;
-    if ( qi >= a.n_rows )
-        return;
-    int64_t const r = a.row0 + qi;
+    int64_t r;
+    if ( a.rows )
+    {
+        if ( qi >= *a.n_rows_dev )
+            return;
+        r = a.rows[qi];
+    }
+    else
+    {
+        if ( qi >= a.n_rows )
+            return;
+        r = a.row0 + qi;
+    }
     int32_t const t = a.order[r];
     int64_t so = 0, np = a.n_src;
     int32_t root = 0;
@@ -612,6 +598,13 @@ __global__ void __launch_bounds__( kQueryBlock )
                 next = near0 ? c0 : c1;
                 if ( STACK > 0 )
                 {
+                    if ( sp == STACK )
+                    {
+                        // deeper than the LDS stack: left to the re-run
+                        a.overflow_rows[atomicAdd( a.n_overflow, 1 )] =
+                            (int32_t)r;
+                        return;
+                    }
                     st_node[sp * kQueryBlock + threadIdx.x] = far;
                     st_d[sp * kQueryBlock + threadIdx.x] = fs;
                 }
@@ -620,7 +613,7 @@ __global__ void __launch_bounds__( kQueryBlock )
                     int2 e;
                     e.x = far;
                     e.y = __float_as_int( (float)fs );
-                    a.gstack[(int64_t)sp * a.n_rows + qi] = e;
+                    a.gstack[sp * a.gstride + qi] = e;
                 }
                 ++sp;
             }
@@ -642,7 +635,7 @@ __global__ void __launch_bounds__( kQueryBlock )
                     }
                     else
                     {
-                        int2 e = a.gstack[(int64_t)sp * a.n_rows + qi];
+                        int2 e = a.gstack[sp * a.gstride + qi];
                         cand = e.x;
                         // the float copy of a double bound is rounded; a
                         // rounded-down copy could prune wrongly, so it is
@@ -674,6 +667,14 @@ __global__ void __launch_bounds__( kQueryBlock )
         }
 }
 
+// Queries run with an LDS stack of kLdsStack entries; the rare one whose
+// descent needs more is re-run with a global stack of kMaxDepth entries.
+// Every level of the hierarchy lengthens the common prefix of the node's
+// keys (63 Morton bits, then 32 index bits for equal codes), so no path
+// from the root has more than 96 internal levels.
+constexpr int kLdsStack = 32;
+constexpr int kMaxDepth = 97;
+
 template <typename T>
 struct QueryLauncher
 {
@@ -681,10 +682,8 @@ struct QueryLauncher
     static void run( int stack, QueryArgs<T> const &a, hipStream_t s )
     {
         dim3 grid( ceil_div( a.n_rows, kQueryBlock ) ), block( kQueryBlock );
-        if ( stack == 32 )
-            knn_query_kernel<T, K, 32><<<grid, block, 0, s>>>( a );
-        else if ( stack == 64 )
-            knn_query_kernel<T, K, 64><<<grid, block, 0, s>>>( a );
+        if ( stack == kLdsStack )
+            knn_query_kernel<T, K, kLdsStack><<<grid, block, 0, s>>>( a );
         else
             knn_query_kernel<T, K, 0><<<grid, block, 0, s>>>( a );
         DTKX_HIP( hipGetLastError() );
@@ -725,10 +724,36 @@ inline int query_bucket( int k )
     return 64;
 }
 
-// traversal stack: LDS for up to 64 levels, else global memory
-inline int stack_kind( int height )
+// The query launch with the LDS stack, then the re-run of its overflow.
+template <typename T>
+void run_queries( int kmax, QueryArgs<T> qa, int64_t n_rows, KnnInfo *info,
+                  hipStream_t s )
 {
-    return height <= 32 ? 32 : ( height <= 64 ? 64 : 0 );
+    if ( n_rows == 0 )
+        return;
+    DeviceBuffer<int32_t> rows( n_rows, s ), count( 1, s );
+    DTKX_HIP( hipMemsetAsync( count.get(), 0, sizeof( int32_t ), s ) );
+    qa.overflow_rows = rows.get();
+    qa.n_overflow = count.get();
+    QueryLauncher<T>::dispatch( kmax, kLdsStack, qa, s );
+    int32_t n_over = 0;
+    DTKX_HIP( hipMemcpyAsync( &n_over, count.get(), sizeof( int32_t ),
+                              hipMemcpyDeviceToHost, s ) );
+    DTKX_HIP( hipStreamSynchronize( s ) );
+    if ( info )
+        info->n_overflow += n_over;
+    if ( n_over == 0 )
+        return;
+    DeviceBuffer<int2> gstack( static_cast<std::size_t>( kMaxDepth ) * n_over,
+                               s );
+    qa.overflow_rows = nullptr;
+    qa.n_overflow = nullptr;
+    qa.rows = rows.get();
+    qa.n_rows_dev = count.get();
+    qa.n_rows = n_over;
+    qa.gstack = gstack.get();
+    qa.gstride = n_over;
+    QueryLauncher<T>::dispatch( kmax, 0, qa, s );
 }
 
 template <typename T>
@@ -786,28 +811,16 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     int64_t const n_int = n_src - 1;
     DeviceBuffer<Node<T>> nodes( std::max<int64_t>( n_int, 1 ), s );
     DeviceBuffer<int32_t> par_int( std::max<int64_t>( n_int, 1 ), s ),
-        par_leaf( n_src, s ), height( 1, s );
-    int h_tree = 0;
+        par_leaf( n_src, s );
     if ( n_int > 0 )
     {
-        DTKX_HIP( hipMemsetAsync( height.get(), 0, sizeof( int32_t ), s ) );
         karras_kernel<T, V><<<ceil_div( n_int, kBlock ), kBlock, 0, s>>>(
             codes_sorted.get(), ids_sorted, src, n_int, box.get(), nullptr, 1,
             nodes.get(), par_int.get(), par_leaf.get() );
-        leaf_depth_kernel<<<ceil_div( n_src, kBlock ), kBlock, 0, s>>>(
-            par_int.get(), par_leaf.get(), n_src, nullptr, nullptr,
-            height.get() );
         DTKX_HIP( hipGetLastError() );
-        DTKX_HIP( hipMemcpyAsync( &h_tree, height.get(), sizeof( int32_t ),
-                                  hipMemcpyDeviceToHost, s ) );
-        DTKX_HIP( hipStreamSynchronize( s ) );
     }
     if ( n_tgt == 0 )
-    {
-        if ( info )
-            info->tree_height = h_tree;
         return;
-    }
 
     // Morton order of the targets (same scene box)
     DeviceBuffer<int32_t> order_buf;
@@ -828,12 +841,6 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
             tids.get(), order, static_cast<int>( n_tgt ), 0, 63, s ) );
     }
 
-    // a descent pushes at most one node per internal level below the root
-    int const need = h_tree;
-    int const stack = stack_kind( need );
-    DeviceBuffer<int2> gstack;
-    if ( stack == 0 )
-        gstack.resize( static_cast<std::size_t>( need + 1 ) * n_tgt, s );
     QueryArgs<T> qa;
     qa.nodes = nodes.get();
     qa.src = src;
@@ -846,17 +853,11 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     qa.out_idx = out.indices;
     qa.out_dist = out.dist;
     qa.src_pos = out.morton_layout ? src_pos : nullptr;
-    qa.gstack = gstack.get();
     if ( info && info->query_begin )
         DTKX_HIP( hipEventRecord( info->query_begin, s ) );
-    QueryLauncher<T>::dispatch( k, stack, qa, s );
+    run_queries<T>( k, qa, n_tgt, info, s );
     if ( info && info->query_end )
         DTKX_HIP( hipEventRecord( info->query_end, s ) );
-    if ( info )
-    {
-        info->tree_height = h_tree;
-        info->stack_used = stack;
-    }
     // buffers are released at scope exit, ordered on the stream
 }
 
@@ -1010,25 +1011,15 @@ void knn_batched_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     int64_t const n_int = n_src - P;
     DeviceBuffer<Node<T>> nodes( std::max<int64_t>( n_int, 1 ), s );
     DeviceBuffer<int32_t> par_int( std::max<int64_t>( n_int, 1 ), s ),
-        par_leaf( n_src, s ), height( 1, s );
-    int h_tree = 0;
+        par_leaf( n_src, s );
     if ( n_int > 0 )
     {
-        DTKX_HIP( hipMemsetAsync( height.get(), 0, sizeof( int32_t ), s ) );
         karras_kernel<T, V><<<ceil_div( n_int, kBlock ), kBlock, 0, s>>>(
             codes_sorted.get(), out.src_perm, src, n_int, boxes.get(),
             b.src_off, P, nodes.get(), par_int.get(), par_leaf.get() );
-        leaf_depth_kernel<<<ceil_div( n_src, kBlock ), kBlock, 0, s>>>(
-            par_int.get(), par_leaf.get(), n_src, b.src_off, b.src_prob,
-            height.get() );
         DTKX_HIP( hipGetLastError() );
-        DTKX_HIP( hipMemcpyAsync( &h_tree, height.get(), sizeof( int32_t ),
-                                  hipMemcpyDeviceToHost, s ) );
-        DTKX_HIP( hipStreamSynchronize( s ) );
     }
     codes_sorted.release();
-    if ( info )
-        info->tree_height = h_tree;
     if ( n_tgt == 0 )
         return;
 
@@ -1043,13 +1034,6 @@ void knn_batched_impl( void const *src_v, int64_t n_src, void const *tgt_v,
         b.nnz, out.row_off );
     DTKX_HIP( hipGetLastError() );
 
-    int const stack = stack_kind( h_tree );
-    int64_t max_rows = 0;
-    for ( auto const &l : b.launches )
-        max_rows = std::max( max_rows, l.end - l.begin );
-    DeviceBuffer<int2> gstack;
-    if ( stack == 0 )
-        gstack.resize( static_cast<std::size_t>( h_tree + 1 ) * max_rows, s );
     if ( info && info->query_begin )
         DTKX_HIP( hipEventRecord( info->query_begin, s ) );
     for ( auto const &l : b.launches )
@@ -1067,13 +1051,10 @@ void knn_batched_impl( void const *src_v, int64_t n_src, void const *tgt_v,
         qa.row_off = out.row_off;
         qa.out_idx = out.indices;
         qa.src_pos = out.src_pos;
-        qa.gstack = gstack.get();
-        QueryLauncher<T>::dispatch( l.kmax, stack, qa, s );
+        run_queries<T>( l.kmax, qa, l.end - l.begin, info, s );
     }
     if ( info && info->query_end )
         DTKX_HIP( hipEventRecord( info->query_end, s ) );
-    if ( info )
-        info->stack_used = stack;
 }
 
 } // namespace

@@ -19,8 +19,7 @@ namespace dtkx
 
 struct KnnInfo
 {
-    int tree_height = 0; // internal levels on the longest root-leaf path
-    int stack_used = 0;  // traversal stack capacity chosen
+    int n_overflow = 0; // queries re-run with the global traversal stack
     // (optional) recorded on the stream around the query kernels
     hipEvent_t query_begin = nullptr, query_end = nullptr;
 };

@@ -152,7 +152,8 @@ typedef struct dtkx_mls_stats {
     double ms_knn;              /* stage times of the last create, measured */
     double ms_setup;            /* with HIP events on the handle's stream   */
     int32_t group_size_used;
-    int32_t tree_height;
+    int32_t n_stack_overflow;   /* kNN queries deeper than the 32-entry LDS
+                                   traversal stack, re-run with a global one */
     double ms_knn_query;        /* the kNN query kernels alone */
     int64_t n_eigen_fallback;   /* Cholesky solver: systems not certified,
                                    set up by the cyclic eigen-solver */

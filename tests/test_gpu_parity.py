@@ -87,6 +87,34 @@ def test_knn_large(dtk, oracle_lib):
     assert np.array_equal(idx, gi)
 
 
+@pytest.mark.parametrize("geometry", ["float", "double"])
+def test_knn_deep_tree_overflow_rerun(dtk, oracle_lib, geometry):
+    """A hierarchy deeper than the 32-entry LDS traversal stack: pairs of
+    points halving their distance to the origin (one Morton level each, and
+    a pair is an internal node, so the traversal pushes it) ending in 65536
+    copies of the origin (16 more levels, split by index bits). Queries at
+    the origin push past 32 entries before the list fills; they are re-run
+    with the global stack and must still equal brute force."""
+    rng = np.random.default_rng(4)
+    chain = np.repeat(np.array([[2.0 ** -i] * 3 for i in range(1, 22)]), 2, axis=0)
+    dup = np.zeros((65536, 3))
+    src = np.concatenate([rng.random((3000, 3)), chain, dup])
+    tgt = np.concatenate([np.zeros((64, 3)), rng.random((500, 3)) * 1e-3,
+                          rng.random((2000, 3))])
+    g = 0 if geometry == "float" else 1
+    for k in (16, 32):
+        gi, gd = oracle_lib.knn(src, tgt, k, geometry=g)
+        idx, dist = dtk.knn(cuda(src), cuda(tgt), k, geometry=geometry, with_distances=True)
+        assert np.array_equal(idx.cpu().numpy(), gi)
+        assert np.array_equal(bits(dist.cpu().numpy()), bits(gd))
+    op = dtk.MovingLeastSquaresOperator(cuda(src), cuda(tgt), basis="constant", k=16,
+                                        geometry=geometry)
+    assert op.stats()["n_stack_overflow"] > 0
+    _, idx, _ = op.graph()
+    gi, _ = oracle_lib.knn(src, tgt, 16, geometry=g)
+    assert np.array_equal(idx.cpu().numpy(), gi)
+
+
 # --------------------------------------------------------------------- SVD
 @pytest.mark.parametrize("case,shape,deficient", [
     ("svd_full_rank", (10, 32, 32), ()),
