@@ -13,7 +13,11 @@
 // Pruning uses squared distances against a threshold slightly above the
 // square of the current k-th distance, which never drops a candidate that
 // could tie or beat it; the kept set is therefore exactly the k smallest
-// (distance, index) pairs, as brute force gives.
+// (distance, index) pairs, as brute force gives. For larger k (and double
+// coordinates, whose key comparisons cost more) the list is replaced by two
+// traversals: the exact k-th squared distance from a list of distances
+// alone (one median-of-three per slot), then a fixed-radius pass that
+// collects the row, sorted at the end by (distance, index).
 #include "knn.hpp"
 
 #include <hipcub/hipcub.hpp>
@@ -483,6 +487,32 @@ __device__ inline double sqrt_rn( double s ) { return __builtin_sqrt( s ); }
 
 constexpr int kQueryBlock = 64;
 
+// Queries run with an LDS stack of kLdsStack entries; the rare one whose
+// descent needs more is re-run with a global stack of kMaxDepth entries.
+// Every level of the hierarchy lengthens the common prefix of the node's
+// keys (63 Morton bits, then 32 index bits for equal codes), so no path
+// from the root has more than 96 internal levels.
+#ifndef DTKX_KNN_LDS_STACK
+#define DTKX_KNN_LDS_STACK 32
+#endif
+constexpr int kLdsStack = DTKX_KNN_LDS_STACK;
+constexpr int kMaxDepth = 97;
+// the largest K whose two-pass rows are sorted inside the traversal kernel
+constexpr int kFusedSortK = 32;
+// launch mode of the separate row sort
+constexpr int kSortRows = -1;
+// The smallest K searched in two passes. Measured for 1M uniform sources
+// and targets: with float coordinates one pass is faster up to k = 16
+// (1.65 against 1.99 ms) and two passes from k = 32 (2.99 against 3.11 ms;
+// 5.5 against 7.8 ms at k = 64); with double coordinates, whose (distance,
+// index) comparisons cost three operations, two passes win from k = 16
+// (3.2 against 4.2 ms; 5.1 against 11.9 ms at k = 32).
+template <typename T>
+constexpr int two_pass_k()
+{
+    return sizeof( T ) == 4 ? 32 : 16;
+}
+
 // Arguments of a query launch. One problem: rows [row0, row0 + n_rows) of
 // the Morton-ordered targets against n_src sources, kk results per row.
 // Batch (src_off set): target t belongs to problem tgt_prob[t], whose
@@ -667,13 +697,414 @@ __global__ void __launch_bounds__( kQueryBlock )
         }
 }
 
-// Queries run with an LDS stack of kLdsStack entries; the rare one whose
-// descent needs more is re-run with a global stack of kMaxDepth entries.
-// Every level of the hierarchy lengthens the common prefix of the node's
-// keys (63 Morton bits, then 32 index bits for equal codes), so no path
-// from the root has more than 96 internal levels.
-constexpr int kLdsStack = 32;
-constexpr int kMaxDepth = 97;
+// ------------------------------------------------- two-pass k-best search
+// For larger k the sorted (distance, index) list costs five operations per
+// slot for every candidate, and a wave runs it whenever any of its lanes
+// meets one. Two traversals do the same work with fewer operations:
+// pass 1 keeps only the k smallest SQUARED distances, a sorted register
+// array updated with one median-of-three per slot; its last entry s_k gives
+// the k-th distance d_k = sqrt(s_k) (a correctly rounded square root is
+// monotone, so it maps order statistics to order statistics). Exactly the
+// pass-1 entries whose root is below d_k are sources at distance < d_k, say
+// c_lt of them. Pass 2 is a fixed-radius traversal that writes those c_lt
+// sources and the k - c_lt smallest indices among the sources at distance
+// == d_k to the row; the row is then sorted by (distance, index) in
+// registers. The result is the set and order the sorted list gives.
+template <typename T>
+__device__ inline T med3( T a, T b, T c );
+template <>
+__device__ inline float med3( float a, float b, float c )
+{
+    return __builtin_amdgcn_fmed3f( a, b, c );
+}
+template <>
+__device__ inline double med3( double a, double b, double c )
+{
+    return fmin( c, fmax( a, b ) ); // a <= c
+}
+
+// The k smallest squared distances, ascending. Slots below K - kk hold -1,
+// so the last slot is the kk-th smallest for any kk <= K.
+template <typename T, int K>
+struct KthList
+{
+    T d[K];
+    __device__ void init( int kk )
+    {
+#pragma unroll
+        for ( int j = 0; j < K; ++j )
+            d[j] = j < K - kk ? T( -1 ) : T( INFINITY );
+    }
+    // new d[j] = clamp(s, d[j - 1], d[j]): the j-th of the sorted union
+    __device__ void insert( T s )
+    {
+#pragma unroll
+        for ( int j = K - 1; j > 0; --j )
+            d[j] = med3( d[j - 1], s, d[j] );
+        d[0] = fmin( d[0], s );
+    }
+    __device__ T last() const { return d[K - 1]; }
+};
+
+// A squared-distance bound above every s whose rounded root is <= w.
+__device__ inline float root_bound( float w )
+{
+    return w * w * 1.00000095367431640625f; // (1 + 2^-20)
+}
+__device__ inline double root_bound( double w )
+{
+    return w * w * 1.0000000000000009; // (1 + 2^-50)
+}
+
+// (distance, index) keys of the final sort
+template <typename T>
+struct SortKey;
+template <>
+struct SortKey<float>
+{
+    using type = uint64_t;
+    __device__ static type make( float d, int32_t id )
+    {
+        return ( (uint64_t)__float_as_uint( d ) << 32 ) | (uint32_t)id;
+    }
+    __device__ static type none() { return ~0ull; }
+    __device__ static bool less( type a, type b ) { return a < b; }
+    __device__ static int32_t index( type a ) { return (int32_t)( a & 0xffffffffu ); }
+    __device__ static double distance( type a )
+    {
+        return (double)__uint_as_float( (uint32_t)( a >> 32 ) );
+    }
+};
+struct DKey
+{
+    double d;
+    int32_t id;
+};
+template <>
+struct SortKey<double>
+{
+    using type = DKey;
+    __device__ static type make( double d, int32_t id ) { return {d, id}; }
+    __device__ static type none() { return {INFINITY, INT32_MAX}; }
+    __device__ static bool less( type a, type b )
+    {
+        return a.d < b.d || ( a.d == b.d && a.id < b.id );
+    }
+    __device__ static int32_t index( type a ) { return a.id; }
+    __device__ static double distance( type a ) { return a.d; }
+};
+
+constexpr int pow2_at_least( int k )
+{
+    int n = 1;
+    while ( n < k )
+        n <<= 1;
+    return n;
+}
+
+// ascending bitonic sorting network, all indices compile-time constants
+template <typename T, int N>
+__device__ inline void bitonic_sort( typename SortKey<T>::type ( &a )[N] )
+{
+    using SK = SortKey<T>;
+#pragma unroll
+    for ( int k = 2; k <= N; k <<= 1 )
+#pragma unroll
+        for ( int j = k >> 1; j > 0; j >>= 1 )
+#pragma unroll
+            for ( int i = 0; i < N; ++i )
+            {
+                int const l = i ^ j;
+                if ( l > i )
+                {
+                    bool const up = ( i & k ) == 0;
+                    auto const x = a[i], y = a[l];
+                    bool const swap = up ? SK::less( y, x ) : SK::less( x, y );
+                    a[i] = swap ? y : x;
+                    a[l] = swap ? x : y;
+                }
+            }
+}
+
+// A two-pass row sorted by (distance, index), distances recomputed exactly
+// as the traversals computed them, indices then mapped to positions for the
+// Morton layout. (Fused: the LDS stack, not the network's registers, limits
+// the kernel's occupancy.)
+template <typename T, int K>
+__device__ inline void sort_row( QueryArgs<T> const &a, T const *q,
+                                 int64_t base, int kk, int32_t *row )
+{
+    using SK = SortKey<T>;
+    constexpr int N = pow2_at_least( K );
+    typename SK::type key[N];
+#pragma unroll
+    for ( int j = 0; j < N; ++j )
+    {
+        key[j] = SK::none();
+        if ( j < kk )
+        {
+            int32_t const id = row[j];
+            typename Vec4<T>::type const p = a.src[id];
+            T const b[3] = {p.x, p.y, p.z};
+            key[j] = SK::make( sqrt_rn( box_sq( q, b, b ) ), id );
+        }
+    }
+    bitonic_sort<T, N>( key );
+#pragma unroll
+    for ( int j = 0; j < K; ++j )
+        if ( j < kk )
+        {
+            int32_t const id = SK::index( key[j] );
+            row[j] = a.src_pos ? a.src_pos[id] : id;
+            if ( a.out_dist )
+                a.out_dist[base + j] = SK::distance( key[j] );
+        }
+}
+
+template <typename T, int K, int STACK>
+__global__ void __launch_bounds__( kQueryBlock )
+    knn_query2_kernel( QueryArgs<T> const a )
+{
+    using SK = SortKey<T>;
+    constexpr int LS = STACK > 0 ? STACK : 1;
+    __shared__ int32_t st_node[LS * kQueryBlock];
+    __shared__ T st_d[LS * kQueryBlock];
+    int64_t const qi = xcd_tile() * kQueryBlock + threadIdx.x;
+    int64_t r;
+    if ( a.rows )
+    {
+        if ( qi >= *a.n_rows_dev )
+            return;
+        r = a.rows[qi];
+    }
+    else
+    {
+        if ( qi >= a.n_rows )
+            return;
+        r = a.row0 + qi;
+    }
+    int32_t const t = a.order[r];
+    int64_t so = 0, np = a.n_src;
+    int32_t root = 0;
+    int kk = a.kk;
+    if ( a.src_off )
+    {
+        int const pb = a.tgt_prob[t];
+        so = a.src_off[pb];
+        np = a.src_off[pb + 1] - so;
+        root = (int32_t)( so - pb );
+        kk = a.kk_prob[pb];
+    }
+    typename Vec4<T>::type qp = a.tgt[t];
+    T const q[3] = {qp.x, qp.y, qp.z};
+    int64_t const base =
+        a.row_off ? a.row_off[r] : ( a.src_pos ? r : (int64_t)t ) * kk;
+    int32_t *row = a.out_idx + base;
+
+    auto overflow = [&]() {
+        a.overflow_rows[atomicAdd( a.n_overflow, 1 )] = (int32_t)r;
+    };
+
+    // pass 1: the kk-th smallest squared distance
+    KthList<T, K> list;
+    list.init( kk );
+    if ( np == 1 )
+    {
+        typename Vec4<T>::type sp0 = a.src[so];
+        T const b[3] = {sp0.x, sp0.y, sp0.z};
+        list.insert( box_sq( q, b, b ) );
+    }
+    else
+    {
+        T thr = list.last();
+        int sp = 0;
+        int32_t node = root;
+        while ( node >= 0 )
+        {
+            Node<T> const nd = a.nodes[node];
+            T const s0 = box_sq( q, nd.lo[0], nd.hi[0] );
+            T const s1 = box_sq( q, nd.lo[1], nd.hi[1] );
+            int32_t const c0 = nd.child[0], c1 = nd.child[1];
+            // an entry equal to the bound leaves the list's values unchanged
+            if ( c0 < 0 && s0 < thr )
+            {
+                list.insert( s0 );
+                thr = list.last();
+            }
+            if ( c1 < 0 && s1 < thr )
+            {
+                list.insert( s1 );
+                thr = list.last();
+            }
+            bool const g0 = c0 >= 0 && s0 < thr;
+            bool const g1 = c1 >= 0 && s1 < thr;
+            int32_t next = -1;
+            if ( g0 && g1 )
+            {
+                bool const near0 = s0 <= s1;
+                int32_t const far = near0 ? c1 : c0;
+                T const fs = near0 ? s1 : s0;
+                next = near0 ? c0 : c1;
+                if ( STACK > 0 )
+                {
+                    if ( sp == STACK )
+                    {
+                        overflow();
+                        return;
+                    }
+                    st_node[sp * kQueryBlock + threadIdx.x] = far;
+                    st_d[sp * kQueryBlock + threadIdx.x] = fs;
+                }
+                else
+                {
+                    int2 e;
+                    e.x = far;
+                    e.y = __float_as_int( (float)fs );
+                    a.gstack[sp * a.gstride + qi] = e;
+                }
+                ++sp;
+            }
+            else if ( g0 )
+                next = c0;
+            else if ( g1 )
+                next = c1;
+            else
+            {
+                while ( sp > 0 )
+                {
+                    --sp;
+                    int32_t cand;
+                    T cs;
+                    if ( STACK > 0 )
+                    {
+                        cand = st_node[sp * kQueryBlock + threadIdx.x];
+                        cs = st_d[sp * kQueryBlock + threadIdx.x];
+                    }
+                    else
+                    {
+                        int2 e = a.gstack[sp * a.gstride + qi];
+                        cand = e.x;
+                        // a rounded copy of a double bound only orders
+                        cs = sizeof( T ) == 4 ? (T)__int_as_float( e.y ) : 0;
+                    }
+                    if ( cs < thr )
+                    {
+                        next = cand;
+                        break;
+                    }
+                }
+            }
+            node = next;
+        }
+    }
+
+    // pass 2: the sources at distance < d_k, and the smallest indices among
+    // those at distance == d_k, written to the row unordered
+    T const dk = sqrt_rn( list.last() );
+    int c_lt = 0;
+#pragma unroll
+    for ( int j = 0; j < K; ++j )
+        c_lt += ( list.d[j] >= 0 && sqrt_rn( list.d[j] ) < dk ) ? 1 : 0;
+    T const s_hi = root_bound( dk );
+    int n_lt = 0, n_eq = 0;
+    auto take = [&]( T s, int32_t id ) {
+        T const d = sqrt_rn( s );
+        if ( d < dk )
+        {
+            if ( n_lt < c_lt )
+                row[n_lt++] = id;
+        }
+        else if ( d == dk )
+        {
+            if ( c_lt + n_eq < kk )
+                row[c_lt + n_eq++] = id;
+            else
+            {
+                // all tie slots taken: replace the largest index if larger
+                int jm = c_lt;
+                for ( int j = c_lt + 1; j < kk; ++j )
+                    jm = row[j] > row[jm] ? j : jm;
+                if ( id < row[jm] )
+                    row[jm] = id;
+            }
+        }
+    };
+    if ( np == 1 )
+    {
+        typename Vec4<T>::type sp0 = a.src[so];
+        T const b[3] = {sp0.x, sp0.y, sp0.z};
+        take( box_sq( q, b, b ), (int32_t)so );
+    }
+    else
+    {
+        int sp = 0;
+        int32_t node = root;
+        while ( node >= 0 )
+        {
+            Node<T> const nd = a.nodes[node];
+            T const s0 = box_sq( q, nd.lo[0], nd.hi[0] );
+            T const s1 = box_sq( q, nd.lo[1], nd.hi[1] );
+            int32_t const c0 = nd.child[0], c1 = nd.child[1];
+            if ( c0 < 0 && s0 <= s_hi )
+                take( s0, ~c0 );
+            if ( c1 < 0 && s1 <= s_hi )
+                take( s1, ~c1 );
+            bool const g0 = c0 >= 0 && s0 <= s_hi;
+            bool const g1 = c1 >= 0 && s1 <= s_hi;
+            int32_t next = -1;
+            if ( g0 && g1 )
+            {
+                next = c0;
+                if ( STACK > 0 )
+                {
+                    if ( sp == STACK )
+                    {
+                        overflow();
+                        return;
+                    }
+                    st_node[sp * kQueryBlock + threadIdx.x] = c1;
+                }
+                else
+                    a.gstack[sp * a.gstride + qi].x = c1;
+                ++sp;
+            }
+            else if ( g0 )
+                next = c0;
+            else if ( g1 )
+                next = c1;
+            else if ( sp > 0 )
+            {
+                --sp;
+                next = STACK > 0 ? st_node[sp * kQueryBlock + threadIdx.x]
+                                 : a.gstack[sp * a.gstride + qi].x;
+            }
+            node = next;
+        }
+    }
+    if constexpr ( K <= kFusedSortK )
+        sort_row<T, K>( a, q, base, kk, row );
+}
+
+// The row sort of a two-pass launch for K > kFusedSortK, whose sorting
+// network would take the traversal's registers.
+template <typename T, int K>
+__global__ void __launch_bounds__( kQueryBlock )
+    knn_sort_rows_kernel( QueryArgs<T> const a )
+{
+    int64_t const qi = xcd_tile() * kQueryBlock + threadIdx.x;
+    if ( qi >= a.n_rows )
+        return;
+    int64_t const r = a.row0 + qi;
+    int32_t const t = a.order[r];
+    int kk = a.kk;
+    if ( a.src_off )
+        kk = a.kk_prob[a.tgt_prob[t]];
+    typename Vec4<T>::type qp = a.tgt[t];
+    T const q[3] = {qp.x, qp.y, qp.z};
+    int64_t const base =
+        a.row_off ? a.row_off[r] : ( a.src_pos ? r : (int64_t)t ) * kk;
+    sort_row<T, K>( a, q, base, kk, a.out_idx + base );
+}
 
 template <typename T>
 struct QueryLauncher
@@ -682,9 +1113,18 @@ struct QueryLauncher
     static void run( int stack, QueryArgs<T> const &a, hipStream_t s )
     {
         dim3 grid( ceil_div( a.n_rows, kQueryBlock ) ), block( kQueryBlock );
-        if ( stack == kLdsStack )
+        if constexpr ( K >= two_pass_k<T>() )
+        {
+            if ( stack == kLdsStack )
+                knn_query2_kernel<T, K, kLdsStack><<<grid, block, 0, s>>>( a );
+            else if ( stack == 0 )
+                knn_query2_kernel<T, K, 0><<<grid, block, 0, s>>>( a );
+            else if ( K > kFusedSortK && stack == kSortRows )
+                knn_sort_rows_kernel<T, K><<<grid, block, 0, s>>>( a );
+        }
+        else if ( stack == kLdsStack )
             knn_query_kernel<T, K, kLdsStack><<<grid, block, 0, s>>>( a );
-        else
+        else if ( stack == 0 )
             knn_query_kernel<T, K, 0><<<grid, block, 0, s>>>( a );
         DTKX_HIP( hipGetLastError() );
     }
@@ -742,18 +1182,23 @@ void run_queries( int kmax, QueryArgs<T> qa, int64_t n_rows, KnnInfo *info,
     DTKX_HIP( hipStreamSynchronize( s ) );
     if ( info )
         info->n_overflow += n_over;
-    if ( n_over == 0 )
-        return;
-    DeviceBuffer<int2> gstack( static_cast<std::size_t>( kMaxDepth ) * n_over,
-                               s );
-    qa.overflow_rows = nullptr;
-    qa.n_overflow = nullptr;
-    qa.rows = rows.get();
-    qa.n_rows_dev = count.get();
-    qa.n_rows = n_over;
-    qa.gstack = gstack.get();
-    qa.gstride = n_over;
-    QueryLauncher<T>::dispatch( kmax, 0, qa, s );
+    if ( n_over > 0 )
+    {
+        DeviceBuffer<int2> gstack(
+            static_cast<std::size_t>( kMaxDepth ) * n_over, s );
+        QueryArgs<T> ra = qa;
+        ra.overflow_rows = nullptr;
+        ra.n_overflow = nullptr;
+        ra.rows = rows.get();
+        ra.n_rows_dev = count.get();
+        ra.n_rows = n_over;
+        ra.gstack = gstack.get();
+        ra.gstride = n_over;
+        QueryLauncher<T>::dispatch( kmax, 0, ra, s );
+    }
+    // two-pass rows of large K are written unordered: sort them (a no-op
+    // launch mode for every other K)
+    QueryLauncher<T>::dispatch( kmax, kSortRows, qa, s );
 }
 
 template <typename T>

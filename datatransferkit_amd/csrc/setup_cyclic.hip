@@ -35,8 +35,8 @@ constexpr int kBlock = 64;
 // the same rotations on it (identical values, so both agree), and owns
 // R = M / H rows of V.
 template <int M, int H, bool F32, bool STORE>
-__global__ void __launch_bounds__( kBlock )
-    setup_cyclic_kernel( double4 const *__restrict__ src,
+__device__ __forceinline__ void
+    cyclic_system( int64_t const li, double4 const *__restrict__ src,
                          double4 const *__restrict__ tgt, int64_t n_tgt,
                          int32_t const *__restrict__ indices,
                          int32_t const *__restrict__ order, int64_t row0,
@@ -54,7 +54,6 @@ __global__ void __launch_bounds__( kBlock )
     constexpr int R = M / H;
     constexpr int SPB = kBlock / H; // systems per block
     int const h = threadIdx.x % H;
-    int64_t const li = xcd_tile() * SPB + threadIdx.x / H;
     bool valid;
     int64_t slot;
     if ( rows )
@@ -251,6 +250,46 @@ __global__ void __launch_bounds__( kBlock )
     }
 }
 
+// Row-list mode (a device-side list of rows, its length known only on the
+// device) runs a bounded grid that strides over the list, so an empty list
+// costs a few thousand empty blocks, not one per row of the operator.
+constexpr int kListBlocks = 2048;
+
+template <int M, int H, bool F32, bool STORE, bool LIST>
+__global__ void __launch_bounds__( kBlock )
+    setup_cyclic_kernel( double4 const *__restrict__ src,
+                         double4 const *__restrict__ tgt, int64_t n_tgt,
+                         int32_t const *__restrict__ indices,
+                         int32_t const *__restrict__ order, int64_t row0,
+                         int64_t const *__restrict__ row_off,
+                         int32_t const *__restrict__ rows,
+                         int32_t const *__restrict__ n_rows, int kk_uniform,
+                         int rbf, int geometry, int truncation, double rcond,
+                         int max_sweeps, double *__restrict__ coeffs,
+                         float *__restrict__ coeffs32,
+                         double *__restrict__ pinv_out,
+                         unsigned long long *__restrict__ counters )
+{
+    constexpr int SPB = kBlock / H;
+    auto one = [&]( int64_t tile ) {
+        cyclic_system<M, H, F32, STORE>(
+            tile * SPB + threadIdx.x / H, src, tgt, n_tgt, indices, order, row0,
+            row_off, rows, n_rows, kk_uniform, rbf, geometry, truncation, rcond,
+            max_sweeps, coeffs, coeffs32, pinv_out, counters );
+    };
+    if constexpr ( LIST )
+    {
+        // the trip count is uniform over the block (the pair shuffles and
+        // the wave-wide convergence vote need every lane); the loop costs
+        // registers, so the whole-operator form below has none
+        int64_t const n = *n_rows;
+        for ( int64_t tile = xcd_tile(); tile * SPB < n; tile += gridDim.x )
+            one( tile );
+    }
+    else
+        one( xcd_tile() );
+}
+
 template <int M, int H>
 void launch_cyclic( double4 const *src, double4 const *tgt, int64_t n_tgt,
                     int32_t const *indices, int32_t const *order,
@@ -258,10 +297,16 @@ void launch_cyclic( double4 const *src, double4 const *tgt, int64_t n_tgt,
                     double *pinv, unsigned long long *counters, hipStream_t s )
 {
     int const sweeps = p.max_rot > 0 ? static_cast<int>( p.max_rot ) : 30;
-    dim3 const grid( ceil_div( n_tgt - p.row0, kBlock / H ) ), block( kBlock );
+    int64_t blocks = ceil_div( n_tgt - p.row0, kBlock / H );
+    if ( p.rows )
+        blocks = blocks < kListBlocks ? blocks : kListBlocks;
+    dim3 const grid( blocks ), block( kBlock );
     auto run = [&]( auto f32, auto store ) {
-        setup_cyclic_kernel<M, H, decltype( f32 )::value, decltype( store )::value>
-            <<<grid, block, 0, s>>>( src, tgt, n_tgt, indices, order, p.row0,
+        auto kernel = p.rows ? setup_cyclic_kernel<M, H, decltype( f32 )::value,
+                                                   decltype( store )::value, true>
+                             : setup_cyclic_kernel<M, H, decltype( f32 )::value,
+                                                   decltype( store )::value, false>;
+        kernel<<<grid, block, 0, s>>>( src, tgt, n_tgt, indices, order, p.row0,
                                      p.row_off, p.rows, p.n_rows, p.kk,
                                      p.rbf, p.geometry, p.truncation, p.rcond,
                                      sweeps, coeffs, coeffs32, pinv, counters );

@@ -55,10 +55,13 @@ def test_knn_ties_duplicates_and_layouts(dtk, oracle_lib):
     lattice = grid_points((12, 12, 12), (0, 0, 0)) / 11.0
     src = np.concatenate([lattice, lattice[:300], rng.random((2000, 3))])
     tgt = np.concatenate([lattice[::7], rng.random((1000, 3)), src[:40]])
-    for k in (1, 6, 27):
+    for k in (1, 6, 27, 40):
         gi, gd = oracle_lib.knn(src, tgt, k)
         idx = dtk.knn(cuda(src), cuda(tgt), k)
         assert np.array_equal(idx.cpu().numpy(), gi)
+        gi_d, _ = oracle_lib.knn(src, tgt, k, geometry=1)
+        idx = dtk.knn(cuda(src), cuda(tgt), k, geometry="double")
+        assert np.array_equal(idx.cpu().numpy(), gi_d)
         idx = dtk.knn(cuda(src.T.copy()), cuda(tgt.T.copy()), k, layout="soa")
         assert np.array_equal(idx.cpu().numpy(), gi)
         idx = dtk.knn(src, tgt, k)  # host pointers are staged
@@ -75,6 +78,45 @@ def test_knn_small_source_sets(dtk, oracle_lib):
         assert idx.shape == (100, min(4, ns))
         assert np.array_equal(idx.cpu().numpy(), gi)
         assert np.array_equal(bits(dist.cpu().numpy()), bits(gd))
+
+
+@pytest.mark.parametrize("geometry", ["float", "double"])
+def test_knn_rounded_distance_ties(dtk, oracle_lib, geometry):
+    """Sources on a sphere of radius 1/4 around a target, 1e-7 apart in
+    radius: in float arithmetic thousands of them share a handful of
+    rounded distances, many from different squared distances. The k-th
+    result sits inside such a group, so the selection must take the
+    smallest indices among every source whose rounded distance ties the
+    k-th, whatever its squared distance (k from 16 up covers both the
+    one-pass list and the two-pass search with its separate row sort)."""
+    rng = np.random.default_rng(7)
+    centre = np.array([0.5, 0.5, 0.5])
+    d = rng.normal(size=(4000, 3))
+    d /= np.linalg.norm(d, axis=1)[:, None]
+    src = (centre + 0.25 * d * (1 + rng.integers(-3, 4, size=(4000, 1)) * 1e-7))
+    src = src.astype(np.float32).astype(np.float64)
+    tgt = np.concatenate([centre[None, :], rng.random((300, 3))])
+    g = 0 if geometry == "float" else 1
+    for k in (16, 24, 32, 40, 64):
+        gi, gd = oracle_lib.knn(src, tgt, k, geometry=g)
+        idx, dist = dtk.knn(cuda(src), cuda(tgt), k, geometry=geometry, with_distances=True)
+        assert np.array_equal(idx.cpu().numpy(), gi), k
+        assert np.array_equal(bits(dist.cpu().numpy()), bits(gd)), k
+
+
+@pytest.mark.parametrize("geometry", ["float", "double"])
+def test_knn_fewer_sources_than_k(dtk, oracle_lib, geometry):
+    rng = np.random.default_rng(5)
+    tgt = rng.random((200, 3))
+    g = 0 if geometry == "float" else 1
+    for ns in (1, 2, 7, 33):
+        src = rng.random((ns, 3))
+        for k in (20, 40, 64):
+            gi, gd = oracle_lib.knn(src, tgt, k, geometry=g)
+            idx, dist = dtk.knn(cuda(src), cuda(tgt), k, geometry=geometry, with_distances=True)
+            assert idx.shape == (200, min(k, ns))
+            assert np.array_equal(idx.cpu().numpy(), gi)
+            assert np.array_equal(bits(dist.cpu().numpy()), bits(gd))
 
 
 def test_knn_large(dtk, oracle_lib):
