@@ -64,6 +64,28 @@ void cache_release();
 // stream, such as graph capture.
 hipStream_t library_stream();
 
+// Work forked from stream s onto a second stream of the library's own
+// (default priority, one per device): the side stream first waits for
+// everything queued on s so far, and join() (or the destructor) makes s
+// wait for everything queued on the side stream since. Buffers the side
+// work uses are allocated on s before the fork and released on s after the
+// join, so the block cache's stream order holds for them.
+class Fork
+{
+  public:
+    explicit Fork( hipStream_t s );
+    ~Fork();
+    Fork( Fork const & ) = delete;
+    Fork &operator=( Fork const & ) = delete;
+    hipStream_t side() const { return side_; }
+    void join();
+
+  private:
+    hipStream_t main_, side_;
+    hipEvent_t fork_ = nullptr, join_ = nullptr;
+    bool joined_ = false;
+};
+
 // Owning device buffer, move-only. With a stream it draws from the block
 // cache of that stream; without one it uses hipMalloc / hipFree.
 template <typename T>

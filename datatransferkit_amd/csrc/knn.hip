@@ -142,9 +142,11 @@ __device__ inline uint64_t expand21( uint32_t v )
 }
 
 // prob (nullable): problem of each point, whose scene box is box + 6 prob
-template <typename T, typename V>
+// Key uint64_t: the 63-bit code; uint32_t: its top 30 bits (10 per axis),
+// the same order at a coarser grid.
+template <typename T, typename V, typename Key = uint64_t>
 __global__ void __launch_bounds__( kBlock )
-    morton_kernel( V const *pts, int64_t n, T const *boxes, uint64_t *codes,
+    morton_kernel( V const *pts, int64_t n, T const *boxes, Key *codes,
                    int32_t *ids, int32_t const *prob )
 {
     int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
@@ -161,8 +163,9 @@ __global__ void __launch_bounds__( kBlock )
         u = u < 0 ? 0 : ( u > 2097151.0 ? 2097151.0 : u );
         q[d] = (uint32_t)u;
     }
-    codes[i] = ( expand21( q[0] ) << 2 ) | ( expand21( q[1] ) << 1 ) |
-               expand21( q[2] );
+    uint64_t const code = ( expand21( q[0] ) << 2 ) |
+                          ( expand21( q[1] ) << 1 ) | expand21( q[2] );
+    codes[i] = sizeof( Key ) == 8 ? (Key)code : (Key)( code >> 33 );
     ids[i] = (int32_t)i;
 }
 
@@ -261,13 +264,12 @@ __device__ inline int node_problem( int32_t const *src_off, int n_prob,
 // Batch: n_int internal nodes in all; node i belongs to problem p, whose
 // sorted points are [src_off[p], src_off[p + 1]) with scene box scene + 6 p;
 // the hierarchy of each problem is built over its own range (child node
-// indices, parents and leaf codes are global).
+// indices and leaf codes are global).
 template <typename T, typename V>
 __global__ void __launch_bounds__( kBlock )
     karras_kernel( uint64_t const *codes_all, int32_t const *sorted_all,
                    V const *pts, int64_t n_int, T const *scenes,
-                   int32_t const *src_off, int n_prob, Node<T> *nodes,
-                   int32_t *parent_internal, int32_t *parent_leaf )
+                   int32_t const *src_off, int n_prob, Node<T> *nodes )
 {
     int64_t const gi = blockIdx.x * (int64_t)kBlock + threadIdx.x;
     if ( gi >= n_int )
@@ -322,12 +324,10 @@ __global__ void __launch_bounds__( kBlock )
             nd.lo[c][1] = nd.hi[c][1] = p.y;
             nd.lo[c][2] = nd.hi[c][2] = p.z;
             nd.child[c] = ~orig;
-            parent_leaf[so + b] = (int32_t)( ( gi << 1 ) | c );
             continue;
         }
         int64_t const child = io + ( c == 0 ? gamma : gamma + 1 );
         nd.child[c] = (int32_t)child;
-        parent_internal[child] = (int32_t)( ( gi << 1 ) | c );
         if ( e - b + 1 <= kTight )
         {
             T lo[3] = {INFINITY, INFINITY, INFINITY};
@@ -486,31 +486,51 @@ __device__ inline float sqrt_rn( float s ) { return __builtin_sqrtf( s ); }
 __device__ inline double sqrt_rn( double s ) { return __builtin_sqrt( s ); }
 
 constexpr int kQueryBlock = 64;
+#ifndef DTKX_KNN_MIN_WAVES
+#define DTKX_KNN_MIN_WAVES 1
+#endif
+constexpr int kQueryMinWaves = DTKX_KNN_MIN_WAVES;
 
 // Queries run with an LDS stack of kLdsStack entries; the rare one whose
 // descent needs more is re-run with a global stack of kMaxDepth entries.
+// The LDS stack bounds the occupancy (8 bytes per entry and lane): 24
+// entries fit 13 waves per CU, and no query of 1M uniform points overflows
+// them (20 entries overflow 0.35 % of the queries, whose re-run costs more
+// than the occupancy gains; 32 entries fit 9 waves and run 15 % slower).
 // Every level of the hierarchy lengthens the common prefix of the node's
 // keys (63 Morton bits, then 32 index bits for equal codes), so no path
 // from the root has more than 96 internal levels.
 #ifndef DTKX_KNN_LDS_STACK
-#define DTKX_KNN_LDS_STACK 32
+#define DTKX_KNN_LDS_STACK 24
 #endif
 constexpr int kLdsStack = DTKX_KNN_LDS_STACK;
 constexpr int kMaxDepth = 97;
 // the largest K whose two-pass rows are sorted inside the traversal kernel
-constexpr int kFusedSortK = 32;
+#ifndef DTKX_KNN_FUSED_SORT_K
+#define DTKX_KNN_FUSED_SORT_K 32
+#endif
+constexpr int kFusedSortK = DTKX_KNN_FUSED_SORT_K;
+// depth of a global traversal stack used in place of the LDS one (0: LDS)
+#ifndef DTKX_KNN_GLOBAL_FIRST
+#define DTKX_KNN_GLOBAL_FIRST 0
+#endif
+constexpr int kGlobalFirst = DTKX_KNN_GLOBAL_FIRST;
 // launch mode of the separate row sort
 constexpr int kSortRows = -1;
 // The smallest K searched in two passes. Measured for 1M uniform sources
-// and targets: with float coordinates one pass is faster up to k = 16
-// (1.65 against 1.99 ms) and two passes from k = 32 (2.99 against 3.11 ms;
-// 5.5 against 7.8 ms at k = 64); with double coordinates, whose (distance,
-// index) comparisons cost three operations, two passes win from k = 16
-// (3.2 against 4.2 ms; 5.1 against 11.9 ms at k = 32).
+// and targets with the 24-entry LDS stack: with float coordinates one pass
+// is faster up to k = 24 (1.35 against 1.67 ms at k = 16, 1.93 against
+// 2.09 ms at k = 24) and two passes from k = 32 (2.53 against 2.68 ms);
+// with double coordinates, whose (distance, index) comparisons cost three
+// operations, two passes win from k = 16 (with the 32-entry stack: 3.2
+// against 4.2 ms at k = 16, 5.1 against 11.9 ms at k = 32).
+#ifndef DTKX_KNN_TWO_PASS_KF
+#define DTKX_KNN_TWO_PASS_KF 32
+#endif
 template <typename T>
 constexpr int two_pass_k()
 {
-    return sizeof( T ) == 4 ? 32 : 16;
+    return sizeof( T ) == 4 ? DTKX_KNN_TWO_PASS_KF : 16;
 }
 
 // Arguments of a query launch. One problem: rows [row0, row0 + n_rows) of
@@ -545,6 +565,9 @@ struct QueryArgs
     int32_t const *n_rows_dev = nullptr;
     int2 *gstack = nullptr;
     int64_t gstride = 0;
+    // global-stack launch with overflow_rows set: its depth, past which a
+    // row is listed for the re-run
+    int gdepth = 0;
 };
 
 // One lane per query. STACK > 0: LDS stack of that many entries per lane
@@ -552,7 +575,7 @@ struct QueryArgs
 // per-lane stack in global memory (gstack, laid out [level][lane]) deep
 // enough for any tree.
 template <typename T, int K, int STACK>
-__global__ void __launch_bounds__( kQueryBlock )
+__global__ void __launch_bounds__( kQueryBlock, kQueryMinWaves )
     knn_query_kernel( QueryArgs<T> const a )
 {
     constexpr int LS = STACK > 0 ? STACK : 1;
@@ -640,6 +663,12 @@ __global__ void __launch_bounds__( kQueryBlock )
                 }
                 else
                 {
+                    if ( a.overflow_rows && sp == a.gdepth )
+                    {
+                        a.overflow_rows[atomicAdd( a.n_overflow, 1 )] =
+                            (int32_t)r;
+                        return;
+                    }
                     int2 e;
                     e.x = far;
                     e.y = __float_as_int( (float)fs );
@@ -862,7 +891,7 @@ __device__ inline void sort_row( QueryArgs<T> const &a, T const *q,
 }
 
 template <typename T, int K, int STACK>
-__global__ void __launch_bounds__( kQueryBlock )
+__global__ void __launch_bounds__( kQueryBlock, kQueryMinWaves )
     knn_query2_kernel( QueryArgs<T> const a )
 {
     using SK = SortKey<T>;
@@ -957,6 +986,11 @@ __global__ void __launch_bounds__( kQueryBlock )
                 }
                 else
                 {
+                    if ( a.overflow_rows && sp == a.gdepth )
+                    {
+                        overflow();
+                        return;
+                    }
                     int2 e;
                     e.x = far;
                     e.y = __float_as_int( (float)fs );
@@ -1065,7 +1099,14 @@ __global__ void __launch_bounds__( kQueryBlock )
                     st_node[sp * kQueryBlock + threadIdx.x] = c1;
                 }
                 else
+                {
+                    if ( a.overflow_rows && sp == a.gdepth )
+                    {
+                        overflow();
+                        return;
+                    }
                     a.gstack[sp * a.gstride + qi].x = c1;
+                }
                 ++sp;
             }
             else if ( g0 )
@@ -1175,7 +1216,18 @@ void run_queries( int kmax, QueryArgs<T> qa, int64_t n_rows, KnnInfo *info,
     DTKX_HIP( hipMemsetAsync( count.get(), 0, sizeof( int32_t ), s ) );
     qa.overflow_rows = rows.get();
     qa.n_overflow = count.get();
-    QueryLauncher<T>::dispatch( kmax, kLdsStack, qa, s );
+    DeviceBuffer<int2> gfirst;
+    if ( kGlobalFirst > 0 )
+    {
+        gfirst.resize( static_cast<std::size_t>( kGlobalFirst ) * n_rows, s );
+        QueryArgs<T> ga = qa;
+        ga.gstack = gfirst.get();
+        ga.gstride = n_rows;
+        ga.gdepth = kGlobalFirst;
+        QueryLauncher<T>::dispatch( kmax, 0, ga, s );
+    }
+    else
+        QueryLauncher<T>::dispatch( kmax, kLdsStack, qa, s );
     int32_t n_over = 0;
     DTKX_HIP( hipMemcpyAsync( &n_over, count.get(), sizeof( int32_t ),
                               hipMemcpyDeviceToHost, s ) );
@@ -1201,6 +1253,37 @@ void run_queries( int kmax, QueryArgs<T> qa, int64_t n_rows, KnnInfo *info,
     QueryLauncher<T>::dispatch( kmax, kSortRows, qa, s );
 }
 
+// Radix sort of (key, index) pairs over key bits [0, bits). rocprim picks a
+// merge sort below 2^20 + 1 keys, about 22 launches at 2^20 whose host-side
+// cost (some 9 us each) paces the whole build; the onesweep radix sort
+// needs a histogram launch and one launch per 8 bits.
+struct OnesweepConfig
+    : rocprim::radix_sort_config<rocprim::default_config,
+                                 rocprim::default_config,
+                                 rocprim::default_config, 0>
+{
+};
+
+template <typename Key>
+std::size_t sort_pairs_bytes( int64_t n, int bits, hipStream_t s )
+{
+    std::size_t bytes = 0;
+    DTKX_HIP( rocprim::radix_sort_pairs<OnesweepConfig>(
+        nullptr, bytes, (Key *)nullptr, (Key *)nullptr, (int32_t *)nullptr,
+        (int32_t *)nullptr, static_cast<std::size_t>( n ), 0, bits, s ) );
+    return bytes;
+}
+
+template <typename Key>
+void sort_pairs( void *temp, std::size_t bytes, Key const *keys, Key *keys_out,
+                 int32_t const *vals, int32_t *vals_out, int64_t n, int bits,
+                 hipStream_t s )
+{
+    DTKX_HIP( rocprim::radix_sort_pairs<OnesweepConfig>(
+        temp, bytes, keys, keys_out, vals, vals_out,
+        static_cast<std::size_t>( n ), 0, bits, s ) );
+}
+
 template <typename T>
 void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
                int64_t n_tgt, int k, KnnOutput const &out, KnnInfo *info,
@@ -1219,6 +1302,33 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     bbox_final<T><<<1, kBlock, 0, s>>>( partial.get(), nparts, box.get() );
     DTKX_HIP( hipGetLastError() );
 
+    // Morton order of the targets (same scene box) on the side stream: it
+    // needs only the box, and both sorts are latency-bound at these sizes,
+    // so it runs beside the sources' sort and hierarchy build
+    DeviceBuffer<int32_t> order_buf;
+    int32_t *order = out.query_order;
+    if ( !order && n_tgt > 0 )
+    {
+        order_buf.resize( n_tgt, s );
+        order = order_buf.get();
+    }
+    // (the order only has to be local, so 30-bit keys do)
+    DeviceBuffer<uint32_t> tcodes( n_tgt, s ), tcodes_sorted( n_tgt, s );
+    DeviceBuffer<int32_t> tids( n_tgt, s );
+    std::size_t const ttemp_bytes = sort_pairs_bytes<uint32_t>( n_tgt, 30, s );
+    DeviceBuffer<char> ttemp( ttemp_bytes, s );
+    Fork fork( s );
+    if ( n_tgt > 0 )
+    {
+        hipStream_t const s2 = fork.side();
+        morton_kernel<T, V, uint32_t>
+            <<<ceil_div( n_tgt, kBlock ), kBlock, 0, s2>>>(
+                tgt, n_tgt, box.get(), tcodes.get(), tids.get(), nullptr );
+        DTKX_HIP( hipGetLastError() );
+        sort_pairs( ttemp.get(), ttemp_bytes, tcodes.get(), tcodes_sorted.get(),
+                    tids.get(), order, n_tgt, 30, s2 );
+    }
+
     // Morton codes and sort: sources
     DeviceBuffer<uint64_t> codes( n_src, s ), codes_sorted( n_src, s );
     DeviceBuffer<int32_t> ids( n_src, s ), ids_sorted_buf;
@@ -1231,15 +1341,21 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     morton_kernel<T, V><<<ceil_div( n_src, kBlock ), kBlock, 0, s>>>(
         src, n_src, box.get(), codes.get(), ids.get(), nullptr );
     DTKX_HIP( hipGetLastError() );
+#if DTKX_KNN_SRC_ONESWEEP
+    std::size_t const temp_bytes = sort_pairs_bytes<uint64_t>( n_src, 63, s );
+    DeviceBuffer<char> temp( temp_bytes, s );
+    sort_pairs( temp.get(), temp_bytes, codes.get(), codes_sorted.get(),
+                ids.get(), ids_sorted, n_src, 63, s );
+#else
     std::size_t temp_bytes = 0;
-    int64_t const nmax = std::max( n_src, n_tgt );
     DTKX_HIP( hipcub::DeviceRadixSort::SortPairs(
         nullptr, temp_bytes, codes.get(), codes_sorted.get(), ids.get(),
-        ids_sorted, static_cast<int>( nmax ), 0, 63, s ) );
+        ids_sorted, static_cast<int>( n_src ), 0, 63, s ) );
     DeviceBuffer<char> temp( temp_bytes, s );
     DTKX_HIP( hipcub::DeviceRadixSort::SortPairs(
         temp.get(), temp_bytes, codes.get(), codes_sorted.get(), ids.get(),
         ids_sorted, static_cast<int>( n_src ), 0, 63, s ) );
+#endif
     DeviceBuffer<int32_t> pos_buf;
     int32_t *src_pos = out.src_pos;
     if ( out.morton_layout && !src_pos )
@@ -1255,36 +1371,16 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     // hierarchy + refit
     int64_t const n_int = n_src - 1;
     DeviceBuffer<Node<T>> nodes( std::max<int64_t>( n_int, 1 ), s );
-    DeviceBuffer<int32_t> par_int( std::max<int64_t>( n_int, 1 ), s ),
-        par_leaf( n_src, s );
     if ( n_int > 0 )
     {
         karras_kernel<T, V><<<ceil_div( n_int, kBlock ), kBlock, 0, s>>>(
             codes_sorted.get(), ids_sorted, src, n_int, box.get(), nullptr, 1,
-            nodes.get(), par_int.get(), par_leaf.get() );
+            nodes.get() );
         DTKX_HIP( hipGetLastError() );
     }
+    fork.join();
     if ( n_tgt == 0 )
         return;
-
-    // Morton order of the targets (same scene box)
-    DeviceBuffer<int32_t> order_buf;
-    int32_t *order = out.query_order;
-    if ( !order )
-    {
-        order_buf.resize( n_tgt, s );
-        order = order_buf.get();
-    }
-    {
-        DeviceBuffer<uint64_t> tcodes( n_tgt, s ), tcodes_sorted( n_tgt, s );
-        DeviceBuffer<int32_t> tids( n_tgt, s );
-        morton_kernel<T, V><<<ceil_div( n_tgt, kBlock ), kBlock, 0, s>>>(
-            tgt, n_tgt, box.get(), tcodes.get(), tids.get(), nullptr );
-        DTKX_HIP( hipGetLastError() );
-        DTKX_HIP( hipcub::DeviceRadixSort::SortPairs(
-            temp.get(), temp_bytes, tcodes.get(), tcodes_sorted.get(),
-            tids.get(), order, static_cast<int>( n_tgt ), 0, 63, s ) );
-    }
 
     QueryArgs<T> qa;
     qa.nodes = nodes.get();
@@ -1455,13 +1551,11 @@ void knn_batched_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     // one hierarchy per problem, all in one launch
     int64_t const n_int = n_src - P;
     DeviceBuffer<Node<T>> nodes( std::max<int64_t>( n_int, 1 ), s );
-    DeviceBuffer<int32_t> par_int( std::max<int64_t>( n_int, 1 ), s ),
-        par_leaf( n_src, s );
     if ( n_int > 0 )
     {
         karras_kernel<T, V><<<ceil_div( n_int, kBlock ), kBlock, 0, s>>>(
             codes_sorted.get(), out.src_perm, src, n_int, boxes.get(),
-            b.src_off, P, nodes.get(), par_int.get(), par_leaf.get() );
+            b.src_off, P, nodes.get() );
         DTKX_HIP( hipGetLastError() );
     }
     codes_sorted.release();

@@ -139,6 +139,53 @@ hipStream_t library_stream()
     return st;
 }
 
+namespace
+{
+hipStream_t side_stream()
+{
+    static std::mutex mu;
+    static std::map<int, hipStream_t> streams;
+    int dev = 0;
+    DTKX_HIP( hipGetDevice( &dev ) );
+    std::lock_guard<std::mutex> lock( mu );
+    auto it = streams.find( dev );
+    if ( it != streams.end() )
+        return it->second;
+    hipStream_t st = nullptr;
+    DTKX_HIP( hipStreamCreateWithFlags( &st, hipStreamNonBlocking ) );
+    streams[dev] = st;
+    return st;
+}
+} // namespace
+
+Fork::Fork( hipStream_t s )
+    : main_( s )
+    , side_( side_stream() )
+{
+    DTKX_HIP( hipEventCreateWithFlags( &fork_, hipEventDisableTiming ) );
+    DTKX_HIP( hipEventCreateWithFlags( &join_, hipEventDisableTiming ) );
+    DTKX_HIP( hipEventRecord( fork_, main_ ) );
+    DTKX_HIP( hipStreamWaitEvent( side_, fork_, 0 ) );
+}
+
+void Fork::join()
+{
+    if ( joined_ )
+        return;
+    joined_ = true;
+    DTKX_HIP( hipEventRecord( join_, side_ ) );
+    DTKX_HIP( hipStreamWaitEvent( main_, join_, 0 ) );
+}
+
+Fork::~Fork()
+{
+    // a failed join leaves nothing better to do than wait for the side work
+    if ( !joined_ && hipEventRecord( join_, side_ ) == hipSuccess )
+        (void)hipStreamWaitEvent( main_, join_, 0 );
+    (void)hipEventDestroy( fork_ );
+    (void)hipEventDestroy( join_ );
+}
+
 void cache_release()
 {
     int dev = 0;
