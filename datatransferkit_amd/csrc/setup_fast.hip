@@ -29,6 +29,10 @@ namespace
 
 using namespace cyc;
 constexpr int kBlock = 64;
+#ifndef DTKX_SETUP_GATHER
+#define DTKX_SETUP_GATHER 4
+#endif
+constexpr int kGather = DTKX_SETUP_GATHER;
 
 // The coefficients of a wave's rows are written with the lanes walking the
 // wave's entries in memory order (consecutive rows hold consecutive
@@ -73,12 +77,27 @@ __global__ void __launch_bounds__( kBlock, 2 )
     int32_t const *nbr = indices + rbeg;
     double4 const tp = valid ? tgt[t] : double4{0, 0, 0, 0};
 
+    // the neighbours are gathered kGather at a time, so that many loads of
+    // a lane are in flight together (one at a time, the two passes wait on
+    // 2 kk dependent index-then-point loads)
+    auto gather = [&]( int l0, double4 ( &p )[kGather] ) {
+#pragma unroll
+        for ( int j = 0; j < kGather; ++j )
+            p[j] = src[nbr[l0 + j < kk ? l0 + j : kk - 1]];
+    };
+
     // radius (Impl.hpp:118-134)
     double dmax = 10. * DBL_EPSILON;
-    for ( int l = 0; l < kk; ++l )
+    for ( int l0 = 0; l0 < kk; l0 += kGather )
     {
-        double const d = nbr_distance<M, F32>( geometry, src[nbr[l]], tp );
-        dmax = d > dmax ? d : dmax;
+        double4 p[kGather];
+        gather( l0, p );
+#pragma unroll
+        for ( int j = 0; j < kGather; ++j )
+        {
+            double const d = nbr_distance<M, F32>( geometry, p[j], tp );
+            dmax = d > dmax ? d : dmax; // a repeated last entry changes nothing
+        }
     }
     double const radius = F32 ? (double)( 1.1f * (float)dmax ) : 1.1 * dmax;
 
@@ -87,18 +106,26 @@ __global__ void __launch_bounds__( kBlock, 2 )
 #pragma unroll
     for ( int e = 0; e < S::size; ++e )
         A[e] = 0.0;
-    for ( int l = 0; l < kk; ++l )
+    for ( int l0 = 0; l0 < kk; l0 += kGather )
     {
-        double P[M];
-        double const phi =
-            nbr_row<M, F32>( geometry, rbf, radius, src[nbr[l]], tp, P );
+        double4 p[kGather];
+        gather( l0, p );
 #pragma unroll
-        for ( int r = 0; r < M; ++r )
+        for ( int j = 0; j < kGather; ++j )
         {
-            double const pr = P[r] * phi;
+            if ( l0 + j >= kk )
+                break;
+            double P[M];
+            double const phi =
+                nbr_row<M, F32>( geometry, rbf, radius, p[j], tp, P );
 #pragma unroll
-            for ( int c = r; c < M; ++c )
-                A[S::at( r, c )] = fma( pr, P[c], A[S::at( r, c )] );
+            for ( int r = 0; r < M; ++r )
+            {
+                double const pr = P[r] * phi;
+#pragma unroll
+                for ( int c = r; c < M; ++c )
+                    A[S::at( r, c )] = fma( pr, P[c], A[S::at( r, c )] );
+            }
         }
     }
     double trace = 0.0;
@@ -210,26 +237,42 @@ __global__ void __launch_bounds__( kBlock, 2 )
     int const nrows = n_tgt - slot0 < kBlock ? (int)( n_tgt - slot0 ) : kBlock;
     int64_t const wbeg = sh.beg[0];
     int64_t const wend = sh.beg[nrows - 1] + sh.kk[nrows - 1];
+    // (kGather entries per lane and round, their loads issued together)
     int row = 0;
-    for ( int64_t g = wbeg + threadIdx.x; g < wend; g += kBlock )
+    for ( int64_t g0 = wbeg + threadIdx.x; g0 < wend;
+          g0 += kGather * kBlock )
     {
-        // the row holding entry g: the last row beginning at or before g
-        while ( row + 1 < nrows && sh.beg[row + 1] <= g )
-            ++row;
-        double const *xr = sh.x[row];
-        double const rr = sh.geo[row][0];
-        double4 const tr = {sh.geo[row][1], sh.geo[row][2], sh.geo[row][3], 0.0};
-        double P[M];
-        double const phi = nbr_row<M, F32>( geometry, rbf, rr, src[indices[g]],
-                                            tr, P );
-        double acc = 0.0;
+        double4 p[kGather];
 #pragma unroll
-        for ( int j = 0; j < M; ++j )
-            acc = fma( xr[j], P[j], acc );
-        if constexpr ( F32 )
-            coeffs32[g] = (float)( acc * phi );
-        else
-            coeffs[g] = acc * phi;
+        for ( int j = 0; j < kGather; ++j )
+        {
+            int64_t const g = g0 + j * kBlock;
+            p[j] = src[indices[g < wend ? g : wend - 1]];
+        }
+#pragma unroll
+        for ( int j = 0; j < kGather; ++j )
+        {
+            int64_t const g = g0 + j * kBlock;
+            if ( g >= wend )
+                break;
+            // the row holding entry g: the last row beginning at or before g
+            while ( row + 1 < nrows && sh.beg[row + 1] <= g )
+                ++row;
+            double const *xr = sh.x[row];
+            double const rr = sh.geo[row][0];
+            double4 const tr = {sh.geo[row][1], sh.geo[row][2], sh.geo[row][3],
+                                0.0};
+            double P[M];
+            double const phi = nbr_row<M, F32>( geometry, rbf, rr, p[j], tr, P );
+            double acc = 0.0;
+#pragma unroll
+            for ( int q = 0; q < M; ++q )
+                acc = fma( xr[q], P[q], acc );
+            if constexpr ( F32 )
+                coeffs32[g] = (float)( acc * phi );
+            else
+                coeffs[g] = acc * phi;
+        }
     }
 }
 
