@@ -147,6 +147,15 @@ def load_traffic(key):
         return None
 
 
+def query_kernel(geo, k):
+    """The query kernel the library launches for k results: the K bucket,
+    the one- or two-pass form (two passes from K = 32 for float and K = 16
+    for double coordinates) and the 24-entry LDS stack."""
+    kb = next(v for v in (1, 4, 8, 10, 16, 24, 32, 48, 64) if k <= v)
+    two = kb >= (32 if geo == "float" else 16)
+    return f"knn_query{'2' if two else ''}_kernel<{geo},{kb},24>"
+
+
 def kernel_rooflines(form, r, n, m, kk, nnz, fields):
     """The three stages' kernels against their rooflines: the kNN query and
     the apply against HBM, the setup against FP64."""
@@ -159,10 +168,10 @@ def kernel_rooflines(form, r, n, m, kk, nnz, fields):
     geo = "float"
     out = {
         "knn_query": {
-            "bound": "hbm", "kernel": f"knn_query_kernel<{geo},{kk},32>",
+            "bound": "hbm", "kernel": query_kernel(geo, kk),
             "ms": r["ms_knn_query"], "achieved": round(query, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(query / HBM_PEAK_GBS, 5),
-            "traffic": load_traffic(f"knn_query_kernel<{geo},{kk},32>@{n}x{n}"),
+            "traffic": load_traffic(f"{query_kernel(geo, kk)}@{n}x{n}"),
             "algorithmic_per_unit": f"B_knn = 24 N_s + N_t (24 + 4k) = {b_knn(n, n, kk)} bytes "
                                     f"per launch ({n} queries)",
         },

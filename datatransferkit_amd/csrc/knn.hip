@@ -504,6 +504,10 @@ constexpr int kQueryMinWaves = DTKX_KNN_MIN_WAVES;
 #define DTKX_KNN_LDS_STACK 24
 #endif
 constexpr int kLdsStack = DTKX_KNN_LDS_STACK;
+// stack entries of the fixed-radius pass of a two-pass search when its row
+// is collected in the LDS stack's other slots (its radius is final, so it
+// descends far less deep than the first pass)
+constexpr int kPass2Stack = 16;
 constexpr int kMaxDepth = 97;
 // the largest K whose two-pass rows are sorted inside the traversal kernel
 #ifndef DTKX_KNN_FUSED_SORT_K
@@ -855,13 +859,15 @@ __device__ inline void bitonic_sort( typename SortKey<T>::type ( &a )[N] )
             }
 }
 
-// A two-pass row sorted by (distance, index), distances recomputed exactly
-// as the traversals computed them, indices then mapped to positions for the
-// Morton layout. (Fused: the LDS stack, not the network's registers, limits
-// the kernel's occupancy.)
-template <typename T, int K>
+// A two-pass row (its unordered indices read through get(j)) sorted by
+// (distance, index) into row, distances recomputed exactly as the
+// traversals computed them, indices then mapped to positions for the Morton
+// layout. (Fused: the LDS stack, not the network's registers, limits the
+// kernel's occupancy.)
+template <typename T, int K, typename Get>
 __device__ inline void sort_row( QueryArgs<T> const &a, T const *q,
-                                 int64_t base, int kk, int32_t *row )
+                                 int64_t base, int kk, int32_t *row,
+                                 Get const &get )
 {
     using SK = SortKey<T>;
     constexpr int N = pow2_at_least( K );
@@ -872,7 +878,7 @@ __device__ inline void sort_row( QueryArgs<T> const &a, T const *q,
         key[j] = SK::none();
         if ( j < kk )
         {
-            int32_t const id = row[j];
+            int32_t const id = get( j );
             typename Vec4<T>::type const p = a.src[id];
             T const b[3] = {p.x, p.y, p.z};
             key[j] = SK::make( sqrt_rn( box_sq( q, b, b ) ), id );
@@ -1040,26 +1046,46 @@ __global__ void __launch_bounds__( kQueryBlock, kQueryMinWaves )
     for ( int j = 0; j < K; ++j )
         c_lt += ( list.d[j] >= 0 && sqrt_rn( list.d[j] ) < dk ) ? 1 : 0;
     T const s_hi = root_bound( dk );
+    // The row is collected in LDS when it fits beside pass 2's stack (whose
+    // pass-1 distances are dead): the stack keeps the first kPass2Stack node
+    // slots, the row takes the distance slots (as ints) and the node slots
+    // above. Scattered 4-byte stores to the row in global memory, spread
+    // over the traversal, would leave partly written lines for the L2 to
+    // evict (8 times the row's bytes, measured).
+    constexpr int kDistSlots = STACK * (int)( sizeof( T ) / 4 );
+    constexpr bool kRowLds = STACK > kPass2Stack && K <= kFusedSortK &&
+                             K <= kDistSlots + STACK - kPass2Stack;
+    constexpr int kStack2 = kRowLds ? kPass2Stack : STACK;
+    int32_t *const ids_d = reinterpret_cast<int32_t *>( st_d );
+    auto slot = [&]( int j ) -> int32_t & {
+        if constexpr ( kRowLds )
+            return j < kDistSlots
+                       ? ids_d[j * kQueryBlock + threadIdx.x]
+                       : st_node[( kPass2Stack + j - kDistSlots ) * kQueryBlock +
+                                 threadIdx.x];
+        else
+            return row[j];
+    };
     int n_lt = 0, n_eq = 0;
     auto take = [&]( T s, int32_t id ) {
         T const d = sqrt_rn( s );
         if ( d < dk )
         {
             if ( n_lt < c_lt )
-                row[n_lt++] = id;
+                slot( n_lt++ ) = id;
         }
         else if ( d == dk )
         {
             if ( c_lt + n_eq < kk )
-                row[c_lt + n_eq++] = id;
+                slot( c_lt + n_eq++ ) = id;
             else
             {
                 // all tie slots taken: replace the largest index if larger
                 int jm = c_lt;
                 for ( int j = c_lt + 1; j < kk; ++j )
-                    jm = row[j] > row[jm] ? j : jm;
-                if ( id < row[jm] )
-                    row[jm] = id;
+                    jm = slot( j ) > slot( jm ) ? j : jm;
+                if ( id < slot( jm ) )
+                    slot( jm ) = id;
             }
         }
     };
@@ -1091,7 +1117,7 @@ __global__ void __launch_bounds__( kQueryBlock, kQueryMinWaves )
                 next = c0;
                 if ( STACK > 0 )
                 {
-                    if ( sp == STACK )
+                    if ( sp == kStack2 )
                     {
                         overflow();
                         return;
@@ -1123,7 +1149,7 @@ __global__ void __launch_bounds__( kQueryBlock, kQueryMinWaves )
         }
     }
     if constexpr ( K <= kFusedSortK )
-        sort_row<T, K>( a, q, base, kk, row );
+        sort_row<T, K>( a, q, base, kk, row, slot );
 }
 
 // The row sort of a two-pass launch for K > kFusedSortK, whose sorting
@@ -1144,7 +1170,8 @@ __global__ void __launch_bounds__( kQueryBlock )
     T const q[3] = {qp.x, qp.y, qp.z};
     int64_t const base =
         a.row_off ? a.row_off[r] : ( a.src_pos ? r : (int64_t)t ) * kk;
-    sort_row<T, K>( a, q, base, kk, a.out_idx + base );
+    int32_t *row = a.out_idx + base;
+    sort_row<T, K>( a, q, base, kk, row, [row]( int j ) { return row[j]; } );
 }
 
 template <typename T>
