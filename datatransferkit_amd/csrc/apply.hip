@@ -72,13 +72,35 @@ __global__ void __launch_bounds__( kBlock )
     xs[p] = v;
 }
 
+// The same packing driven by the sources: each lane reads its source's
+// fields (coalesced across lanes) and writes one packed record to the
+// source's position, instead of reading fields at permuted addresses.
+template <typename X, typename XS, int NF>
+__global__ void __launch_bounds__( kBlock )
+    scatter_fields_kernel( X const *__restrict__ x, int64_t ld, int nf,
+                           int32_t const *__restrict__ spos, int64_t n,
+                           Pack<XS, NF> *__restrict__ xs )
+{
+    int64_t const s = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( s >= n )
+        return;
+    Pack<XS, NF> v;
+#pragma unroll
+    for ( int f = 0; f < NF; ++f )
+        v.v[f] = f < nf ? (XS)x[f * ld + s] : XS( 0 );
+    xs[spos[s]] = v;
+}
+
 // One lane per stored row (consecutive lanes: spatially adjacent targets);
 // every field of the row reuses the row's coefficients and indices.
 // Sequential sum per row, j ascending. Each wave owns 64 consecutive rows
 // and stages their indices and coefficients through LDS kChunk columns at
 // a time, so the HBM reads are runs of whole rows rather than one strided
 // element per lane.
-constexpr int kChunk = 8;
+#ifndef DTKX_APPLY_CHUNK
+#define DTKX_APPLY_CHUNK 16
+#endif
+constexpr int kChunk = DTKX_APPLY_CHUNK;
 constexpr int kPitch = kChunk + 1; // odd row pitch: no LDS bank conflicts
 
 template <typename C, typename XS, typename X, typename Acc, int NF>
@@ -152,9 +174,14 @@ void apply_pass( RowLayout const &g, C const *coeffs, X const *x,
                  hipStream_t s )
 {
     auto *xs = reinterpret_cast<Pack<XS, NF> *>( scratch );
-    gather_fields_kernel<X, XS, NF>
-        <<<ceil_div( g.n_src, kBlock ), kBlock, 0, s>>>( x, ld_src, nf,
-                                                         g.perm, g.n_src, xs );
+    if ( g.spos )
+        scatter_fields_kernel<X, XS, NF>
+            <<<ceil_div( g.n_src, kBlock ), kBlock, 0, s>>>(
+                x, ld_src, nf, g.spos, g.n_src, xs );
+    else
+        gather_fields_kernel<X, XS, NF>
+            <<<ceil_div( g.n_src, kBlock ), kBlock, 0, s>>>(
+                x, ld_src, nf, g.perm, g.n_src, xs );
     crs_rows_kernel<C, XS, X, Acc, NF>
         <<<ceil_div( g.n_tgt, kBlock ), kBlock, 0, s>>>(
             g.idx, coeffs, g.n_tgt, g.kk, xs, g.order, y, ld_tgt, nf );

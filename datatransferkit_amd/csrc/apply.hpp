@@ -33,6 +33,7 @@ struct RowLayout
     int32_t const *order = nullptr; // row -> target (nullable)
     int32_t const *perm = nullptr;  // position -> source (nullable)
     int32_t const *trow = nullptr;  // target -> row (nullable, inverse of order)
+    int32_t const *spos = nullptr;  // source -> position (nullable, inverse of perm)
     int64_t n_src = 0, n_tgt = 0;
     int kk = 0;
 };

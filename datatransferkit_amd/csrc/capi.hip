@@ -27,7 +27,7 @@ struct dtkx_mls
     // CRS rows in the Morton order of the targets (row r: target order[r],
     // trow the inverse), columns as Morton positions of the sources
     // (position p: source perm[p]); empty buffers mean caller order
-    dtkx::DeviceBuffer<int32_t> offsets, indices, order, trow, perm;
+    dtkx::DeviceBuffer<int32_t> offsets, indices, order, trow, perm, spos;
     dtkx::DeviceBuffer<double> coeffs;
     // per-row pseudo-inverse, packed upper triangle (store_pinv)
     dtkx::DeviceBuffer<double> pinv;
@@ -50,6 +50,7 @@ struct dtkx_mls
         g.order = order.get();
         g.perm = perm.get();
         g.trow = trow.get();
+        g.spos = spos.get();
         g.n_src = n_src;
         g.n_tgt = n_tgt;
         g.kk = kk;
@@ -406,9 +407,11 @@ int dtkx_mls_create( const double *src_xyz, int64_t n_src,
             int const k = opts->k > 0 ? opts->k : h->m;
             dtkx::KnnOutput kout;
             h->perm.resize( n_src, h->stream );
+            h->spos.resize( n_src, h->stream );
             kout.indices = h->indices.get();
             kout.query_order = h->order.get();
             kout.src_perm = h->perm.get();
+            kout.src_pos = h->spos.get();
             kout.morton_layout = true;
             if ( opts->geometry == DTKX_GEOMETRY_FLOAT )
             {
