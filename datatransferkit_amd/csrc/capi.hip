@@ -28,6 +28,8 @@ struct dtkx_mls
     // trow the inverse), columns as Morton positions of the sources
     // (position p: source perm[p]); empty buffers mean caller order
     dtkx::DeviceBuffer<int32_t> offsets, indices, order, trow, perm, spos;
+    // create only: each row's largest neighbour distance (FP32 bulk)
+    dtkx::DeviceBuffer<float> row_max;
     dtkx::DeviceBuffer<double> coeffs;
     // per-row pseudo-inverse, packed upper triangle (store_pinv)
     dtkx::DeviceBuffer<double> pinv;
@@ -258,6 +260,7 @@ void run_setup( dtkx_mls *h, unsigned long long *counters_d )
         // the FP32-bulk form uses the cyclic solver unless Cholesky is asked
         p.max_rot = h->opts.max_rotations; // sweeps; 0 = default
         bool const f32 = h->opts.precision == DTKX_PRECISION_FP32_BULK;
+        p.row_max = f32 ? h->row_max.get() : nullptr;
         auto *setup = h->opts.solver == DTKX_SOLVER_CHOLESKY
                           ? &dtkx::setup_fast
                           : &dtkx::setup_cyclic;
@@ -408,6 +411,12 @@ int dtkx_mls_create( const double *src_xyz, int64_t n_src,
             dtkx::KnnOutput kout;
             h->perm.resize( n_src, h->stream );
             h->spos.resize( n_src, h->stream );
+            // the FP32-bulk setup takes its radius from the search
+            if ( opts->precision == DTKX_PRECISION_FP32_BULK )
+            {
+                h->row_max.resize( n_tgt, h->stream );
+                kout.row_max = h->row_max.get();
+            }
             kout.indices = h->indices.get();
             kout.query_order = h->order.get();
             kout.src_perm = h->perm.get();
@@ -442,6 +451,7 @@ int dtkx_mls_create( const double *src_xyz, int64_t n_src,
             ev.mark( 1 );
             h->stats.n_stack_overflow = info.n_overflow;
             run_setup( h.get(), counters.get() );
+            h->row_max = {};
             ev.mark( 2 );
         }
         finish_create( h.get(), counters.get() );
@@ -1077,6 +1087,13 @@ int dtkx_mls_batch_create( int32_t n_problems, const dtkx_mls_problem *problems,
         ko.src_pos = b->pos.get();
         ko.row_off = b->row_off.get();
         ko.indices = b->indices.get();
+        // the FP32-bulk setup takes its radius from the search
+        dtkx::DeviceBuffer<float> row_max;
+        if ( opts->precision == DTKX_PRECISION_FP32_BULK && nt > 0 )
+        {
+            row_max.resize( nt, s );
+            ko.row_max = row_max.get();
+        }
         dtkx::KnnInfo info;
         if ( nt > 0 )
             ev.pass_query_events( info );
@@ -1118,6 +1135,7 @@ int dtkx_mls_batch_create( int32_t n_problems, const dtkx_mls_problem *problems,
             {
                 p.max_rot = opts->max_rotations;
                 bool const f32 = opts->precision == DTKX_PRECISION_FP32_BULK;
+                p.row_max = f32 ? row_max.get() : nullptr;
                 auto *setup = opts->solver == DTKX_SOLVER_CHOLESKY
                                   ? &dtkx::setup_fast
                                   : &dtkx::setup_cyclic;

@@ -559,6 +559,8 @@ struct QueryArgs
     double *out_dist = nullptr;
     // Morton layout: row r written at row r with indices as positions
     int32_t const *src_pos = nullptr;
+    // nullable: row r's largest neighbour distance, written at row r
+    float *row_max = nullptr;
     // LDS-stack launch: rows whose descent would push past the LDS stack
     // are listed here (and written by the re-run) instead of finished
     int32_t *overflow_rows = nullptr;
@@ -727,6 +729,8 @@ __global__ void __launch_bounds__( kQueryBlock, kQueryMinWaves )
             a.out_idx[base + j] = a.src_pos ? a.src_pos[id] : id;
             if ( a.out_dist )
                 a.out_dist[base + j] = best.distance( j );
+            if ( a.row_max && j == kk - 1 )
+                a.row_max[r] = (float)best.distance( j );
         }
 }
 
@@ -866,8 +870,8 @@ __device__ inline void bitonic_sort( typename SortKey<T>::type ( &a )[N] )
 // kernel's occupancy.)
 template <typename T, int K, typename Get>
 __device__ inline void sort_row( QueryArgs<T> const &a, T const *q,
-                                 int64_t base, int kk, int32_t *row,
-                                 Get const &get )
+                                 int64_t r, int64_t base, int kk,
+                                 int32_t *row, Get const &get )
 {
     using SK = SortKey<T>;
     constexpr int N = pow2_at_least( K );
@@ -893,6 +897,8 @@ __device__ inline void sort_row( QueryArgs<T> const &a, T const *q,
             row[j] = a.src_pos ? a.src_pos[id] : id;
             if ( a.out_dist )
                 a.out_dist[base + j] = SK::distance( key[j] );
+            if ( a.row_max && j == kk - 1 )
+                a.row_max[r] = (float)SK::distance( key[j] );
         }
 }
 
@@ -1149,7 +1155,7 @@ __global__ void __launch_bounds__( kQueryBlock, kQueryMinWaves )
         }
     }
     if constexpr ( K <= kFusedSortK )
-        sort_row<T, K>( a, q, base, kk, row, slot );
+        sort_row<T, K>( a, q, r, base, kk, row, slot );
 }
 
 // The row sort of a two-pass launch for K > kFusedSortK, whose sorting
@@ -1171,7 +1177,8 @@ __global__ void __launch_bounds__( kQueryBlock )
     int64_t const base =
         a.row_off ? a.row_off[r] : ( a.src_pos ? r : (int64_t)t ) * kk;
     int32_t *row = a.out_idx + base;
-    sort_row<T, K>( a, q, base, kk, row, [row]( int j ) { return row[j]; } );
+    sort_row<T, K>( a, q, r, base, kk, row,
+                    [row]( int j ) { return row[j]; } );
 }
 
 template <typename T>
@@ -1421,6 +1428,7 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     qa.out_idx = out.indices;
     qa.out_dist = out.dist;
     qa.src_pos = out.morton_layout ? src_pos : nullptr;
+    qa.row_max = out.morton_layout ? out.row_max : nullptr;
     if ( info && info->query_begin )
         DTKX_HIP( hipEventRecord( info->query_begin, s ) );
     run_queries<T>( k, qa, n_tgt, info, s );
@@ -1617,6 +1625,7 @@ void knn_batched_impl( void const *src_v, int64_t n_src, void const *tgt_v,
         qa.row_off = out.row_off;
         qa.out_idx = out.indices;
         qa.src_pos = out.src_pos;
+        qa.row_max = out.row_max;
         run_queries<T>( l.kmax, qa, l.end - l.begin, info, s );
     }
     if ( info && info->query_end )

@@ -40,6 +40,9 @@ struct KnnOutput
     int32_t *src_perm = nullptr;
     int32_t *src_pos = nullptr;
     bool morton_layout = false;
+    // nullable, Morton layout: each row's largest neighbour distance (the
+    // k-th, as the search computes it)
+    float *row_max = nullptr;
 };
 
 // src/tgt: float4 (geometry FLOAT) or double4 (geometry DOUBLE) device
@@ -87,6 +90,7 @@ struct KnnBatchOutput
     int32_t *src_pos = nullptr;  // [n_src] source -> position
     int64_t *row_off = nullptr;  // [n_tgt + 1]
     int32_t *indices = nullptr;  // [nnz]
+    float *row_max = nullptr;    // nullable [n_tgt]: largest distance per row
 };
 
 void knn_search_batched( int geometry, void const *src, int64_t n_src,

@@ -31,6 +31,9 @@ struct SetupParams
     // i < *n_rows (device count, at most n_tgt - row0) instead of a range
     int32_t const *rows = nullptr;
     int32_t const *n_rows = nullptr;
+    // FP32 bulk (nullable): each row's largest neighbour distance from the
+    // kNN, which then gives the radius without a pass over the neighbours
+    float const *row_max = nullptr;
 };
 
 // counters[0] = n_underdetermined, [1] = n_unconverged, [2] = rotations

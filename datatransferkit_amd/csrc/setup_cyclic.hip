@@ -48,7 +48,8 @@ __device__ __forceinline__ void
                          int max_sweeps, double *__restrict__ coeffs,
                          float *__restrict__ coeffs32,
                          double *__restrict__ pinv_out,
-                         unsigned long long *__restrict__ counters )
+                         unsigned long long *__restrict__ counters,
+                         float const *__restrict__ row_max )
 {
     using S = SymIdx<M>;
     constexpr int R = M / H;
@@ -75,9 +76,15 @@ __device__ __forceinline__ void
     int32_t const *nbr = indices + rbeg;
     double4 const tp = valid ? tgt[t] : double4{0, 0, 0, 0};
 
-    // radius (Impl.hpp:118-134): the largest neighbour distance
+    // radius (Impl.hpp:118-134): the largest neighbour distance, for the
+    // FP32-bulk form possibly taken from the kNN
     double dmax = 10. * DBL_EPSILON;
-    if ( valid )
+    if ( F32 && row_max )
+    {
+        double const d = valid ? (double)row_max[slot] : 0.0;
+        dmax = d > dmax ? d : dmax;
+    }
+    else if ( valid )
         for ( int l = 0; l < kk; ++l )
         {
             double const d = nbr_distance<M, F32>( geometry, src[nbr[l]], tp );
@@ -268,14 +275,15 @@ __global__ void __launch_bounds__( kBlock )
                          int max_sweeps, double *__restrict__ coeffs,
                          float *__restrict__ coeffs32,
                          double *__restrict__ pinv_out,
-                         unsigned long long *__restrict__ counters )
+                         unsigned long long *__restrict__ counters,
+                         float const *__restrict__ row_max )
 {
     constexpr int SPB = kBlock / H;
     auto one = [&]( int64_t tile ) {
         cyclic_system<M, H, F32, STORE>(
             tile * SPB + threadIdx.x / H, src, tgt, n_tgt, indices, order, row0,
             row_off, rows, n_rows, kk_uniform, rbf, geometry, truncation, rcond,
-            max_sweeps, coeffs, coeffs32, pinv_out, counters );
+            max_sweeps, coeffs, coeffs32, pinv_out, counters, row_max );
     };
     if constexpr ( LIST )
     {
@@ -309,7 +317,8 @@ void launch_cyclic( double4 const *src, double4 const *tgt, int64_t n_tgt,
         kernel<<<grid, block, 0, s>>>( src, tgt, n_tgt, indices, order, p.row0,
                                      p.row_off, p.rows, p.n_rows, p.kk,
                                      p.rbf, p.geometry, p.truncation, p.rcond,
-                                     sweeps, coeffs, coeffs32, pinv, counters );
+                                     sweeps, coeffs, coeffs32, pinv, counters,
+                                     p.row_max );
     };
     using T = std::true_type;
     using F = std::false_type;

@@ -61,7 +61,8 @@ __global__ void __launch_bounds__( kBlock, 2 )
                            double *__restrict__ pinv_out,
                            int32_t *__restrict__ fallback,
                            int32_t *__restrict__ n_fallback,
-                           unsigned long long *__restrict__ counters )
+                           unsigned long long *__restrict__ counters,
+                           float const *__restrict__ row_max )
 {
     using S = SymIdx<M>;
     __shared__ RowShare<M> sh;
@@ -86,17 +87,27 @@ __global__ void __launch_bounds__( kBlock, 2 )
             p[j] = src[nbr[l0 + j < kk ? l0 + j : kk - 1]];
     };
 
-    // radius (Impl.hpp:118-134)
+    // radius (Impl.hpp:118-134); the FP32-bulk form may take the largest
+    // distance from the kNN
     double dmax = 10. * DBL_EPSILON;
-    for ( int l0 = 0; l0 < kk; l0 += kGather )
+    if ( F32 && row_max )
     {
-        double4 p[kGather];
-        gather( l0, p );
-#pragma unroll
-        for ( int j = 0; j < kGather; ++j )
+        double const d = valid ? (double)row_max[slot] : 0.0;
+        dmax = d > dmax ? d : dmax;
+    }
+    else
+    {
+        for ( int l0 = 0; l0 < kk; l0 += kGather )
         {
-            double const d = nbr_distance<M, F32>( geometry, p[j], tp );
-            dmax = d > dmax ? d : dmax; // a repeated last entry changes nothing
+            double4 p[kGather];
+            gather( l0, p );
+#pragma unroll
+            for ( int j = 0; j < kGather; ++j )
+            {
+                double const d = nbr_distance<M, F32>( geometry, p[j], tp );
+                // a repeated last entry changes nothing
+                dmax = d > dmax ? d : dmax;
+            }
         }
     }
     double const radius = F32 ? (double)( 1.1f * (float)dmax ) : 1.1 * dmax;
@@ -290,7 +301,7 @@ void launch_fast( double4 const *src, double4 const *tgt, int64_t n_tgt,
             <<<grid, block, 0, s>>>( src, tgt, n_tgt, indices, order, p.row0,
                                      p.row_off, p.kk, p.rbf, p.geometry,
                                      p.truncation, p.rcond, coeffs, coeffs32,
-                                     pinv, list, count, counters );
+                                     pinv, list, count, counters, p.row_max );
     };
     using T = std::true_type;
     using F = std::false_type;
