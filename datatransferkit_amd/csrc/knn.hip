@@ -461,6 +461,18 @@ struct KBest<double, K>
     using type = KBestD<K>;
 };
 
+// q clamped to [lo, hi] (lo <= hi): one median-of-three for float, a max
+// and a min for double, the values of q < lo ? lo : (q > hi ? hi : q) (a
+// signed-zero difference between the forms squares to the same zero).
+__device__ inline float clamp_to( float q, float lo, float hi )
+{
+    return __builtin_amdgcn_fmed3f( q, lo, hi );
+}
+__device__ inline double clamp_to( double q, double lo, double hi )
+{
+    return fmin( fmax( q, lo ), hi );
+}
+
 // Squared distance from q to its projection onto [lo, hi]: the operation
 // sequence of the point distance (tmp = b - a; s += tmp * tmp), so for a
 // degenerate box it is that distance exactly and otherwise a lower bound.
@@ -472,8 +484,7 @@ __device__ inline T box_sq( T const *q, T const *lo, T const *hi )
 #pragma unroll
     for ( int d = 0; d < 3; ++d )
     {
-        T c = q[d] < lo[d] ? lo[d] : ( q[d] > hi[d] ? hi[d] : q[d] );
-        T tmp = c - q[d];
+        T tmp = clamp_to( q[d], lo[d], hi[d] ) - q[d];
         s += tmp * tmp;
     }
     return s;
@@ -486,10 +497,6 @@ __device__ inline float sqrt_rn( float s ) { return __builtin_sqrtf( s ); }
 __device__ inline double sqrt_rn( double s ) { return __builtin_sqrt( s ); }
 
 constexpr int kQueryBlock = 64;
-#ifndef DTKX_KNN_MIN_WAVES
-#define DTKX_KNN_MIN_WAVES 1
-#endif
-constexpr int kQueryMinWaves = DTKX_KNN_MIN_WAVES;
 
 // Queries run with an LDS stack of kLdsStack entries; the rare one whose
 // descent needs more is re-run with a global stack of kMaxDepth entries.
@@ -514,11 +521,7 @@ constexpr int kMaxDepth = 97;
 #define DTKX_KNN_FUSED_SORT_K 32
 #endif
 constexpr int kFusedSortK = DTKX_KNN_FUSED_SORT_K;
-// depth of a global traversal stack used in place of the LDS one (0: LDS)
-#ifndef DTKX_KNN_GLOBAL_FIRST
-#define DTKX_KNN_GLOBAL_FIRST 0
-#endif
-constexpr int kGlobalFirst = DTKX_KNN_GLOBAL_FIRST;
+
 // launch mode of the separate row sort
 constexpr int kSortRows = -1;
 // The smallest K searched in two passes. Measured for 1M uniform sources
@@ -565,15 +568,12 @@ struct QueryArgs
     // are listed here (and written by the re-run) instead of finished
     int32_t *overflow_rows = nullptr;
     int32_t *n_overflow = nullptr;
-    // global-stack re-run: rows rows[i] for i < *n_rows_dev, stack depth
-    // gdepth per lane laid out [level][lane]
+    // global-stack re-run: rows rows[i] for i < *n_rows_dev, a stack of
+    // kMaxDepth entries per lane laid out [level][lane] (stride gstride)
     int32_t const *rows = nullptr;
     int32_t const *n_rows_dev = nullptr;
     int2 *gstack = nullptr;
     int64_t gstride = 0;
-    // global-stack launch with overflow_rows set: its depth, past which a
-    // row is listed for the re-run
-    int gdepth = 0;
 };
 
 // One lane per query. STACK > 0: LDS stack of that many entries per lane
@@ -581,7 +581,7 @@ struct QueryArgs
 // per-lane stack in global memory (gstack, laid out [level][lane]) deep
 // enough for any tree.
 template <typename T, int K, int STACK>
-__global__ void __launch_bounds__( kQueryBlock, kQueryMinWaves )
+__global__ void __launch_bounds__( kQueryBlock )
     knn_query_kernel( QueryArgs<T> const a )
 {
     constexpr int LS = STACK > 0 ? STACK : 1;
@@ -669,12 +669,6 @@ __global__ void __launch_bounds__( kQueryBlock, kQueryMinWaves )
                 }
                 else
                 {
-                    if ( a.overflow_rows && sp == a.gdepth )
-                    {
-                        a.overflow_rows[atomicAdd( a.n_overflow, 1 )] =
-                            (int32_t)r;
-                        return;
-                    }
                     int2 e;
                     e.x = far;
                     e.y = __float_as_int( (float)fs );
@@ -903,7 +897,7 @@ __device__ inline void sort_row( QueryArgs<T> const &a, T const *q,
 }
 
 template <typename T, int K, int STACK>
-__global__ void __launch_bounds__( kQueryBlock, kQueryMinWaves )
+__global__ void __launch_bounds__( kQueryBlock )
     knn_query2_kernel( QueryArgs<T> const a )
 {
     using SK = SortKey<T>;
@@ -998,11 +992,6 @@ __global__ void __launch_bounds__( kQueryBlock, kQueryMinWaves )
                 }
                 else
                 {
-                    if ( a.overflow_rows && sp == a.gdepth )
-                    {
-                        overflow();
-                        return;
-                    }
                     int2 e;
                     e.x = far;
                     e.y = __float_as_int( (float)fs );
@@ -1131,14 +1120,7 @@ __global__ void __launch_bounds__( kQueryBlock, kQueryMinWaves )
                     st_node[sp * kQueryBlock + threadIdx.x] = c1;
                 }
                 else
-                {
-                    if ( a.overflow_rows && sp == a.gdepth )
-                    {
-                        overflow();
-                        return;
-                    }
                     a.gstack[sp * a.gstride + qi].x = c1;
-                }
                 ++sp;
             }
             else if ( g0 )
@@ -1250,18 +1232,7 @@ void run_queries( int kmax, QueryArgs<T> qa, int64_t n_rows, KnnInfo *info,
     DTKX_HIP( hipMemsetAsync( count.get(), 0, sizeof( int32_t ), s ) );
     qa.overflow_rows = rows.get();
     qa.n_overflow = count.get();
-    DeviceBuffer<int2> gfirst;
-    if ( kGlobalFirst > 0 )
-    {
-        gfirst.resize( static_cast<std::size_t>( kGlobalFirst ) * n_rows, s );
-        QueryArgs<T> ga = qa;
-        ga.gstack = gfirst.get();
-        ga.gstride = n_rows;
-        ga.gdepth = kGlobalFirst;
-        QueryLauncher<T>::dispatch( kmax, 0, ga, s );
-    }
-    else
-        QueryLauncher<T>::dispatch( kmax, kLdsStack, qa, s );
+    QueryLauncher<T>::dispatch( kmax, kLdsStack, qa, s );
     int32_t n_over = 0;
     DTKX_HIP( hipMemcpyAsync( &n_over, count.get(), sizeof( int32_t ),
                               hipMemcpyDeviceToHost, s ) );
