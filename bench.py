@@ -479,6 +479,86 @@ def bench_shell(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_nn(args):
+    """The nearest-neighbour operator (the reference's
+    NearestNeighborOperator): create (k = 1 search over float points) and
+    one apply of the fields per step, 1M sources -> 1M targets, with the
+    apply's bandwidth against HBM and the oracle's k = 1 search and copy
+    timed on the host beside it."""
+    import numpy as np
+    import torch
+
+    import datatransferkit_amd as dtk
+
+    n, nf = args.n, args.fields
+    rng = np.random.default_rng(5)
+    src = rng.random((n, 3))
+    tgt = 0.05 + 0.9 * rng.random((n, 3))
+    vals = rng.standard_normal((nf, n))
+    d_src, d_tgt, x = (torch.from_numpy(a).cuda() for a in (src, tgt, vals))
+    y = torch.empty((nf, n), dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def step():
+        op = dtk.NearestNeighborOperator(d_src, d_tgt, stream=stream)
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record()
+        op.apply(x, y)
+        b.record()
+        b.synchronize()
+        return op, a.elapsed_time(b)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ms_apply = []
+    for _ in range(args.steps):
+        op, ms = step()
+        ms_apply.append(ms)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    idx = op.indices().cpu().numpy()
+    exact = bool(np.array_equal(y.cpu().numpy(), vals[:, idx]))
+    ms_a = float(np.mean(ms_apply))
+    b_apply = 4 * n + nf * 16 * n
+    out = {
+        "metric": "nearest-neighbour transfer target pts/s (create + apply)",
+        "value": n / dt, "unit": "target pts/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1000 * dt,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 values, f32 search", "data": "synthetic",
+        "config": {"workload": f"NearestNeighborOperator {n} sources U[0,1]^3 -> {n} "
+                               f"targets U[0.05,0.95]^3, {nf} fields", "n": n,
+                   "fields": nf},
+        "ms_apply": ms_a,
+        "apply_roofline": {"bound": "hbm", "achieved": round(b_apply / (ms_a * 1e-3) / 1e9, 1),
+                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(b_apply / (ms_a * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                           "algorithmic": f"4 N_t + 16 F N_t = {b_apply} bytes"},
+        "values_are_copies": exact,
+    }
+    if not args.no_cpu_baseline:
+        import oracle
+
+        sample = min(n, args.cpu_sample or n)
+        threads = min(os.cpu_count() or 1, int(os.environ.get("DTKX_CPU_THREADS", "16")))
+        oracle.set_num_threads(threads)
+        t0 = time.perf_counter()
+        gi, _ = oracle.knn(src, tgt[:sample], 1)
+        ref = vals[:, gi[:, 0]]
+        dtc = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": sample / dtc, "unit": "target pts/s", "cores": threads,
+            "kind": "port",
+            "sample": f"first {sample} of {n} targets, all {n} sources, k-d tree "
+                      f"k = 1 search + copy of {nf} fields ({dtc:.1f} s)"}
+        out["parity_vs_oracle_sample"] = bool(np.array_equal(idx[:sample], gi[:, 0])
+                                              and np.array_equal(y.cpu().numpy()[:, :sample], ref))
+    print(json.dumps(out), flush=True)
+
+
 def bench_copy(args):
     """Calibration for HBM counters: a plain device-to-device copy of
     1 GiB (reads 1 GiB, writes 1 GiB per launch), repeated."""
@@ -510,10 +590,11 @@ def main():
                     help="targets timed on the CPU oracle (0: all)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="transfer",
-                    choices=["transfer", "batched", "adjoint", "shell", "copy"],
+                    choices=["transfer", "batched", "adjoint", "shell", "nn", "copy"],
                     help="transfer: one 1M x 1M operator (default); batched: many "
                          "uneven operators; adjoint: forward + backward; shell: "
-                         "FP32 bulk on a thin shell")
+                         "FP32 bulk on a thin shell; nn: the nearest-neighbour "
+                         "operator")
     ap.add_argument("--problems", type=int, default=4096)
     ap.add_argument("--max-src", type=int, default=65536)
     args = ap.parse_args()
@@ -530,6 +611,9 @@ def main():
         return
     if args.workload == "copy":
         bench_copy(args)
+        return
+    if args.workload == "nn":
+        bench_nn(args)
         return
     if args.workload == "shell":
         if args.n == 1_000_000:

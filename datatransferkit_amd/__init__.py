@@ -7,8 +7,9 @@ Jacobi pseudo-inverse, CRS apply and transpose) behind a C ABI
 """
 from ._lib import DtkxError  # noqa: F401
 from .operators import (  # noqa: F401
-    MovingLeastSquaresBatch, MovingLeastSquaresOperator, knn,
-    release_cached_memory, svd_pseudo_inverse)
+    MovingLeastSquaresBatch, MovingLeastSquaresOperator,
+    NearestNeighborOperator, knn, release_cached_memory, svd_pseudo_inverse)
 
-__all__ = ["MovingLeastSquaresOperator", "MovingLeastSquaresBatch", "knn",
-           "svd_pseudo_inverse", "release_cached_memory", "DtkxError"]
+__all__ = ["MovingLeastSquaresOperator", "MovingLeastSquaresBatch",
+           "NearestNeighborOperator", "knn", "svd_pseudo_inverse",
+           "release_cached_memory", "DtkxError"]

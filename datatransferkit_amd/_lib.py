@@ -46,6 +46,7 @@ EXPORTED = [
     "dtkx_mls_destroy", "dtkx_knn", "dtkx_svd_pseudo_inverse",
     "dtkx_mls_batch_create", "dtkx_mls_batch_apply",
     "dtkx_mls_batch_get_stats", "dtkx_mls_batch_destroy",
+    "dtkx_nn_create", "dtkx_nn_apply", "dtkx_nn_get_indices", "dtkx_nn_destroy",
 ]
 
 
@@ -134,6 +135,11 @@ def load():
     L.dtkx_mls_batch_apply.argtypes = [_vp, _vp, _vp]
     L.dtkx_mls_batch_get_stats.argtypes = [_vp, ctypes.POINTER(Stats)]
     L.dtkx_mls_batch_destroy.argtypes = [_vp]
+    L.dtkx_nn_create.argtypes = [_vp, _i64, _vp, _i64, _i32, _i32, _vp,
+                                 ctypes.POINTER(_vp)]
+    L.dtkx_nn_apply.argtypes = [_vp, _vp, _vp, _i32, _i64, _i64]
+    L.dtkx_nn_get_indices.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_i64)]
+    L.dtkx_nn_destroy.argtypes = [_vp]
     for name in EXPORTED:
         f = getattr(L, name)
         if f.restype is ctypes.c_int or f.restype is None:

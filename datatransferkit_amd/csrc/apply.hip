@@ -514,6 +514,35 @@ void crs_apply_transpose_f64( RowLayout const &g, CscIndex const &csc,
     }
 }
 
+namespace
+{
+// one lane per target: its index is read once for all fields, the reads
+// and writes of a field run along the targets
+__global__ void __launch_bounds__( kBlock )
+    nn_gather_kernel( int32_t const *__restrict__ idx, int64_t n_tgt,
+                      double const *__restrict__ x, int64_t ld_src,
+                      double *__restrict__ y, int64_t ld_tgt, int nf )
+{
+    int64_t const t = xcd_tile() * kBlock + threadIdx.x;
+    if ( t >= n_tgt )
+        return;
+    int64_t const s = idx[t];
+    for ( int f = 0; f < nf; ++f )
+        y[f * ld_tgt + t] = x[f * ld_src + s];
+}
+} // namespace
+
+void nn_gather_f64( int32_t const *idx, int64_t n_tgt, double const *x,
+                    int64_t ld_src, double *y, int64_t ld_tgt, int n_fields,
+                    hipStream_t s )
+{
+    if ( n_tgt == 0 || n_fields == 0 )
+        return;
+    nn_gather_kernel<<<ceil_div( n_tgt, kBlock ), kBlock, 0, s>>>(
+        idx, n_tgt, x, ld_src, y, ld_tgt, n_fields );
+    DTKX_HIP( hipGetLastError() );
+}
+
 void gather_fields_f64( double const *x, int64_t ld, int nf,
                         int32_t const *perm, int64_t n, double *out, int pitch,
                         hipStream_t s )

@@ -38,6 +38,12 @@ struct RowLayout
     int kk = 0;
 };
 
+// Nearest-neighbour transfer: y[f][t] = x[f][idx[t]] for every field, a
+// copy of each value (signed zeros and NaN payloads included).
+void nn_gather_f64( int32_t const *idx, int64_t n_tgt, double const *x,
+                    int64_t ld_src, double *y, int64_t ld_tgt, int n_fields,
+                    hipStream_t s );
+
 void crs_apply_f64( RowLayout const &g, double const *coeffs, double const *x,
                     int64_t ld_src, double *y, int64_t ld_tgt, int n_fields,
                     hipStream_t s );

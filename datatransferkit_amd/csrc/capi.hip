@@ -155,7 +155,9 @@ struct Staged
 };
 
 // Output view: written in place if device memory, else through a staging
-// buffer copied back by finish().
+// buffer copied back by finish(). With gaps (fields at a leading dimension
+// above their length) the staging buffer starts as a copy of the caller's
+// array, so the entries between fields come back unchanged.
 template <typename T>
 struct StagedOut
 {
@@ -163,7 +165,7 @@ struct StagedOut
     T *user = nullptr;
     T *ptr = nullptr;
     std::size_t n = 0;
-    StagedOut( T *p, std::size_t count, hipStream_t s )
+    StagedOut( T *p, std::size_t count, hipStream_t s, bool gaps = false )
         : user( p )
         , n( count )
     {
@@ -174,6 +176,9 @@ struct StagedOut
         }
         buf.resize( n, s );
         ptr = buf.get();
+        if ( gaps )
+            DTKX_HIP( hipMemcpyAsync( ptr, p, n * sizeof( T ),
+                                      hipMemcpyHostToDevice, s ) );
     }
     void finish( hipStream_t s )
     {
@@ -530,7 +535,7 @@ int dtkx_mls_apply( const dtkx_mls *h, const double *src_vals,
         std::size_t const nin = (std::size_t)( n_fields - 1 ) * ld_src + h->n_src;
         std::size_t const nout = (std::size_t)( n_fields - 1 ) * ld_tgt + h->n_tgt;
         Staged<double> x( src_vals, nin, h->stream );
-        StagedOut<double> y( tgt_vals, nout, h->stream );
+        StagedOut<double> y( tgt_vals, nout, h->stream, ld_tgt > h->n_tgt );
         if ( h->opts.precision == DTKX_PRECISION_FP32_BULK )
             dtkx::crs_apply_f32_dvals( h->layout(), h->coeffs32.get(), x.ptr,
                                        ld_src, y.ptr, ld_tgt, n_fields,
@@ -562,7 +567,7 @@ int dtkx_mls_apply_fp32( const dtkx_mls *h, const float *src_vals,
         std::size_t const nin = (std::size_t)( n_fields - 1 ) * ld_src + h->n_src;
         std::size_t const nout = (std::size_t)( n_fields - 1 ) * ld_tgt + h->n_tgt;
         Staged<float> x( src_vals, nin, h->stream );
-        StagedOut<float> y( tgt_vals, nout, h->stream );
+        StagedOut<float> y( tgt_vals, nout, h->stream, ld_tgt > h->n_tgt );
         dtkx::crs_apply_f32( h->layout(), h->coeffs32.get(), x.ptr, ld_src,
                              y.ptr, ld_tgt, n_fields, h->stream );
         y.finish( h->stream );
@@ -591,7 +596,7 @@ int dtkx_mls_apply_transpose( const dtkx_mls *h, const double *tgt_bar,
         std::size_t const nin = (std::size_t)( n_fields - 1 ) * ld_tgt + h->n_tgt;
         std::size_t const nout = (std::size_t)( n_fields - 1 ) * ld_src + h->n_src;
         Staged<double> yb( tgt_bar, nin, h->stream );
-        StagedOut<double> xb( src_bar, nout, h->stream );
+        StagedOut<double> xb( src_bar, nout, h->stream, ld_src > h->n_src );
         dtkx::crs_apply_transpose_f64( h->layout(), h->csc, h->coeffs.get(),
                                        yb.ptr, ld_tgt, xb.ptr, ld_src,
                                        n_fields, h->stream );
@@ -758,6 +763,109 @@ int dtkx_knn( const double *src_xyz, int64_t n_src, const double *tgt_xyz,
         oi.finish( s );
         od.finish( s );
         DTKX_HIP( hipStreamSynchronize( s ) );
+        return DTKX_SUCCESS;
+    } );
+}
+
+// ------------------------------------------ nearest-neighbour operator
+struct dtkx_nn
+{
+    int64_t n_src = 0, n_tgt = 0;
+    hipStream_t stream = nullptr;
+    dtkx::DeviceBuffer<int32_t> idx; // nearest source of each target
+};
+
+int dtkx_nn_create( const double *src_xyz, int64_t n_src,
+                    const double *tgt_xyz, int64_t n_tgt, int32_t geometry,
+                    int32_t layout, void *stream, dtkx_nn **out )
+{
+    return guarded( [&]() {
+        REQ( out != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        *out = nullptr;
+        hipStream_t s = static_cast<hipStream_t>( stream );
+        REQ( n_src > 0, DTKX_ERR_EMPTY_SOURCE );
+        REQ( n_tgt >= 0 && src_xyz != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( n_tgt == 0 || tgt_xyz != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( n_src <= INT32_MAX && n_tgt <= INT32_MAX, DTKX_ERR_UNSUPPORTED );
+        REQ( geometry == DTKX_GEOMETRY_FLOAT ||
+                 geometry == DTKX_GEOMETRY_DOUBLE,
+             DTKX_ERR_INVALID_ARGUMENT );
+        REQ( layout == DTKX_LAYOUT_AOS || layout == DTKX_LAYOUT_SOA,
+             DTKX_ERR_INVALID_ARGUMENT );
+        auto h = std::make_unique<dtkx_nn>();
+        h->n_src = n_src;
+        h->n_tgt = n_tgt;
+        h->stream = s;
+        if ( n_tgt > 0 )
+        {
+            h->idx.resize( n_tgt, s );
+            Staged<double> sx( src_xyz, 3 * n_src, s );
+            Staged<double> tx( tgt_xyz, 3 * n_tgt, s );
+            dtkx::KnnOutput kout;
+            kout.indices = h->idx.get();
+            if ( geometry == DTKX_GEOMETRY_FLOAT )
+            {
+                dtkx::DeviceBuffer<float4> sf( n_src, s ), tf( n_tgt, s );
+                dtkx::points_to_float4( sx.ptr, n_src, layout, sf.get(), s );
+                dtkx::points_to_float4( tx.ptr, n_tgt, layout, tf.get(), s );
+                dtkx::knn_search( geometry, sf.get(), n_src, tf.get(), n_tgt,
+                                  1, kout, nullptr, s );
+            }
+            else
+            {
+                dtkx::DeviceBuffer<double4> sd( n_src, s ), td( n_tgt, s );
+                dtkx::points_to_double4( sx.ptr, n_src, layout, sd.get(), s );
+                dtkx::points_to_double4( tx.ptr, n_tgt, layout, td.get(), s );
+                dtkx::knn_search( geometry, sd.get(), n_src, td.get(), n_tgt,
+                                  1, kout, nullptr, s );
+            }
+        }
+        DTKX_HIP( hipStreamSynchronize( s ) );
+        *out = h.release();
+        return DTKX_SUCCESS;
+    } );
+}
+
+int dtkx_nn_apply( const dtkx_nn *h, const double *src_vals, double *tgt_vals,
+                   int32_t n_fields, int64_t ld_src, int64_t ld_tgt )
+{
+    return guarded( [&]() {
+        REQ( h != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( n_fields >= 1, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( ld_src >= h->n_src, DTKX_ERR_SIZE_MISMATCH );
+        REQ( ld_tgt >= h->n_tgt, DTKX_ERR_SIZE_MISMATCH );
+        REQ( src_vals != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( h->n_tgt == 0 || tgt_vals != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        if ( h->n_tgt == 0 )
+            return DTKX_SUCCESS;
+        std::size_t const nin = (std::size_t)( n_fields - 1 ) * ld_src + h->n_src;
+        std::size_t const nout = (std::size_t)( n_fields - 1 ) * ld_tgt + h->n_tgt;
+        Staged<double> x( src_vals, nin, h->stream );
+        StagedOut<double> y( tgt_vals, nout, h->stream, ld_tgt > h->n_tgt );
+        dtkx::nn_gather_f64( h->idx.get(), h->n_tgt, x.ptr, ld_src, y.ptr,
+                             ld_tgt, n_fields, h->stream );
+        y.finish( h->stream );
+        DTKX_HIP( hipStreamSynchronize( h->stream ) );
+        return DTKX_SUCCESS;
+    } );
+}
+
+int dtkx_nn_get_indices( const dtkx_nn *h, const int32_t **indices,
+                         int64_t *n_tgt )
+{
+    return guarded( [&]() {
+        REQ( h != nullptr && indices != nullptr && n_tgt != nullptr,
+             DTKX_ERR_INVALID_ARGUMENT );
+        *indices = h->idx.get();
+        *n_tgt = h->n_tgt;
+        return DTKX_SUCCESS;
+    } );
+}
+
+int dtkx_nn_destroy( dtkx_nn *h )
+{
+    return guarded( [&]() {
+        delete h;
         return DTKX_SUCCESS;
     } );
 }

@@ -52,6 +52,21 @@ def _check_points(p, name):
     return shape
 
 
+def _fields(vals, n, name):
+    """(fields, leading dimension) of a [n] or [fields, n] value array."""
+    shape = tuple(vals.shape)
+    if len(shape) == 1:
+        nf, ld = 1, shape[0]
+    elif len(shape) == 2:
+        nf, ld = shape
+    else:
+        raise DtkxError(_lib.ERR_INVALID_ARGUMENT, f"{name} must be 1-D or 2-D")
+    if ld != n:
+        raise DtkxError(_lib.ERR_SIZE_MISMATCH,
+                        f"{name}.extent(0) == {n} (got {ld})")
+    return nf, ld
+
+
 class MovingLeastSquaresOperator:
     """MLS transfer operator on one GPU (one-rank communicator)."""
 
@@ -109,17 +124,7 @@ class MovingLeastSquaresOperator:
 
     # ---------------------------------------------------------------- apply
     def _fields(self, vals, n, name):
-        shape = tuple(vals.shape)
-        if len(shape) == 1:
-            nf, ld = 1, shape[0]
-        elif len(shape) == 2:
-            nf, ld = shape
-        else:
-            raise DtkxError(_lib.ERR_INVALID_ARGUMENT, f"{name} must be 1-D or 2-D")
-        if ld != n:
-            raise DtkxError(_lib.ERR_SIZE_MISMATCH,
-                            f"{name}.extent(0) == {n} (got {ld})")
-        return nf, ld
+        return _fields(vals, n, name)
 
     def apply(self, source_values, target_values):
         """target_values = C source_values (fields along the leading axis)."""
@@ -289,6 +294,67 @@ def _memcpy_d2d(dst_tensor, src_ptr, nbytes):
                        ctypes.c_void_p(src_ptr), nbytes, 3)  # DeviceToDevice
     if rc != 0:
         raise DtkxError(_lib.ERR_HIP, f"hipMemcpy failed ({rc})")
+
+
+class NearestNeighborOperator:
+    """Nearest-neighbour transfer on one GPU: each target takes the value of
+    its nearest source (packages/Meshfree/src/DTK_NearestNeighborOperator_
+    decl.hpp:24-50, _def.hpp:23-86; ties go to the smallest source index).
+
+        op = NearestNeighborOperator(source_points, target_points)
+        op.apply(source_values, target_values)
+    """
+
+    def __init__(self, source_points, target_points, *, geometry="float",
+                 layout="aos", stream=None):
+        if layout == "aos":
+            ns, ds = _check_points(source_points, "source_points")
+            nt, dt = _check_points(target_points, "target_points")
+        else:
+            ds, ns = _check_points(source_points, "source_points")
+            dt, nt = _check_points(target_points, "target_points")
+        if ds != 3 or dt != 3:
+            raise DtkxError(_lib.ERR_DIMENSION, "points must be 3-D")
+        self._n_src, self._n_tgt = ns, nt
+        sp, ka = _ptr(source_points)
+        tp, kb = _ptr(target_points)
+        h = ctypes.c_void_p()
+        check(_lib.load().dtkx_nn_create(
+            sp, ns, tp, nt, _lib.GEOMETRY[geometry], _lib.LAYOUT[layout],
+            ctypes.c_void_p(stream) if stream else None, ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.load().dtkx_nn_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def apply(self, source_values, target_values):
+        """target_values[f, t] = source_values[f, nearest(t)]."""
+        nf, lds = _fields(source_values, self._n_src, "source_values")
+        nf2, ldt = _fields(target_values, self._n_tgt, "target_values")
+        if nf != nf2:
+            raise DtkxError(_lib.ERR_SIZE_MISMATCH, "field counts differ")
+        xp, ka = _ptr(source_values)
+        yp, kb = _ptr(target_values)
+        check(_lib.load().dtkx_nn_apply(self._h, xp, yp, nf, lds, ldt))
+        return target_values
+
+    def indices(self):
+        """Copy of the nearest source of every target (int32 CUDA tensor)."""
+        import torch
+
+        p, n = ctypes.c_void_p(), ctypes.c_int64()
+        check(_lib.load().dtkx_nn_get_indices(self._h, ctypes.byref(p),
+                                              ctypes.byref(n)))
+        out = torch.empty(n.value, dtype=torch.int32, device="cuda")
+        if n.value:
+            _memcpy_d2d(out, p.value, 4 * n.value)
+        return out
 
 
 def knn(source_points, target_points, k, geometry="float", layout="aos",

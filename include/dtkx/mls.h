@@ -290,6 +290,32 @@ int dtkx_mls_batch_apply(const dtkx_mls_batch *b,
 int dtkx_mls_batch_get_stats(const dtkx_mls_batch *b, dtkx_mls_stats *stats);
 int dtkx_mls_batch_destroy(dtkx_mls_batch *b);
 
+/* ------------------------------------------ nearest-neighbour operator */
+/* NearestNeighborOperator (DTK_NearestNeighborOperator_decl.hpp:24-50,
+ * _def.hpp:23-86): each target takes the value of its nearest source. The
+ * search is dtkx_knn with k = 1 (ArborX Nearest<Point> with k = 1 in the
+ * reference, _def.hpp:46-53; ties go to the smallest source index); apply
+ * copies x[f][nearest(t)] into y[f][t], bit for bit (the reference's fetch
+ * and deep_copy, _def.hpp:78-82). One rank. */
+typedef struct dtkx_nn dtkx_nn;
+
+/* geometry: dtkx_geometry (FLOAT: the reference's float points); layout:
+ * dtkx_layout; stream: hipStream_t or NULL. Fails with
+ * DTKX_ERR_EMPTY_SOURCE when n_src == 0 (the reference's
+ * DTK_CHECK(!search_tree.empty()), _def.hpp:43). */
+int dtkx_nn_create(const double *src_xyz, int64_t n_src,
+                   const double *tgt_xyz, int64_t n_tgt, int32_t geometry,
+                   int32_t layout, void *stream, dtkx_nn **out);
+/* Fields are [n_fields][ld]; ld_src >= n_src, ld_tgt >= n_tgt
+ * (the reference's size preconditions, _def.hpp:73-76). Synchronous. */
+int dtkx_nn_apply(const dtkx_nn *h, const double *src_vals, double *tgt_vals,
+                  int32_t n_fields, int64_t ld_src, int64_t ld_tgt);
+/* The nearest source of every target (device array of n_tgt, owned by the
+ * handle): the reference's _indices member (_decl.hpp:46). */
+int dtkx_nn_get_indices(const dtkx_nn *h, const int32_t **indices,
+                        int64_t *n_tgt);
+int dtkx_nn_destroy(dtkx_nn *h);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

@@ -1,6 +1,7 @@
-"""The C++ drop-in (include/dtkx/MovingLeastSquaresOperator.hpp) compiles
-with a plain C++17 compiler against the C ABI, and on the GPU passes the
-reference's own operator tests restated in tests/native/test_mls_cpp.cpp."""
+"""The C++ drop-ins (include/dtkx/MovingLeastSquaresOperator.hpp,
+NearestNeighborOperator.hpp) compile with a plain C++17 compiler against the
+C ABI, and on the GPU pass the reference's own operator tests restated in
+tests/native/test_mls_cpp.cpp and test_nn_cpp.cpp."""
 import os
 import subprocess
 
@@ -9,13 +10,14 @@ import pytest
 from conftest import REPO
 
 SRC = os.path.join(REPO, "tests", "native", "test_mls_cpp.cpp")
+SRC_NN = os.path.join(REPO, "tests", "native", "test_nn_cpp.cpp")
 LIBDIR = os.path.join(REPO, "datatransferkit_amd")
 
 
-def _build(tmp_path):
-    exe = tmp_path / "test_mls_cpp"
+def _build(tmp_path, src=SRC, name="test_mls_cpp"):
+    exe = tmp_path / name
     subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-I",
-                    os.path.join(REPO, "include"), SRC, "-L", LIBDIR, "-ldtkx",
+                    os.path.join(REPO, "include"), src, "-L", LIBDIR, "-ldtkx",
                     f"-Wl,-rpath,{LIBDIR}", "-o", str(exe)], check=True)
     return exe
 
@@ -23,6 +25,17 @@ def _build(tmp_path):
 def test_dropin_header_compiles_and_links(tmp_path):
     exe = _build(tmp_path)
     assert exe.exists()
+    assert _build(tmp_path, SRC_NN, "test_nn_cpp").exists()
+
+
+@pytest.mark.gpu
+def test_dropin_nearest_neighbor_reference_tests(tmp_path):
+    exe = _build(tmp_path, SRC_NN, "test_nn_cpp")
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "FAIL" not in r.stdout
+    assert r.stdout.count("PASS") == 5
 
 
 @pytest.mark.gpu
