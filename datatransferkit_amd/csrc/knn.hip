@@ -1322,17 +1322,10 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     DeviceBuffer<int32_t> tids( n_tgt, s );
     std::size_t const ttemp_bytes = sort_pairs_bytes<uint32_t>( n_tgt, 30, s );
     DeviceBuffer<char> ttemp( ttemp_bytes, s );
+    // the side stream waits for the box only; its work is queued after the
+    // sources' chain below, since queueing (a few microseconds of host time
+    // per launch) paces this build and the sources' chain is the longer
     Fork fork( s );
-    if ( n_tgt > 0 )
-    {
-        hipStream_t const s2 = fork.side();
-        morton_kernel<T, V, uint32_t>
-            <<<ceil_div( n_tgt, kBlock ), kBlock, 0, s2>>>(
-                tgt, n_tgt, box.get(), tcodes.get(), tids.get(), nullptr );
-        DTKX_HIP( hipGetLastError() );
-        sort_pairs( ttemp.get(), ttemp_bytes, tcodes.get(), tcodes_sorted.get(),
-                    tids.get(), order, n_tgt, 30, s2 );
-    }
 
     // Morton codes and sort: sources
     DeviceBuffer<uint64_t> codes( n_src, s ), codes_sorted( n_src, s );
@@ -1382,6 +1375,16 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
             codes_sorted.get(), ids_sorted, src, n_int, box.get(), nullptr, 1,
             nodes.get() );
         DTKX_HIP( hipGetLastError() );
+    }
+    if ( n_tgt > 0 )
+    {
+        hipStream_t const s2 = fork.side();
+        morton_kernel<T, V, uint32_t>
+            <<<ceil_div( n_tgt, kBlock ), kBlock, 0, s2>>>(
+                tgt, n_tgt, box.get(), tcodes.get(), tids.get(), nullptr );
+        DTKX_HIP( hipGetLastError() );
+        sort_pairs( ttemp.get(), ttemp_bytes, tcodes.get(), tcodes_sorted.get(),
+                    tids.get(), order, n_tgt, 30, s2 );
     }
     fork.join();
     if ( n_tgt == 0 )
