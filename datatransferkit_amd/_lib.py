@@ -42,7 +42,8 @@ EXPORTED = [
     "dtkx_mls_options_init",
     "dtkx_mls_create", "dtkx_mls_create_with_neighbors", "dtkx_mls_apply",
     "dtkx_mls_apply_fp32", "dtkx_mls_apply_transpose",
-    "dtkx_mls_target_gradient", "dtkx_mls_get_graph", "dtkx_mls_get_stats",
+    "dtkx_mls_target_gradient", "dtkx_mls_get_graph", "dtkx_mls_write_graph",
+    "dtkx_mls_get_stats",
     "dtkx_mls_destroy", "dtkx_knn", "dtkx_svd_pseudo_inverse",
     "dtkx_mls_batch_create", "dtkx_mls_batch_apply",
     "dtkx_mls_batch_get_stats", "dtkx_mls_batch_destroy",
@@ -125,6 +126,7 @@ def load():
     L.dtkx_mls_target_gradient.argtypes = [_vp, _vp, _vp, _i32, _i64, _i64, _vp]
     L.dtkx_mls_get_graph.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                      ctypes.POINTER(_vp), ctypes.POINTER(_i64)]
+    L.dtkx_mls_write_graph.argtypes = [_vp, ctypes.c_char_p, _i64]
     L.dtkx_mls_get_stats.argtypes = [_vp, ctypes.POINTER(Stats)]
     L.dtkx_mls_destroy.argtypes = [_vp]
     L.dtkx_knn.argtypes = [_vp, _i64, _vp, _i64, _i32, _i32, _i32, _vp, _vp, _vp]

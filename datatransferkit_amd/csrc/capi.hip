@@ -15,6 +15,7 @@
 #include <vector>
 #include <type_traits>
 #include <cstring>
+#include <cstdio>
 #include <memory>
 #include <string>
 
@@ -701,6 +702,56 @@ int dtkx_mls_get_graph( const dtkx_mls *h, const int32_t **offsets,
     } );
 }
 
+int dtkx_mls_write_graph( const dtkx_mls *h, const char *path,
+                          int64_t col_base )
+{
+    return guarded( [&]() {
+        REQ( h != nullptr && path != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        int32_t const *offsets = nullptr, *indices = nullptr;
+        void const *coeffs = nullptr;
+        int64_t nnz = 0;
+        int const st = dtkx_mls_get_graph( h, &offsets, &indices, &coeffs, &nnz );
+        if ( st != DTKX_SUCCESS )
+            return static_cast<dtkx_status>( st );
+        bool const f32 = h->opts.precision == DTKX_PRECISION_FP32_BULK;
+        int32_t const cbytes = f32 ? 4 : 8;
+        std::vector<int32_t> off( h->n_tgt + 1 ), idx( nnz );
+        std::vector<char> co( (std::size_t)nnz * cbytes );
+        DTKX_HIP( hipMemcpyAsync( off.data(), offsets,
+                                  off.size() * sizeof( int32_t ),
+                                  hipMemcpyDeviceToHost, h->stream ) );
+        if ( nnz > 0 )
+        {
+            DTKX_HIP( hipMemcpyAsync( idx.data(), indices,
+                                      idx.size() * sizeof( int32_t ),
+                                      hipMemcpyDeviceToHost, h->stream ) );
+            DTKX_HIP( hipMemcpyAsync( co.data(), coeffs, co.size(),
+                                      hipMemcpyDeviceToHost, h->stream ) );
+        }
+        DTKX_HIP( hipStreamSynchronize( h->stream ) );
+        std::vector<int64_t> off64( off.begin(), off.end() ), col( nnz );
+        for ( int64_t e = 0; e < nnz; ++e )
+            col[e] = col_base + idx[e];
+        std::FILE *f = std::fopen( path, "wb" );
+        REQ( f != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        int64_t const head[4] = {h->n_tgt, h->n_src, nnz, col_base};
+        int32_t const head2[2] = {cbytes, 0};
+        bool ok = std::fwrite( "DTKXCSR1", 1, 8, f ) == 8 &&
+                  std::fwrite( head, sizeof( head ), 1, f ) == 1 &&
+                  std::fwrite( head2, sizeof( head2 ), 1, f ) == 1 &&
+                  std::fwrite( off64.data(), sizeof( int64_t ), off64.size(),
+                               f ) == off64.size() &&
+                  ( nnz == 0 ||
+                    ( std::fwrite( col.data(), sizeof( int64_t ), col.size(),
+                                   f ) == col.size() &&
+                      std::fwrite( co.data(), 1, co.size(), f ) ==
+                          co.size() ) );
+        ok = ( std::fclose( f ) == 0 ) && ok;
+        REQ( ok, DTKX_ERR_INVALID_ARGUMENT );
+        return DTKX_SUCCESS;
+    } );
+}
+
 int dtkx_mls_get_stats( const dtkx_mls *h, dtkx_mls_stats *stats )
 {
     return guarded( [&]() {
@@ -736,7 +787,7 @@ int dtkx_knn( const double *src_xyz, int64_t n_src, const double *tgt_xyz,
              DTKX_ERR_INVALID_ARGUMENT );
         if ( n_tgt == 0 )
             return DTKX_SUCCESS;
-            int const kk = k < n_src ? k : static_cast<int>( n_src );
+        int const kk = k < n_src ? k : static_cast<int>( n_src );
         Staged<double> sx( src_xyz, 3 * n_src, s );
         Staged<double> tx( tgt_xyz, 3 * n_tgt, s );
         StagedOut<int32_t> oi( indices, (std::size_t)n_tgt * kk, s );

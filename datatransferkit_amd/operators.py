@@ -171,6 +171,12 @@ class MovingLeastSquaresOperator:
         check(_lib.load().dtkx_mls_target_gradient(self._h, xp, yp, nf, lds, ldt, gp))
         return out
 
+    def write_graph(self, path, col_base=0):
+        """Write the operator's CSR (caller-order rows, global column ids
+        col_base + source index) in the DTKXCSR1 format; see read_graph."""
+        check(_lib.load().dtkx_mls_write_graph(self._h, str(path).encode(),
+                                               int(col_base)))
+
     # ------------------------------------------------------------ inspection
     def stats(self) -> dict:
         s = _lib.Stats()
@@ -294,6 +300,31 @@ def _memcpy_d2d(dst_tensor, src_ptr, nbytes):
                        ctypes.c_void_p(src_ptr), nbytes, 3)  # DeviceToDevice
     if rc != 0:
         raise DtkxError(_lib.ERR_HIP, f"hipMemcpy failed ({rc})")
+
+
+_CSR_MAGIC = b"DTKXCSR1"
+
+
+def read_graph(path):
+    """Read a DTKXCSR1 file (dtkx_mls_write_graph) into numpy arrays:
+    dict(n_rows, n_cols, col_base, offsets int64 [n_rows + 1], columns int64
+    [nnz], coeffs float64 or float32 [nnz]). Little-endian; the header is
+    validated and nothing in the file is executed."""
+    with open(path, "rb") as fh:
+        if fh.read(8) != _CSR_MAGIC:
+            raise DtkxError(_lib.ERR_INVALID_ARGUMENT, f"{path}: not a DTKXCSR1 file")
+        n_rows, n_cols, nnz, col_base = (int(v) for v in np.fromfile(fh, "<i8", 4))
+        cbytes, _ = (int(v) for v in np.fromfile(fh, "<i4", 2))
+        if cbytes not in (4, 8) or min(n_rows, n_cols, nnz) < 0:
+            raise DtkxError(_lib.ERR_INVALID_ARGUMENT, f"{path}: bad header")
+        offsets = np.fromfile(fh, "<i8", n_rows + 1)
+        columns = np.fromfile(fh, "<i8", nnz)
+        coeffs = np.fromfile(fh, "<f8" if cbytes == 8 else "<f4", nnz)
+    if (len(offsets) != n_rows + 1 or len(columns) != nnz or len(coeffs) != nnz
+            or offsets[0] != 0 or offsets[-1] != nnz):
+        raise DtkxError(_lib.ERR_INVALID_ARGUMENT, f"{path}: truncated or inconsistent")
+    return {"n_rows": n_rows, "n_cols": n_cols, "col_base": col_base,
+            "offsets": offsets, "columns": columns, "coeffs": coeffs}
 
 
 class NearestNeighborOperator:

@@ -234,6 +234,21 @@ int dtkx_mls_target_gradient(const dtkx_mls *h, const double *src_vals,
 int dtkx_mls_get_graph(const dtkx_mls *h, const int32_t **offsets,
                        const int32_t **indices, const void **coeffs,
                        int64_t *nnz);
+/* The operator as a CSR file for downstream assembly (a Tpetra CrsMatrix,
+ * C^T, another process): no counterpart in the reference, whose operator
+ * keeps its graph in memory (..._decl.hpp:57-62; SplineOperator builds a
+ * CrsMatrix from such a graph, DTK_SplineOperator_def.hpp:108-149).
+ * Little-endian, in order:
+ *   char    magic[8]        "DTKXCSR1"
+ *   int64_t n_rows, n_cols, nnz, col_base
+ *   int32_t coeff_bytes     8 (double) or 4 (float, FP32_BULK)
+ *   int32_t reserved        0
+ *   int64_t offsets[n_rows + 1]
+ *   int64_t columns[nnz]    global ids: col_base + source index
+ *   coeffs[nnz]             double or float
+ * Rows are targets in caller order, each ascending by (distance, index). */
+int dtkx_mls_write_graph(const dtkx_mls *h, const char *path,
+                         int64_t col_base);
 int dtkx_mls_get_stats(const dtkx_mls *h, dtkx_mls_stats *stats);
 int dtkx_mls_destroy(dtkx_mls *h);
 
