@@ -368,12 +368,16 @@ def bench_adjoint(args):
     y = torch.zeros((nf, n_tgt), dtype=torch.float64, device="cuda")
     xbar = torch.zeros_like(x)
     stream = torch.cuda.current_stream().cuda_stream
-    kw = dict(basis="quadratic", rbf="wendland2", k=16, solver="cyclic",
-              geometry="double", stream=stream)
+    kw = dict(basis="quadratic", rbf="wendland2", k=16, geometry="double",
+              stream=stream)
     results, grads = {}, {}
-    for mode in ("recompute", "store"):
+    # Cholesky (the headline mode), stored pinv, and the cyclic Jacobi
+    modes = {"recompute": ("cholesky", False), "store": ("cholesky", True),
+             "recompute_cyclic": ("cyclic", False)}
+    for mode, (solver, store) in modes.items():
         def step():
-            op = dtk.MovingLeastSquaresOperator(d_src, d_tgt, store_pinv=(mode == "store"), **kw)
+            op = dtk.MovingLeastSquaresOperator(d_src, d_tgt, store_pinv=store,
+                                                solver=solver, **kw)
             op.apply(x, y)
             op.apply_transpose(ybar, xbar)
             g = op.target_gradient(x, ybar)
@@ -399,11 +403,14 @@ def bench_adjoint(args):
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"MLS adjoint {n_src} sources -> {n_tgt} targets, quadratic, "
                                f"k=16, wendland2, {nf} fields, FP64 double geometry, "
-                               f"cyclic Jacobi",
+                               f"certified Cholesky",
                    "n_src": n_src, "n_tgt": n_tgt, "fields": nf},
         "modes": results,
         "store_vs_recompute_rel": float(np.abs(grads["store"] - grads["recompute"]).max()
                                         / np.abs(grads["recompute"]).max()),
+        "cholesky_vs_cyclic_rel": float(
+            np.abs(grads["recompute_cyclic"] - grads["recompute"]).max()
+            / np.abs(grads["recompute"]).max()),
         "fd_check": "central differences at h = 1e-6 in extended precision on a "
                     "sample at this density: tests/test_gpu_adjoint.py",
     }

@@ -646,6 +646,7 @@ int dtkx_mls_target_gradient( const dtkx_mls *h, const double *src_vals,
         a.max_sweeps = h->opts.solver == DTKX_SOLVER_CYCLIC_JACOBI
                            ? h->opts.max_rotations
                            : 0;
+        a.solver = h->opts.solver;
         a.ld_tgt = ld_tgt;
         a.d_tgt = g.ptr;
         dtkx::target_gradient( a, x.ptr, ld_src, n_fields, yb.ptr, h->stream );

@@ -183,6 +183,94 @@ __device__ inline bool converged( double const *A )
     return !( off > tol * diag );
 }
 
+// Certified Cholesky inverse. A (packed upper triangle, trace tr) becomes
+// V = U^-1 in place, where A = U^T U (left-looking, each pivot's
+// reciprocal kept so a column divides once; non-positive pivots are
+// replaced by one and reported). Returns whether every eigenvalue of A is
+// proven above the truncation threshold: lambda_min(A) = 1 / |V|_2^2 >=
+// 1 / |V|_F^2 > 4 thr + 64 m eps trace, thr = DBL_EPSILON or rcond * trace
+// (>= rcond * lambda_max), the margin covering the rounding of the
+// eigenvalues an eigen-solver would compute. Then A^+ = A^-1 = V V^T.
+template <int M>
+__device__ inline bool cholesky_inverse( double *A, double tr, int truncation,
+                                         double rcond )
+{
+    using S = SymIdx<M>;
+    bool ok = true;
+    double rinv[M];
+#pragma unroll
+    for ( int j = 0; j < M; ++j )
+    {
+#pragma unroll
+        for ( int i = 0; i <= j; ++i )
+        {
+            double s = A[S::at( i, j )];
+#pragma unroll
+            for ( int k = 0; k < i; ++k )
+                s = fma( -A[S::at( k, i )], A[S::at( k, j )], s );
+            if ( i < j )
+                A[S::at( i, j )] = s * rinv[i];
+            else
+            {
+                ok = ok && s > 0.0;
+                double const u = __builtin_sqrt( s > 0.0 ? s : 1.0 );
+                A[S::at( j, j )] = u;
+                rinv[j] = 1.0 / u;
+            }
+        }
+    }
+    // V = U^-1 in place: column j from the columns before it
+#pragma unroll
+    for ( int j = 0; j < M; ++j )
+    {
+#pragma unroll
+        for ( int i = 0; i < j; ++i )
+        {
+            double s = 0.0;
+#pragma unroll
+            for ( int k = i; k < j; ++k )
+                s = fma( A[S::at( i, k )], A[S::at( k, j )], s );
+            A[S::at( i, j )] = -s * rinv[j];
+        }
+        A[S::at( j, j )] = rinv[j];
+    }
+    double fro = 0.0;
+#pragma unroll
+    for ( int e = 0; e < S::size; ++e )
+        fro = fma( A[e], A[e], fro );
+    double const lambda_lb = 1.0 / fro;
+    double const thr =
+        truncation == DTKX_TRUNCATION_RELATIVE ? rcond * tr : DBL_EPSILON;
+    return ok && lambda_lb > 4.0 * thr + 64.0 * M * DBL_EPSILON * tr &&
+           lambda_lb == lambda_lb;
+}
+
+// y = V V^T x for the inverse factor V (upper triangle, packed)
+template <int M>
+__device__ inline void apply_vvt( double const *V, double const *x, double *y )
+{
+    using S = SymIdx<M>;
+    double w[M];
+#pragma unroll
+    for ( int k = 0; k < M; ++k )
+    {
+        double s = 0.0;
+#pragma unroll
+        for ( int i = 0; i <= k; ++i )
+            s = fma( V[S::at( i, k )], x[i], s );
+        w[k] = s;
+    }
+#pragma unroll
+    for ( int i = 0; i < M; ++i )
+    {
+        double s = 0.0;
+#pragma unroll
+        for ( int k = i; k < M; ++k )
+            s = fma( V[S::at( i, k )], w[k], s );
+        y[i] = s;
+    }
+}
+
 } // namespace cyc
 } // namespace dtkx
 

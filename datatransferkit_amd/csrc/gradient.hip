@@ -19,12 +19,15 @@
 // derivative of y = e_0^T A^{-1} b (dy = a0^T (db - dA A^{-1} b)), exact
 // for systems with no truncated eigenvalue.
 //
-// A^+ comes from the register-resident cyclic Jacobi, recomputed per target
-// (no storage), or is read back from the packed triangle the cyclic setup
-// stored (store_pinv). Double geometry only: float-rounded coordinates make
+// A^+ is recomputed per target (no storage) with the register-resident
+// cyclic Jacobi or, for Cholesky handles, the certified Cholesky inverse
+// (the systems it cannot certify go to the cyclic solver), or is read back
+// from the packed triangle the setup stored (store_pinv). Double geometry only: float-rounded coordinates make
 // the operator piecewise constant at the float spacing.
 #include "cyclic_device.hpp"
 #include "gradient.hpp"
+
+#include <tuple>
 
 namespace dtkx
 {
@@ -108,7 +111,18 @@ __device__ inline double group_sum( double v )
     return v;
 }
 
-template <int M, int H, bool STORED>
+// SOLVE: the source of A^+ -- stored (the packed triangle), cyclic Jacobi,
+// or certified Cholesky. rows (nullable): the rows rows[i], i < n_list, in
+// place of [0, n_tgt). A Cholesky system that cannot be certified is listed
+// in fallback for a cyclic launch over the list and written there.
+enum Solve
+{
+    kStored,
+    kCyclic,
+    kCholesky
+};
+
+template <int M, int H, int SOLVE>
 __global__ void __launch_bounds__( kBlock )
     target_gradient_kernel( double4 const *__restrict__ src,
                             double4 const *__restrict__ tgt, int64_t n_tgt,
@@ -119,14 +133,18 @@ __global__ void __launch_bounds__( kBlock )
                             double const *__restrict__ pinv,
                             double const *__restrict__ xs, int pitch, int nf,
                             double const *__restrict__ ybar, int64_t ld_tgt,
-                            double *__restrict__ d_tgt, bool accumulate )
+                            double *__restrict__ d_tgt, bool accumulate,
+                            int32_t const *__restrict__ rows, int64_t n_list,
+                            int32_t *__restrict__ fallback,
+                            int32_t *__restrict__ n_fallback )
 {
     using S = SymIdx<M>;
     constexpr int R = M / H;
     constexpr int SPB = kBlock / H;
     int const h = threadIdx.x % H;
-    int64_t const slot = xcd_tile() * SPB + threadIdx.x / H;
-    bool const valid = slot < n_tgt;
+    int64_t const li = xcd_tile() * SPB + threadIdx.x / H;
+    bool valid = rows ? li < n_list : li < n_tgt;
+    int64_t const slot = rows ? ( valid ? rows[li] : 0 ) : li;
     int64_t const t = valid ? ( order ? order[slot] : slot ) : 0;
     int32_t const *nbr = indices + slot * kk;
     double4 const tp = valid ? tgt[t] : double4{0, 0, 0, 0};
@@ -197,7 +215,7 @@ __global__ void __launch_bounds__( kBlock )
             bbar[i] = group_sum<H>( bbar[i] );
     };
 
-    if constexpr ( STORED )
+    if constexpr ( SOLVE == kStored )
     {
         load_upstream();
         form_bbar();
@@ -212,6 +230,51 @@ __global__ void __launch_bounds__( kBlock )
             gamma[i] = g;
             a0[i] = pt[S::at( 0, i )];
         }
+    }
+    else if constexpr ( SOLVE == kCholesky )
+    {
+        double A[S::size];
+#pragma unroll
+        for ( int e = 0; e < S::size; ++e )
+            A[e] = 0.0;
+        if ( valid )
+            for ( int l = 0; l < kk; ++l )
+            {
+                double P[M];
+                double const phi = nbr_row<M, false>(
+                    DTKX_GEOMETRY_DOUBLE, rbf, radius, src[nbr[l]], tp, P );
+#pragma unroll
+                for ( int r = 0; r < M; ++r )
+                {
+                    double const pr = P[r] * phi;
+#pragma unroll
+                    for ( int c = r; c < M; ++c )
+                        A[S::at( r, c )] = fma( pr, P[c], A[S::at( r, c )] );
+                }
+            }
+        double trace = 0.0;
+#pragma unroll
+        for ( int i = 0; i < M; ++i )
+            trace += A[S::at( i, i )];
+        if ( !valid )
+#pragma unroll
+            for ( int i = 0; i < M; ++i )
+                A[S::at( i, i )] = 1.0; // keeps the factorisation finite
+        bool const ok = cholesky_inverse<M>( A, trace, truncation, rcond );
+        if ( valid && !ok )
+        {
+            // left to the cyclic launch over the list
+            fallback[atomicAdd( n_fallback, 1 )] = (int32_t)slot;
+            valid = false;
+        }
+        load_upstream();
+        form_bbar();
+        double e0[M];
+#pragma unroll
+        for ( int i = 0; i < M; ++i )
+            e0[i] = i == 0 ? 1.0 : 0.0;
+        apply_vvt<M>( A, e0, a0 );
+        apply_vvt<M>( A, bbar, gamma );
     }
     else
     {
@@ -370,23 +433,46 @@ void launch_gradient( GradientArgs const &a, double const *xs, int pitch,
                       hipStream_t s )
 {
     int const sweeps = a.max_sweeps > 0 ? a.max_sweeps : 30;
+    auto args = [&]( int32_t const *rows, int64_t n_list, int32_t *fb,
+                     int32_t *n_fb ) {
+        return std::make_tuple( a.src, a.tgt, a.n_tgt, a.indices, a.order,
+                                a.kk, a.rbf, a.truncation, a.rcond, sweeps,
+                                a.pinv, xs, pitch, nf, ybar, a.ld_tgt, a.d_tgt,
+                                accumulate, rows, n_list, fb, n_fb );
+    };
+    auto launch = [&]( auto kernel, int64_t n, int spb, auto tup ) {
+        if ( n == 0 )
+            return;
+        dim3 const grid( ceil_div( n, spb ) ), block( kBlock );
+        std::apply(
+            [&]( auto... xs_ ) { kernel<<<grid, block, 0, s>>>( xs_... ); },
+            tup );
+        DTKX_HIP( hipGetLastError() );
+    };
     if ( a.pinv )
     {
-        dim3 const grid( ceil_div( a.n_tgt, kBlock ) ), block( kBlock );
-        target_gradient_kernel<M, 1, true><<<grid, block, 0, s>>>(
-            a.src, a.tgt, a.n_tgt, a.indices, a.order, a.kk, a.rbf,
-            a.truncation, a.rcond, sweeps, a.pinv, xs, pitch, nf, ybar,
-            a.ld_tgt, a.d_tgt, accumulate );
+        launch( target_gradient_kernel<M, 1, kStored>, a.n_tgt, kBlock,
+                args( nullptr, 0, nullptr, nullptr ) );
+        return;
     }
-    else
+    if ( a.solver != DTKX_SOLVER_CHOLESKY )
     {
-        dim3 const grid( ceil_div( a.n_tgt, kBlock / H ) ), block( kBlock );
-        target_gradient_kernel<M, H, false><<<grid, block, 0, s>>>(
-            a.src, a.tgt, a.n_tgt, a.indices, a.order, a.kk, a.rbf,
-            a.truncation, a.rcond, sweeps, nullptr, xs, pitch, nf, ybar,
-            a.ld_tgt, a.d_tgt, accumulate );
+        launch( target_gradient_kernel<M, H, kCyclic>, a.n_tgt, kBlock / H,
+                args( nullptr, 0, nullptr, nullptr ) );
+        return;
     }
-    DTKX_HIP( hipGetLastError() );
+    // certified Cholesky, then the cyclic solver on the rows it lists (the
+    // count is read back: the list is usually empty)
+    DeviceBuffer<int32_t> list( a.n_tgt, s ), count( 1, s );
+    DTKX_HIP( hipMemsetAsync( count.get(), 0, sizeof( int32_t ), s ) );
+    launch( target_gradient_kernel<M, 1, kCholesky>, a.n_tgt, kBlock,
+            args( nullptr, 0, list.get(), count.get() ) );
+    int32_t n_list = 0;
+    DTKX_HIP( hipMemcpyAsync( &n_list, count.get(), sizeof( int32_t ),
+                              hipMemcpyDeviceToHost, s ) );
+    DTKX_HIP( hipStreamSynchronize( s ) );
+    launch( target_gradient_kernel<M, H, kCyclic>, n_list, kBlock / H,
+            args( list.get(), n_list, nullptr, nullptr ) );
 }
 
 } // namespace

@@ -18,6 +18,7 @@ struct GradientArgs
     double const *pinv = nullptr;     // stored packed A^+ per row, or null
     int64_t n_src = 0, n_tgt = 0;
     int m = 4, kk = 4, rbf = 0, truncation = 0, max_sweeps = 0;
+    int solver = 0;                   // DTKX_SOLVER_* of the handle
     double rcond = 0.0;
     int64_t ld_tgt = 0;               // leading dimension of ybar
     double *d_tgt = nullptr;          // [n_tgt][3], caller order

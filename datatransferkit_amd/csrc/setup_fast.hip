@@ -151,56 +151,7 @@ __global__ void __launch_bounds__( kBlock, 2 )
             A[S::at( i, i )] = 1.0;
     }
 
-    // A = U^T U in place (left-looking, column by column); each pivot's
-    // reciprocal is kept so a column divides once
-    bool ok = true;
-    double rinv[M];
-#pragma unroll
-    for ( int j = 0; j < M; ++j )
-    {
-#pragma unroll
-        for ( int i = 0; i <= j; ++i )
-        {
-            double s = A[S::at( i, j )];
-#pragma unroll
-            for ( int k = 0; k < i; ++k )
-                s = fma( -A[S::at( k, i )], A[S::at( k, j )], s );
-            if ( i < j )
-                A[S::at( i, j )] = s * rinv[i];
-            else
-            {
-                ok = ok && s > 0.0;
-                double const u = __builtin_sqrt( s > 0.0 ? s : 1.0 );
-                A[S::at( j, j )] = u;
-                rinv[j] = 1.0 / u;
-            }
-        }
-    }
-    // V = U^-1 in place: column j from the columns before it
-#pragma unroll
-    for ( int j = 0; j < M; ++j )
-    {
-#pragma unroll
-        for ( int i = 0; i < j; ++i )
-        {
-            double s = 0.0;
-#pragma unroll
-            for ( int k = i; k < j; ++k )
-                s = fma( A[S::at( i, k )], A[S::at( k, j )], s );
-            A[S::at( i, j )] = -s * rinv[j];
-        }
-        A[S::at( j, j )] = rinv[j];
-    }
-    double fro = 0.0;
-#pragma unroll
-    for ( int e = 0; e < S::size; ++e )
-        fro = fma( A[e], A[e], fro );
-    double const lambda_lb = 1.0 / fro;
-    double const thr = truncation == DTKX_TRUNCATION_RELATIVE
-                           ? rcond * trace
-                           : DBL_EPSILON;
-    ok = ok && lambda_lb > 4.0 * thr + 64.0 * M * DBL_EPSILON * trace &&
-         lambda_lb == lambda_lb;
+    bool const ok = cholesky_inverse<M>( A, trace, truncation, rcond );
     if ( valid && !ok )
     {
         // set up by the eigen-solver afterwards (it also rewrites this

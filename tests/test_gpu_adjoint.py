@@ -119,6 +119,59 @@ def test_store_and_recompute_agree(dtk, solver):
     assert rel < 1e-8
 
 
+def test_cholesky_recompute_matches_cyclic(dtk):
+    # the certified Cholesky inverse and the cyclic Jacobi pseudo-inverse
+    # agree to eps * cond(A) on full-rank systems
+    src, tgt, _ = c1_inputs(20_000, seed=7)
+    tgt = tgt[:4000] * 0.9 + 0.05
+    x = cuda(fields(src, 3, seed=3))
+    ybar = cuda(np.random.default_rng(4).standard_normal((3, len(tgt))))
+    kw = dict(basis="quadratic", rbf="wendland2", k=16, geometry="double")
+    gc = dtk.MovingLeastSquaresOperator(cuda(src), cuda(tgt), solver="cyclic",
+                                        **kw).target_gradient(x, ybar)
+    op = dtk.MovingLeastSquaresOperator(cuda(src), cuda(tgt), solver="cholesky", **kw)
+    gk = op.target_gradient(x, ybar)
+    rel = (gc - gk).abs().max().item() / gc.abs().max().item()
+    print(f"cholesky vs cyclic recompute: {rel:.3e}")
+    assert rel < 1e-8
+
+
+def test_cholesky_gradient_matches_central_differences(dtk):
+    src, tgt, _ = c1_inputs(20_000, seed=96)
+    tgt = tgt[:3000] * 0.9 + 0.05
+    x = cuda(fields(src, 4, seed=6))
+    ybar = cuda(np.random.default_rng(6).standard_normal((4, len(tgt))))
+    opts = dict(basis="quadratic", rbf="wendland2", k=16, solver="cholesky")
+    op = dtk.MovingLeastSquaresOperator(cuda(src), cuda(tgt), geometry="double", **opts)
+    g = op.target_gradient(x, ybar).cpu().numpy()
+    _, nbr, _ = op.graph()
+    h = 1e-6
+    gfd = fd_gradient(dtk, src, tgt, nbr, x, ybar, opts, h=h)
+    smooth = radius_gap(src, tgt, nbr.cpu().numpy()) > 10 * h
+    err = np.abs(g - gfd)[smooth].max() / np.abs(gfd)[smooth].max()
+    print(f"cholesky: |g - g_fd|_inf / |g_fd|_inf = {err:.3e}")
+    assert smooth.mean() > 0.95
+    assert err < 1e-6
+
+
+def test_cholesky_fallback_rows_equal_cyclic(dtk):
+    # points on a plane: every quadratic moment matrix is singular, so no
+    # system is certified and every row takes the cyclic path, bit for bit
+    rng = np.random.default_rng(8)
+    src = np.concatenate([rng.random((3000, 2)), np.full((3000, 1), 0.5)], axis=1)
+    tgt = np.concatenate([0.1 + 0.8 * rng.random((500, 2)), np.full((500, 1), 0.5)],
+                         axis=1)
+    x = cuda(fields(src, 2, seed=9))
+    ybar = cuda(rng.standard_normal((2, len(tgt))))
+    kw = dict(basis="quadratic", rbf="wendland2", k=16, geometry="double")
+    gc = dtk.MovingLeastSquaresOperator(cuda(src), cuda(tgt), solver="cyclic",
+                                        **kw).target_gradient(x, ybar)
+    op = dtk.MovingLeastSquaresOperator(cuda(src), cuda(tgt), solver="cholesky", **kw)
+    assert op.stats()["n_eigen_fallback"] == len(tgt)
+    gk = op.target_gradient(x, ybar)
+    assert torch.equal(gc, gk)
+
+
 def test_more_than_eight_fields_accumulate(dtk):
     # fields go through the adjoint in chunks of eight; linear in the fields
     src, tgt, _ = c1_inputs(8000, seed=6)
