@@ -96,7 +96,9 @@ __global__ void __launch_bounds__( kBlock )
 // Sequential sum per row, j ascending. Each wave owns 64 consecutive rows
 // and stages their indices and coefficients through LDS kChunk columns at
 // a time, so the HBM reads are runs of whole rows rather than one strided
-// element per lane.
+// element per lane. 16 columns (64-byte runs of a 4-byte array) measured
+// best: the FP32 apply of 1M rows of 32 takes 0.23 ms, against 0.30 ms at
+// 8 columns, 0.34 ms at 32 (LDS-bound occupancy) and 0.44 ms at 4.
 #ifndef DTKX_APPLY_CHUNK
 #define DTKX_APPLY_CHUNK 16
 #endif

@@ -1339,12 +1339,8 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     morton_kernel<T, V><<<ceil_div( n_src, kBlock ), kBlock, 0, s>>>(
         src, n_src, box.get(), codes.get(), ids.get(), nullptr );
     DTKX_HIP( hipGetLastError() );
-#if DTKX_KNN_SRC_ONESWEEP
-    std::size_t const temp_bytes = sort_pairs_bytes<uint64_t>( n_src, 63, s );
-    DeviceBuffer<char> temp( temp_bytes, s );
-    sort_pairs( temp.get(), temp_bytes, codes.get(), codes_sorted.get(),
-                ids.get(), ids_sorted, n_src, 63, s );
-#else
+    // 63-bit keys: rocprim's merge sort at these sizes (its onesweep radix
+    // sort takes longer on 64-bit keys below 2^21)
     std::size_t temp_bytes = 0;
     DTKX_HIP( hipcub::DeviceRadixSort::SortPairs(
         nullptr, temp_bytes, codes.get(), codes_sorted.get(), ids.get(),
@@ -1353,7 +1349,6 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     DTKX_HIP( hipcub::DeviceRadixSort::SortPairs(
         temp.get(), temp_bytes, codes.get(), codes_sorted.get(), ids.get(),
         ids_sorted, static_cast<int>( n_src ), 0, 63, s ) );
-#endif
     DeviceBuffer<int32_t> pos_buf;
     int32_t *src_pos = out.src_pos;
     if ( out.morton_layout && !src_pos )

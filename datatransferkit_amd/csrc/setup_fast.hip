@@ -29,6 +29,8 @@ namespace
 
 using namespace cyc;
 constexpr int kBlock = 64;
+// neighbours gathered per lane and round (0.69 ms for one at a time, 0.48 ms
+// for four, about the same for eight: 1M rows of 32, the FP32-bulk form)
 #ifndef DTKX_SETUP_GATHER
 #define DTKX_SETUP_GATHER 4
 #endif
