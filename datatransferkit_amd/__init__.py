@@ -8,9 +8,9 @@ Jacobi pseudo-inverse, CRS apply and transpose) behind a C ABI
 from ._lib import DtkxError  # noqa: F401
 from .operators import (  # noqa: F401
     MovingLeastSquaresBatch, MovingLeastSquaresOperator,
-    NearestNeighborOperator, knn, read_graph, release_cached_memory,
+    NearestNeighborOperator, SplineOperator, knn, read_graph, release_cached_memory,
     svd_pseudo_inverse)
 
 __all__ = ["MovingLeastSquaresOperator", "MovingLeastSquaresBatch",
-           "NearestNeighborOperator", "knn", "read_graph", "svd_pseudo_inverse",
+           "NearestNeighborOperator", "SplineOperator", "knn", "read_graph", "svd_pseudo_inverse",
            "release_cached_memory", "DtkxError"]

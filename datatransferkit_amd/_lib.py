@@ -48,6 +48,9 @@ EXPORTED = [
     "dtkx_mls_batch_create", "dtkx_mls_batch_apply",
     "dtkx_mls_batch_get_stats", "dtkx_mls_batch_destroy",
     "dtkx_nn_create", "dtkx_nn_apply", "dtkx_nn_get_indices", "dtkx_nn_destroy",
+    "dtkx_spline_options_init", "dtkx_spline_create", "dtkx_spline_apply",
+    "dtkx_spline_solve", "dtkx_spline_evaluate", "dtkx_spline_system_apply",
+    "dtkx_spline_get_basis", "dtkx_spline_get_info", "dtkx_spline_destroy",
 ]
 
 
@@ -84,6 +87,20 @@ class Problem(ctypes.Structure):
         ("tgt_xyz", ctypes.c_void_p), ("n_tgt", ctypes.c_int64),
         ("basis", ctypes.c_int32), ("k", ctypes.c_int32),
     ]
+
+
+class SplineOptions(ctypes.Structure):
+    _fields_ = [
+        ("rbf", ctypes.c_int32), ("geometry", ctypes.c_int32),
+        ("layout", ctypes.c_int32), ("restart", ctypes.c_int32),
+        ("max_iterations", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("tolerance", ctypes.c_double), ("stream", ctypes.c_void_p),
+    ]
+
+
+class SplineInfo(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("converged", ctypes.c_int32),
+                ("residual", ctypes.c_double)]
 
 
 class DtkxError(RuntimeError):
@@ -142,6 +159,19 @@ def load():
     L.dtkx_nn_apply.argtypes = [_vp, _vp, _vp, _i32, _i64, _i64]
     L.dtkx_nn_get_indices.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_i64)]
     L.dtkx_nn_destroy.argtypes = [_vp]
+    L.dtkx_spline_options_init.argtypes = [ctypes.POINTER(SplineOptions)]
+    L.dtkx_spline_create.argtypes = [_vp, _i64, _vp, _i64,
+                                     ctypes.POINTER(SplineOptions),
+                                     ctypes.POINTER(_vp)]
+    L.dtkx_spline_apply.argtypes = [_vp, _vp, _vp, _i32, _i64, _i64]
+    L.dtkx_spline_solve.argtypes = [_vp, _vp, _vp]
+    L.dtkx_spline_evaluate.argtypes = [_vp, _vp, _vp]
+    L.dtkx_spline_system_apply.argtypes = [_vp, _vp, _vp]
+    L.dtkx_spline_get_basis.argtypes = [_vp, _i32, ctypes.POINTER(_vp),
+                                        ctypes.POINTER(_vp), ctypes.POINTER(_i64),
+                                        ctypes.POINTER(_i32)]
+    L.dtkx_spline_get_info.argtypes = [_vp, ctypes.POINTER(SplineInfo)]
+    L.dtkx_spline_destroy.argtypes = [_vp]
     for name in EXPORTED:
         f = getattr(L, name)
         if f.restype is ctypes.c_int or f.restype is None:
@@ -155,6 +185,12 @@ def check(status: int) -> None:
     if status != SUCCESS:
         msg = load().dtkx_last_error().decode(errors="replace")
         raise DtkxError(status, msg)
+
+
+def default_spline_options() -> SplineOptions:
+    o = SplineOptions()
+    check(load().dtkx_spline_options_init(ctypes.byref(o)))
+    return o
 
 
 def default_options() -> Options:

@@ -27,6 +27,7 @@ SOURCES = [
     "setup_fast.hip",
     "apply.hip",
     "gradient.hip",
+    "spline.hip",
     "capi.hip",
 ]
 

@@ -10,10 +10,12 @@
 #include "gradient.hpp"
 #include "knn.hpp"
 #include "setup.hpp"
+#include "spline.hpp"
 
 #include <algorithm>
 #include <vector>
 #include <type_traits>
+#include <cmath>
 #include <cstring>
 #include <cstdio>
 #include <memory>
@@ -915,6 +917,286 @@ int dtkx_nn_get_indices( const dtkx_nn *h, const int32_t **indices,
 }
 
 int dtkx_nn_destroy( dtkx_nn *h )
+{
+    return guarded( [&]() {
+        delete h;
+        return DTKX_SUCCESS;
+    } );
+}
+
+// ------------------------------------------------------- RBF spline
+struct dtkx_spline
+{
+    dtkx_spline_options opts{};
+    int64_t n_src = 0, n_tgt = 0;
+    int kk = 0;
+    hipStream_t stream = nullptr;
+    // basis operators M (sources' rows) and N (targets' rows)
+    dtkx::DeviceBuffer<int32_t> m_idx, n_idx;
+    dtkx::DeviceBuffer<double> m_phi, n_phi;
+    // Vandermonde rows: the points as the search holds them
+    dtkx::DeviceBuffer<float4> src_f, tgt_f;
+    dtkx::DeviceBuffer<double4> src_d, tgt_d;
+    mutable dtkx::SplineSolver solver;
+    mutable dtkx_spline_info info{};
+    dtkx::SplineSystem system() const
+    {
+        dtkx::SplineSystem y;
+        y.n_src = n_src;
+        y.n_tgt = n_tgt;
+        y.kk = kk;
+        y.geometry = opts.geometry;
+        bool const f = opts.geometry == DTKX_GEOMETRY_FLOAT;
+        y.src_pts = f ? (void const *)src_f.get() : (void const *)src_d.get();
+        y.tgt_pts = f ? (void const *)tgt_f.get() : (void const *)tgt_d.get();
+        y.m_idx = m_idx.get();
+        y.m_phi = m_phi.get();
+        y.n_idx = n_idx.get();
+        y.n_phi = n_phi.get();
+        return y;
+    }
+    dtkx::GmresOptions gmres() const
+    {
+        dtkx::GmresOptions g;
+        g.tolerance = opts.tolerance;
+        g.restart = opts.restart;
+        g.max_iterations = opts.max_iterations;
+        return g;
+    }
+};
+
+int dtkx_spline_options_init( dtkx_spline_options *opts )
+{
+    return guarded( [&]() {
+        REQ( opts != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        *opts = dtkx_spline_options{};
+        opts->rbf = DTKX_RBF_WENDLAND0;
+        opts->geometry = DTKX_GEOMETRY_FLOAT;
+        opts->layout = DTKX_LAYOUT_AOS;
+        opts->restart = 300;
+        opts->max_iterations = 1000;
+        opts->tolerance = 1e-10;
+        opts->stream = nullptr;
+        return DTKX_SUCCESS;
+    } );
+}
+
+int dtkx_spline_create( const double *src_xyz, int64_t n_src,
+                        const double *tgt_xyz, int64_t n_tgt,
+                        const dtkx_spline_options *opts, dtkx_spline **out )
+{
+    return guarded( [&]() {
+        REQ( out != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        *out = nullptr;
+        dtkx_spline_options o;
+        dtkx_spline_options_init( &o );
+        if ( opts )
+            o = *opts;
+        REQ( o.rbf >= DTKX_RBF_WENDLAND0 && o.rbf <= DTKX_RBF_BUHMANN4,
+             DTKX_ERR_INVALID_ARGUMENT );
+        REQ( o.geometry == DTKX_GEOMETRY_FLOAT ||
+                 o.geometry == DTKX_GEOMETRY_DOUBLE,
+             DTKX_ERR_INVALID_ARGUMENT );
+        REQ( o.layout == DTKX_LAYOUT_AOS || o.layout == DTKX_LAYOUT_SOA,
+             DTKX_ERR_INVALID_ARGUMENT );
+        REQ( o.restart >= 1 && o.restart <= 512, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( o.max_iterations >= 1, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( o.tolerance >= 0. && std::isfinite( o.tolerance ),
+             DTKX_ERR_INVALID_ARGUMENT );
+        REQ( n_src > 0, DTKX_ERR_EMPTY_SOURCE );
+        REQ( n_tgt >= 0 && src_xyz != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( n_tgt == 0 || tgt_xyz != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( n_src <= INT32_MAX - dtkx::spline_k && n_tgt <= INT32_MAX,
+             DTKX_ERR_UNSUPPORTED );
+        hipStream_t s = static_cast<hipStream_t>( o.stream );
+        auto h = std::make_unique<dtkx_spline>();
+        h->opts = o;
+        h->n_src = n_src;
+        h->n_tgt = n_tgt;
+        h->stream = s;
+        h->kk = (int)std::min<int64_t>( dtkx::spline_k, n_src );
+        int const kk = h->kk;
+        Staged<double> sx( src_xyz, 3 * n_src, s );
+        Staged<double> tx( tgt_xyz, 3 * n_tgt, s );
+        h->src_d.resize( n_src, s );
+        dtkx::points_to_double4( sx.ptr, n_src, o.layout, h->src_d.get(), s );
+        if ( n_tgt > 0 )
+        {
+            h->tgt_d.resize( n_tgt, s );
+            dtkx::points_to_double4( tx.ptr, n_tgt, o.layout, h->tgt_d.get(),
+                                     s );
+        }
+        h->m_idx.resize( (size_t)n_src * kk, s );
+        h->m_phi.resize( (size_t)n_src * kk, s );
+        h->n_idx.resize( (size_t)n_tgt * kk, s );
+        h->n_phi.resize( (size_t)n_tgt * kk, s );
+        dtkx::KnnOutput km, kn;
+        km.indices = h->m_idx.get();
+        kn.indices = h->n_idx.get();
+        bool const fl = o.geometry == DTKX_GEOMETRY_FLOAT;
+        if ( fl )
+        {
+            h->src_f.resize( n_src, s );
+            dtkx::double4_to_float4( h->src_d.get(), n_src, h->src_f.get(), s );
+            if ( n_tgt > 0 )
+            {
+                h->tgt_f.resize( n_tgt, s );
+                dtkx::double4_to_float4( h->tgt_d.get(), n_tgt,
+                                         h->tgt_f.get(), s );
+            }
+        }
+        void const *sp = fl ? (void const *)h->src_f.get()
+                            : (void const *)h->src_d.get();
+        void const *tp = fl ? (void const *)h->tgt_f.get()
+                            : (void const *)h->tgt_d.get();
+        // M: the sources' neighbours among the sources (def.hpp:211-212);
+        // N: the targets' (def.hpp:215-216)
+        dtkx::knn_search( o.geometry, sp, n_src, sp, n_src, dtkx::spline_k, km,
+                          nullptr, s );
+        if ( n_tgt > 0 )
+            dtkx::knn_search( o.geometry, sp, n_src, tp, n_tgt,
+                              dtkx::spline_k, kn, nullptr, s );
+        dtkx::spline_basis_weights( h->src_d.get(), h->src_d.get(), n_src, kk,
+                                    h->m_idx.get(), o.rbf, o.geometry,
+                                    h->m_phi.get(), s );
+        dtkx::spline_basis_weights( h->src_d.get(), h->tgt_d.get(), n_tgt, kk,
+                                    h->n_idx.get(), o.rbf, o.geometry,
+                                    h->n_phi.get(), s );
+        DTKX_HIP( hipStreamSynchronize( s ) );
+        if ( fl )
+        {
+            h->src_d.release();
+            h->tgt_d.release();
+        }
+        *out = h.release();
+        return DTKX_SUCCESS;
+    } );
+}
+
+int dtkx_spline_apply( const dtkx_spline *h, const double *src_vals,
+                       double *tgt_vals, int32_t n_fields, int64_t ld_src,
+                       int64_t ld_tgt )
+{
+    return guarded( [&]() {
+        REQ( h != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( n_fields >= 1, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( ld_src >= h->n_src, DTKX_ERR_SIZE_MISMATCH );
+        REQ( ld_tgt >= h->n_tgt, DTKX_ERR_SIZE_MISMATCH );
+        REQ( src_vals != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( h->n_tgt == 0 || tgt_vals != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        h->info = dtkx_spline_info{};
+        if ( h->n_tgt == 0 )
+            return DTKX_SUCCESS;
+        hipStream_t s = h->stream;
+        std::size_t const nin = (std::size_t)( n_fields - 1 ) * ld_src + h->n_src;
+        std::size_t const nout = (std::size_t)( n_fields - 1 ) * ld_tgt + h->n_tgt;
+        Staged<double> x( src_vals, nin, s );
+        StagedOut<double> y( tgt_vals, nout, s, ld_tgt > h->n_tgt );
+        dtkx::DeviceBuffer<double> z( h->n_src + dtkx::spline_k, s );
+        auto const sys = h->system();
+        dtkx_spline_info agg{};
+        agg.converged = 1;
+        for ( int f = 0; f < n_fields; ++f )
+        {
+            auto const r = h->solver.solve( sys, x.ptr + f * ld_src, z.get(),
+                                            h->gmres(), s );
+            agg.iterations = std::max( agg.iterations, r.iterations );
+            agg.residual = std::max( agg.residual, r.residual );
+            agg.converged = agg.converged && r.converged;
+            dtkx::spline_evaluate( sys, z.get(), y.ptr + f * ld_tgt, s );
+        }
+        y.finish( s );
+        DTKX_HIP( hipStreamSynchronize( s ) );
+        h->info = agg;
+        return DTKX_SUCCESS;
+    } );
+}
+
+int dtkx_spline_solve( const dtkx_spline *h, const double *src_vals,
+                       double *coeffs )
+{
+    return guarded( [&]() {
+        REQ( h != nullptr && src_vals != nullptr && coeffs != nullptr,
+             DTKX_ERR_INVALID_ARGUMENT );
+        hipStream_t s = h->stream;
+        std::size_t const nz = h->n_src + dtkx::spline_k;
+        Staged<double> x( src_vals, h->n_src, s );
+        StagedOut<double> z( coeffs, nz, s );
+        auto const r = h->solver.solve( h->system(), x.ptr, z.ptr, h->gmres(),
+                                        s );
+        z.finish( s );
+        DTKX_HIP( hipStreamSynchronize( s ) );
+        h->info.iterations = r.iterations;
+        h->info.residual = r.residual;
+        h->info.converged = r.converged;
+        return DTKX_SUCCESS;
+    } );
+}
+
+int dtkx_spline_evaluate( const dtkx_spline *h, const double *coeffs,
+                          double *tgt_vals )
+{
+    return guarded( [&]() {
+        REQ( h != nullptr && coeffs != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( h->n_tgt == 0 || tgt_vals != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        if ( h->n_tgt == 0 )
+            return DTKX_SUCCESS;
+        hipStream_t s = h->stream;
+        Staged<double> z( coeffs, h->n_src + dtkx::spline_k, s );
+        StagedOut<double> y( tgt_vals, h->n_tgt, s );
+        dtkx::spline_evaluate( h->system(), z.ptr, y.ptr, s );
+        y.finish( s );
+        DTKX_HIP( hipStreamSynchronize( s ) );
+        return DTKX_SUCCESS;
+    } );
+}
+
+int dtkx_spline_system_apply( const dtkx_spline *h, const double *z,
+                              double *y )
+{
+    return guarded( [&]() {
+        REQ( h != nullptr && z != nullptr && y != nullptr,
+             DTKX_ERR_INVALID_ARGUMENT );
+        hipStream_t s = h->stream;
+        std::size_t const n = h->n_src + dtkx::spline_k;
+        Staged<double> zi( z, n, s );
+        StagedOut<double> yo( y, n, s );
+        dtkx::spline_system_apply( h->system(), zi.ptr, yo.ptr, s );
+        yo.finish( s );
+        DTKX_HIP( hipStreamSynchronize( s ) );
+        return DTKX_SUCCESS;
+    } );
+}
+
+int dtkx_spline_get_basis( const dtkx_spline *h, int32_t which,
+                           const int32_t **indices, const double **values,
+                           int64_t *n_rows, int32_t *kk )
+{
+    return guarded( [&]() {
+        REQ( h != nullptr && ( which == 0 || which == 1 ),
+             DTKX_ERR_INVALID_ARGUMENT );
+        if ( indices )
+            *indices = which == 0 ? h->m_idx.get() : h->n_idx.get();
+        if ( values )
+            *values = which == 0 ? h->m_phi.get() : h->n_phi.get();
+        if ( n_rows )
+            *n_rows = which == 0 ? h->n_src : h->n_tgt;
+        if ( kk )
+            *kk = h->kk;
+        return DTKX_SUCCESS;
+    } );
+}
+
+int dtkx_spline_get_info( const dtkx_spline *h, dtkx_spline_info *info )
+{
+    return guarded( [&]() {
+        REQ( h != nullptr && info != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        *info = h->info;
+        return DTKX_SUCCESS;
+    } );
+}
+
+int dtkx_spline_destroy( dtkx_spline *h )
 {
     return guarded( [&]() {
         delete h;

@@ -1,0 +1,525 @@
+// Compactly supported RBF spline: basis weights, the saddle-point operator
+// C = P + M + P^T, the target evaluation (Q + N) and a restarted GMRES
+// whose Krylov basis lives in device memory.
+//
+// Layout: an extended vector is [n_src source entries | 4 polynomial
+// entries], the order of the reference's prolongation range map
+// (DTK_DetailsSplineProlongationOperator.hpp:47-61). The basis is
+// (restart + 1) such vectors at a stride ld (a multiple of 64 doubles).
+// Every reduction runs over a fixed grid and sums its per-block partials in
+// block order, so a solve is deterministic.
+#include "mls_device.hpp"
+#include "spline.hpp"
+
+#include <cfloat>
+#include <cmath>
+#include <vector>
+
+namespace dtkx
+{
+namespace
+{
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+// Krylov kernels: rows per thread and per block
+constexpr int kRows = 4;
+constexpr int kTile = kBlock * kRows;
+// largest number of basis vectors one orthogonalisation pass handles
+constexpr int kMaxVectors = 512;
+// grid of the operator kernels (grid-stride over the rows)
+constexpr int kRowBlocks = 2048;
+
+__device__ inline double wave_sum( double a )
+{
+    for ( int o = 32; o > 0; o >>= 1 )
+        a += __shfl_xor( a, o, 64 );
+    return a;
+}
+
+__device__ inline double4 as_double4( float4 p )
+{
+    return make_double4( p.x, p.y, p.z, 0. );
+}
+__device__ inline double4 as_double4( double4 p ) { return p; }
+
+// ------------------------------------------------------------ weights
+// One thread per row point: transformed coordinates in double
+// (Impl.hpp:92-95), distances as computeRadius / computeWeights form them
+// (Impl.hpp:121-125, 162-165), R = 1.1 max(10 DBL_EPSILON, max_l d_l)
+// (Impl.hpp:118-134), phi_l = rbf(d_l / R).
+__global__ void __launch_bounds__( kBlock )
+    basis_weights_kernel( double4 const *src, double4 const *pts, int64_t n,
+                          int kk, int32_t const *idx, int rbf, int geometry,
+                          double *phi )
+{
+    int64_t const i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( i >= n )
+        return;
+    double4 const t = pts[i];
+    double dist[spline_k];
+    double dmax = 10. * DBL_EPSILON;
+    for ( int l = 0; l < kk; ++l )
+    {
+        double4 const s = src[idx[i * kk + l]];
+        double const xr[3] = {s.x - t.x, s.y - t.y, s.z - t.z};
+        dist[l] = dev::rel_distance( geometry, xr );
+        if ( dist[l] > dmax )
+            dmax = dist[l];
+    }
+    double const radius = 1.1 * dmax;
+    for ( int l = 0; l < kk; ++l )
+        phi[i * kk + l] = dev::rbf_eval( rbf, dist[l] / radius );
+}
+
+// ------------------------------------------------------------ operators
+// Rows of (V + B) z for a basis operator B (kk entries per row) and the
+// Vandermonde rows V(i) = {1, x_i, y_i, z_i} of the row points:
+// y[i] = V(i) zp + sum_l phi[i][l] zs[idx[i][l]], the order of
+// Thyra::add(P, M) (def.hpp:230-231). TRANS (the sources' rows only) also
+// forms the block partials of P^T zs = sum_i V(i) zs[i]
+// (DTK_DetailsPolynomialMatrix.hpp:136-190) into partial[p][block].
+template <typename Pt, bool TRANS>
+__global__ void __launch_bounds__( kBlock )
+    rows_kernel( Pt const *pts, int64_t n_rows, int kk,
+                 int32_t const *idx, double const *phi, double const *zs,
+                 double const *zp, double *y, double *partial )
+{
+    double const p0 = zp[0], p1 = zp[1], p2 = zp[2], p3 = zp[3];
+    double acc[4] = {0., 0., 0., 0.};
+    int64_t const stride = (int64_t)gridDim.x * kBlock;
+    for ( int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n_rows;
+          i += stride )
+    {
+        double4 const v = as_double4( pts[i] );
+        double yi = p0 + v.x * p1 + v.y * p2 + v.z * p3;
+        double b = 0.;
+        for ( int l = 0; l < kk; ++l )
+            b += phi[i * kk + l] * zs[idx[i * kk + l]];
+        y[i] = yi + b;
+        if ( TRANS )
+        {
+            double const zi = zs[i];
+            acc[0] += zi;
+            acc[1] += v.x * zi;
+            acc[2] += v.y * zi;
+            acc[3] += v.z * zi;
+        }
+    }
+    if ( !TRANS )
+        return;
+    __shared__ double lds[4][kWaves];
+    int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for ( int p = 0; p < 4; ++p )
+    {
+        double const a = wave_sum( acc[p] );
+        if ( lane == 0 )
+            lds[p][wave] = a;
+    }
+    __syncthreads();
+    if ( threadIdx.x < 4 )
+    {
+        double a = 0.;
+        for ( int w = 0; w < kWaves; ++w )
+            a += lds[threadIdx.x][w];
+        partial[threadIdx.x * (int64_t)gridDim.x + blockIdx.x] = a;
+    }
+}
+
+// out[v] = sum_b partial[v][b] over nb blocks, one wave per v.
+__global__ void __launch_bounds__( kBlock )
+    reduce_kernel( double const *partial, int nb, int nv, double *out )
+{
+    int const v = blockIdx.x * kWaves + ( threadIdx.x >> 6 );
+    int const lane = threadIdx.x & 63;
+    if ( v >= nv )
+        return;
+    double a = 0.;
+    for ( int b = lane; b < nb; b += 64 )
+        a += partial[(int64_t)v * nb + b];
+    a = wave_sum( a );
+    if ( lane == 0 )
+        out[v] = a;
+}
+
+// partial[v][block] = sum over the block's rows of q_v[i] w[i], v < nv.
+__global__ void __launch_bounds__( kBlock )
+    gemv_t_kernel( double const *basis, int64_t ld, int nv, int64_t n,
+                   double const *w, double *partial )
+{
+    __shared__ double lds[kWaves][kMaxVectors];
+    int64_t const base = blockIdx.x * (int64_t)kTile + threadIdx.x;
+    double wr[kRows];
+#pragma unroll
+    for ( int r = 0; r < kRows; ++r )
+    {
+        int64_t const i = base + r * kBlock;
+        wr[r] = i < n ? w[i] : 0.;
+    }
+    int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for ( int v = 0; v < nv; ++v )
+    {
+        double const *q = basis + v * ld;
+        double a = 0.;
+#pragma unroll
+        for ( int r = 0; r < kRows; ++r )
+        {
+            int64_t const i = base + r * kBlock;
+            if ( i < n )
+                a += q[i] * wr[r];
+        }
+        a = wave_sum( a );
+        if ( lane == 0 )
+            lds[wave][v] = a;
+    }
+    __syncthreads();
+    for ( int v = threadIdx.x; v < nv; v += kBlock )
+    {
+        double a = 0.;
+        for ( int w2 = 0; w2 < kWaves; ++w2 )
+            a += lds[w2][v];
+        partial[(int64_t)v * gridDim.x + blockIdx.x] = a;
+    }
+}
+
+// w[i] -= sum_v q_v[i] h[v], v < nv; with norm_partial, the block's
+// sum of the new w[i]^2 as well.
+__global__ void __launch_bounds__( kBlock )
+    gemv_n_kernel( double const *basis, int64_t ld, int nv, int64_t n,
+                   double const *h, double *w, double *norm_partial )
+{
+    __shared__ double hs[kMaxVectors];
+    __shared__ double lds[kWaves];
+    for ( int v = threadIdx.x; v < nv; v += kBlock )
+        hs[v] = h[v];
+    __syncthreads();
+    int64_t const base = blockIdx.x * (int64_t)kTile + threadIdx.x;
+    double acc[kRows];
+#pragma unroll
+    for ( int r = 0; r < kRows; ++r )
+    {
+        int64_t const i = base + r * kBlock;
+        acc[r] = i < n ? w[i] : 0.;
+    }
+    for ( int v = 0; v < nv; ++v )
+    {
+        double const *q = basis + v * ld;
+        double const hv = hs[v];
+#pragma unroll
+        for ( int r = 0; r < kRows; ++r )
+        {
+            int64_t const i = base + r * kBlock;
+            if ( i < n )
+                acc[r] -= q[i] * hv;
+        }
+    }
+    double s2 = 0.;
+#pragma unroll
+    for ( int r = 0; r < kRows; ++r )
+    {
+        int64_t const i = base + r * kBlock;
+        if ( i < n )
+        {
+            w[i] = acc[r];
+            s2 += acc[r] * acc[r];
+        }
+    }
+    if ( !norm_partial )
+        return;
+    s2 = wave_sum( s2 );
+    int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if ( lane == 0 )
+        lds[wave] = s2;
+    __syncthreads();
+    if ( threadIdx.x == 0 )
+    {
+        double a = 0.;
+        for ( int w2 = 0; w2 < kWaves; ++w2 )
+            a += lds[w2];
+        norm_partial[blockIdx.x] = a;
+    }
+}
+
+// out[i] = in[i] * alpha
+__global__ void __launch_bounds__( kBlock )
+    scale_kernel( double const *in, int64_t n, double alpha, double *out )
+{
+    int64_t const i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( i < n )
+        out[i] = in[i] * alpha;
+}
+
+// r[i] = b[i] - r[i] and the block's sum of r[i]^2 (same tiling as gemv).
+__global__ void __launch_bounds__( kBlock )
+    residual_kernel( double const *b, int64_t n, double *r,
+                     double *norm_partial )
+{
+    __shared__ double lds[kWaves];
+    int64_t const base = blockIdx.x * (int64_t)kTile + threadIdx.x;
+    double s2 = 0.;
+#pragma unroll
+    for ( int k = 0; k < kRows; ++k )
+    {
+        int64_t const i = base + k * kBlock;
+        if ( i < n )
+        {
+            double const v = b[i] - r[i];
+            r[i] = v;
+            s2 += v * v;
+        }
+    }
+    s2 = wave_sum( s2 );
+    int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if ( lane == 0 )
+        lds[wave] = s2;
+    __syncthreads();
+    if ( threadIdx.x == 0 )
+    {
+        double a = 0.;
+        for ( int w2 = 0; w2 < kWaves; ++w2 )
+            a += lds[w2];
+        norm_partial[blockIdx.x] = a;
+    }
+}
+
+int row_blocks( int64_t n )
+{
+    int64_t const b = ( n + kBlock - 1 ) / kBlock;
+    return (int)std::max<int64_t>( 1, std::min<int64_t>( b, kRowBlocks ) );
+}
+
+template <bool TRANS>
+void launch_rows( int geometry, void const *pts, int64_t n_rows, int kk,
+                  int32_t const *idx, double const *phi, double const *zs,
+                  double const *zp, double *y, double *partial, int nb,
+                  hipStream_t s )
+{
+    if ( geometry == DTKX_GEOMETRY_FLOAT )
+        rows_kernel<float4, TRANS><<<nb, kBlock, 0, s>>>(
+            static_cast<float4 const *>( pts ), n_rows, kk, idx, phi, zs, zp,
+            y, partial );
+    else
+        rows_kernel<double4, TRANS><<<nb, kBlock, 0, s>>>(
+            static_cast<double4 const *>( pts ), n_rows, kk, idx, phi, zs, zp,
+            y, partial );
+    DTKX_HIP( hipGetLastError() );
+}
+
+// C z into y; partial holds 4 * row_blocks(n_src) doubles.
+void system_apply( SplineSystem const &sys, double const *z, double *y,
+                   double *partial, hipStream_t s )
+{
+    int const nb = row_blocks( sys.n_src );
+    launch_rows<true>( sys.geometry, sys.src_pts, sys.n_src, sys.kk,
+                       sys.m_idx, sys.m_phi, z, z + sys.n_src, y, partial, nb,
+                       s );
+    reduce_kernel<<<1, kBlock, 0, s>>>( partial, nb, spline_k,
+                                        y + sys.n_src );
+    DTKX_HIP( hipGetLastError() );
+}
+
+int tiles( int64_t n ) { return (int)( ( n + kTile - 1 ) / kTile ); }
+
+} // namespace
+
+void spline_basis_weights( double4 const *src, double4 const *pts, int64_t n,
+                           int kk, int32_t const *idx, int rbf, int geometry,
+                           double *phi, hipStream_t s )
+{
+    if ( n == 0 )
+        return;
+    int64_t const nb = ( n + kBlock - 1 ) / kBlock;
+    basis_weights_kernel<<<nb, kBlock, 0, s>>>( src, pts, n, kk, idx, rbf,
+                                                geometry, phi );
+    DTKX_HIP( hipGetLastError() );
+}
+
+void spline_system_apply( SplineSystem const &sys, double const *z, double *y,
+                          hipStream_t s )
+{
+    DeviceBuffer<double> partial( 4 * (size_t)row_blocks( sys.n_src ), s );
+    system_apply( sys, z, y, partial.get(), s );
+}
+
+void spline_evaluate( SplineSystem const &sys, double const *z, double *y,
+                      hipStream_t s )
+{
+    if ( sys.n_tgt == 0 )
+        return;
+    launch_rows<false>( sys.geometry, sys.tgt_pts, sys.n_tgt, sys.kk,
+                        sys.n_idx, sys.n_phi, z, z + sys.n_src, y, nullptr,
+                        row_blocks( sys.n_tgt ), s );
+}
+
+SplineSolver::~SplineSolver()
+{
+    if ( host_ )
+        (void)hipHostFree( host_ );
+}
+
+void SplineSolver::reserve( int64_t n, int restart, hipStream_t s )
+{
+    int64_t const ld = ( n + 63 ) / 64 * 64;
+    if ( ld == ld_ && restart <= restart_ )
+        return;
+    restart = std::max( restart, restart_ );
+    ld_ = ld;
+    restart_ = restart;
+    nblk_ = tiles( n );
+    size_t const nv = (size_t)restart + 1;
+    basis_.resize( nv * ld, s );
+    rhs_.resize( ld, s );
+    res_.resize( ld, s );
+    size_t const rows = 4 * (size_t)row_blocks( n );
+    partial_.resize( std::max( nv * nblk_, rows ), s );
+    dots_.resize( 3 * nv + 1, s );
+    if ( host_ )
+        DTKX_HIP( hipHostFree( host_ ) );
+    host_ = nullptr;
+    DTKX_HIP( hipHostMalloc( (void **)&host_, ( 3 * nv + 1 ) * sizeof( double ),
+                             hipHostMallocDefault ) );
+}
+
+GmresInfo SplineSolver::solve( SplineSystem const &sys, double const *x,
+                               double *z, GmresOptions const &opt,
+                               hipStream_t s )
+{
+    int64_t const n = sys.n_src + spline_k;
+    int const m = opt.restart;
+    reserve( n, m, s );
+    int64_t const ld = ld_;
+    int const nb = nblk_;
+    double *const Q = basis_.get();
+    double *const b = rhs_.get();
+    double *const r = res_.get();
+    double *const part = partial_.get();
+    double *const dots = dots_.get();
+    auto norm2_of = [&]( double const *v ) {
+        gemv_t_kernel<<<nb, kBlock, 0, s>>>( v, ld, 1, n, v, part );
+        reduce_kernel<<<1, kBlock, 0, s>>>( part, nb, 1, dots );
+        DTKX_HIP( hipGetLastError() );
+        DTKX_HIP( hipMemcpyAsync( host_, dots, sizeof( double ),
+                                  hipMemcpyDeviceToHost, s ) );
+        DTKX_HIP( hipStreamSynchronize( s ) );
+        return host_[0];
+    };
+    // b = S x = [x; 0], z = 0 (def.hpp:200-203, 286-291)
+    DTKX_HIP( hipMemcpyAsync( b, x, sys.n_src * sizeof( double ),
+                              hipMemcpyDeviceToDevice, s ) );
+    DTKX_HIP( hipMemsetAsync( b + sys.n_src, 0, spline_k * sizeof( double ),
+                              s ) );
+    DTKX_HIP( hipMemsetAsync( z, 0, n * sizeof( double ), s ) );
+    GmresInfo info;
+    double const beta = std::sqrt( norm2_of( b ) );
+    if ( !( beta > 0. ) )
+    {
+        info.converged = std::isfinite( beta );
+        return info;
+    }
+    double const target = opt.tolerance * beta;
+    std::vector<double> H( (size_t)( m + 1 ) * m ), cs( m ), sn( m ),
+        g( m + 1 ), yv( m );
+    auto Hat = [&]( int i, int j ) -> double & {
+        return H[(size_t)j * ( m + 1 ) + i];
+    };
+    double const *rvec = b;
+    double rnorm = beta;
+    int total = 0;
+    int const gblocks = (int)( ( n + kBlock - 1 ) / kBlock );
+    while ( true )
+    {
+        // q_0 = r / |r|
+        scale_kernel<<<gblocks, kBlock, 0, s>>>( rvec, n, 1. / rnorm, Q );
+        DTKX_HIP( hipGetLastError() );
+        std::fill( g.begin(), g.end(), 0. );
+        g[0] = rnorm;
+        int j = 0;
+        while ( j < m && total < opt.max_iterations )
+        {
+            double *const w = Q + ( j + 1 ) * ld;
+            int const nv = j + 1;
+            system_apply( sys, Q + j * ld, w, part, s );
+            // classical Gram-Schmidt, twice; the second update also
+            // forms |w|^2
+            gemv_t_kernel<<<nb, kBlock, 0, s>>>( Q, ld, nv, n, w, part );
+            reduce_kernel<<<( nv + kWaves - 1 ) / kWaves, kBlock, 0, s>>>(
+                part, nb, nv, dots );
+            gemv_n_kernel<<<nb, kBlock, 0, s>>>( Q, ld, nv, n, dots, w,
+                                                 nullptr );
+            gemv_t_kernel<<<nb, kBlock, 0, s>>>( Q, ld, nv, n, w, part );
+            reduce_kernel<<<( nv + kWaves - 1 ) / kWaves, kBlock, 0, s>>>(
+                part, nb, nv, dots + nv );
+            gemv_n_kernel<<<nb, kBlock, 0, s>>>( Q, ld, nv, n, dots + nv, w,
+                                                 part );
+            reduce_kernel<<<1, kBlock, 0, s>>>( part, nb, 1, dots + 2 * nv );
+            DTKX_HIP( hipGetLastError() );
+            DTKX_HIP( hipMemcpyAsync( host_, dots,
+                                      ( 2 * nv + 1 ) * sizeof( double ),
+                                      hipMemcpyDeviceToHost, s ) );
+            DTKX_HIP( hipStreamSynchronize( s ) );
+            for ( int i = 0; i < nv; ++i )
+                Hat( i, j ) = host_[i] + host_[nv + i];
+            double const hn = std::sqrt( host_[2 * nv] );
+            Hat( j + 1, j ) = hn;
+            // previous rotations, then the one that zeroes H(j+1, j)
+            for ( int i = 0; i < j; ++i )
+            {
+                double const t = cs[i] * Hat( i, j ) + sn[i] * Hat( i + 1, j );
+                Hat( i + 1, j ) = -sn[i] * Hat( i, j ) + cs[i] * Hat( i + 1, j );
+                Hat( i, j ) = t;
+            }
+            double const a = Hat( j, j ), c = Hat( j + 1, j );
+            double const rr = std::hypot( a, c );
+            cs[j] = rr > 0. ? a / rr : 1.;
+            sn[j] = rr > 0. ? c / rr : 0.;
+            Hat( j, j ) = rr;
+            Hat( j + 1, j ) = 0.;
+            g[j + 1] = -sn[j] * g[j];
+            g[j] = cs[j] * g[j];
+            ++total;
+            ++j;
+            if ( !std::isfinite( hn ) )
+                break;
+            if ( std::fabs( g[j] ) <= target || !( hn > 0. ) )
+                break; // converged, or the Krylov space is invariant
+            scale_kernel<<<gblocks, kBlock, 0, s>>>( w, n, 1. / hn, w );
+            DTKX_HIP( hipGetLastError() );
+        }
+        // z += Q_j y, H(0:j, 0:j) y = g(0:j)
+        for ( int i = j - 1; i >= 0; --i )
+        {
+            double t = g[i];
+            for ( int l = i + 1; l < j; ++l )
+                t -= Hat( i, l ) * yv[l];
+            yv[i] = Hat( i, i ) != 0. ? t / Hat( i, i ) : 0.;
+        }
+        if ( j > 0 )
+        {
+            for ( int i = 0; i < j; ++i )
+                host_[i] = -yv[i];
+            DTKX_HIP( hipMemcpyAsync( dots, host_, j * sizeof( double ),
+                                      hipMemcpyHostToDevice, s ) );
+            gemv_n_kernel<<<nb, kBlock, 0, s>>>( Q, ld, j, n, dots, z,
+                                                 nullptr );
+            DTKX_HIP( hipGetLastError() );
+        }
+        // explicit residual r = b - C z
+        system_apply( sys, z, r, part, s );
+        residual_kernel<<<nb, kBlock, 0, s>>>( b, n, r, part );
+        reduce_kernel<<<1, kBlock, 0, s>>>( part, nb, 1, dots );
+        DTKX_HIP( hipGetLastError() );
+        DTKX_HIP( hipMemcpyAsync( host_, dots, sizeof( double ),
+                                  hipMemcpyDeviceToHost, s ) );
+        DTKX_HIP( hipStreamSynchronize( s ) );
+        rnorm = std::sqrt( host_[0] );
+        info.iterations = total;
+        info.residual = rnorm / beta;
+        info.converged = rnorm <= target;
+        if ( info.converged || total >= opt.max_iterations || j == 0 ||
+             !std::isfinite( rnorm ) )
+            return info;
+        rvec = r;
+    }
+}
+
+} // namespace dtkx

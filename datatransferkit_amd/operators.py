@@ -388,6 +388,118 @@ class NearestNeighborOperator:
         return out
 
 
+class SplineOperator:
+    """Compactly supported RBF spline transfer on one GPU
+    (packages/Meshfree/src/DTK_SplineOperator_decl.hpp:40-113,
+    _def.hpp:170-295): y = (Q + N) (P + M + P^T)^-1 S x with the linear
+    polynomial basis and 4 neighbours per row, solved by restarted GMRES on
+    the device to ``tolerance`` (the reference's Belos setting, 1e-10).
+
+        op = SplineOperator(source_points, target_points)
+        op.apply(source_values, target_values)
+    """
+
+    def __init__(self, source_points, target_points, *, rbf="wendland0",
+                 geometry="float", layout="aos", tolerance=1e-10, restart=300,
+                 max_iterations=1000, stream=None):
+        if layout == "aos":
+            ns, ds = _check_points(source_points, "source_points")
+            nt, dt = _check_points(target_points, "target_points")
+        else:
+            ds, ns = _check_points(source_points, "source_points")
+            dt, nt = _check_points(target_points, "target_points")
+        if ds != 3 or dt != 3:
+            raise DtkxError(_lib.ERR_DIMENSION, "points must be 3-D")
+        self._n_src, self._n_tgt = ns, nt
+        o = _lib.default_spline_options()
+        o.rbf = _lib.RBF[rbf]
+        o.geometry = _lib.GEOMETRY[geometry]
+        o.layout = _lib.LAYOUT[layout]
+        o.tolerance = tolerance
+        o.restart = restart
+        o.max_iterations = max_iterations
+        o.stream = stream
+        sp, ka = _ptr(source_points)
+        tp, kb = _ptr(target_points)
+        h = ctypes.c_void_p()
+        check(_lib.load().dtkx_spline_create(sp, ns, tp, nt, ctypes.byref(o),
+                                             ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                _lib.load().dtkx_spline_destroy(h)
+            except Exception:
+                pass
+            self._h = None
+
+    def apply(self, source_values, target_values):
+        """target_values = A source_values ([n] or [fields, n]); one solve
+        per field."""
+        nf, lds = _fields(source_values, self._n_src, "source_values")
+        nf2, ldt = _fields(target_values, self._n_tgt, "target_values")
+        if nf != nf2:
+            raise DtkxError(_lib.ERR_SIZE_MISMATCH, "field counts differ")
+        xp, ka = _ptr(source_values)
+        yp, kb = _ptr(target_values)
+        check(_lib.load().dtkx_spline_apply(self._h, xp, yp, nf, lds, ldt))
+        return target_values
+
+    def solve(self, source_values, coeffs):
+        """coeffs [n_src + 4] = (P + M + P^T)^-1 S source_values."""
+        _fields(source_values, self._n_src, "source_values")
+        _fields(coeffs, self._n_src + 4, "coeffs")
+        xp, ka = _ptr(source_values)
+        zp, kb = _ptr(coeffs)
+        check(_lib.load().dtkx_spline_solve(self._h, xp, zp))
+        return coeffs
+
+    def evaluate(self, coeffs, target_values):
+        """target_values = (Q + N) coeffs."""
+        _fields(coeffs, self._n_src + 4, "coeffs")
+        _fields(target_values, self._n_tgt, "target_values")
+        zp, ka = _ptr(coeffs)
+        yp, kb = _ptr(target_values)
+        check(_lib.load().dtkx_spline_evaluate(self._h, zp, yp))
+        return target_values
+
+    def system_apply(self, z, y):
+        """y = (P + M + P^T) z, both [n_src + 4]."""
+        _fields(z, self._n_src + 4, "z")
+        _fields(y, self._n_src + 4, "y")
+        zp, ka = _ptr(z)
+        yp, kb = _ptr(y)
+        check(_lib.load().dtkx_spline_system_apply(self._h, zp, yp))
+        return y
+
+    def basis(self, which):
+        """(indices [rows, kk] int32, values [rows, kk] float64) CUDA copies
+        of M (which="M": the sources' rows) or N ("N": the targets')."""
+        import torch
+
+        ip, vp = ctypes.c_void_p(), ctypes.c_void_p()
+        rows, kk = ctypes.c_int64(), ctypes.c_int32()
+        check(_lib.load().dtkx_spline_get_basis(
+            self._h, {"M": 0, "N": 1}[which], ctypes.byref(ip), ctypes.byref(vp),
+            ctypes.byref(rows), ctypes.byref(kk)))
+        n = rows.value * kk.value
+        idx = torch.empty((rows.value, kk.value), dtype=torch.int32, device="cuda")
+        val = torch.empty((rows.value, kk.value), dtype=torch.float64, device="cuda")
+        if n:
+            _memcpy_d2d(idx, ip.value, 4 * n)
+            _memcpy_d2d(val, vp.value, 8 * n)
+        return idx, val
+
+    def info(self) -> dict:
+        """Last solve: iterations, explicit relative residual, converged."""
+        s = _lib.SplineInfo()
+        check(_lib.load().dtkx_spline_get_info(self._h, ctypes.byref(s)))
+        return {"iterations": s.iterations, "residual": s.residual,
+                "converged": bool(s.converged)}
+
+
 def knn(source_points, target_points, k, geometry="float", layout="aos",
         with_distances=False, stream=None):
     """k nearest sources of each target: indices [n_tgt, min(k, n_src)]

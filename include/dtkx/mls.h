@@ -331,6 +331,72 @@ int dtkx_nn_get_indices(const dtkx_nn *h, const int32_t **indices,
                         int64_t *n_tgt);
 int dtkx_nn_destroy(dtkx_nn *h);
 
+/* ----------------------------------------------- RBF spline operator */
+/* SplineOperator<Wendland<0>, Linear3> (DTK_SplineOperator_decl.hpp:40-113,
+ * _def.hpp:170-295): A = (Q + N) (P + M + P^T)^-1 S. M (sources x sources)
+ * and N (targets x sources) hold, per row, the 4 nearest sources
+ * (k = PolynomialBasis::size, _def.hpp:188) with the values
+ * rbf(|x_l - t| / R), R = 1.1 max_l |x_l - t| (buildBasisOperator,
+ * _def.hpp:46-149); P and Q are the Vandermonde rows {1, x, y, z} of the
+ * sources and targets (buildPolynomialOperator, _def.hpp:151-167, and
+ * DTK_DetailsPolynomialMatrix.hpp:71-196); S appends four zero polynomial
+ * entries to a source vector (DTK_DetailsSplineProlongationOperator.hpp
+ * :41-108). The saddle-point system C z = S x, C = [[M, P], [P^T, 0]], is
+ * solved on the GPU by restarted GMRES from z = 0, as the reference does
+ * with Belos pseudo-block GMRES (_def.hpp:248-263). One rank. */
+typedef struct dtkx_spline dtkx_spline;
+
+typedef struct dtkx_spline_options {
+  int32_t rbf;            /* dtkx_rbf; default Wendland<0> (_def.hpp:319) */
+  int32_t geometry;       /* dtkx_geometry; FLOAT = the reference's points */
+  int32_t layout;         /* dtkx_layout of the point arrays */
+  int32_t restart;        /* Krylov vectors per GMRES cycle, 1..512 (300) */
+  int32_t max_iterations; /* GMRES iterations per solve (1000) */
+  int32_t reserved;
+  double tolerance;       /* relative residual |S x - C z| / |S x|: 1e-10
+                           * ("Convergence Tolerance", _def.hpp:258) */
+  void *stream;           /* hipStream_t or NULL */
+} dtkx_spline_options;
+
+/* Report of a handle's last solve (one per field of the last apply/solve:
+ * the largest iteration count and residual, and whether all converged). */
+typedef struct dtkx_spline_info {
+  int32_t iterations;
+  int32_t converged;
+  double residual;
+} dtkx_spline_info;
+
+int dtkx_spline_options_init(dtkx_spline_options *opts);
+/* Fails with DTKX_ERR_EMPTY_SOURCE when n_src == 0 (the reference's
+ * DTK_CHECK(!distributed_tree.empty()), _def.hpp:70). */
+int dtkx_spline_create(const double *src_xyz, int64_t n_src,
+                       const double *tgt_xyz, int64_t n_tgt,
+                       const dtkx_spline_options *opts, dtkx_spline **out);
+/* y = A x for n_fields fields stored [n_fields][ld] (ld_src >= n_src,
+ * ld_tgt >= n_tgt): one GMRES solve per field (apply, _def.hpp:266-295).
+ * Synchronous. Not safe to call concurrently on one handle. */
+int dtkx_spline_apply(const dtkx_spline *h, const double *src_vals,
+                      double *tgt_vals, int32_t n_fields, int64_t ld_src,
+                      int64_t ld_tgt);
+/* The spline coefficients z = C^-1 S x (n_src + 4 doubles: the source
+ * entries, then the four polynomial entries) for one field. */
+int dtkx_spline_solve(const dtkx_spline *h, const double *src_vals,
+                      double *coeffs);
+/* y = (Q + N) z for coefficients z of n_src + 4 doubles. */
+int dtkx_spline_evaluate(const dtkx_spline *h, const double *coeffs,
+                         double *tgt_vals);
+/* y = C z = (P + M + P^T) z, both of n_src + 4 doubles. */
+int dtkx_spline_system_apply(const dtkx_spline *h, const double *z,
+                             double *y);
+/* The basis operators (device arrays owned by the handle): which = 0 for M
+ * (n_rows = n_src), 1 for N (n_rows = n_tgt); row i holds kk = min(4, n_src)
+ * source indices ascending by (distance, index) and their rbf values. */
+int dtkx_spline_get_basis(const dtkx_spline *h, int32_t which,
+                          const int32_t **indices, const double **values,
+                          int64_t *n_rows, int32_t *kk);
+int dtkx_spline_get_info(const dtkx_spline *h, dtkx_spline_info *info);
+int dtkx_spline_destroy(dtkx_spline *h);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif

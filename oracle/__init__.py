@@ -83,6 +83,9 @@ def _declare(L):
         _dp, _i64, _dp, _i64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
         ctypes.c_long, _ip, _dp, ctypes.POINTER(Counters)]
+    L.dtkx_oracle_basis_weights.argtypes = [
+        _dp, _i64, _dp, _i64, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int,
+        ctypes.c_int, _dp]
     L.dtkx_oracle_set_libm_mode.argtypes = [ctypes.c_int]
     L.dtkx_oracle_set_libm_mode.restype = None
     L.dtkx_oracle_get_libm_mode.restype = ctypes.c_int
@@ -225,6 +228,22 @@ def mls_setup(src, tgt, indices, m, rbf_name="wendland0",
     if want_a0:
         return coeffs, c, a0
     return coeffs, c
+
+
+def basis_weights(src, tgt, indices, rbf_name="wendland0",
+                  geometry=GEOMETRY_FLOAT, layout=LAYOUT_AOS):
+    """Values of the spline's basis operator rows: phi [nt, kk] for the
+    given neighbour rows (DTK_SplineOperator_def.hpp:91-109)."""
+    src = _f64(src)
+    tgt = _f64(tgt)
+    indices = np.ascontiguousarray(indices, dtype=np.int32)
+    nt = tgt.size // 3
+    kk = indices.shape[1] if indices.ndim == 2 else 0
+    phi = np.empty((nt, kk))
+    _check(lib().dtkx_oracle_basis_weights(
+        _d(src), src.size // 3, _d(tgt), nt, layout, _i(indices), kk,
+        RBF[rbf_name], geometry, _d(phi)))
+    return phi
 
 
 def mls_create(src, tgt, m, k=0, rbf_name="wendland0", geometry=GEOMETRY_FLOAT,

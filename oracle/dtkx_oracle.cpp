@@ -974,6 +974,58 @@ int dtkx_oracle_mls_setup( double const *src, int64_t ns, double const *tgt,
     return 0;
 }
 
+// Values of SplineOperator::buildBasisOperator (DTK_SplineOperator_def.hpp
+// :91-109, 149-163): for each row point t with its kk neighbours, the
+// transformed coordinates (Impl.hpp:92-95), the radius (Impl.hpp:118-134)
+// and the weights phi_l = rbf(|x_l - t| / R) (Impl.hpp:161-165). phi is
+// [nt][kk] in the neighbour order given.
+int dtkx_oracle_basis_weights( double const *src, int64_t ns,
+                               double const *tgt, int64_t nt, int layout,
+                               int32_t const *indices, int kk, int rbf,
+                               int geometry, double *phi )
+{
+    if ( rbf < 0 || rbf > DTKX_RBF_BUHMANN4 || kk < 0 )
+        return DTKX_ERR_INVALID_ARGUMENT;
+#pragma omp parallel
+    {
+        std::vector<double> dist( std::max( kk, 1 ) );
+#pragma omp for schedule( static )
+        for ( int64_t t = 0; t < nt; ++t )
+        {
+            double tp[3];
+            load_point( tgt, nt, layout, t, tp );
+            for ( int l = 0; l < kk; ++l )
+            {
+                double s[3], xr[3];
+                load_point( src, ns, layout, indices[t * kk + l], s );
+                for ( int d = 0; d < 3; ++d )
+                    xr[d] = s[d] - tp[d];
+                if ( geometry == DTKX_GEOMETRY_FLOAT )
+                {
+                    float const pf[3] = {static_cast<float>( xr[0] ),
+                                         static_cast<float>( xr[1] ),
+                                         static_cast<float>( xr[2] )};
+                    float const zf[3] = {0.f, 0.f, 0.f};
+                    dist[l] = distance_f( pf, zf );
+                }
+                else
+                {
+                    double const zd[3] = {0., 0., 0.};
+                    dist[l] = distance_d( xr, zd );
+                }
+            }
+            double distance = 10. * DBL_EPSILON;
+            for ( int l = 0; l < kk; ++l )
+                if ( dist[l] > distance )
+                    distance = dist[l];
+            double const radius = 1.1 * distance;
+            for ( int l = 0; l < kk; ++l )
+                phi[t * kk + l] = rbf_radial( rbf, radius, dist[l] );
+        }
+    }
+    return 0;
+}
+
 // computeTargetValues (Impl.hpp:58-66) for n_fields fields with leading
 // dimensions; the row sum starts at 0 and runs j ascending.
 int dtkx_oracle_apply( int64_t nt, int kk, int32_t const *indices,
