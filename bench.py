@@ -566,6 +566,126 @@ def bench_nn(args):
     print(json.dumps(out), flush=True)
 
 
+def spline_solve_bytes(n_src, info, restart=300):
+    """Algorithmic HBM bytes of one GMRES solve, 8 bytes per vector entry,
+    n = n_src + 4 entries. Per iteration with nv basis vectors: the
+    operator C (index 16, value 32, point 16, own entry 8 and result
+    8 bytes per source row); the first Gram-Schmidt pass, whose dots read
+    the nv vectors, w as vector nv (for |w|^2) and w, and whose update reads
+    the nv vectors and w and writes w; the second pass when taken (dots:
+    nv + 1 reads, update: nv + 2); the normalisation (read + write w). The
+    second passes are spread evenly over the iterations."""
+    n = n_src + 4
+    its = info["iterations"]
+    frac2 = info["reorthogonalized"] / max(its, 1)
+    total = 0.0
+    for it in range(its):
+        nv = it % restart + 1
+        total += 80 * n_src + 8 * n * ((nv + 2) + (nv + 2)) + 16 * n
+        total += frac2 * 8 * n * ((nv + 1) + (nv + 2))
+    return int(total)
+
+
+def bench_spline(args):
+    """The RBF spline operator (the reference's SplineOperator): create
+    (two k = 4 searches and the basis weights) and one apply (GMRES solve of
+    the saddle-point system to 1e-10 and the target evaluation) per step,
+    n sources -> n targets, one field. The CPU baseline is the oracle on a
+    smaller cloud of the same distribution: its k-d tree searches and
+    weights, and scipy's restarted GMRES (restart 300, 1e-10) on the same
+    system."""
+    import numpy as np
+    import torch
+
+    import datatransferkit_amd as dtk
+
+    n = args.n
+    rng = np.random.default_rng(5)
+    src = rng.random((n, 3))
+    tgt = 0.05 + 0.9 * rng.random((n, 3))
+    vals = np.sin(2 * np.pi * src[:, 0]) * np.cos(2 * np.pi * src[:, 1]) + src[:, 2] ** 2
+    d_src, d_tgt, x = (torch.from_numpy(a).cuda() for a in (src, tgt, vals))
+    y = torch.empty(n, dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def step():
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        e[0].record()
+        op = dtk.SplineOperator(d_src, d_tgt, stream=stream)
+        e[1].record()
+        op.apply(x, y)
+        e[2].record()
+        e[2].synchronize()
+        return op, e[0].elapsed_time(e[1]), e[1].elapsed_time(e[2])
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ms_c, ms_a = [], []
+    for _ in range(args.steps):
+        op, c, a = step()
+        ms_c.append(c)
+        ms_a.append(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    info = op.info()
+    ms_solve = float(np.mean(ms_a))
+    nbytes = spline_solve_bytes(n, info)
+    gbs = nbytes / (ms_solve * 1e-3) / 1e9
+    out = {
+        "metric": "spline transfer target pts/s (create + apply)",
+        "value": n / dt, "unit": "target pts/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1000 * dt,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 (f32 search)", "data": "synthetic",
+        "config": {"workload": f"SplineOperator<Wendland0, Linear3> {n} sources U[0,1]^3 -> "
+                               f"{n} targets U[0.05,0.95]^3, 1 field, GMRES tol 1e-10",
+                   "n": n},
+        "ms_create": float(np.mean(ms_c)), "ms_apply": ms_solve,
+        "gmres": info,
+        "apply_roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                           "algorithmic": f"{nbytes} bytes per solve "
+                                          f"({info['iterations']} iterations, "
+                                          f"{info['reorthogonalized']} second passes)"},
+    }
+    if not args.no_cpu_baseline:
+        import oracle
+        import scipy.sparse as sps
+        import scipy.sparse.linalg as spla
+        from oracle.spline import SplineOracle
+
+        sample = min(n, args.cpu_sample or n)
+        threads = min(os.cpu_count() or 1, int(os.environ.get("DTKX_CPU_THREADS", "16")))
+        oracle.set_num_threads(threads)
+        r2 = np.random.default_rng(6)
+        cs = r2.random((sample, 3))
+        ct = 0.05 + 0.9 * r2.random((sample, 3))
+        cx = np.sin(2 * np.pi * cs[:, 0]) * np.cos(2 * np.pi * cs[:, 1]) + cs[:, 2] ** 2
+        t0 = time.perf_counter()
+        o = SplineOracle(cs, ct)
+        vs = sps.csr_matrix(o.Vs)
+        cmat = sps.bmat([[o.M, vs], [vs.T, None]], format="csr")
+        z, rc = spla.gmres(cmat, np.concatenate([cx, np.zeros(4)]), rtol=1e-10,
+                           restart=300, maxiter=1000)
+        cy = o.Vt @ z[sample:] + o.N @ z[:sample]
+        dtc = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": sample / dtc, "unit": "target pts/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{sample} sources -> {sample} targets of the same distributions: "
+                      f"oracle k-d tree searches ({threads} threads) and weights, scipy "
+                      f"GMRES (restart 300, 1e-10, converged={rc == 0}) and evaluation "
+                      f"({dtc:.1f} s)"}
+        gop = dtk.SplineOperator(torch.from_numpy(cs).cuda(), torch.from_numpy(ct).cuda())
+        gy = torch.empty(sample, dtype=torch.float64, device="cuda")
+        gop.apply(torch.from_numpy(cx).cuda(), gy)
+        out["sample_max_rel_diff_vs_cpu"] = float(
+            np.max(np.abs(gy.cpu().numpy() - cy)) / np.max(np.abs(cy)))
+    print(json.dumps(out), flush=True)
+
+
 def bench_copy(args):
     """Calibration for HBM counters: a plain device-to-device copy of
     1 GiB (reads 1 GiB, writes 1 GiB per launch), repeated."""
@@ -597,11 +717,12 @@ def main():
                     help="targets timed on the CPU oracle (0: all)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="transfer",
-                    choices=["transfer", "batched", "adjoint", "shell", "nn", "copy"],
+                    choices=["transfer", "batched", "adjoint", "shell", "nn", "spline",
+                             "copy"],
                     help="transfer: one 1M x 1M operator (default); batched: many "
                          "uneven operators; adjoint: forward + backward; shell: "
                          "FP32 bulk on a thin shell; nn: the nearest-neighbour "
-                         "operator")
+                         "operator; spline: the RBF spline operator")
     ap.add_argument("--problems", type=int, default=4096)
     ap.add_argument("--max-src", type=int, default=65536)
     args = ap.parse_args()
@@ -621,6 +742,9 @@ def main():
         return
     if args.workload == "nn":
         bench_nn(args)
+        return
+    if args.workload == "spline":
+        bench_spline(args)
         return
     if args.workload == "shell":
         if args.n == 1_000_000:

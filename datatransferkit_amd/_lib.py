@@ -100,6 +100,7 @@ class SplineOptions(ctypes.Structure):
 
 class SplineInfo(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_int32), ("converged", ctypes.c_int32),
+                ("reorthogonalized", ctypes.c_int32), ("restarts", ctypes.c_int32),
                 ("residual", ctypes.c_double)]
 
 

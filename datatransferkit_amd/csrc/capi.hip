@@ -1101,6 +1101,9 @@ int dtkx_spline_apply( const dtkx_spline *h, const double *src_vals,
             auto const r = h->solver.solve( sys, x.ptr + f * ld_src, z.get(),
                                             h->gmres(), s );
             agg.iterations = std::max( agg.iterations, r.iterations );
+            agg.reorthogonalized =
+                std::max( agg.reorthogonalized, r.reorthogonalized );
+            agg.restarts = std::max( agg.restarts, r.restarts );
             agg.residual = std::max( agg.residual, r.residual );
             agg.converged = agg.converged && r.converged;
             dtkx::spline_evaluate( sys, z.get(), y.ptr + f * ld_tgt, s );
@@ -1127,6 +1130,8 @@ int dtkx_spline_solve( const dtkx_spline *h, const double *src_vals,
         z.finish( s );
         DTKX_HIP( hipStreamSynchronize( s ) );
         h->info.iterations = r.iterations;
+        h->info.reorthogonalized = r.reorthogonalized;
+        h->info.restarts = r.restarts;
         h->info.residual = r.residual;
         h->info.converged = r.converged;
         return DTKX_SUCCESS;

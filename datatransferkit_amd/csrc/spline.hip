@@ -11,6 +11,7 @@
 #include "mls_device.hpp"
 #include "spline.hpp"
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <vector>
@@ -22,9 +23,15 @@ namespace
 
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
-// Krylov kernels: rows per thread and per block
-constexpr int kRows = 4;
-constexpr int kTile = kBlock * kRows;
+// Krylov kernels. Dots (gemv_t): each block takes kRowsT rows per thread
+// and a group of kGroup basis vectors, so a thread has kGroup
+// independent loads per row and one wave reduction per vector for kRowsT
+// rows. Updates (gemv_n, residual): kRowsN rows per thread, many waves.
+constexpr int kRowsT = 8;
+constexpr int kTileT = kBlock * kRowsT;
+constexpr int kGroup = 16;
+constexpr int kRowsN = 2;
+constexpr int kTileN = kBlock * kRowsN;
 // largest number of basis vectors one orthogonalisation pass handles
 constexpr int kMaxVectors = 512;
 // grid of the operator kernels (grid-stride over the rows)
@@ -94,8 +101,20 @@ __global__ void __launch_bounds__( kBlock )
         double4 const v = as_double4( pts[i] );
         double yi = p0 + v.x * p1 + v.y * p2 + v.z * p3;
         double b = 0.;
-        for ( int l = 0; l < kk; ++l )
-            b += phi[i * kk + l] * zs[idx[i * kk + l]];
+        if ( kk == spline_k )
+        {
+            int4 const c = reinterpret_cast<int4 const *>( idx )[i];
+            double2 const f0 = reinterpret_cast<double2 const *>( phi )[2 * i];
+            double2 const f1 =
+                reinterpret_cast<double2 const *>( phi )[2 * i + 1];
+            b += f0.x * zs[c.x];
+            b += f0.y * zs[c.y];
+            b += f1.x * zs[c.z];
+            b += f1.y * zs[c.w];
+        }
+        else
+            for ( int l = 0; l < kk; ++l )
+                b += phi[i * kk + l] * zs[idx[i * kk + l]];
         y[i] = yi + b;
         if ( TRANS )
         {
@@ -142,43 +161,54 @@ __global__ void __launch_bounds__( kBlock )
         out[v] = a;
 }
 
-// partial[v][block] = sum over the block's rows of q_v[i] w[i], v < nv.
+// partial[v][row block] = sum over the block's rows of q_v[i] w[i] for the
+// vectors v of the block's group (blockIdx.y), v < nv.
 __global__ void __launch_bounds__( kBlock )
     gemv_t_kernel( double const *basis, int64_t ld, int nv, int64_t n,
                    double const *w, double *partial )
 {
-    __shared__ double lds[kWaves][kMaxVectors];
-    int64_t const base = blockIdx.x * (int64_t)kTile + threadIdx.x;
-    double wr[kRows];
+    __shared__ double lds[kWaves][kGroup];
+    int const v0 = blockIdx.y * kGroup;
+    int const cnt = min( kGroup, nv - v0 );
+    int64_t const base = blockIdx.x * (int64_t)kTileT + threadIdx.x;
+    double acc[kGroup];
 #pragma unroll
-    for ( int r = 0; r < kRows; ++r )
+    for ( int g = 0; g < kGroup; ++g )
+        acc[g] = 0.;
+    // loads are unconditional: rows past n read row n - 1 with weight 0 and
+    // vectors past nv read vector nv - 1 into accumulators never stored
+    double const *q[kGroup];
+#pragma unroll
+    for ( int g = 0; g < kGroup; ++g )
+        q[g] = basis + min( v0 + g, nv - 1 ) * ld;
+#pragma unroll 2
+    for ( int r = 0; r < kRowsT; ++r )
     {
         int64_t const i = base + r * kBlock;
-        wr[r] = i < n ? w[i] : 0.;
+        int64_t const ic = min( i, n - 1 );
+        double const wi = i < n ? w[ic] : 0.;
+#pragma unroll
+        for ( int g = 0; g < kGroup; ++g )
+            acc[g] += q[g][ic] * wi;
     }
     int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for ( int v = 0; v < nv; ++v )
-    {
-        double const *q = basis + v * ld;
-        double a = 0.;
 #pragma unroll
-        for ( int r = 0; r < kRows; ++r )
+    for ( int g = 0; g < kGroup; ++g )
+    {
+        if ( g < cnt )
         {
-            int64_t const i = base + r * kBlock;
-            if ( i < n )
-                a += q[i] * wr[r];
+            double const a = wave_sum( acc[g] );
+            if ( lane == 0 )
+                lds[wave][g] = a;
         }
-        a = wave_sum( a );
-        if ( lane == 0 )
-            lds[wave][v] = a;
     }
     __syncthreads();
-    for ( int v = threadIdx.x; v < nv; v += kBlock )
+    if ( threadIdx.x < cnt )
     {
         double a = 0.;
         for ( int w2 = 0; w2 < kWaves; ++w2 )
-            a += lds[w2][v];
-        partial[(int64_t)v * gridDim.x + blockIdx.x] = a;
+            a += lds[w2][threadIdx.x];
+        partial[(int64_t)( v0 + threadIdx.x ) * gridDim.x + blockIdx.x] = a;
     }
 }
 
@@ -193,29 +223,28 @@ __global__ void __launch_bounds__( kBlock )
     for ( int v = threadIdx.x; v < nv; v += kBlock )
         hs[v] = h[v];
     __syncthreads();
-    int64_t const base = blockIdx.x * (int64_t)kTile + threadIdx.x;
-    double acc[kRows];
+    int64_t const base = blockIdx.x * (int64_t)kTileN + threadIdx.x;
+    double acc[kRowsN];
+    int64_t ii[kRowsN];
 #pragma unroll
-    for ( int r = 0; r < kRows; ++r )
+    for ( int r = 0; r < kRowsN; ++r )
     {
-        int64_t const i = base + r * kBlock;
-        acc[r] = i < n ? w[i] : 0.;
+        // rows past n read row n - 1 and are not written back
+        ii[r] = min( base + r * kBlock, n - 1 );
+        acc[r] = w[ii[r]];
     }
+#pragma unroll 8
     for ( int v = 0; v < nv; ++v )
     {
         double const *q = basis + v * ld;
         double const hv = hs[v];
 #pragma unroll
-        for ( int r = 0; r < kRows; ++r )
-        {
-            int64_t const i = base + r * kBlock;
-            if ( i < n )
-                acc[r] -= q[i] * hv;
-        }
+        for ( int r = 0; r < kRowsN; ++r )
+            acc[r] -= q[ii[r]] * hv;
     }
     double s2 = 0.;
 #pragma unroll
-    for ( int r = 0; r < kRows; ++r )
+    for ( int r = 0; r < kRowsN; ++r )
     {
         int64_t const i = base + r * kBlock;
         if ( i < n )
@@ -249,16 +278,16 @@ __global__ void __launch_bounds__( kBlock )
         out[i] = in[i] * alpha;
 }
 
-// r[i] = b[i] - r[i] and the block's sum of r[i]^2 (same tiling as gemv).
+// r[i] = b[i] - r[i] and the block's sum of r[i]^2 (gemv_n's tiling).
 __global__ void __launch_bounds__( kBlock )
     residual_kernel( double const *b, int64_t n, double *r,
                      double *norm_partial )
 {
     __shared__ double lds[kWaves];
-    int64_t const base = blockIdx.x * (int64_t)kTile + threadIdx.x;
+    int64_t const base = blockIdx.x * (int64_t)kTileN + threadIdx.x;
     double s2 = 0.;
 #pragma unroll
-    for ( int k = 0; k < kRows; ++k )
+    for ( int k = 0; k < kRowsN; ++k )
     {
         int64_t const i = base + k * kBlock;
         if ( i < n )
@@ -318,7 +347,9 @@ void system_apply( SplineSystem const &sys, double const *z, double *y,
     DTKX_HIP( hipGetLastError() );
 }
 
-int tiles( int64_t n ) { return (int)( ( n + kTile - 1 ) / kTile ); }
+int tiles_t( int64_t n ) { return (int)( ( n + kTileT - 1 ) / kTileT ); }
+int tiles_n( int64_t n ) { return (int)( ( n + kTileN - 1 ) / kTileN ); }
+int groups( int nv ) { return ( nv + kGroup - 1 ) / kGroup; }
 
 } // namespace
 
@@ -365,18 +396,18 @@ void SplineSolver::reserve( int64_t n, int restart, hipStream_t s )
     restart = std::max( restart, restart_ );
     ld_ = ld;
     restart_ = restart;
-    nblk_ = tiles( n );
-    size_t const nv = (size_t)restart + 1;
-    basis_.resize( nv * ld, s );
+    size_t const nv = (size_t)restart + 2;
+    basis_.resize( ( nv - 1 ) * ld, s );
     rhs_.resize( ld, s );
     res_.resize( ld, s );
     size_t const rows = 4 * (size_t)row_blocks( n );
-    partial_.resize( std::max( nv * nblk_, rows ), s );
-    dots_.resize( 3 * nv + 1, s );
+    partial_.resize( std::max( { nv * tiles_t( n ), (size_t)tiles_n( n ), rows } ),
+                     s );
+    dots_.resize( 3 * nv, s );
     if ( host_ )
         DTKX_HIP( hipHostFree( host_ ) );
     host_ = nullptr;
-    DTKX_HIP( hipHostMalloc( (void **)&host_, ( 3 * nv + 1 ) * sizeof( double ),
+    DTKX_HIP( hipHostMalloc( (void **)&host_, 3 * nv * sizeof( double ),
                              hipHostMallocDefault ) );
 }
 
@@ -388,20 +419,32 @@ GmresInfo SplineSolver::solve( SplineSystem const &sys, double const *x,
     int const m = opt.restart;
     reserve( n, m, s );
     int64_t const ld = ld_;
-    int const nb = nblk_;
+    int const nbt = tiles_t( n ), nbn = tiles_n( n );
     double *const Q = basis_.get();
     double *const b = rhs_.get();
     double *const r = res_.get();
     double *const part = partial_.get();
     double *const dots = dots_.get();
-    auto norm2_of = [&]( double const *v ) {
-        gemv_t_kernel<<<nb, kBlock, 0, s>>>( v, ld, 1, n, v, part );
-        reduce_kernel<<<1, kBlock, 0, s>>>( part, nb, 1, dots );
+    // dots[0..nv) = Q[0..nv)^T w
+    auto project = [&]( int nv, double const *w, double *out ) {
+        gemv_t_kernel<<<dim3( nbt, groups( nv ) ), kBlock, 0, s>>>( Q, ld, nv,
+                                                                     n, w, part );
+        reduce_kernel<<<( nv + kWaves - 1 ) / kWaves, kBlock, 0, s>>>(
+            part, nbt, nv, out );
         DTKX_HIP( hipGetLastError() );
-        DTKX_HIP( hipMemcpyAsync( host_, dots, sizeof( double ),
+    };
+    // w -= Q[0..nv) h; *norm2 = |w|^2
+    auto update = [&]( int nv, double const *h, double *w, double *norm2 ) {
+        gemv_n_kernel<<<nbn, kBlock, 0, s>>>( Q, ld, nv, n, h, w,
+                                              norm2 ? part : nullptr );
+        if ( norm2 )
+            reduce_kernel<<<1, kBlock, 0, s>>>( part, nbn, 1, norm2 );
+        DTKX_HIP( hipGetLastError() );
+    };
+    auto fetch = [&]( double const *src, int count ) {
+        DTKX_HIP( hipMemcpyAsync( host_, src, count * sizeof( double ),
                                   hipMemcpyDeviceToHost, s ) );
         DTKX_HIP( hipStreamSynchronize( s ) );
-        return host_[0];
     };
     // b = S x = [x; 0], z = 0 (def.hpp:200-203, 286-291)
     DTKX_HIP( hipMemcpyAsync( b, x, sys.n_src * sizeof( double ),
@@ -410,13 +453,21 @@ GmresInfo SplineSolver::solve( SplineSystem const &sys, double const *x,
                               s ) );
     DTKX_HIP( hipMemsetAsync( z, 0, n * sizeof( double ), s ) );
     GmresInfo info;
-    double const beta = std::sqrt( norm2_of( b ) );
+    // |b|^2: b as the only basis vector of a projection
+    gemv_t_kernel<<<dim3( nbt, 1 ), kBlock, 0, s>>>( b, ld, 1, n, b, part );
+    reduce_kernel<<<1, kBlock, 0, s>>>( part, nbt, 1, dots );
+    DTKX_HIP( hipGetLastError() );
+    fetch( dots, 1 );
+    double const beta = std::sqrt( host_[0] );
     if ( !( beta > 0. ) )
     {
         info.converged = std::isfinite( beta );
         return info;
     }
     double const target = opt.tolerance * beta;
+    // second Gram-Schmidt pass when a pass leaves less than this fraction of
+    // the norm (the DGKS criterion)
+    double const eta = 0.70710678118654752;
     std::vector<double> H( (size_t)( m + 1 ) * m ), cs( m ), sn( m ),
         g( m + 1 ), yv( m );
     auto Hat = [&]( int i, int j ) -> double & {
@@ -439,27 +490,25 @@ GmresInfo SplineSolver::solve( SplineSystem const &sys, double const *x,
             double *const w = Q + ( j + 1 ) * ld;
             int const nv = j + 1;
             system_apply( sys, Q + j * ld, w, part, s );
-            // classical Gram-Schmidt, twice; the second update also
-            // forms |w|^2
-            gemv_t_kernel<<<nb, kBlock, 0, s>>>( Q, ld, nv, n, w, part );
-            reduce_kernel<<<( nv + kWaves - 1 ) / kWaves, kBlock, 0, s>>>(
-                part, nb, nv, dots );
-            gemv_n_kernel<<<nb, kBlock, 0, s>>>( Q, ld, nv, n, dots, w,
-                                                 nullptr );
-            gemv_t_kernel<<<nb, kBlock, 0, s>>>( Q, ld, nv, n, w, part );
-            reduce_kernel<<<( nv + kWaves - 1 ) / kWaves, kBlock, 0, s>>>(
-                part, nb, nv, dots + nv );
-            gemv_n_kernel<<<nb, kBlock, 0, s>>>( Q, ld, nv, n, dots + nv, w,
-                                                 part );
-            reduce_kernel<<<1, kBlock, 0, s>>>( part, nb, 1, dots + 2 * nv );
-            DTKX_HIP( hipGetLastError() );
-            DTKX_HIP( hipMemcpyAsync( host_, dots,
-                                      ( 2 * nv + 1 ) * sizeof( double ),
-                                      hipMemcpyDeviceToHost, s ) );
-            DTKX_HIP( hipStreamSynchronize( s ) );
+            // classical Gram-Schmidt; w is vector nv of the basis array, so
+            // the projection over nv + 1 vectors also gives |w|^2
+            project( nv + 1, w, dots );
+            update( nv, dots, w, dots + nv + 1 );
+            fetch( dots, nv + 2 );
             for ( int i = 0; i < nv; ++i )
-                Hat( i, j ) = host_[i] + host_[nv + i];
-            double const hn = std::sqrt( host_[2 * nv] );
+                Hat( i, j ) = host_[i];
+            double n0 = host_[nv], n1 = host_[nv + 1];
+            if ( n1 < eta * eta * n0 )
+            {
+                project( nv, w, dots );
+                update( nv, dots, w, dots + nv );
+                fetch( dots, nv + 1 );
+                for ( int i = 0; i < nv; ++i )
+                    Hat( i, j ) += host_[i];
+                n1 = host_[nv];
+                ++info.reorthogonalized;
+            }
+            double const hn = std::sqrt( n1 );
             Hat( j + 1, j ) = hn;
             // previous rotations, then the one that zeroes H(j+1, j)
             for ( int i = 0; i < j; ++i )
@@ -499,18 +548,14 @@ GmresInfo SplineSolver::solve( SplineSystem const &sys, double const *x,
                 host_[i] = -yv[i];
             DTKX_HIP( hipMemcpyAsync( dots, host_, j * sizeof( double ),
                                       hipMemcpyHostToDevice, s ) );
-            gemv_n_kernel<<<nb, kBlock, 0, s>>>( Q, ld, j, n, dots, z,
-                                                 nullptr );
-            DTKX_HIP( hipGetLastError() );
+            update( j, dots, z, nullptr );
         }
         // explicit residual r = b - C z
         system_apply( sys, z, r, part, s );
-        residual_kernel<<<nb, kBlock, 0, s>>>( b, n, r, part );
-        reduce_kernel<<<1, kBlock, 0, s>>>( part, nb, 1, dots );
+        residual_kernel<<<nbn, kBlock, 0, s>>>( b, n, r, part );
+        reduce_kernel<<<1, kBlock, 0, s>>>( part, nbn, 1, dots );
         DTKX_HIP( hipGetLastError() );
-        DTKX_HIP( hipMemcpyAsync( host_, dots, sizeof( double ),
-                                  hipMemcpyDeviceToHost, s ) );
-        DTKX_HIP( hipStreamSynchronize( s ) );
+        fetch( dots, 1 );
         rnorm = std::sqrt( host_[0] );
         info.iterations = total;
         info.residual = rnorm / beta;
@@ -519,6 +564,7 @@ GmresInfo SplineSolver::solve( SplineSystem const &sys, double const *x,
              !std::isfinite( rnorm ) )
             return info;
         rvec = r;
+        ++info.restarts;
     }
 }
 

@@ -64,6 +64,8 @@ struct GmresOptions
 struct GmresInfo
 {
     int iterations = 0;
+    int reorthogonalized = 0; // iterations with a second Gram-Schmidt pass
+    int restarts = 0;
     double residual = 0.0; // explicit relative residual of the result
     bool converged = false;
 };
@@ -84,10 +86,9 @@ class SplineSolver
   private:
     void reserve( int64_t n, int restart, hipStream_t s );
     int64_t ld_ = 0;
-    int nblk_ = 0;
     int restart_ = 0;
     DeviceBuffer<double> basis_, rhs_, res_, partial_, dots_;
-    double *host_ = nullptr; // pinned: h1 | h2 | norm^2
+    double *host_ = nullptr; // pinned: projections and norms
 };
 
 } // namespace dtkx

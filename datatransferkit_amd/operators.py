@@ -497,7 +497,8 @@ class SplineOperator:
         s = _lib.SplineInfo()
         check(_lib.load().dtkx_spline_get_info(self._h, ctypes.byref(s)))
         return {"iterations": s.iterations, "residual": s.residual,
-                "converged": bool(s.converged)}
+                "converged": bool(s.converged), "reorthogonalized": s.reorthogonalized,
+                "restarts": s.restarts}
 
 
 def knn(source_points, target_points, k, geometry="float", layout="aos",

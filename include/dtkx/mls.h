@@ -154,8 +154,8 @@ typedef struct dtkx_mls_stats {
     double ms_knn;              /* stage times of the last create, measured */
     double ms_setup;            /* with HIP events on the handle's stream   */
     int32_t group_size_used;
-    int32_t n_stack_overflow;   /* kNN queries deeper than the 32-entry LDS
-                                   traversal stack, re-run with a global one */
+    int32_t n_stack_overflow;   /* kNN queries deeper than the LDS traversal
+                                   stack, re-run with a global one */
     double ms_knn_query;        /* the kNN query kernels alone */
     int64_t n_eigen_fallback;   /* Cholesky solver: systems not certified,
                                    set up by the cyclic eigen-solver */
@@ -363,6 +363,8 @@ typedef struct dtkx_spline_options {
 typedef struct dtkx_spline_info {
   int32_t iterations;
   int32_t converged;
+  int32_t reorthogonalized; /* iterations that took a second Gram-Schmidt pass */
+  int32_t restarts;         /* GMRES cycles after the first */
   double residual;
 } dtkx_spline_info;
 

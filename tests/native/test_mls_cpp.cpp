@@ -7,11 +7,15 @@
 //       Linear3, Quadratic3} (:464-503)
 //   packages/Meshfree/test/tstMeshfreeOperatorsSimpleProblem.cpp:113-147
 //       corner_cases (1e-12)
+//   the same four tests for SplineOperator<Wendland<0>, Linear3>
+//       (tstMeshfreeOperators.cpp:504-516) with the Spline tolerances:
+//       same_npoints_and_basis 2e-7, grid 1e-9, line 2e-9, finite
 // The random point sets are the fixtures regenerated bit-for-bit from the
 // reference's std::default_random_engine (tests/golden/). Host pointers are
 // used, as the reference's Serial node would.
 // Build: g++ -std=c++17 -I include test_mls_cpp.cpp -L<dir> -ldtkx
 #include "dtkx/MovingLeastSquaresOperator.hpp"
+#include "dtkx/SplineOperator.hpp"
 
 #include <array>
 #include <cmath>
@@ -100,10 +104,16 @@ static std::vector<double> transfer( std::vector<Point> const &src,
     return out;
 }
 
-template <typename Basis>
-static void run_basis( std::string const &golden )
+// tolerances of tstMeshfreeOperators.cpp:214-217, 290-293, 366-369
+struct Tolerances
 {
-    using Op = dtkx::MovingLeastSquaresOperator<dtkx::Wendland<0>, Basis>;
+    double same_npoints, grid, line;
+};
+
+template <typename Op>
+static void run_operator( std::string const &golden, Tolerances tol,
+                          std::string const &name )
+{
     int const m = Op::polynomial_basis::size;
     auto const f = poly( m );
     auto eval = [&]( std::vector<Point> const &p ) {
@@ -112,24 +122,25 @@ static void run_basis( std::string const &golden )
             v.push_back( f( x ) );
         return v;
     };
-    std::string const tag = " m=" + std::to_string( m );
+    std::string const tag = " " + name + " m=" + std::to_string( m );
     {
         auto src = read_points( golden + "/mls_points_m" + std::to_string( m ) + "_src.f64" );
         auto tgt = read_points( golden + "/mls_points_m" + std::to_string( m ) + "_tgt.f64" );
-        check_close( transfer<Op>( src, tgt, eval( src ) ), eval( tgt ), 1e-7,
+        check_close( transfer<Op>( src, tgt, eval( src ) ), eval( tgt ),
+                     tol.same_npoints,
                      ( "same_npoints_and_basis" + tag ).c_str() );
     }
     {
         auto src = grid( {40, 40, 1}, {0., 0., 0.} );
         auto tgt = grid( {1, 1, 1}, {19., 19., 0.} );
-        check_close( transfer<Op>( src, tgt, eval( src ) ), eval( tgt ), 1e-14,
-                     ( "grid" + tag ).c_str() );
+        check_close( transfer<Op>( src, tgt, eval( src ) ), eval( tgt ),
+                     tol.grid, ( "grid" + tag ).c_str() );
     }
     {
         auto src = grid( {40, 1, 1}, {0., 0., 0.} );
         auto tgt = grid( {39, 1, 1}, {0.5, 0., 0.} );
-        check_close( transfer<Op>( src, tgt, eval( src ) ), eval( tgt ), 1e-14,
-                     ( "line" + tag ).c_str() );
+        check_close( transfer<Op>( src, tgt, eval( src ) ), eval( tgt ),
+                     tol.line, ( "line" + tag ).c_str() );
     }
     {
         auto src = read_points( golden + "/mls_points_m1_src.f64" );
@@ -148,9 +159,16 @@ static void run_basis( std::string const &golden )
 int main( int argc, char **argv )
 {
     std::string const golden = argc > 1 ? argv[1] : "tests/golden";
-    run_basis<dtkx::MultivariatePolynomialBasis<dtkx::Constant, 3>>( golden );
-    run_basis<dtkx::MultivariatePolynomialBasis<dtkx::Linear, 3>>( golden );
-    run_basis<dtkx::MultivariatePolynomialBasis<dtkx::Quadratic, 3>>( golden );
+    Tolerances const mls{1e-7, 1e-14, 1e-14}, spline{2e-7, 1e-9, 2e-9};
+    using W0 = dtkx::Wendland<0>;
+    using dtkx::MultivariatePolynomialBasis;
+    run_operator<dtkx::MovingLeastSquaresOperator<
+        W0, MultivariatePolynomialBasis<dtkx::Constant, 3>>>( golden, mls, "MLS" );
+    run_operator<dtkx::MovingLeastSquaresOperator<
+        W0, MultivariatePolynomialBasis<dtkx::Linear, 3>>>( golden, mls, "MLS" );
+    run_operator<dtkx::MovingLeastSquaresOperator<
+        W0, MultivariatePolynomialBasis<dtkx::Quadratic, 3>>>( golden, mls, "MLS" );
+    run_operator<dtkx::SplineOperator<>>( golden, spline, "Spline" );
 
     using Op = dtkx::MovingLeastSquaresOperator<>; // Wendland<0>, Linear3
     check_close( transfer<Op>( {{1., 1., 1.}, {2., 2., 2.}}, {{1.5, 1.5, 1.5}},
@@ -170,6 +188,18 @@ int main( int argc, char **argv )
         threw = std::string( e.what() ).find( "failed in" ) != std::string::npos;
     }
     std::printf( "%s empty source throws\n", threw ? "PASS" : "FAIL" );
+    if ( !threw )
+        ++g_failures;
+    threw = false;
+    try
+    {
+        dtkx::SplineOperator<> sp( nullptr, 0, nullptr, 0 );
+    }
+    catch ( std::logic_error const &e )
+    {
+        threw = std::string( e.what() ).find( "failed in" ) != std::string::npos;
+    }
+    std::printf( "%s spline empty source throws\n", threw ? "PASS" : "FAIL" );
     if ( !threw )
         ++g_failures;
 

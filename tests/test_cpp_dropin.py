@@ -1,7 +1,8 @@
 """The C++ drop-ins (include/dtkx/MovingLeastSquaresOperator.hpp,
-NearestNeighborOperator.hpp) compile with a plain C++17 compiler against the
-C ABI, and on the GPU pass the reference's own operator tests restated in
-tests/native/test_mls_cpp.cpp and test_nn_cpp.cpp."""
+NearestNeighborOperator.hpp, SplineOperator.hpp) compile with a plain C++17
+compiler against the C ABI, and on the GPU pass the reference's own operator
+tests restated in tests/native/test_mls_cpp.cpp (MLS and Spline) and
+test_nn_cpp.cpp."""
 import os
 import subprocess
 
@@ -46,4 +47,4 @@ def test_dropin_reference_operator_tests(tmp_path):
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "FAIL" not in r.stdout
-    assert r.stdout.count("PASS") == 15
+    assert r.stdout.count("PASS") == 20
