@@ -568,22 +568,18 @@ def bench_nn(args):
 
 def spline_solve_bytes(n_src, info, restart=300):
     """Algorithmic HBM bytes of one GMRES solve, 8 bytes per vector entry,
-    n = n_src + 4 entries. Per iteration with nv basis vectors: the
-    operator C (index 16, value 32, point 16, own entry 8 and result
-    8 bytes per source row); the first Gram-Schmidt pass, whose dots read
-    the nv vectors, w as vector nv (for |w|^2) and w, and whose update reads
-    the nv vectors and w and writes w; the second pass when taken (dots:
-    nv + 1 reads, update: nv + 2); the normalisation (read + write w). The
-    second passes are spread evenly over the iterations."""
+    n = n_src + 4 entries. Per iteration with nv basis vectors (the newest
+    one not yet reorthogonalised): the operator C (index 16, value 32,
+    point 16, own entry 8 and result 8 bytes per source row); the fused
+    dots of the nv vectors against it and its image (nv + 1 vectors read);
+    the fused update, which reads nv - 1 vectors, the newest one and the
+    image and writes two (nv + 3)."""
     n = n_src + 4
-    its = info["iterations"]
-    frac2 = info["reorthogonalized"] / max(its, 1)
-    total = 0.0
-    for it in range(its):
+    total = 0
+    for it in range(info["iterations"]):
         nv = it % restart + 1
-        total += 80 * n_src + 8 * n * ((nv + 2) + (nv + 2)) + 16 * n
-        total += frac2 * 8 * n * ((nv + 1) + (nv + 2))
-    return int(total)
+        total += 80 * n_src + 8 * n * ((nv + 1) + (nv + 3))
+    return total
 
 
 def bench_spline(args):
@@ -647,8 +643,7 @@ def bench_spline(args):
         "apply_roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
                            "algorithmic": f"{nbytes} bytes per solve "
-                                          f"({info['iterations']} iterations, "
-                                          f"{info['reorthogonalized']} second passes)"},
+                                          f"({info['iterations']} iterations)"},
     }
     if not args.no_cpu_baseline:
         import oracle

@@ -25,12 +25,24 @@ constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 // Krylov kernels. Dots (gemv_t): each block takes kRowsT rows per thread
 // and a group of kGroup basis vectors, so a thread has kGroup
-// independent loads per row and one wave reduction per vector for kRowsT
-// rows. Updates (gemv_n, residual): kRowsN rows per thread, many waves.
-constexpr int kRowsT = 8;
+// independent loads per row and two wave reductions per vector for kRowsT
+// rows. Updates (gemv_n, dcgs2_update, residual): kRowsN rows per thread,
+// many waves. Solve at 1M sources (91 iterations), rows x group:
+// 8 x 16 21.6 ms, 4 x 16 22.6, 8 x 8 20.0, 4 x 8 20.5, 16 x 8 19.9,
+// 32 x 8 20.4, 8 x 4 19.8, 16 x 4 19.8; kRowsN 1, 2, 4: 21.8, 21.6, 21.9.
+#ifndef DTKX_SPLINE_ROWS_T
+#define DTKX_SPLINE_ROWS_T 16
+#endif
+#ifndef DTKX_SPLINE_GROUP
+#define DTKX_SPLINE_GROUP 8
+#endif
+#ifndef DTKX_SPLINE_ROWS_N
+#define DTKX_SPLINE_ROWS_N 2
+#endif
+constexpr int kRowsT = DTKX_SPLINE_ROWS_T;
 constexpr int kTileT = kBlock * kRowsT;
-constexpr int kGroup = 16;
-constexpr int kRowsN = 2;
+constexpr int kGroup = DTKX_SPLINE_GROUP;
+constexpr int kRowsN = DTKX_SPLINE_ROWS_N;
 constexpr int kTileN = kBlock * kRowsN;
 // largest number of basis vectors one orthogonalisation pass handles
 constexpr int kMaxVectors = 512;
@@ -161,20 +173,24 @@ __global__ void __launch_bounds__( kBlock )
         out[v] = a;
 }
 
-// partial[v][row block] = sum over the block's rows of q_v[i] w[i] for the
-// vectors v of the block's group (blockIdx.y), v < nv.
+// partial[2 v + k][row block] = sum over the block's rows of q_v[i] r_k[i]
+// for the two right-hand sides r_0, r_1 and the vectors v of the block's
+// group (blockIdx.y), v < nv.
 __global__ void __launch_bounds__( kBlock )
     gemv_t_kernel( double const *basis, int64_t ld, int nv, int64_t n,
-                   double const *w, double *partial )
+                   double const *r0, double const *r1, double *partial )
 {
-    __shared__ double lds[kWaves][kGroup];
+    __shared__ double lds[kWaves][2 * kGroup];
     int const v0 = blockIdx.y * kGroup;
     int const cnt = min( kGroup, nv - v0 );
     int64_t const base = blockIdx.x * (int64_t)kTileT + threadIdx.x;
-    double acc[kGroup];
+    double a0[kGroup], a1[kGroup];
 #pragma unroll
     for ( int g = 0; g < kGroup; ++g )
-        acc[g] = 0.;
+    {
+        a0[g] = 0.;
+        a1[g] = 0.;
+    }
     // loads are unconditional: rows past n read row n - 1 with weight 0 and
     // vectors past nv read vector nv - 1 into accumulators never stored
     double const *q[kGroup];
@@ -186,10 +202,15 @@ __global__ void __launch_bounds__( kBlock )
     {
         int64_t const i = base + r * kBlock;
         int64_t const ic = min( i, n - 1 );
-        double const wi = i < n ? w[ic] : 0.;
+        double const x0 = i < n ? r0[ic] : 0.;
+        double const x1 = i < n ? r1[ic] : 0.;
 #pragma unroll
         for ( int g = 0; g < kGroup; ++g )
-            acc[g] += q[g][ic] * wi;
+        {
+            double const qv = q[g][ic];
+            a0[g] += qv * x0;
+            a1[g] += qv * x1;
+        }
     }
     int const lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -197,18 +218,76 @@ __global__ void __launch_bounds__( kBlock )
     {
         if ( g < cnt )
         {
-            double const a = wave_sum( acc[g] );
+            double const b0 = wave_sum( a0[g] );
+            double const b1 = wave_sum( a1[g] );
             if ( lane == 0 )
-                lds[wave][g] = a;
+            {
+                lds[wave][2 * g] = b0;
+                lds[wave][2 * g + 1] = b1;
+            }
         }
     }
     __syncthreads();
-    if ( threadIdx.x < cnt )
+    if ( threadIdx.x < 2 * cnt )
     {
         double a = 0.;
         for ( int w2 = 0; w2 < kWaves; ++w2 )
             a += lds[w2][threadIdx.x];
-        partial[(int64_t)( v0 + threadIdx.x ) * gridDim.x + blockIdx.x] = a;
+        partial[(int64_t)( 2 * v0 + threadIdx.x ) * gridDim.x + blockIdx.x] = a;
+    }
+}
+
+// One step of the delayed reorthogonalisation, rows of kRowsN per thread:
+//   q_j = (u - sum_{v<j} q_v s_v) / rho      (u is basis vector j)
+//   v   = (w - sum_{v<j} q_v t_v - q_j gamma) / rho   (w is vector j + 1)
+// with s_v = st[2 v], t_v = st[2 v + 1]; q_j and v overwrite u and w.
+__global__ void __launch_bounds__( kBlock )
+    dcgs2_update_kernel( double *basis, int64_t ld, int j, int64_t n,
+                         double const *st, double inv_rho, double gamma )
+{
+    __shared__ double ss[kMaxVectors], ts[kMaxVectors];
+    for ( int v = threadIdx.x; v < j; v += kBlock )
+    {
+        ss[v] = st[2 * v];
+        ts[v] = st[2 * v + 1];
+    }
+    __syncthreads();
+    double *const u = basis + j * ld;
+    double *const w = u + ld;
+    int64_t const base = blockIdx.x * (int64_t)kTileN + threadIdx.x;
+    int64_t ii[kRowsN];
+    double au[kRowsN], aw[kRowsN];
+#pragma unroll
+    for ( int r = 0; r < kRowsN; ++r )
+    {
+        // rows past n read row n - 1 and are not written back
+        ii[r] = min( base + r * kBlock, n - 1 );
+        au[r] = u[ii[r]];
+        aw[r] = w[ii[r]];
+    }
+#pragma unroll 8
+    for ( int v = 0; v < j; ++v )
+    {
+        double const *q = basis + v * ld;
+        double const sv = ss[v], tv = ts[v];
+#pragma unroll
+        for ( int r = 0; r < kRowsN; ++r )
+        {
+            double const qv = q[ii[r]];
+            au[r] -= qv * sv;
+            aw[r] -= qv * tv;
+        }
+    }
+#pragma unroll
+    for ( int r = 0; r < kRowsN; ++r )
+    {
+        int64_t const i = base + r * kBlock;
+        if ( i < n )
+        {
+            double const qj = au[r] * inv_rho;
+            u[i] = qj;
+            w[i] = ( aw[r] - qj * gamma ) * inv_rho;
+        }
     }
 }
 
@@ -401,8 +480,9 @@ void SplineSolver::reserve( int64_t n, int restart, hipStream_t s )
     rhs_.resize( ld, s );
     res_.resize( ld, s );
     size_t const rows = 4 * (size_t)row_blocks( n );
-    partial_.resize( std::max( { nv * tiles_t( n ), (size_t)tiles_n( n ), rows } ),
-                     s );
+    // two right-hand sides per basis vector in the projections
+    partial_.resize(
+        std::max( { 2 * nv * tiles_t( n ), (size_t)tiles_n( n ), rows } ), s );
     dots_.resize( 3 * nv, s );
     if ( host_ )
         DTKX_HIP( hipHostFree( host_ ) );
@@ -425,26 +505,18 @@ GmresInfo SplineSolver::solve( SplineSystem const &sys, double const *x,
     double *const r = res_.get();
     double *const part = partial_.get();
     double *const dots = dots_.get();
-    // dots[0..nv) = Q[0..nv)^T w
-    auto project = [&]( int nv, double const *w, double *out ) {
-        gemv_t_kernel<<<dim3( nbt, groups( nv ) ), kBlock, 0, s>>>( Q, ld, nv,
-                                                                     n, w, part );
-        reduce_kernel<<<( nv + kWaves - 1 ) / kWaves, kBlock, 0, s>>>(
-            part, nbt, nv, out );
-        DTKX_HIP( hipGetLastError() );
-    };
-    // w -= Q[0..nv) h; *norm2 = |w|^2
-    auto update = [&]( int nv, double const *h, double *w, double *norm2 ) {
-        gemv_n_kernel<<<nbn, kBlock, 0, s>>>( Q, ld, nv, n, h, w,
-                                              norm2 ? part : nullptr );
-        if ( norm2 )
-            reduce_kernel<<<1, kBlock, 0, s>>>( part, nbn, 1, norm2 );
-        DTKX_HIP( hipGetLastError() );
-    };
     auto fetch = [&]( double const *src, int count ) {
         DTKX_HIP( hipMemcpyAsync( host_, src, count * sizeof( double ),
                                   hipMemcpyDeviceToHost, s ) );
         DTKX_HIP( hipStreamSynchronize( s ) );
+    };
+    // dots[2 v + k] = Q[v]^T r_k, v < nv
+    auto project = [&]( int nv, double const *r0, double const *r1 ) {
+        gemv_t_kernel<<<dim3( nbt, groups( nv ) ), kBlock, 0, s>>>(
+            Q, ld, nv, n, r0, r1, part );
+        reduce_kernel<<<( 2 * nv + kWaves - 1 ) / kWaves, kBlock, 0, s>>>(
+            part, nbt, 2 * nv, dots );
+        DTKX_HIP( hipGetLastError() );
     };
     // b = S x = [x; 0], z = 0 (def.hpp:200-203, 286-291)
     DTKX_HIP( hipMemcpyAsync( b, x, sys.n_src * sizeof( double ),
@@ -454,7 +526,7 @@ GmresInfo SplineSolver::solve( SplineSystem const &sys, double const *x,
     DTKX_HIP( hipMemsetAsync( z, 0, n * sizeof( double ), s ) );
     GmresInfo info;
     // |b|^2: b as the only basis vector of a projection
-    gemv_t_kernel<<<dim3( nbt, 1 ), kBlock, 0, s>>>( b, ld, 1, n, b, part );
+    gemv_t_kernel<<<dim3( nbt, 1 ), kBlock, 0, s>>>( b, ld, 1, n, b, b, part );
     reduce_kernel<<<1, kBlock, 0, s>>>( part, nbt, 1, dots );
     DTKX_HIP( hipGetLastError() );
     fetch( dots, 1 );
@@ -465,90 +537,125 @@ GmresInfo SplineSolver::solve( SplineSystem const &sys, double const *x,
         return info;
     }
     double const target = opt.tolerance * beta;
-    // second Gram-Schmidt pass when a pass leaves less than this fraction of
-    // the norm (the DGKS criterion)
-    double const eta = 0.70710678118654752;
-    std::vector<double> H( (size_t)( m + 1 ) * m ), cs( m ), sn( m ),
-        g( m + 1 ), yv( m );
-    auto Hat = [&]( int i, int j ) -> double & {
-        return H[(size_t)j * ( m + 1 ) + i];
+    // Arnoldi with classical Gram-Schmidt and delayed reorthogonalisation
+    // (DCGS2): iteration j applies C to the not yet reorthogonalised,
+    // unnormalised u = Q[j]; one pass of dots gives s = Q_{j-1}^T u,
+    // |u|^2, t = Q_{j-1}^T w and u^T w for w = C u; one update pass then
+    // reorthogonalises u into q_j = (u - Q_{j-1} s) / rho,
+    // rho^2 = |u|^2 - |s|^2, and forms the next u = (I - Q_j Q_j^T) w / rho.
+    // Column j - 1 of the Hessenberg matrix is final once rho is known:
+    // H(0:j, j-1) += s, H(j, j-1) = rho. Column j is
+    // ([t; gamma] - H(0:j, 0:j-1) s) / rho with H(j+1, j) = 1 until the
+    // next iteration, gamma = (u^T w - s^T t) / rho. The basis is read
+    // twice per iteration.
+    std::vector<double> Ha( (size_t)( m + 1 ) * m ), R( (size_t)( m + 1 ) * m ),
+        cs( m ), sn( m ), g( m + 1 ), yv( m ), hs( m + 1 );
+    auto at = [&]( std::vector<double> &M, int i, int j ) -> double & {
+        return M[(size_t)j * ( m + 1 ) + i];
     };
     double const *rvec = b;
     double rnorm = beta;
     int total = 0;
     int const gblocks = (int)( ( n + kBlock - 1 ) / kBlock );
+    int const ublocks = tiles_n( n );
     while ( true )
     {
-        // q_0 = r / |r|
+        // u = r / |r|
         scale_kernel<<<gblocks, kBlock, 0, s>>>( rvec, n, 1. / rnorm, Q );
         DTKX_HIP( hipGetLastError() );
         std::fill( g.begin(), g.end(), 0. );
-        g[0] = rnorm;
-        int j = 0;
-        while ( j < m && total < opt.max_iterations )
+        int j = 0; // u = Q[j]
+        int k = 0; // final columns
+        while ( true )
         {
-            double *const w = Q + ( j + 1 ) * ld;
-            int const nv = j + 1;
-            system_apply( sys, Q + j * ld, w, part, s );
-            // classical Gram-Schmidt; w is vector nv of the basis array, so
-            // the projection over nv + 1 vectors also gives |w|^2
-            project( nv + 1, w, dots );
-            update( nv, dots, w, dots + nv + 1 );
-            fetch( dots, nv + 2 );
-            for ( int i = 0; i < nv; ++i )
-                Hat( i, j ) = host_[i];
-            double n0 = host_[nv], n1 = host_[nv + 1];
-            if ( n1 < eta * eta * n0 )
-            {
-                project( nv, w, dots );
-                update( nv, dots, w, dots + nv );
-                fetch( dots, nv + 1 );
-                for ( int i = 0; i < nv; ++i )
-                    Hat( i, j ) += host_[i];
-                n1 = host_[nv];
-                ++info.reorthogonalized;
-            }
-            double const hn = std::sqrt( n1 );
-            Hat( j + 1, j ) = hn;
-            // previous rotations, then the one that zeroes H(j+1, j)
-            for ( int i = 0; i < j; ++i )
-            {
-                double const t = cs[i] * Hat( i, j ) + sn[i] * Hat( i + 1, j );
-                Hat( i + 1, j ) = -sn[i] * Hat( i, j ) + cs[i] * Hat( i + 1, j );
-                Hat( i, j ) = t;
-            }
-            double const a = Hat( j, j ), c = Hat( j + 1, j );
-            double const rr = std::hypot( a, c );
-            cs[j] = rr > 0. ? a / rr : 1.;
-            sn[j] = rr > 0. ? c / rr : 0.;
-            Hat( j, j ) = rr;
-            Hat( j + 1, j ) = 0.;
-            g[j + 1] = -sn[j] * g[j];
-            g[j] = cs[j] * g[j];
-            ++total;
-            ++j;
-            if ( !std::isfinite( hn ) )
+            bool const last = j == m || total >= opt.max_iterations;
+            if ( last && j == 0 )
                 break;
-            if ( std::fabs( g[j] ) <= target || !( hn > 0. ) )
-                break; // converged, or the Krylov space is invariant
-            scale_kernel<<<gblocks, kBlock, 0, s>>>( w, n, 1. / hn, w );
+            double *const u = Q + j * ld;
+            double *const w = u + ld;
+            if ( !last )
+                system_apply( sys, u, w, part, s );
+            project( j + 1, u, last ? u : w );
+            fetch( dots, 2 * ( j + 1 ) );
+            double ss = 0., st = 0.;
+            for ( int v = 0; v < j; ++v )
+            {
+                ss += host_[2 * v] * host_[2 * v];
+                st += host_[2 * v] * host_[2 * v + 1];
+            }
+            double const sigma = host_[2 * j], tau = host_[2 * j + 1];
+            double const rho2 = sigma - ss;
+            double const rho = rho2 > 0. ? std::sqrt( rho2 ) : 0.;
+            if ( j == 0 )
+                g[0] = rnorm * rho; // r = rnorm u = rnorm rho q_0
+            else
+            {
+                // column j - 1 is final
+                int const c = j - 1;
+                for ( int v = 0; v < j; ++v )
+                    at( Ha, v, c ) += host_[2 * v];
+                at( Ha, j, c ) = rho;
+                for ( int v = 0; v <= j; ++v )
+                    at( R, v, c ) = at( Ha, v, c );
+                for ( int i = 0; i < c; ++i )
+                {
+                    double const t0 = cs[i] * at( R, i, c ) + sn[i] * at( R, i + 1, c );
+                    at( R, i + 1, c ) =
+                        -sn[i] * at( R, i, c ) + cs[i] * at( R, i + 1, c );
+                    at( R, i, c ) = t0;
+                }
+                double const a = at( R, c, c ), bb = at( R, c + 1, c );
+                double const rr = std::hypot( a, bb );
+                cs[c] = rr > 0. ? a / rr : 1.;
+                sn[c] = rr > 0. ? bb / rr : 0.;
+                at( R, c, c ) = rr;
+                at( R, c + 1, c ) = 0.;
+                g[c + 1] = -sn[c] * g[c];
+                g[c] = cs[c] * g[c];
+                ++total;
+                ++info.reorthogonalized;
+                k = j;
+                if ( !std::isfinite( rho ) || std::fabs( g[j] ) <= target ||
+                     !( rho > 0. ) || last )
+                    break; // converged, invariant Krylov space, or done
+            }
+            if ( !( rho > 0. ) || !std::isfinite( rho ) )
+                break;
+            // provisional column j: ([t; gamma] - H(0:j, 0:j-1) s) / rho
+            double const gamma = ( tau - st ) / rho;
+            for ( int v = 0; v <= j; ++v )
+            {
+                double a = 0.;
+                for ( int c = 0; c < j && c + 1 >= v; ++c )
+                    a += at( Ha, v, c ) * host_[2 * c];
+                hs[v] = a;
+            }
+            for ( int v = 0; v < j; ++v )
+                at( Ha, v, j ) = ( host_[2 * v + 1] - hs[v] ) / rho;
+            at( Ha, j, j ) = ( gamma - hs[j] ) / rho;
+            at( Ha, j + 1, j ) = 1.;
+            dcgs2_update_kernel<<<ublocks, kBlock, 0, s>>>( Q, ld, j, n, dots,
+                                                            1. / rho, gamma );
             DTKX_HIP( hipGetLastError() );
+            ++j;
         }
-        // z += Q_j y, H(0:j, 0:j) y = g(0:j)
-        for ( int i = j - 1; i >= 0; --i )
+        // z += Q_k y, R(0:k, 0:k) y = g(0:k)
+        for ( int i = k - 1; i >= 0; --i )
         {
             double t = g[i];
-            for ( int l = i + 1; l < j; ++l )
-                t -= Hat( i, l ) * yv[l];
-            yv[i] = Hat( i, i ) != 0. ? t / Hat( i, i ) : 0.;
+            for ( int l = i + 1; l < k; ++l )
+                t -= at( R, i, l ) * yv[l];
+            yv[i] = at( R, i, i ) != 0. ? t / at( R, i, i ) : 0.;
         }
-        if ( j > 0 )
+        if ( k > 0 )
         {
-            for ( int i = 0; i < j; ++i )
+            for ( int i = 0; i < k; ++i )
                 host_[i] = -yv[i];
-            DTKX_HIP( hipMemcpyAsync( dots, host_, j * sizeof( double ),
+            DTKX_HIP( hipMemcpyAsync( dots, host_, k * sizeof( double ),
                                       hipMemcpyHostToDevice, s ) );
-            update( j, dots, z, nullptr );
+            gemv_n_kernel<<<nbn, kBlock, 0, s>>>( Q, ld, k, n, dots, z,
+                                                  nullptr );
+            DTKX_HIP( hipGetLastError() );
         }
         // explicit residual r = b - C z
         system_apply( sys, z, r, part, s );
@@ -560,7 +667,7 @@ GmresInfo SplineSolver::solve( SplineSystem const &sys, double const *x,
         info.iterations = total;
         info.residual = rnorm / beta;
         info.converged = rnorm <= target;
-        if ( info.converged || total >= opt.max_iterations || j == 0 ||
+        if ( info.converged || total >= opt.max_iterations || k == 0 ||
              !std::isfinite( rnorm ) )
             return info;
         rvec = r;
