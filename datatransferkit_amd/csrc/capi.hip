@@ -931,12 +931,18 @@ struct dtkx_spline
     int64_t n_src = 0, n_tgt = 0;
     int kk = 0;
     hipStream_t stream = nullptr;
-    // basis operators M (sources' rows) and N (targets' rows)
+    // basis operators M (sources' rows) and N (targets' rows) in source
+    // order, as dtkx_spline_get_basis returns them
     dtkx::DeviceBuffer<int32_t> m_idx, n_idx;
     dtkx::DeviceBuffer<double> m_phi, n_phi;
-    // Vandermonde rows: the points as the search holds them
+    // Vandermonde rows: the points as the search holds them (the sources'
+    // in Morton position order)
     dtkx::DeviceBuffer<float4> src_f, tgt_f;
     dtkx::DeviceBuffer<double4> src_d, tgt_d;
+    // the system the solver runs on: sources renumbered by Morton position
+    dtkx::DeviceBuffer<int32_t> perm; // position -> source
+    dtkx::DeviceBuffer<int32_t> m_idx_p, n_idx_p;
+    dtkx::DeviceBuffer<double> m_phi_p;
     mutable dtkx::SplineSolver solver;
     mutable dtkx_spline_info info{};
     dtkx::SplineSystem system() const
@@ -949,12 +955,13 @@ struct dtkx_spline
         bool const f = opts.geometry == DTKX_GEOMETRY_FLOAT;
         y.src_pts = f ? (void const *)src_f.get() : (void const *)src_d.get();
         y.tgt_pts = f ? (void const *)tgt_f.get() : (void const *)tgt_d.get();
-        y.m_idx = m_idx.get();
-        y.m_phi = m_phi.get();
-        y.n_idx = n_idx.get();
+        y.m_idx = m_idx_p.get();
+        y.m_phi = m_phi_p.get();
+        y.n_idx = n_idx_p.get();
         y.n_phi = n_phi.get();
         return y;
     }
+    std::size_t n_ext() const { return n_src + dtkx::spline_k; }
     dtkx::GmresOptions gmres() const
     {
         dtkx::GmresOptions g;
@@ -1033,6 +1040,10 @@ int dtkx_spline_create( const double *src_xyz, int64_t n_src,
         dtkx::KnnOutput km, kn;
         km.indices = h->m_idx.get();
         kn.indices = h->n_idx.get();
+        h->perm.resize( n_src, s );
+        dtkx::DeviceBuffer<int32_t> pos( n_src, s );
+        km.src_perm = h->perm.get();
+        km.src_pos = pos.get();
         bool const fl = o.geometry == DTKX_GEOMETRY_FLOAT;
         if ( fl )
         {
@@ -1062,6 +1073,30 @@ int dtkx_spline_create( const double *src_xyz, int64_t n_src,
         dtkx::spline_basis_weights( h->src_d.get(), h->tgt_d.get(), n_tgt, kk,
                                     h->n_idx.get(), o.rbf, o.geometry,
                                     h->n_phi.get(), s );
+        // the solver's system: sources in Morton position order
+        h->m_idx_p.resize( (size_t)n_src * kk, s );
+        h->m_phi_p.resize( (size_t)n_src * kk, s );
+        h->n_idx_p.resize( (size_t)n_tgt * kk, s );
+        if ( fl )
+        {
+            dtkx::DeviceBuffer<float4> sp_p( n_src, s );
+            dtkx::spline_renumber( o.geometry, n_src, kk, h->perm.get(),
+                                   pos.get(), h->m_idx.get(), h->m_phi.get(),
+                                   h->src_f.get(), h->m_idx_p.get(),
+                                   h->m_phi_p.get(), sp_p.get(), n_tgt,
+                                   h->n_idx.get(), h->n_idx_p.get(), s );
+            h->src_f = std::move( sp_p );
+        }
+        else
+        {
+            dtkx::DeviceBuffer<double4> sp_p( n_src, s );
+            dtkx::spline_renumber( o.geometry, n_src, kk, h->perm.get(),
+                                   pos.get(), h->m_idx.get(), h->m_phi.get(),
+                                   h->src_d.get(), h->m_idx_p.get(),
+                                   h->m_phi_p.get(), sp_p.get(), n_tgt,
+                                   h->n_idx.get(), h->n_idx_p.get(), s );
+            h->src_d = std::move( sp_p );
+        }
         DTKX_HIP( hipStreamSynchronize( s ) );
         if ( fl )
         {
@@ -1092,13 +1127,15 @@ int dtkx_spline_apply( const dtkx_spline *h, const double *src_vals,
         std::size_t const nout = (std::size_t)( n_fields - 1 ) * ld_tgt + h->n_tgt;
         Staged<double> x( src_vals, nin, s );
         StagedOut<double> y( tgt_vals, nout, s, ld_tgt > h->n_tgt );
-        dtkx::DeviceBuffer<double> z( h->n_src + dtkx::spline_k, s );
+        dtkx::DeviceBuffer<double> z( h->n_ext(), s ), xp( h->n_src, s );
         auto const sys = h->system();
         dtkx_spline_info agg{};
         agg.converged = 1;
         for ( int f = 0; f < n_fields; ++f )
         {
-            auto const r = h->solver.solve( sys, x.ptr + f * ld_src, z.get(),
+            dtkx::spline_to_positions( x.ptr + f * ld_src, h->perm.get(),
+                                       h->n_src, h->n_src, xp.get(), s );
+            auto const r = h->solver.solve( sys, xp.get(), z.get(),
                                             h->gmres(), s );
             agg.iterations = std::max( agg.iterations, r.iterations );
             agg.reorthogonalized =
@@ -1122,11 +1159,16 @@ int dtkx_spline_solve( const dtkx_spline *h, const double *src_vals,
         REQ( h != nullptr && src_vals != nullptr && coeffs != nullptr,
              DTKX_ERR_INVALID_ARGUMENT );
         hipStream_t s = h->stream;
-        std::size_t const nz = h->n_src + dtkx::spline_k;
+        std::size_t const nz = h->n_ext();
         Staged<double> x( src_vals, h->n_src, s );
         StagedOut<double> z( coeffs, nz, s );
-        auto const r = h->solver.solve( h->system(), x.ptr, z.ptr, h->gmres(),
-                                        s );
+        dtkx::DeviceBuffer<double> xp( h->n_src, s ), zp( nz, s );
+        dtkx::spline_to_positions( x.ptr, h->perm.get(), h->n_src, h->n_src,
+                                   xp.get(), s );
+        auto const r = h->solver.solve( h->system(), xp.get(), zp.get(),
+                                        h->gmres(), s );
+        dtkx::spline_to_sources( zp.get(), h->perm.get(), h->n_src, nz, z.ptr,
+                                 s );
         z.finish( s );
         DTKX_HIP( hipStreamSynchronize( s ) );
         h->info.iterations = r.iterations;
@@ -1147,9 +1189,12 @@ int dtkx_spline_evaluate( const dtkx_spline *h, const double *coeffs,
         if ( h->n_tgt == 0 )
             return DTKX_SUCCESS;
         hipStream_t s = h->stream;
-        Staged<double> z( coeffs, h->n_src + dtkx::spline_k, s );
+        Staged<double> z( coeffs, h->n_ext(), s );
         StagedOut<double> y( tgt_vals, h->n_tgt, s );
-        dtkx::spline_evaluate( h->system(), z.ptr, y.ptr, s );
+        dtkx::DeviceBuffer<double> zp( h->n_ext(), s );
+        dtkx::spline_to_positions( z.ptr, h->perm.get(), h->n_src, h->n_ext(),
+                                   zp.get(), s );
+        dtkx::spline_evaluate( h->system(), zp.get(), y.ptr, s );
         y.finish( s );
         DTKX_HIP( hipStreamSynchronize( s ) );
         return DTKX_SUCCESS;
@@ -1163,10 +1208,15 @@ int dtkx_spline_system_apply( const dtkx_spline *h, const double *z,
         REQ( h != nullptr && z != nullptr && y != nullptr,
              DTKX_ERR_INVALID_ARGUMENT );
         hipStream_t s = h->stream;
-        std::size_t const n = h->n_src + dtkx::spline_k;
+        std::size_t const n = h->n_ext();
         Staged<double> zi( z, n, s );
         StagedOut<double> yo( y, n, s );
-        dtkx::spline_system_apply( h->system(), zi.ptr, yo.ptr, s );
+        dtkx::DeviceBuffer<double> zp( n, s ), yp( n, s );
+        dtkx::spline_to_positions( zi.ptr, h->perm.get(), h->n_src, n,
+                                   zp.get(), s );
+        dtkx::spline_system_apply( h->system(), zp.get(), yp.get(), s );
+        dtkx::spline_to_sources( yp.get(), h->perm.get(), h->n_src, n, yo.ptr,
+                                 s );
         yo.finish( s );
         DTKX_HIP( hipStreamSynchronize( s ) );
         return DTKX_SUCCESS;

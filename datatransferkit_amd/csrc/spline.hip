@@ -390,6 +390,57 @@ __global__ void __launch_bounds__( kBlock )
     }
 }
 
+// ------------------------------------------------------ renumbering
+// Sources renumbered by their Morton position (perm: position -> source,
+// pos: source -> position): row p of the position-ordered M is row
+// perm[p] with its indices mapped through pos, and the point rows follow.
+template <typename Pt>
+__global__ void __launch_bounds__( kBlock )
+    permute_rows_kernel( int64_t n_src, int kk, int32_t const *perm,
+                         int32_t const *pos, int32_t const *idx,
+                         double const *phi, Pt const *pts, int32_t *idx_p,
+                         double *phi_p, Pt *pts_p )
+{
+    int64_t const p = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( p >= n_src )
+        return;
+    int64_t const r = perm[p];
+    for ( int l = 0; l < kk; ++l )
+    {
+        idx_p[p * kk + l] = pos[idx[r * kk + l]];
+        phi_p[p * kk + l] = phi[r * kk + l];
+    }
+    pts_p[p] = pts[r];
+}
+
+// out[e] = pos[idx[e]]
+__global__ void __launch_bounds__( kBlock )
+    remap_kernel( int64_t count, int32_t const *pos, int32_t const *idx,
+                  int32_t *out )
+{
+    int64_t const e = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( e < count )
+        out[e] = pos[idx[e]];
+}
+
+// Extended vectors between source order and position order: entries below
+// n_src move through perm, the polynomial entries after them are copied.
+template <bool TO_POSITIONS>
+__global__ void __launch_bounds__( kBlock )
+    permute_vector_kernel( double const *in, int32_t const *perm,
+                           int64_t n_src, int64_t n, double *out )
+{
+    int64_t const i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( i >= n )
+        return;
+    if ( i >= n_src )
+        out[i] = in[i];
+    else if ( TO_POSITIONS )
+        out[i] = in[perm[i]];
+    else
+        out[perm[i]] = in[i];
+}
+
 int row_blocks( int64_t n )
 {
     int64_t const b = ( n + kBlock - 1 ) / kBlock;
@@ -431,6 +482,51 @@ int tiles_n( int64_t n ) { return (int)( ( n + kTileN - 1 ) / kTileN ); }
 int groups( int nv ) { return ( nv + kGroup - 1 ) / kGroup; }
 
 } // namespace
+
+void spline_renumber( int geometry, int64_t n_src, int kk,
+                      int32_t const *perm, int32_t const *pos,
+                      int32_t const *m_idx, double const *m_phi,
+                      void const *src_pts, int32_t *m_idx_p, double *m_phi_p,
+                      void *src_pts_p, int64_t n_tgt, int32_t const *n_idx,
+                      int32_t *n_idx_p, hipStream_t s )
+{
+    int64_t const nb = ( n_src + kBlock - 1 ) / kBlock;
+    if ( geometry == DTKX_GEOMETRY_FLOAT )
+        permute_rows_kernel<float4><<<nb, kBlock, 0, s>>>(
+            n_src, kk, perm, pos, m_idx, m_phi,
+            static_cast<float4 const *>( src_pts ), m_idx_p, m_phi_p,
+            static_cast<float4 *>( src_pts_p ) );
+    else
+        permute_rows_kernel<double4><<<nb, kBlock, 0, s>>>(
+            n_src, kk, perm, pos, m_idx, m_phi,
+            static_cast<double4 const *>( src_pts ), m_idx_p, m_phi_p,
+            static_cast<double4 *>( src_pts_p ) );
+    DTKX_HIP( hipGetLastError() );
+    int64_t const count = n_tgt * kk;
+    if ( count > 0 )
+    {
+        remap_kernel<<<( count + kBlock - 1 ) / kBlock, kBlock, 0, s>>>(
+            count, pos, n_idx, n_idx_p );
+        DTKX_HIP( hipGetLastError() );
+    }
+}
+
+void spline_to_positions( double const *in, int32_t const *perm,
+                          int64_t n_src, int64_t n, double *out,
+                          hipStream_t s )
+{
+    permute_vector_kernel<true><<<( n + kBlock - 1 ) / kBlock, kBlock, 0, s>>>(
+        in, perm, n_src, n, out );
+    DTKX_HIP( hipGetLastError() );
+}
+
+void spline_to_sources( double const *in, int32_t const *perm, int64_t n_src,
+                        int64_t n, double *out, hipStream_t s )
+{
+    permute_vector_kernel<false><<<( n + kBlock - 1 ) / kBlock, kBlock, 0, s>>>(
+        in, perm, n_src, n, out );
+    DTKX_HIP( hipGetLastError() );
+}
 
 void spline_basis_weights( double4 const *src, double4 const *pts, int64_t n,
                            int kk, int32_t const *idx, int rbf, int geometry,

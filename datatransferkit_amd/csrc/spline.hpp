@@ -46,6 +46,25 @@ void spline_basis_weights( double4 const *src, double4 const *pts, int64_t n,
                            int kk, int32_t const *idx, int rbf, int geometry,
                            double *phi, hipStream_t s );
 
+// The sources renumbered by their Morton position (perm: position ->
+// source, pos: its inverse), so that the operator's gathers stay local:
+// M's rows and the source points are reordered and their indices mapped,
+// N's indices are mapped. Every buffer is device memory.
+void spline_renumber( int geometry, int64_t n_src, int kk,
+                      int32_t const *perm, int32_t const *pos,
+                      int32_t const *m_idx, double const *m_phi,
+                      void const *src_pts, int32_t *m_idx_p, double *m_phi_p,
+                      void *src_pts_p, int64_t n_tgt, int32_t const *n_idx,
+                      int32_t *n_idx_p, hipStream_t s );
+
+// Vectors of n entries, the first n_src of which are source entries,
+// between source order and position order (the rest are copied).
+void spline_to_positions( double const *in, int32_t const *perm,
+                          int64_t n_src, int64_t n, double *out,
+                          hipStream_t s );
+void spline_to_sources( double const *in, int32_t const *perm, int64_t n_src,
+                        int64_t n, double *out, hipStream_t s );
+
 // y = C z = P z + M z + P^T z over the extended vector (n_src + 4 entries).
 void spline_system_apply( SplineSystem const &sys, double const *z, double *y,
                           hipStream_t s );

@@ -187,3 +187,27 @@ def test_corner_cases(dtk, spl):
         dtk.SplineOperator(cuda(np.zeros((0, 3))), cuda(tgt))
     with pytest.raises(dtk.DtkxError):
         dtk.SplineOperator(cuda(src), cuda(tgt), restart=513)
+
+
+def test_restarts_and_iteration_cap(dtk, spl):
+    """Restarted cycles (restart 10) still reach the tolerance and the same
+    transfer; an iteration cap below what the tolerance needs ends the solve
+    unconverged with its explicit residual reported."""
+    src, tgt, x = cloud(8_000, 5_000, 17)
+    o = spl.SplineOracle(src, tgt)
+    yo = o.apply(x)
+    op = dtk.SplineOperator(cuda(src), cuda(tgt), restart=10)
+    y = _apply(op, x, len(tgt))
+    info = op.info()
+    assert info["converged"] and info["restarts"] >= 3 and info["residual"] <= 1e-10, info
+    assert np.max(np.abs(y - yo)) <= 1e-7 * np.max(np.abs(yo))
+    op = dtk.SplineOperator(cuda(src), cuda(tgt), max_iterations=5)
+    y = _apply(op, x, len(tgt))
+    info = op.info()
+    assert not info["converged"] and info["iterations"] == 5, info
+    assert 1e-10 < info["residual"] < 1.0
+    z = torch.empty(len(src) + 4, dtype=torch.float64, device="cuda")
+    op.solve(cuda(x), z)
+    b = np.concatenate([x, np.zeros(4)])
+    r = np.linalg.norm(o.saddle_apply(z.cpu().numpy()) - b) / np.linalg.norm(b)
+    assert abs(r - op.info()["residual"]) <= 1e-12 + 1e-6 * r
