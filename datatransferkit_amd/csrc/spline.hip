@@ -664,8 +664,9 @@ GmresInfo SplineSolver::solve( SplineSystem const &sys, double const *x,
         int k = 0; // final columns
         while ( true )
         {
-            bool const last = j == m || total >= opt.max_iterations;
-            if ( last && j == 0 )
+            // j == m: only column m - 1 is left to finalise
+            bool const last = j == m;
+            if ( j == 0 && total >= opt.max_iterations )
                 break;
             double *const u = Q + j * ld;
             double *const w = u + ld;
@@ -712,7 +713,7 @@ GmresInfo SplineSolver::solve( SplineSystem const &sys, double const *x,
                 ++info.reorthogonalized;
                 k = j;
                 if ( !std::isfinite( rho ) || std::fabs( g[j] ) <= target ||
-                     !( rho > 0. ) || last )
+                     !( rho > 0. ) || last || total >= opt.max_iterations )
                     break; // converged, invariant Krylov space, or done
             }
             if ( !( rho > 0. ) || !std::isfinite( rho ) )
