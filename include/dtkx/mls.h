@@ -350,7 +350,10 @@ typedef struct dtkx_spline_options {
   int32_t rbf;            /* dtkx_rbf; default Wendland<0> (_def.hpp:319) */
   int32_t geometry;       /* dtkx_geometry; FLOAT = the reference's points */
   int32_t layout;         /* dtkx_layout of the point arrays */
-  int32_t restart;        /* Krylov vectors per GMRES cycle, 1..512 (300) */
+  int32_t restart;        /* Krylov vectors per GMRES cycle, 1..512 (300);
+                           * the basis takes (restart + 1) * 8 * (n_src + 4)
+                           * bytes of device memory (2.4 GB at 1M sources):
+                           * lower it for very large clouds */
   int32_t max_iterations; /* GMRES iterations per solve (1000) */
   int32_t reserved;
   double tolerance;       /* relative residual |S x - C z| / |S x|: 1e-10

@@ -99,12 +99,13 @@ def test_system_apply_and_evaluate(dtk, spl):
                                atol=1e-14 * np.max(np.abs(ref)) * 8)
 
 
-@pytest.mark.parametrize("n", [2_000, 30_000])
-def test_solve_residual_and_transfer_match_oracle(dtk, spl, n):
+@pytest.mark.parametrize("n,geometry", [(2_000, "float"), (30_000, "float"),
+                                        (10_000, "double")])
+def test_solve_residual_and_transfer_match_oracle(dtk, spl, n, geometry):
     src, tgt, x = cloud(n, n, 9)
-    print(f"spline solve n={n}", flush=True)
-    o = spl.SplineOracle(src, tgt)
-    op = dtk.SplineOperator(cuda(src), cuda(tgt))
+    print(f"spline solve n={n} {geometry}", flush=True)
+    o = spl.SplineOracle(src, tgt, geometry=0 if geometry == "float" else 1)
+    op = dtk.SplineOperator(cuda(src), cuda(tgt), geometry=geometry)
     z = torch.empty(n + 4, dtype=torch.float64, device="cuda")
     op.solve(cuda(x), z)
     info = op.info()
