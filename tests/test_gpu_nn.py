@@ -120,3 +120,29 @@ def test_nn_preconditions(dtk):
     empty = dtk.NearestNeighborOperator(cuda(np.random.rand(10, 3)),
                                         cuda(np.zeros((0, 3))))
     assert empty.indices().numel() == 0
+
+
+@pytest.mark.parametrize("shift", [0, 1, 7])
+def test_fetch_semantics_one_rank(dtk, shift):
+    """The reference's fetch test (tstDetailsCommunicationHelpers.cpp:167-222)
+    on one rank is a gather, out[i] = v[indices[i]], of a rank-1 view (v)
+    and a rank-2 view (w, DIM = 2 columns). Here the gather is the
+    nearest-neighbour apply: target i sits on source indices[i], the cyclic
+    shift indices[i] = (shift + i) % P of the test's plan, so its nearest
+    source is that one (distance 0, lattice points distinct)."""
+    P = 125
+    src = grid_points((5, 5, 5), (0, 0, 0)) * 0.25
+    indices = (shift + np.arange(P)) % P
+    tgt = src[indices]
+    op = dtk.NearestNeighborOperator(cuda(src), cuda(tgt))
+    assert np.array_equal(op.indices().cpu().numpy(), indices)
+    v = np.arange(P, dtype=np.float64)                  # iota
+    y = torch.empty(P, dtype=torch.float64, device="cuda")
+    op.apply(cuda(v), y)
+    assert np.array_equal(y.cpu().numpy(), indices.astype(np.float64))
+    # w(i, j) = i + j P, as fields [j][i]; w_ref(i, j) = indices(i) + j P
+    w = np.stack([np.arange(P) + j * P for j in range(2)]).astype(np.float64)
+    yw = torch.empty((2, P), dtype=torch.float64, device="cuda")
+    op.apply(cuda(w), yw)
+    ref = np.stack([indices + j * P for j in range(2)]).astype(np.float64)
+    assert np.array_equal(yw.cpu().numpy(), ref)
