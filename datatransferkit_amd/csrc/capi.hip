@@ -931,20 +931,27 @@ struct dtkx_spline
     int64_t n_src = 0, n_tgt = 0;
     int kk = 0;
     hipStream_t stream = nullptr;
-    // basis operators M (sources' rows) and N (targets' rows) in source
+    // points [sources | targets]: double copy (geometry DOUBLE) or float
+    // copy (FLOAT); the targets' part holds the Vandermonde rows of Q
+    dtkx::DeviceBuffer<double4> pts_d;
+    dtkx::DeviceBuffer<float4> pts_f;
+    // basis rows of M (the sources' rows) then N (the targets'), in source
     // order, as dtkx_spline_get_basis returns them
-    dtkx::DeviceBuffer<int32_t> m_idx, n_idx;
-    dtkx::DeviceBuffer<double> m_phi, n_phi;
-    // Vandermonde rows: the points as the search holds them (the sources'
-    // in Morton position order)
-    dtkx::DeviceBuffer<float4> src_f, tgt_f;
-    dtkx::DeviceBuffer<double4> src_d, tgt_d;
+    dtkx::DeviceBuffer<int32_t> nbr;
+    dtkx::DeviceBuffer<double> phi;
     // the system the solver runs on: sources renumbered by Morton position
     dtkx::DeviceBuffer<int32_t> perm; // position -> source
     dtkx::DeviceBuffer<int32_t> m_idx_p, n_idx_p;
     dtkx::DeviceBuffer<double> m_phi_p;
+    dtkx::DeviceBuffer<double4> src_d_p;
+    dtkx::DeviceBuffer<float4> src_f_p;
     mutable dtkx::SplineSolver solver;
     mutable dtkx_spline_info info{};
+    bool float_geometry() const { return opts.geometry == DTKX_GEOMETRY_FLOAT; }
+    int32_t const *m_idx() const { return nbr.get(); }
+    int32_t const *n_idx() const { return nbr.get() + (size_t)n_src * kk; }
+    double const *m_phi() const { return phi.get(); }
+    double const *n_phi() const { return phi.get() + (size_t)n_src * kk; }
     dtkx::SplineSystem system() const
     {
         dtkx::SplineSystem y;
@@ -952,13 +959,20 @@ struct dtkx_spline
         y.n_tgt = n_tgt;
         y.kk = kk;
         y.geometry = opts.geometry;
-        bool const f = opts.geometry == DTKX_GEOMETRY_FLOAT;
-        y.src_pts = f ? (void const *)src_f.get() : (void const *)src_d.get();
-        y.tgt_pts = f ? (void const *)tgt_f.get() : (void const *)tgt_d.get();
+        if ( float_geometry() )
+        {
+            y.src_pts = src_f_p.get();
+            y.tgt_pts = pts_f.get() + n_src;
+        }
+        else
+        {
+            y.src_pts = src_d_p.get();
+            y.tgt_pts = pts_d.get() + n_src;
+        }
         y.m_idx = m_idx_p.get();
         y.m_phi = m_phi_p.get();
         y.n_idx = n_idx_p.get();
-        y.n_phi = n_phi.get();
+        y.n_phi = n_phi();
         return y;
     }
     std::size_t n_ext() const { return n_src + dtkx::spline_k; }
@@ -1025,84 +1039,58 @@ int dtkx_spline_create( const double *src_xyz, int64_t n_src,
         int const kk = h->kk;
         Staged<double> sx( src_xyz, 3 * n_src, s );
         Staged<double> tx( tgt_xyz, 3 * n_tgt, s );
-        h->src_d.resize( n_src, s );
-        dtkx::points_to_double4( sx.ptr, n_src, o.layout, h->src_d.get(), s );
+        int64_t const n_all = n_src + n_tgt;
+        h->pts_d.resize( n_all, s );
+        dtkx::points_to_double4( sx.ptr, n_src, o.layout, h->pts_d.get(), s );
         if ( n_tgt > 0 )
-        {
-            h->tgt_d.resize( n_tgt, s );
-            dtkx::points_to_double4( tx.ptr, n_tgt, o.layout, h->tgt_d.get(),
-                                     s );
-        }
-        h->m_idx.resize( (size_t)n_src * kk, s );
-        h->m_phi.resize( (size_t)n_src * kk, s );
-        h->n_idx.resize( (size_t)n_tgt * kk, s );
-        h->n_phi.resize( (size_t)n_tgt * kk, s );
-        dtkx::KnnOutput km, kn;
-        km.indices = h->m_idx.get();
-        kn.indices = h->n_idx.get();
-        h->perm.resize( n_src, s );
-        dtkx::DeviceBuffer<int32_t> pos( n_src, s );
-        km.src_perm = h->perm.get();
-        km.src_pos = pos.get();
-        bool const fl = o.geometry == DTKX_GEOMETRY_FLOAT;
+            dtkx::points_to_double4( tx.ptr, n_tgt, o.layout,
+                                     h->pts_d.get() + n_src, s );
+        bool const fl = h->float_geometry();
         if ( fl )
         {
-            h->src_f.resize( n_src, s );
-            dtkx::double4_to_float4( h->src_d.get(), n_src, h->src_f.get(), s );
-            if ( n_tgt > 0 )
-            {
-                h->tgt_f.resize( n_tgt, s );
-                dtkx::double4_to_float4( h->tgt_d.get(), n_tgt,
-                                         h->tgt_f.get(), s );
-            }
+            h->pts_f.resize( n_all, s );
+            dtkx::double4_to_float4( h->pts_d.get(), n_all, h->pts_f.get(), s );
         }
-        void const *sp = fl ? (void const *)h->src_f.get()
-                            : (void const *)h->src_d.get();
-        void const *tp = fl ? (void const *)h->tgt_f.get()
-                            : (void const *)h->tgt_d.get();
-        // M: the sources' neighbours among the sources (def.hpp:211-212);
-        // N: the targets' (def.hpp:215-216)
-        dtkx::knn_search( o.geometry, sp, n_src, sp, n_src, dtkx::spline_k, km,
-                          nullptr, s );
-        if ( n_tgt > 0 )
-            dtkx::knn_search( o.geometry, sp, n_src, tp, n_tgt,
-                              dtkx::spline_k, kn, nullptr, s );
-        dtkx::spline_basis_weights( h->src_d.get(), h->src_d.get(), n_src, kk,
-                                    h->m_idx.get(), o.rbf, o.geometry,
-                                    h->m_phi.get(), s );
-        dtkx::spline_basis_weights( h->src_d.get(), h->tgt_d.get(), n_tgt, kk,
-                                    h->n_idx.get(), o.rbf, o.geometry,
-                                    h->n_phi.get(), s );
+        void const *pts = fl ? (void const *)h->pts_f.get()
+                             : (void const *)h->pts_d.get();
+        // One search over the queries [sources | targets]: rows below n_src
+        // are M's, the sources' neighbours among the sources
+        // (def.hpp:211-212), the rest N's, the targets' (def.hpp:215-216).
+        h->nbr.resize( (size_t)n_all * kk, s );
+        h->phi.resize( (size_t)n_all * kk, s );
+        h->perm.resize( n_src, s );
+        dtkx::DeviceBuffer<int32_t> pos( n_src, s );
+        dtkx::KnnOutput ko;
+        ko.indices = h->nbr.get();
+        ko.src_perm = h->perm.get();
+        ko.src_pos = pos.get();
+        dtkx::knn_search( o.geometry, pts, n_src, pts, n_all, dtkx::spline_k,
+                          ko, nullptr, s );
+        dtkx::spline_basis_weights( h->pts_d.get(), h->pts_d.get(), n_all, kk,
+                                    h->nbr.get(), o.rbf, o.geometry,
+                                    h->phi.get(), s );
         // the solver's system: sources in Morton position order
         h->m_idx_p.resize( (size_t)n_src * kk, s );
         h->m_phi_p.resize( (size_t)n_src * kk, s );
         h->n_idx_p.resize( (size_t)n_tgt * kk, s );
+        void *src_p;
         if ( fl )
         {
-            dtkx::DeviceBuffer<float4> sp_p( n_src, s );
-            dtkx::spline_renumber( o.geometry, n_src, kk, h->perm.get(),
-                                   pos.get(), h->m_idx.get(), h->m_phi.get(),
-                                   h->src_f.get(), h->m_idx_p.get(),
-                                   h->m_phi_p.get(), sp_p.get(), n_tgt,
-                                   h->n_idx.get(), h->n_idx_p.get(), s );
-            h->src_f = std::move( sp_p );
+            h->src_f_p.resize( n_src, s );
+            src_p = h->src_f_p.get();
         }
         else
         {
-            dtkx::DeviceBuffer<double4> sp_p( n_src, s );
-            dtkx::spline_renumber( o.geometry, n_src, kk, h->perm.get(),
-                                   pos.get(), h->m_idx.get(), h->m_phi.get(),
-                                   h->src_d.get(), h->m_idx_p.get(),
-                                   h->m_phi_p.get(), sp_p.get(), n_tgt,
-                                   h->n_idx.get(), h->n_idx_p.get(), s );
-            h->src_d = std::move( sp_p );
+            h->src_d_p.resize( n_src, s );
+            src_p = h->src_d_p.get();
         }
+        dtkx::spline_renumber( o.geometry, n_src, kk, h->perm.get(), pos.get(),
+                               h->m_idx(), h->m_phi(), pts, h->m_idx_p.get(),
+                               h->m_phi_p.get(), src_p, n_tgt, h->n_idx(),
+                               h->n_idx_p.get(), s );
         DTKX_HIP( hipStreamSynchronize( s ) );
         if ( fl )
-        {
-            h->src_d.release();
-            h->tgt_d.release();
-        }
+            h->pts_d.release();
         *out = h.release();
         return DTKX_SUCCESS;
     } );
@@ -1231,9 +1219,9 @@ int dtkx_spline_get_basis( const dtkx_spline *h, int32_t which,
         REQ( h != nullptr && ( which == 0 || which == 1 ),
              DTKX_ERR_INVALID_ARGUMENT );
         if ( indices )
-            *indices = which == 0 ? h->m_idx.get() : h->n_idx.get();
+            *indices = which == 0 ? h->m_idx() : h->n_idx();
         if ( values )
-            *values = which == 0 ? h->m_phi.get() : h->n_phi.get();
+            *values = which == 0 ? h->m_phi() : h->n_phi();
         if ( n_rows )
             *n_rows = which == 0 ? h->n_src : h->n_tgt;
         if ( kk )
