@@ -74,10 +74,9 @@ typedef enum dtkx_rbf {
 typedef enum dtkx_precision {
     /* everything in FP64 (geometry per dtkx_geometry) */
     DTKX_PRECISION_FP64 = 0,
-    /* FP32 weights, Vandermonde, coefficients and apply (compensated FP32
-     * sums); FP64 moments P^T W P, solve and singular-value truncation. The
-     * radius is 1.1 x the k-th neighbour's distance as the kNN computes it
-     * (float coordinates for DTKX_GEOMETRY_FLOAT) */
+    /* FP32 relative coordinates, radius, weights, Vandermonde,
+     * coefficients and apply (compensated FP32 sums); FP64 moments
+     * P^T W P, solve and singular-value truncation */
     DTKX_PRECISION_FP32_BULK = 1
 } dtkx_precision;
 
