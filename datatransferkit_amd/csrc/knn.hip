@@ -269,7 +269,8 @@ template <typename T, typename V>
 __global__ void __launch_bounds__( kBlock )
     karras_kernel( uint64_t const *codes_all, int32_t const *sorted_all,
                    V const *pts, int64_t n_int, T const *scenes,
-                   int32_t const *src_off, int n_prob, Node<T> *nodes )
+                   int32_t const *src_off, int n_prob, Node<T> *nodes,
+                   int32_t *sizes )
 {
     int64_t const gi = blockIdx.x * (int64_t)kBlock + threadIdx.x;
     if ( gi >= n_int )
@@ -310,6 +311,7 @@ __global__ void __launch_bounds__( kBlock )
     } while ( t > 1 );
     int64_t const gamma = i + s * d + ( d < 0 ? -1 : 0 );
     int64_t const first = i < j ? i : j, last = i < j ? j : i;
+    sizes[gi] = (int32_t)( last - first + 1 );
     int64_t const rb[2] = {first, gamma + 1};
     int64_t const re[2] = {gamma, last};
     Node<T> nd;
@@ -490,6 +492,25 @@ __device__ inline T box_sq( T const *q, T const *lo, T const *hi )
     return s;
 }
 
+// Squared distance from q to the farthest corner of [lo, hi], in
+// box_sq's operations: per axis the larger of the two rounded differences
+// q - lo and hi - q. Rounding is monotonic, so for every point p of the
+// box the rounded |p - q| is at most that axis term, and the sum bounds
+// box_sq(q, p, p) from above.
+template <typename T>
+__device__ inline T far_sq( T const *q, T const *lo, T const *hi )
+{
+#pragma clang fp contract( off )
+    T s = 0;
+#pragma unroll
+    for ( int d = 0; d < 3; ++d )
+    {
+        T const tmp = fmax( q[d] - lo[d], hi[d] - q[d] );
+        s += tmp * tmp;
+    }
+    return s;
+}
+
 // Correctly rounded square roots: llvm.sqrt without fast-math flags (the
 // library is built with -fhip-fp32-correctly-rounded-divide-sqrt). HIP's
 // __fsqrt_rn maps to the native 1-ulp instruction and must not be used.
@@ -497,6 +518,13 @@ __device__ inline float sqrt_rn( float s ) { return __builtin_sqrtf( s ); }
 __device__ inline double sqrt_rn( double s ) { return __builtin_sqrt( s ); }
 
 constexpr int kQueryBlock = 64;
+
+// Seed the two-pass search's k-th distance from the deepest node on the
+// query's path holding k sources.
+#ifndef DTKX_KNN_SEED
+#define DTKX_KNN_SEED 1
+#endif
+constexpr bool kSeed = DTKX_KNN_SEED != 0;
 
 // Queries run with an LDS stack of kLdsStack entries; the rare one whose
 // descent needs more is re-run with a global stack of kMaxDepth entries.
@@ -550,6 +578,8 @@ template <typename T>
 struct QueryArgs
 {
     Node<T> const *nodes = nullptr;
+    // nullable: points under each internal node (seeds the k-th distance)
+    int32_t const *node_size = nullptr;
     typename Vec4<T>::type const *src = nullptr, *tgt = nullptr;
     int32_t const *order = nullptr; // row -> target
     int64_t row0 = 0, n_rows = 0;
@@ -775,6 +805,16 @@ struct KthList
         d[0] = fmin( d[0], s );
     }
     __device__ T last() const { return d[K - 1]; }
+    // kk entries of value b (padding stays -1): b acts as kk candidates
+    // at squared distance b, so the list ends at the true kk-th value
+    // whenever at least kk sources lie at or below b
+    __device__ void seed( int kk, T b )
+    {
+#pragma unroll
+        for ( int j = 0; j < K; ++j )
+            if ( j >= K - kk )
+                d[j] = b;
+    }
 };
 
 // A squared-distance bound above every s whose rounded root is <= w.
@@ -943,6 +983,41 @@ __global__ void __launch_bounds__( kQueryBlock )
     // pass 1: the kk-th smallest squared distance
     KthList<T, K> list;
     list.init( kk );
+    if ( a.node_size && np > 1 )
+    {
+        // Seed: follow the query's nearer child down to the deepest node
+        // still holding kk sources; the farthest corner of its box bounds
+        // the kk-th distance, so pruning starts there and the descent
+        // pushes no far child beyond it.
+        int32_t cur = root;
+        Node<T> nd = a.nodes[root];
+        T lo[3], hi[3];
+#pragma unroll
+        for ( int d = 0; d < 3; ++d )
+        {
+            lo[d] = fmin( nd.lo[0][d], nd.lo[1][d] );
+            hi[d] = fmax( nd.hi[0][d], nd.hi[1][d] );
+        }
+        while ( true )
+        {
+            int const c =
+                box_sq( q, nd.lo[1], nd.hi[1] ) < box_sq( q, nd.lo[0], nd.hi[0] )
+                    ? 1
+                    : 0;
+            int32_t const ch = nd.child[c];
+            if ( ch < 0 || a.node_size[ch] < kk )
+                break;
+#pragma unroll
+            for ( int d = 0; d < 3; ++d )
+            {
+                lo[d] = nd.lo[c][d];
+                hi[d] = nd.hi[c][d];
+            }
+            cur = ch;
+            nd = a.nodes[cur];
+        }
+        list.seed( kk, root_bound( sqrt_rn( far_sq( q, lo, hi ) ) ) );
+    }
     if ( np == 1 )
     {
         typename Vec4<T>::type sp0 = a.src[so];
@@ -1364,11 +1439,12 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     // hierarchy + refit
     int64_t const n_int = n_src - 1;
     DeviceBuffer<Node<T>> nodes( std::max<int64_t>( n_int, 1 ), s );
+    DeviceBuffer<int32_t> sizes( std::max<int64_t>( n_int, 1 ), s );
     if ( n_int > 0 )
     {
         karras_kernel<T, V><<<ceil_div( n_int, kBlock ), kBlock, 0, s>>>(
             codes_sorted.get(), ids_sorted, src, n_int, box.get(), nullptr, 1,
-            nodes.get() );
+            nodes.get(), sizes.get() );
         DTKX_HIP( hipGetLastError() );
     }
     if ( n_tgt > 0 )
@@ -1387,6 +1463,7 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
 
     QueryArgs<T> qa;
     qa.nodes = nodes.get();
+    qa.node_size = kSeed ? sizes.get() : nullptr;
     qa.src = src;
     qa.tgt = tgt;
     qa.order = order;
@@ -1555,11 +1632,12 @@ void knn_batched_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     // one hierarchy per problem, all in one launch
     int64_t const n_int = n_src - P;
     DeviceBuffer<Node<T>> nodes( std::max<int64_t>( n_int, 1 ), s );
+    DeviceBuffer<int32_t> sizes( std::max<int64_t>( n_int, 1 ), s );
     if ( n_int > 0 )
     {
         karras_kernel<T, V><<<ceil_div( n_int, kBlock ), kBlock, 0, s>>>(
             codes_sorted.get(), out.src_perm, src, n_int, boxes.get(),
-            b.src_off, P, nodes.get() );
+            b.src_off, P, nodes.get(), sizes.get() );
         DTKX_HIP( hipGetLastError() );
     }
     codes_sorted.release();
@@ -1583,6 +1661,8 @@ void knn_batched_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     {
         QueryArgs<T> qa;
         qa.nodes = nodes.get();
+        qa.node_size = kSeed ? sizes.get() : nullptr;
+    qa.node_size = kSeed ? sizes.get() : nullptr;
         qa.src = src;
         qa.tgt = tgt;
         qa.order = out.order;
