@@ -150,10 +150,10 @@ def load_traffic(key):
 def query_kernel(geo, k):
     """The query kernel the library launches for k results: the K bucket,
     the one- or two-pass form (two passes from K = 32 for float and K = 16
-    for double coordinates) and the 24-entry LDS stack."""
+    for double coordinates) and the 22-entry LDS stack."""
     kb = next(v for v in (1, 4, 8, 10, 16, 24, 32, 48, 64) if k <= v)
     two = kb >= (32 if geo == "float" else 16)
-    return f"knn_query{'2' if two else ''}_kernel<{geo},{kb},24>"
+    return f"knn_query{'2' if two else ''}_kernel<{geo},{kb},22>"
 
 
 def kernel_rooflines(form, r, n, m, kk, nnz, fields):

@@ -369,6 +369,15 @@ struct KBestF
             key[j] = ~0ull;
     }
     // squared-distance threshold above which nothing can enter
+    // every slot a candidate at distance w with an index above any source
+    // (kept out of the results as long as kk sources lie within w)
+    __device__ void seed( float w )
+    {
+        uint64_t const v = ( (uint64_t)__float_as_uint( w ) << 32 ) | 0xffffffffu;
+#pragma unroll
+        for ( int j = 0; j < K; ++j )
+            key[j] = v;
+    }
     __device__ float threshold() const
     {
         if ( key[K - 1] == ~0ull )
@@ -411,6 +420,15 @@ struct KBestD
         for ( int j = 0; j < K; ++j )
         {
             dk[j] = INFINITY;
+            ik[j] = INT32_MAX;
+        }
+    }
+    __device__ void seed( double w )
+    {
+#pragma unroll
+        for ( int j = 0; j < K; ++j )
+        {
+            dk[j] = w;
             ik[j] = INT32_MAX;
         }
     }
@@ -511,6 +529,42 @@ __device__ inline T far_sq( T const *q, T const *lo, T const *hi )
     return s;
 }
 
+// Seed of a search for kk neighbours below internal node `root` (whose
+// subtree holds at least kk sources): follow the query's nearer child down
+// to the deepest node still holding kk sources and return far_sq of its
+// box, an upper bound of the kk-th squared distance as the traversal
+// computes it. sizes: sources under each internal node.
+template <typename T>
+__device__ inline T seed_far_sq( Node<T> const *nodes, int32_t const *sizes,
+                                 int32_t root, T const *q, int kk )
+{
+    Node<T> nd = nodes[root];
+    T lo[3], hi[3];
+#pragma unroll
+    for ( int d = 0; d < 3; ++d )
+    {
+        lo[d] = fmin( nd.lo[0][d], nd.lo[1][d] );
+        hi[d] = fmax( nd.hi[0][d], nd.hi[1][d] );
+    }
+    while ( true )
+    {
+        int const c =
+            box_sq( q, nd.lo[1], nd.hi[1] ) < box_sq( q, nd.lo[0], nd.hi[0] ) ? 1
+                                                                              : 0;
+        int32_t const ch = nd.child[c];
+        if ( ch < 0 || sizes[ch] < kk )
+            break;
+#pragma unroll
+        for ( int d = 0; d < 3; ++d )
+        {
+            lo[d] = nd.lo[c][d];
+            hi[d] = nd.hi[c][d];
+        }
+        nd = nodes[ch];
+    }
+    return far_sq( q, lo, hi );
+}
+
 // Correctly rounded square roots: llvm.sqrt without fast-math flags (the
 // library is built with -fhip-fp32-correctly-rounded-divide-sqrt). HIP's
 // __fsqrt_rn maps to the native 1-ulp instruction and must not be used.
@@ -528,21 +582,29 @@ constexpr bool kSeed = DTKX_KNN_SEED != 0;
 
 // Queries run with an LDS stack of kLdsStack entries; the rare one whose
 // descent needs more is re-run with a global stack of kMaxDepth entries.
-// The LDS stack bounds the occupancy (8 bytes per entry and lane): 24
-// entries fit 13 waves per CU, and no query of 1M uniform points overflows
-// them (20 entries overflow 0.35 % of the queries, whose re-run costs more
-// than the occupancy gains; 32 entries fit 9 waves and run 15 % slower).
+// The LDS stack bounds the occupancy (8 bytes per entry and lane). With
+// the seeded k-th distance (seed_far_sq) a descent pushes only far
+// children inside the seed's bound, so shallow stacks suffice: 22 entries
+// (14 waves per CU) against 24 at 1M x 1M: k = 32 2.11 / 2.12 ms, k = 16
+// 1.03 / 1.13, k = 10 0.73 / 0.81; 20 entries (with an 8-entry second
+// pass) 13 % slower at k = 32, and 82 clustered queries overflow. Before
+// seeding, 24 entries were needed, and 40 % of the queries of 1M points in
+// 200 Gaussian clusters (sigma 1e-3) overflowed them; none do now.
 // Every level of the hierarchy lengthens the common prefix of the node's
 // keys (63 Morton bits, then 32 index bits for equal codes), so no path
 // from the root has more than 96 internal levels.
 #ifndef DTKX_KNN_LDS_STACK
-#define DTKX_KNN_LDS_STACK 24
+#define DTKX_KNN_LDS_STACK 22
 #endif
 constexpr int kLdsStack = DTKX_KNN_LDS_STACK;
 // stack entries of the fixed-radius pass of a two-pass search when its row
 // is collected in the LDS stack's other slots (its radius is final, so it
-// descends far less deep than the first pass)
-constexpr int kPass2Stack = 16;
+// descends far less deep than the first pass); the row needs
+// K <= 2 kLdsStack - kPass2Stack slots for float coordinates
+#ifndef DTKX_KNN_PASS2_STACK
+#define DTKX_KNN_PASS2_STACK 12
+#endif
+constexpr int kPass2Stack = DTKX_KNN_PASS2_STACK;
 constexpr int kMaxDepth = 97;
 // the largest K whose two-pass rows are sorted inside the traversal kernel
 #ifndef DTKX_KNN_FUSED_SORT_K
@@ -647,7 +709,10 @@ __global__ void __launch_bounds__( kQueryBlock )
     T const q[3] = {qp.x, qp.y, qp.z};
     typename KBest<T, K>::type best;
     best.init();
-    T thr = INFINITY;
+    if ( a.node_size && np > 1 )
+        best.seed(
+            sqrt_rn( seed_far_sq( a.nodes, a.node_size, root, q, kk ) ) );
+    T thr = best.threshold();
 
     if ( np == 1 )
     {
@@ -984,40 +1049,10 @@ __global__ void __launch_bounds__( kQueryBlock )
     KthList<T, K> list;
     list.init( kk );
     if ( a.node_size && np > 1 )
-    {
-        // Seed: follow the query's nearer child down to the deepest node
-        // still holding kk sources; the farthest corner of its box bounds
-        // the kk-th distance, so pruning starts there and the descent
-        // pushes no far child beyond it.
-        int32_t cur = root;
-        Node<T> nd = a.nodes[root];
-        T lo[3], hi[3];
-#pragma unroll
-        for ( int d = 0; d < 3; ++d )
-        {
-            lo[d] = fmin( nd.lo[0][d], nd.lo[1][d] );
-            hi[d] = fmax( nd.hi[0][d], nd.hi[1][d] );
-        }
-        while ( true )
-        {
-            int const c =
-                box_sq( q, nd.lo[1], nd.hi[1] ) < box_sq( q, nd.lo[0], nd.hi[0] )
-                    ? 1
-                    : 0;
-            int32_t const ch = nd.child[c];
-            if ( ch < 0 || a.node_size[ch] < kk )
-                break;
-#pragma unroll
-            for ( int d = 0; d < 3; ++d )
-            {
-                lo[d] = nd.lo[c][d];
-                hi[d] = nd.hi[c][d];
-            }
-            cur = ch;
-            nd = a.nodes[cur];
-        }
-        list.seed( kk, root_bound( sqrt_rn( far_sq( q, lo, hi ) ) ) );
-    }
+        // pruning starts at the seed, so the descent pushes no far child
+        // beyond it
+        list.seed( kk, root_bound( sqrt_rn(
+                           seed_far_sq( a.nodes, a.node_size, root, q, kk ) ) ) );
     if ( np == 1 )
     {
         typename Vec4<T>::type sp0 = a.src[so];
