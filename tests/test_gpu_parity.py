@@ -131,12 +131,14 @@ def test_knn_large(dtk, oracle_lib):
 
 @pytest.mark.parametrize("geometry", ["float", "double"])
 def test_knn_deep_tree_overflow_rerun(dtk, oracle_lib, geometry):
-    """A hierarchy deeper than the 32-entry LDS traversal stack: pairs of
-    points halving their distance to the origin (one Morton level each, and
-    a pair is an internal node, so the traversal pushes it) ending in 65536
-    copies of the origin (16 more levels, split by index bits). Queries at
-    the origin push past 32 entries before the list fills; they are re-run
-    with the global stack and must still equal brute force."""
+    """A hierarchy deeper than the LDS traversal stack: pairs of points
+    halving their distance to the origin (one Morton level each, and a pair
+    is an internal node, so the traversal pushes it) ending in 65536 copies
+    of the origin (16 more levels, split by index bits). For queries at the
+    origin the seeded bound is 0 and every duplicate's node lies at
+    distance 0, so both children of each pass and the descent pushes past
+    the LDS stack; those queries are re-run with the global stack and must
+    still equal brute force."""
     rng = np.random.default_rng(4)
     chain = np.repeat(np.array([[2.0 ** -i] * 3 for i in range(1, 22)]), 2, axis=0)
     dup = np.zeros((65536, 3))
