@@ -620,7 +620,10 @@ constexpr int kSortRows = -1;
 // 2.09 ms at k = 24) and two passes from k = 32 (2.53 against 2.68 ms);
 // with double coordinates, whose (distance, index) comparisons cost three
 // operations, two passes win from k = 16 (with the 32-entry stack: 3.2
-// against 4.2 ms at k = 16, 5.1 against 11.9 ms at k = 32).
+// against 4.2 ms at k = 16, 5.1 against 11.9 ms at k = 32). With the
+// seeded k-th distance and the 22-entry stack the crossover is unchanged:
+// one pass 1.54 against 1.75 ms at k = 24, two passes 2.12 against 2.21 ms
+// at k = 32.
 #ifndef DTKX_KNN_TWO_PASS_KF
 #define DTKX_KNN_TWO_PASS_KF 32
 #endif
