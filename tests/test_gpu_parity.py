@@ -130,6 +130,33 @@ def test_knn_large(dtk, oracle_lib):
 
 
 @pytest.mark.parametrize("geometry", ["float", "double"])
+def test_knn_seeded_bound_edge_geometry(dtk, oracle_lib, geometry):
+    """The search seeds its k-th distance with the farthest corner of the
+    deepest node on the query's path holding k sources. Geometry where that
+    bound is loose, zero or tight: 50 Gaussian clusters (sigma 1e-4) with
+    exact duplicates, targets on duplicates, inside clusters, between
+    clusters and far outside the sources' box; k across the one-pass and
+    two-pass kernels. Indices and distances must equal brute force."""
+    rng = np.random.default_rng(21)
+    centres = rng.random((50, 3))
+    src = centres[rng.integers(0, 50, 20_000)] + rng.normal(scale=1e-4, size=(20_000, 3))
+    src[:2000] = src[rng.integers(2000, 20_000, 2000)]  # duplicates
+    tgt = np.concatenate([src[rng.integers(0, 20_000, 300)],
+                          centres[rng.integers(0, 50, 300)]
+                          + rng.normal(scale=2e-4, size=(300, 3)),
+                          rng.random((300, 3)),
+                          rng.random((100, 3)) * 20 - 10])
+    g = 0 if geometry == "float" else 1
+    for k in (1, 4, 16, 32, 48):
+        print(f"seeded knn {geometry} k={k}", flush=True)
+        gi, gd = oracle_lib.knn(src, tgt, k, geometry=g)
+        idx, dist = dtk.knn(cuda(src), cuda(tgt), k, geometry=geometry,
+                            with_distances=True)
+        assert np.array_equal(idx.cpu().numpy(), gi), k
+        assert np.array_equal(bits(dist.cpu().numpy()), bits(gd)), k
+
+
+@pytest.mark.parametrize("geometry", ["float", "double"])
 def test_knn_deep_tree_overflow_rerun(dtk, oracle_lib, geometry):
     """A hierarchy deeper than the LDS traversal stack: pairs of points
     halving their distance to the origin (one Morton level each, and a pair
