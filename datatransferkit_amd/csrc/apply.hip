@@ -360,7 +360,12 @@ __global__ void __launch_bounds__( kBlock )
     xs[p] = (XS)x[q][s - src_off[q]];
 }
 
-// One lane per row, the row's entries in order (sequential sum).
+// One lane per row, the row's entries in order (sequential sum), staged
+// as in crs_rows_kernel: each step copies the next kChunk entries of every
+// row of the wave into LDS (rows are contiguous in the CSR arrays, so the
+// copy runs in row-long coalesced pieces), then each lane sums its row's
+// part. Rows differ in length across problems; the wave runs to its
+// longest row.
 template <typename C, typename XS, typename Acc>
 __global__ void __launch_bounds__( kBlock )
     batch_rows_kernel( int32_t const *__restrict__ idx,
@@ -374,14 +379,59 @@ __global__ void __launch_bounds__( kBlock )
 {
 #pragma clang fp contract( off )
     using Sum = typename SumOf<Acc>::type;
-    int64_t const r = xcd_tile() * kBlock + threadIdx.x;
-    if ( r >= n_rows )
-        return;
-    int64_t const b = row_off[r], e = row_off[r + 1];
+    constexpr int kWaves = kBlock / 64;
+    __shared__ int32_t s_idx[kWaves][64 * kPitch];
+    __shared__ C s_c[kWaves][64 * kPitch];
+    __shared__ int64_t s_beg[kWaves][64];
+    __shared__ int32_t s_len[kWaves][64];
+    int const wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    int64_t const r0 = xcd_tile() * kBlock + wave * 64;
+    if ( r0 >= n_rows )
+        return; // whole wave: no block-level barrier below
+    int const nrows = n_rows - r0 < 64 ? (int)( n_rows - r0 ) : 64;
+    int32_t *const si = s_idx[wave];
+    C *const sc = s_c[wave];
+    int len = 0;
+    if ( lane < nrows )
+    {
+        int64_t const b = row_off[r0 + lane];
+        len = (int)( row_off[r0 + lane + 1] - b );
+        s_beg[wave][lane] = b;
+        s_len[wave][lane] = len;
+    }
+    int maxlen = len;
+    for ( int o = 32; o > 0; o >>= 1 )
+        maxlen = max( maxlen, __shfl_xor( maxlen, o, 64 ) );
+    __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
     Sum acc;
-    for ( int64_t j = b; j < e; ++j )
-        acc.add( (Acc)coeffs[j] * (Acc)xs[idx[j]] );
-    int32_t const t = order[r];
+    for ( int j0 = 0; j0 < maxlen; j0 += kChunk )
+    {
+        for ( int e = lane; e < nrows * kChunk; e += 64 )
+        {
+            int const rr = e / kChunk, jj = e - rr * kChunk;
+            if ( j0 + jj < s_len[wave][rr] )
+            {
+                int64_t const g = s_beg[wave][rr] + j0 + jj;
+                si[rr * kPitch + jj] = idx[g];
+                sc[rr * kPitch + jj] = coeffs[g];
+            }
+        }
+        __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
+        int const ch = len - j0 < kChunk ? len - j0 : kChunk;
+        for ( int jj = 0; jj < ch; ++jj )
+            acc.add( (Acc)sc[lane * kPitch + jj] *
+                     (Acc)xs[si[lane * kPitch + jj]] );
+        __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
+    }
+    if ( lane >= nrows )
+        return;
+    int32_t const t = order[r0 + lane];
     int32_t const q = tgt_prob[t];
     y[q][t - tgt_off[q]] = (double)acc.value();
 }
