@@ -261,38 +261,100 @@ __global__ void __launch_bounds__( kBlock )
 
 // One lane per source position: its column's entries in canonical order,
 // each times the row's gathered ybar values.
+// xbar[s] = sum over column s of coef * ybar[row], one lane per column in
+// the CSC's canonical order. A wave's 64 columns are contiguous in the CSC
+// arrays; each step copies the next kChunk entries of every column into
+// LDS (coalesced), then each lane sums its column's part. The wave runs to
+// its longest column.
 template <int NF>
 __global__ void __launch_bounds__( kBlock )
     transpose_kernel( int32_t const *__restrict__ off,
-                      int32_t const *__restrict__ ent,
-                      double const *__restrict__ coeffs, int kk, int64_t n_src,
+                      int32_t const *__restrict__ rows,
+                      double const *__restrict__ coefs, int64_t n_src,
                       Pack<double, NF> const *__restrict__ ys,
                       int32_t const *__restrict__ perm,
                       double *__restrict__ xbar, int64_t ld_src, int nf )
 {
 #pragma clang fp contract( off )
-    int64_t const p = xcd_tile() * kBlock + threadIdx.x;
-    if ( p >= n_src )
-        return;
-    int32_t const b = off[p], e = off[p + 1];
+    constexpr int kWaves = kBlock / 64;
+    __shared__ int32_t s_row[kWaves][64 * kPitch];
+    __shared__ double s_c[kWaves][64 * kPitch];
+    __shared__ int32_t s_beg[kWaves][64];
+    __shared__ int32_t s_len[kWaves][64];
+    int const wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    int64_t const p0 = xcd_tile() * kBlock + wave * 64;
+    if ( p0 >= n_src )
+        return; // whole wave: no block-level barrier below
+    int const ncols = n_src - p0 < 64 ? (int)( n_src - p0 ) : 64;
+    int32_t *const sr = s_row[wave];
+    double *const sc = s_c[wave];
+    int len = 0;
+    if ( lane < ncols )
+    {
+        int32_t const b = off[p0 + lane];
+        len = off[p0 + lane + 1] - b;
+        s_beg[wave][lane] = b;
+        s_len[wave][lane] = len;
+    }
+    int maxlen = len;
+    for ( int o = 32; o > 0; o >>= 1 )
+        maxlen = max( maxlen, __shfl_xor( maxlen, o, 64 ) );
+    __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
     double acc[NF];
 #pragma unroll
     for ( int f = 0; f < NF; ++f )
         acc[f] = 0.;
-    for ( int32_t q = b; q < e; ++q )
+    for ( int j0 = 0; j0 < maxlen; j0 += kChunk )
     {
-        int32_t const l = ent[q];
-        double const c = coeffs[l];
-        Pack<double, NF> const v = ys[l / kk];
+        for ( int e = lane; e < ncols * kChunk; e += 64 )
+        {
+            int const cc = e / kChunk, jj = e - cc * kChunk;
+            if ( j0 + jj < s_len[wave][cc] )
+            {
+                int32_t const q = s_beg[wave][cc] + j0 + jj;
+                sr[cc * kPitch + jj] = rows[q];
+                sc[cc * kPitch + jj] = coefs[q];
+            }
+        }
+        __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
+        int const ch = len - j0 < kChunk ? len - j0 : kChunk;
+        for ( int jj = 0; jj < ch; ++jj )
+        {
+            double const c = sc[lane * kPitch + jj];
+            Pack<double, NF> const v = ys[sr[lane * kPitch + jj]];
 #pragma unroll
-        for ( int f = 0; f < NF; ++f )
-            acc[f] += c * v.v[f];
+            for ( int f = 0; f < NF; ++f )
+                acc[f] += c * v.v[f];
+        }
+        __builtin_amdgcn_fence( __ATOMIC_RELEASE, "wavefront" );
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence( __ATOMIC_ACQUIRE, "wavefront" );
     }
-    int64_t const s = perm ? perm[p] : p;
+    if ( lane >= ncols )
+        return;
+    int64_t const s = perm ? perm[p0 + lane] : p0 + lane;
 #pragma unroll
     for ( int f = 0; f < NF; ++f )
         if ( f < nf )
             xbar[f * ld_src + s] = acc[f];
+}
+
+// rows[q] = entries[q] / kk, coefs[q] = coeffs[entries[q]]
+__global__ void __launch_bounds__( kBlock )
+    csc_pack_kernel( int32_t const *__restrict__ entries,
+                     double const *__restrict__ coeffs, int kk, int64_t nnz,
+                     int32_t *__restrict__ rows, double *__restrict__ coefs )
+{
+    int64_t const q = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( q >= nnz )
+        return;
+    int32_t const l = entries[q];
+    rows[q] = l / kk;
+    coefs[q] = coeffs[l];
 }
 
 template <int NF>
@@ -305,8 +367,9 @@ void transpose_pass( RowLayout const &g, CscIndex const &csc,
     gather_fields_kernel<double, double, NF>
         <<<ceil_div( g.n_tgt, kBlock ), kBlock, 0, s>>>(
             ybar, ld_tgt, nf, g.order, g.n_tgt, ys );
+    (void)coeffs; // read through the CSC's column-ordered copy
     transpose_kernel<NF><<<ceil_div( g.n_src, kBlock ), kBlock, 0, s>>>(
-        csc.col_offsets.get(), csc.entries.get(), coeffs, g.kk, g.n_src, ys,
+        csc.col_offsets.get(), csc.rows.get(), csc.coefs.get(), g.n_src, ys,
         g.perm, xbar, ld_src, nf );
     DTKX_HIP( hipGetLastError() );
 }
@@ -493,12 +556,15 @@ void crs_apply_f32_dvals( RowLayout const &g, float const *coeffs,
                                                ld_tgt, n_fields, s );
 }
 
-void build_csc( RowLayout const &g, CscIndex &csc, hipStream_t s )
+void build_csc( RowLayout const &g, double const *coeffs, CscIndex &csc,
+                hipStream_t s )
 {
     int64_t const nnz = g.n_tgt * g.kk;
     DTKX_REQUIRE( nnz < INT32_MAX, DTKX_ERR_UNSUPPORTED );
     csc.col_offsets.resize( g.n_src + 1, s );
     csc.entries.resize( std::max<int64_t>( nnz, 1 ), s );
+    csc.rows.resize( std::max<int64_t>( nnz, 1 ), s );
+    csc.coefs.resize( std::max<int64_t>( nnz, 1 ), s );
     if ( nnz == 0 )
     {
         DTKX_HIP( hipMemsetAsync( csc.col_offsets.get(), 0,
@@ -524,6 +590,9 @@ void build_csc( RowLayout const &g, CscIndex &csc, hipStream_t s )
         csc.entries.get(), static_cast<int>( nnz ), 0, end_bit, s ) );
     col_offsets_kernel<<<ceil_div( nnz + 1, kBlock ), kBlock, 0, s>>>(
         keys_sorted.get(), nnz, g.n_src, csc.col_offsets.get() );
+    csc_pack_kernel<<<ceil_div( nnz, kBlock ), kBlock, 0, s>>>(
+        csc.entries.get(), coeffs, g.kk, nnz, csc.rows.get(),
+        csc.coefs.get() );
     DTKX_HIP( hipGetLastError() );
 }
 

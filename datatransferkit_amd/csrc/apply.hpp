@@ -62,9 +62,14 @@ struct CscIndex
 {
     DeviceBuffer<int32_t> col_offsets; // [n_src + 1], by source position
     DeviceBuffer<int32_t> entries;     // [nnz] storage slots, canonical order
+    // per CSC entry: its row (storage slot / kk) and its coefficient
+    DeviceBuffer<int32_t> rows;
+    DeviceBuffer<double> coefs;
 };
 
-void build_csc( RowLayout const &g, CscIndex &csc, hipStream_t s );
+// coeffs: the handle's FP64 coefficients, copied into column order
+void build_csc( RowLayout const &g, double const *coeffs, CscIndex &csc,
+                hipStream_t s );
 
 void crs_apply_transpose_f64( RowLayout const &g, CscIndex const &csc,
                               double const *coeffs, double const *ybar,

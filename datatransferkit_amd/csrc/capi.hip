@@ -604,7 +604,8 @@ int dtkx_mls_apply_transpose( const dtkx_mls *h, const double *tgt_bar,
         REQ( h->opts.precision == DTKX_PRECISION_FP64, DTKX_ERR_UNSUPPORTED );
         if ( !h->csc_built )
         {
-            dtkx::build_csc( h->layout(), h->csc, h->stream );
+            dtkx::build_csc( h->layout(), h->coeffs.get(), h->csc,
+                             h->stream );
             h->csc_built = true;
         }
         std::size_t const nin = (std::size_t)( n_fields - 1 ) * ld_tgt + h->n_tgt;
