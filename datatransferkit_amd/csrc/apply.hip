@@ -374,6 +374,20 @@ void transpose_pass( RowLayout const &g, CscIndex const &csc,
     DTKX_HIP( hipGetLastError() );
 }
 
+// out[spos[s] * pitch + f] = x[f][s], one lane per source
+__global__ void __launch_bounds__( kBlock )
+    scatter_pitched_kernel( double const *__restrict__ x, int64_t ld, int nf,
+                            int32_t const *__restrict__ spos, int64_t n,
+                            double *__restrict__ out, int pitch )
+{
+    int64_t const s = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( s >= n )
+        return;
+    double *const o = out + (int64_t)spos[s] * pitch;
+    for ( int f = 0; f < nf; ++f )
+        o[f] = x[f * ld + s];
+}
+
 __global__ void __launch_bounds__( kBlock )
     gather_pitched_kernel( double const *__restrict__ x, int64_t ld, int nf,
                            int32_t const *__restrict__ perm, int64_t n,
@@ -665,13 +679,17 @@ void nn_gather_f64( int32_t const *idx, int64_t n_tgt, double const *x,
 }
 
 void gather_fields_f64( double const *x, int64_t ld, int nf,
-                        int32_t const *perm, int64_t n, double *out, int pitch,
-                        hipStream_t s )
+                        int32_t const *perm, int32_t const *spos, int64_t n,
+                        double *out, int pitch, hipStream_t s )
 {
     if ( n == 0 || nf == 0 )
         return;
-    gather_pitched_kernel<<<ceil_div( n, kBlock ), kBlock, 0, s>>>(
-        x, ld, nf, perm, n, out, pitch );
+    if ( spos )
+        scatter_pitched_kernel<<<ceil_div( n, kBlock ), kBlock, 0, s>>>(
+            x, ld, nf, spos, n, out, pitch );
+    else
+        gather_pitched_kernel<<<ceil_div( n, kBlock ), kBlock, 0, s>>>(
+            x, ld, nf, perm, n, out, pitch );
     DTKX_HIP( hipGetLastError() );
 }
 

@@ -77,9 +77,11 @@ void crs_apply_transpose_f64( RowLayout const &g, CscIndex const &csc,
                               int n_fields, hipStream_t s );
 
 // out[p * pitch + f] = x[f][perm[p]] for f < nf (pitch >= nf), doubles.
+// With spos (perm's inverse) the lanes run over sources instead: coalesced
+// reads of x and one contiguous record written per source.
 void gather_fields_f64( double const *x, int64_t ld, int nf,
-                        int32_t const *perm, int64_t n, double *out, int pitch,
-                        hipStream_t s );
+                        int32_t const *perm, int32_t const *spos, int64_t n,
+                        double *out, int pitch, hipStream_t s );
 
 // Batched operators, one field each: rows in CSR form (row_off), columns
 // as global positions of the concatenated problem-major sources. Problem q

@@ -649,6 +649,7 @@ int dtkx_mls_target_gradient( const dtkx_mls *h, const double *src_vals,
         a.indices = h->indices.get();
         a.order = h->order.get();
         a.perm = h->perm.get();
+        a.spos = h->spos.get();
         a.pinv = h->pinv.get();
         a.n_src = h->n_src;
         a.n_tgt = h->n_tgt;

@@ -489,8 +489,8 @@ void target_gradient( GradientArgs const &a, double const *x, int64_t ld_src,
     for ( int f0 = 0; f0 < n_fields; f0 += kMaxFields )
     {
         int const nf = std::min( kMaxFields, n_fields - f0 );
-        gather_fields_f64( x + f0 * ld_src, ld_src, nf, a.perm, a.n_src,
-                           xs.get(), pitch, s );
+        gather_fields_f64( x + f0 * ld_src, ld_src, nf, a.perm, a.spos,
+                           a.n_src, xs.get(), pitch, s );
         double const *yf = ybar + f0 * a.ld_tgt;
         bool const acc = f0 > 0;
         switch ( a.m )

@@ -15,6 +15,7 @@ struct GradientArgs
     int32_t const *indices = nullptr; // [n_tgt * kk] positions, by row
     int32_t const *order = nullptr;   // row -> target (nullable)
     int32_t const *perm = nullptr;    // position -> source (nullable)
+    int32_t const *spos = nullptr;    // source -> position (nullable)
     double const *pinv = nullptr;     // stored packed A^+ per row, or null
     int64_t n_src = 0, n_tgt = 0;
     int m = 4, kk = 4, rbf = 0, truncation = 0, max_sweeps = 0;
