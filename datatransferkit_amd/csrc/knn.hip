@@ -239,8 +239,14 @@ __device__ inline void cell_box( uint64_t code, int delta, T const *scene,
 // Children of internal node i: ranges, codes, parents and child boxes.
 // A child that is a leaf gets its point as a degenerate box and the code
 // ~source_index; an internal child covering at most kTight points gets its
-// exact box, a larger one the Morton cell of its common prefix.
-constexpr int kTight = 8;
+// exact box, a larger one the Morton cell of its common prefix. Query at
+// 1M x 1M for kTight 8 / 16 / 32: k = 32 2.11 / 2.08 / 2.07 ms, k = 16
+// 1.03 / 1.01 / 1.00, 1M clustered (k = 32) 2.06 / 2.03 / 2.21 (the build
+// takes longer at 32).
+#ifndef DTKX_KNN_TIGHT
+#define DTKX_KNN_TIGHT 16
+#endif
+constexpr int kTight = DTKX_KNN_TIGHT;
 
 // Problem owning internal node i of a batch: the largest p with
 // src_off[p] - p <= i (problem p's internal nodes are
