@@ -95,3 +95,56 @@ def test_error_path_without_gpu(lib):
     rc = L.dtkx_mls_create(None, 0, None, 0, ctypes.byref(o), ctypes.byref(h))
     assert rc == _lib.ERR_EMPTY_SOURCE
     assert ", failed in capi.hip, line" in L.dtkx_last_error().decode()
+
+
+def test_spline_structs_and_defaults(tmp_path, lib):
+    """dtkx_spline_options / dtkx_spline_info: the ctypes layouts match the
+    header, and the defaults are the reference's (Wendland<0>, float
+    points, Belos tolerance 1e-10, DTK_SplineOperator_def.hpp:258)."""
+    src = tmp_path / "spl.c"
+    src.write_text(r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "dtkx/mls.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu\n", sizeof(dtkx_spline_options),
+         offsetof(dtkx_spline_options, tolerance),
+         offsetof(dtkx_spline_options, stream), sizeof(dtkx_spline_info),
+         offsetof(dtkx_spline_info, residual));
+  return 0;
+}
+""")
+    exe = tmp_path / "spl"
+    subprocess.run(["gcc", "-I", os.path.join(REPO, "include"), "-o", str(exe), str(src)],
+                   check=True)
+    a = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True,
+                                        check=True).stdout.split()]
+    from datatransferkit_amd import _lib
+
+    assert a == [ctypes.sizeof(_lib.SplineOptions), _lib.SplineOptions.tolerance.offset,
+                 _lib.SplineOptions.stream.offset, ctypes.sizeof(_lib.SplineInfo),
+                 _lib.SplineInfo.residual.offset]
+    o = _lib.default_spline_options()
+    assert (o.rbf, o.geometry, o.layout) == (0, 0, 0)
+    assert (o.restart, o.max_iterations, o.tolerance) == (300, 1000, 1e-10)
+
+
+def test_spline_error_paths_without_gpu(lib):
+    # option and size checks fail before any device call
+    from datatransferkit_amd import _lib
+
+    L = _lib.load()
+    h = ctypes.c_void_p()
+    o = _lib.default_spline_options()
+    o.restart = 0
+    assert L.dtkx_spline_create(None, 0, None, 0, ctypes.byref(o),
+                                ctypes.byref(h)) == _lib.ERR_INVALID_ARGUMENT
+    o = _lib.default_spline_options()
+    o.tolerance = float("nan")
+    assert L.dtkx_spline_create(None, 0, None, 0, ctypes.byref(o),
+                                ctypes.byref(h)) == _lib.ERR_INVALID_ARGUMENT
+    o = _lib.default_spline_options()
+    assert L.dtkx_spline_create(None, 0, None, 0, ctypes.byref(o),
+                                ctypes.byref(h)) == _lib.ERR_EMPTY_SOURCE
+    assert "n_src > 0" in L.dtkx_last_error().decode()
+    assert L.dtkx_spline_apply(None, None, None, 1, 0, 0) == _lib.ERR_INVALID_ARGUMENT
