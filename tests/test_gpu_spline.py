@@ -182,6 +182,12 @@ def test_corner_cases(dtk, spl):
     y = _apply(op, np.array([255.0, 255.0]), 2)
     np.testing.assert_allclose(y, spl.SplineOracle(src, tgt).apply(np.array([255.0, 255.0])),
                                rtol=1e-9)
+    # one source: kk = 1, V_s of rank 1, the minimum-norm solution
+    one = np.array([[0.3, 0.4, 0.5]])
+    y = _apply(op := dtk.SplineOperator(cuda(one), cuda(tgt)), np.array([7.0]), 2)
+    assert op.info()["converged"]
+    np.testing.assert_allclose(y, spl.SplineOracle(one, tgt).apply(np.array([7.0])),
+                               rtol=1e-9)
     op = dtk.SplineOperator(cuda(src), cuda(np.zeros((0, 3))))
     op.apply(cuda(np.array([1.0, 2.0])), torch.zeros(0, dtype=torch.float64, device="cuda"))
     with pytest.raises(dtk.DtkxError):
