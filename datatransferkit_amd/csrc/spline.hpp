@@ -83,7 +83,7 @@ struct GmresOptions
 struct GmresInfo
 {
     int iterations = 0;
-    int reorthogonalized = 0; // iterations with a second Gram-Schmidt pass
+    int reorthogonalized = 0; // basis vectors reorthogonalised
     int restarts = 0;
     double residual = 0.0; // explicit relative residual of the result
     bool converged = false;

@@ -365,7 +365,8 @@ typedef struct dtkx_spline_options {
 typedef struct dtkx_spline_info {
   int32_t iterations;
   int32_t converged;
-  int32_t reorthogonalized; /* iterations that took a second Gram-Schmidt pass */
+  int32_t reorthogonalized; /* basis vectors reorthogonalised (the delayed
+                             * classical Gram-Schmidt does every one) */
   int32_t restarts;         /* GMRES cycles after the first */
   double residual;
 } dtkx_spline_info;
