@@ -617,6 +617,14 @@ constexpr int kMaxDepth = 97;
 #define DTKX_KNN_FUSED_SORT_K 32
 #endif
 constexpr int kFusedSortK = DTKX_KNN_FUSED_SORT_K;
+// collect the rows of K > kFusedSortK in LDS too (an extra array beside
+// the dead stack slots), written out in one burst per row. Off: the extra
+// LDS costs more occupancy than the scattered row stores cost traffic
+// (1M x 1M query, k = 64: 5.30 against 4.09 ms; k = 48: 3.83 against 3.22).
+#ifndef DTKX_KNN_BIG_ROW_LDS
+#define DTKX_KNN_BIG_ROW_LDS 0
+#endif
+constexpr bool kBigRowLds = DTKX_KNN_BIG_ROW_LDS != 0;
 
 // launch mode of the separate row sort
 constexpr int kSortRows = -1;
@@ -1167,16 +1175,27 @@ __global__ void __launch_bounds__( kQueryBlock )
     // over the traversal, would leave partly written lines for the L2 to
     // evict (8 times the row's bytes, measured).
     constexpr int kDistSlots = STACK * (int)( sizeof( T ) / 4 );
-    constexpr bool kRowLds = STACK > kPass2Stack && K <= kFusedSortK &&
-                             K <= kDistSlots + STACK - kPass2Stack;
+    constexpr int kDeadSlots = kDistSlots + STACK - kPass2Stack;
+    // rows of K > kFusedSortK: the slots beyond the dead ones in an LDS
+    // array of their own, the row copied out at the end
+    constexpr bool kBigRow = kBigRowLds && STACK > kPass2Stack &&
+                             K > kFusedSortK;
+    constexpr bool kRowLds = STACK > kPass2Stack &&
+                             ( kBigRow || ( K <= kFusedSortK && K <= kDeadSlots ) );
+    constexpr int kExtra = kBigRow && K > kDeadSlots ? K - kDeadSlots : 1;
+    __shared__ int32_t st_extra[kBigRow ? kExtra * kQueryBlock : 1];
     constexpr int kStack2 = kRowLds ? kPass2Stack : STACK;
     int32_t *const ids_d = reinterpret_cast<int32_t *>( st_d );
     auto slot = [&]( int j ) -> int32_t & {
         if constexpr ( kRowLds )
-            return j < kDistSlots
-                       ? ids_d[j * kQueryBlock + threadIdx.x]
-                       : st_node[( kPass2Stack + j - kDistSlots ) * kQueryBlock +
-                                 threadIdx.x];
+        {
+            if ( j < kDistSlots )
+                return ids_d[j * kQueryBlock + threadIdx.x];
+            if ( !kBigRow || j < kDeadSlots )
+                return st_node[( kPass2Stack + j - kDistSlots ) * kQueryBlock +
+                               threadIdx.x];
+            return st_extra[( j - kDeadSlots ) * kQueryBlock + threadIdx.x];
+        }
         else
             return row[j];
     };
@@ -1257,6 +1276,9 @@ __global__ void __launch_bounds__( kQueryBlock )
     }
     if constexpr ( K <= kFusedSortK )
         sort_row<T, K>( a, q, r, base, kk, row, slot );
+    else if constexpr ( kBigRow )
+        for ( int j = 0; j < kk; ++j )
+            row[j] = slot( j );
 }
 
 // The row sort of a two-pass launch for K > kFusedSortK, whose sorting
