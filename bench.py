@@ -44,15 +44,17 @@ FORMS = {
     "fp64_cyclic": dict(precision="fp64", solver="cyclic"),
     "fp64_cholesky": dict(precision="fp64", solver="cholesky"),
 }
+BASIS_SIZE = {"constant": 1, "linear": 4, "quadratic": 10}
+
 SETUP_KERNEL = {
-    "fp32_bulk": "setup_cholesky_kernel<10,true,false> (fused MLS setup, FP32 bulk + "
+    "fp32_bulk": "setup_cholesky_kernel<{m},true,false> (fused MLS setup, FP32 bulk + "
                  "certified FP64 Cholesky)",
-    "fp32_bulk_cyclic": "setup_cyclic_kernel<10,2,true,false> (fused MLS setup, FP32 bulk "
+    "fp32_bulk_cyclic": "setup_cyclic_kernel<{m},2,true,false> (fused MLS setup, FP32 bulk "
                         "+ FP64 cyclic Jacobi)",
-    "fp64": "setup_classical_kernel<10,G> (fused MLS setup, FP64 classical Jacobi)",
-    "fp64_cyclic": "setup_cyclic_kernel<10,2,false,false> (fused MLS setup, FP64 cyclic "
+    "fp64": "setup_classical_kernel<{m},G> (fused MLS setup, FP64 classical Jacobi)",
+    "fp64_cyclic": "setup_cyclic_kernel<{m},2,false,false> (fused MLS setup, FP64 cyclic "
                    "Jacobi)",
-    "fp64_cholesky": "setup_cholesky_kernel<10,false,false> (fused MLS setup, certified "
+    "fp64_cholesky": "setup_cholesky_kernel<{m},false,false> (fused MLS setup, certified "
                      "FP64 Cholesky)",
 }
 
@@ -176,10 +178,11 @@ def kernel_rooflines(form, r, n, m, kk, nnz, fields):
                                     f"per launch ({n} queries)",
         },
         "setup": {
-            "bound": "mfma", "kernel": SETUP_KERNEL[form],
+            "bound": "mfma", "kernel": SETUP_KERNEL[form].replace("{m}", str(m)),
             "ms": r["ms_setup"], "achieved": round(setup, 3), "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": round(setup / FP64_PEAK_TFLOPS, 5),
-            "traffic": load_traffic(f"{SETUP_KERNEL[form].split(' ')[0]}@{n}x{n},k{kk}"),
+            "traffic": load_traffic(
+                f"{SETUP_KERNEL[form].replace('{m}', str(m)).split(' ')[0]}@{n}x{n},k{kk}"),
             "algorithmic_per_unit": (f"F_cholesky(m={m},k={kk}) = {f_cholesky(m, kk)}"
                                      if solver == "cholesky" else
                                      f"F_setup(m={m},k={kk}) = {f_setup(m, kk)}")
@@ -208,7 +211,7 @@ def run_form(dtk, torch, name, d_src, d_tgt, d_vals, n_fields, args, stream):
     vdt = torch.float32 if form["precision"] == "fp32_bulk" else torch.float64
     x = d_vals.to(vdt)
     y = torch.zeros((n_fields, d_tgt.shape[0]), dtype=vdt, device=dev)
-    kw = dict(basis="quadratic", rbf=args.rbf, k=args.k, stream=stream, **form)
+    kw = dict(basis=args.basis, rbf=args.rbf, k=args.k, stream=stream, **form)
 
     def step():
         op = dtk.MovingLeastSquaresOperator(d_src, d_tgt, **kw)
@@ -705,6 +708,8 @@ def main():
     ap.add_argument("--k", type=int, default=32)
     ap.add_argument("--fields", type=int, default=4)
     ap.add_argument("--rbf", default="wendland2")
+    ap.add_argument("--basis", default="quadratic", choices=sorted(BASIS_SIZE),
+                    help="polynomial basis of the transfer workload")
     ap.add_argument("--form", default="fp32_bulk", choices=sorted(FORMS))
     ap.add_argument("--compare", default="fp64,fp64_cholesky,fp64_cyclic,fp32_bulk_cyclic",
                     help="comma-separated forms timed beside the headline form ('' for none)")
@@ -753,7 +758,7 @@ def main():
     import datatransferkit_amd as dtk
 
     dev = torch.device("cuda:0")
-    m = 10
+    m = BASIS_SIZE[args.basis]
     n = args.n
     src, tgt, vals = make_inputs(n, n, args.fields)
     d_src = torch.from_numpy(src).to(dev)
@@ -789,7 +794,9 @@ def main():
     roofline = dict(roofs[dominant])
     roofline["stage"] = dominant
     out = {
-        "metric": "MLS setup+apply target pts/s (1M pts, quad basis): plain FP64 vs batched/FP32",
+        "metric": "MLS setup+apply target pts/s (1M pts, quad basis): plain FP64 vs batched/FP32"
+                  if args.basis == "quadratic" else
+                  f"MLS setup+apply target pts/s ({args.basis} basis)",
         "value": head["value"],
         "unit": "target pts/s",
         "n_gpus": 1,
@@ -803,9 +810,9 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"MLS transfer {n} sources U[0,1]^3 -> {n} targets U[0.05,0.95]^3, "
-                        f"quadratic basis, k={kk}, {args.rbf}, {args.fields} fields, "
+                        f"{args.basis} basis, k={kk}, {args.rbf}, {args.fields} fields, "
                         f"form {args.form}",
-            "n_src": n, "n_tgt": n, "basis": "quadratic", "k": kk, "rbf": args.rbf,
+            "n_src": n, "n_tgt": n, "basis": args.basis, "k": kk, "rbf": args.rbf,
             "fields": args.fields, "form": args.form,
         },
         "forms": results,
