@@ -111,7 +111,7 @@ __global__ void __launch_bounds__( kBlock )
                      C const *__restrict__ coeffs, int64_t n_tgt, int kk,
                      Pack<XS, NF> const *__restrict__ xs,
                      int32_t const *__restrict__ order, X *__restrict__ y,
-                     int64_t ld_tgt, int nf )
+                     int64_t ld_tgt, int nf, Pack<X, NF> *__restrict__ yrows )
 {
 #pragma clang fp contract( off )
     using Sum = typename SumOf<Acc>::type;
@@ -157,11 +157,44 @@ __global__ void __launch_bounds__( kBlock )
     if ( lane >= nrows )
         return;
     int64_t const r = r0 + lane;
+    if ( yrows )
+    {
+        // row order: one contiguous record per row
+        Pack<X, NF> v;
+#pragma unroll
+        for ( int f = 0; f < NF; ++f )
+            v.v[f] = (X)acc[f].value();
+        yrows[r] = v;
+        return;
+    }
     int64_t const t = order ? order[r] : r;
 #pragma unroll
     for ( int f = 0; f < NF; ++f )
         if ( f < nf )
             y[f * ld_tgt + t] = (X)acc[f].value();
+}
+
+// y[f][i] = rec[at[i]].v[f]: results stored one record per row (or
+// column) moved to the caller's order with the lanes over the caller's
+// items, so the field stores are coalesced and the random accesses are
+// whole-record reads. Stored straight from the rows, targets in random
+// caller order scatter one value per field and row: at 1M x 1M (k = 32,
+// 4 float fields) the apply took 0.211 ms against 0.142 ms for the same
+// targets given in Morton order, and 0.163 ms staged.
+template <typename X, int NF>
+__global__ void __launch_bounds__( kBlock )
+    records_to_caller_kernel( Pack<X, NF> const *__restrict__ rec,
+                              int32_t const *__restrict__ at, int64_t n,
+                              X *__restrict__ y, int64_t ld, int nf )
+{
+    int64_t const i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if ( i >= n )
+        return;
+    Pack<X, NF> const v = rec[at[i]];
+#pragma unroll
+    for ( int f = 0; f < NF; ++f )
+        if ( f < nf )
+            y[f * ld + i] = v.v[f];
 }
 
 // fields processed per pass: 1, 2, 4 or 8 (the last pass zero-padded)
@@ -173,7 +206,7 @@ inline int pass_width( int remaining )
 template <typename C, typename X, typename XS, typename Acc, int NF>
 void apply_pass( RowLayout const &g, C const *coeffs, X const *x,
                  int64_t ld_src, X *y, int64_t ld_tgt, int nf, XS *scratch,
-                 hipStream_t s )
+                 X *yscratch, hipStream_t s )
 {
     auto *xs = reinterpret_cast<Pack<XS, NF> *>( scratch );
     if ( g.spos )
@@ -184,9 +217,14 @@ void apply_pass( RowLayout const &g, C const *coeffs, X const *x,
         gather_fields_kernel<X, XS, NF>
             <<<ceil_div( g.n_src, kBlock ), kBlock, 0, s>>>(
                 x, ld_src, nf, g.perm, g.n_src, xs );
+    auto *yrows = reinterpret_cast<Pack<X, NF> *>( yscratch );
     crs_rows_kernel<C, XS, X, Acc, NF>
         <<<ceil_div( g.n_tgt, kBlock ), kBlock, 0, s>>>(
-            g.idx, coeffs, g.n_tgt, g.kk, xs, g.order, y, ld_tgt, nf );
+            g.idx, coeffs, g.n_tgt, g.kk, xs, g.order, y, ld_tgt, nf, yrows );
+    if ( yrows )
+        records_to_caller_kernel<X, NF>
+            <<<ceil_div( g.n_tgt, kBlock ), kBlock, 0, s>>>(
+                yrows, g.trow, g.n_tgt, y, ld_tgt, nf );
     DTKX_HIP( hipGetLastError() );
 }
 
@@ -199,6 +237,10 @@ void launch_apply( RowLayout const &g, C const *coeffs, X const *x,
         return;
     DeviceBuffer<XS> scratch(
         (std::size_t)g.n_src * pass_width( n_fields ), s );
+    // rows in Morton order: results staged by row, then moved to targets
+    bool const staged = g.order && g.trow;
+    DeviceBuffer<X> yscratch(
+        staged ? (std::size_t)g.n_tgt * pass_width( n_fields ) : 0, s );
     for ( int f0 = 0; f0 < n_fields; )
     {
         int const w = pass_width( n_fields - f0 );
@@ -209,19 +251,19 @@ void launch_apply( RowLayout const &g, C const *coeffs, X const *x,
         {
         case 1:
             apply_pass<C, X, XS, Acc, 1>( g, coeffs, xf, ld_src, yf, ld_tgt,
-                                          nf, scratch.get(), s );
+                                          nf, scratch.get(), yscratch.get(), s );
             break;
         case 2:
             apply_pass<C, X, XS, Acc, 2>( g, coeffs, xf, ld_src, yf, ld_tgt,
-                                          nf, scratch.get(), s );
+                                          nf, scratch.get(), yscratch.get(), s );
             break;
         case 4:
             apply_pass<C, X, XS, Acc, 4>( g, coeffs, xf, ld_src, yf, ld_tgt,
-                                          nf, scratch.get(), s );
+                                          nf, scratch.get(), yscratch.get(), s );
             break;
         default:
             apply_pass<C, X, XS, Acc, 8>( g, coeffs, xf, ld_src, yf, ld_tgt,
-                                          nf, scratch.get(), s );
+                                          nf, scratch.get(), yscratch.get(), s );
         }
         f0 += nf;
     }
@@ -273,7 +315,8 @@ __global__ void __launch_bounds__( kBlock )
                       double const *__restrict__ coefs, int64_t n_src,
                       Pack<double, NF> const *__restrict__ ys,
                       int32_t const *__restrict__ perm,
-                      double *__restrict__ xbar, int64_t ld_src, int nf )
+                      double *__restrict__ xbar, int64_t ld_src, int nf,
+                      Pack<double, NF> *__restrict__ xrows )
 {
 #pragma clang fp contract( off )
     constexpr int kWaves = kBlock / 64;
@@ -336,6 +379,16 @@ __global__ void __launch_bounds__( kBlock )
     }
     if ( lane >= ncols )
         return;
+    if ( xrows )
+    {
+        // position order: one contiguous record per column
+        Pack<double, NF> v;
+#pragma unroll
+        for ( int f = 0; f < NF; ++f )
+            v.v[f] = acc[f];
+        xrows[p0 + lane] = v;
+        return;
+    }
     int64_t const s = perm ? perm[p0 + lane] : p0 + lane;
 #pragma unroll
     for ( int f = 0; f < NF; ++f )
@@ -361,16 +414,27 @@ template <int NF>
 void transpose_pass( RowLayout const &g, CscIndex const &csc,
                      double const *coeffs, double const *ybar, int64_t ld_tgt,
                      double *xbar, int64_t ld_src, int nf, double *scratch,
-                     hipStream_t s )
+                     double *xscratch, hipStream_t s )
 {
     auto *ys = reinterpret_cast<Pack<double, NF> *>( scratch );
-    gather_fields_kernel<double, double, NF>
-        <<<ceil_div( g.n_tgt, kBlock ), kBlock, 0, s>>>(
-            ybar, ld_tgt, nf, g.order, g.n_tgt, ys );
+    if ( g.order && g.trow )
+        // lanes over targets: coalesced field reads, whole-record stores
+        scatter_fields_kernel<double, double, NF>
+            <<<ceil_div( g.n_tgt, kBlock ), kBlock, 0, s>>>(
+                ybar, ld_tgt, nf, g.trow, g.n_tgt, ys );
+    else
+        gather_fields_kernel<double, double, NF>
+            <<<ceil_div( g.n_tgt, kBlock ), kBlock, 0, s>>>(
+                ybar, ld_tgt, nf, g.order, g.n_tgt, ys );
     (void)coeffs; // read through the CSC's column-ordered copy
+    auto *xrows = reinterpret_cast<Pack<double, NF> *>( xscratch );
     transpose_kernel<NF><<<ceil_div( g.n_src, kBlock ), kBlock, 0, s>>>(
         csc.col_offsets.get(), csc.rows.get(), csc.coefs.get(), g.n_src, ys,
-        g.perm, xbar, ld_src, nf );
+        g.perm, xbar, ld_src, nf, xrows );
+    if ( xrows )
+        records_to_caller_kernel<double, NF>
+            <<<ceil_div( g.n_src, kBlock ), kBlock, 0, s>>>(
+                xrows, g.spos, g.n_src, xbar, ld_src, nf );
     DTKX_HIP( hipGetLastError() );
 }
 
@@ -621,6 +685,11 @@ void crs_apply_transpose_f64( RowLayout const &g, CscIndex const &csc,
         (std::size_t)std::max<int64_t>( g.n_tgt, 1 ) *
             pass_width( n_fields ),
         s );
+    // columns in Morton position order: results staged by position, then
+    // moved to sources
+    bool const staged = g.perm && g.spos;
+    DeviceBuffer<double> xscratch(
+        staged ? (std::size_t)g.n_src * pass_width( n_fields ) : 0, s );
     for ( int f0 = 0; f0 < n_fields; )
     {
         int const w = pass_width( n_fields - f0 );
@@ -631,19 +700,19 @@ void crs_apply_transpose_f64( RowLayout const &g, CscIndex const &csc,
         {
         case 1:
             transpose_pass<1>( g, csc, coeffs, yf, ld_tgt, xf, ld_src, nf,
-                               scratch.get(), s );
+                               scratch.get(), xscratch.get(), s );
             break;
         case 2:
             transpose_pass<2>( g, csc, coeffs, yf, ld_tgt, xf, ld_src, nf,
-                               scratch.get(), s );
+                               scratch.get(), xscratch.get(), s );
             break;
         case 4:
             transpose_pass<4>( g, csc, coeffs, yf, ld_tgt, xf, ld_src, nf,
-                               scratch.get(), s );
+                               scratch.get(), xscratch.get(), s );
             break;
         default:
             transpose_pass<8>( g, csc, coeffs, yf, ld_tgt, xf, ld_src, nf,
-                               scratch.get(), s );
+                               scratch.get(), xscratch.get(), s );
         }
         f0 += nf;
     }

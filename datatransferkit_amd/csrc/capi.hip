@@ -259,6 +259,11 @@ struct StageEvents
     }
 };
 
+// Create counters: the setup's four (underdetermined, unconverged,
+// rotations, eigen fallbacks) and the search's re-run count, read back
+// together at the end of the create.
+constexpr int kCounters = 5;
+
 // Shared tail of the two create paths: neighbours are in h->indices.
 void run_setup( dtkx_mls *h, unsigned long long *counters_d )
 {
@@ -305,7 +310,7 @@ void finish_create( dtkx_mls *h, unsigned long long *counters_d )
     offsets_kernel<<<dtkx::ceil_div( h->n_tgt + 1, 256 ), 256, 0,
                      h->stream>>>( h->offsets.get(), h->n_tgt, h->kk );
     DTKX_HIP( hipGetLastError() );
-    unsigned long long c[4] = {0, 0, 0, 0};
+    unsigned long long c[kCounters] = {0, 0, 0, 0, 0};
     DTKX_HIP( hipMemcpyAsync( c, counters_d, sizeof( c ),
                               hipMemcpyDeviceToHost, h->stream ) );
     DTKX_HIP( hipStreamSynchronize( h->stream ) );
@@ -318,6 +323,8 @@ void finish_create( dtkx_mls *h, unsigned long long *counters_d )
     h->stats.n_unconverged = (int64_t)c[1];
     h->stats.total_rotations = (int64_t)c[2];
     h->stats.n_eigen_fallback = (int64_t)c[3];
+    // an int32 count in the low half of a zeroed slot
+    h->stats.n_stack_overflow = (int64_t)( c[4] & 0xffffffffull );
 }
 
 // coefficient storage of the handle's precision only
@@ -417,14 +424,17 @@ int dtkx_mls_create( const double *src_xyz, int64_t n_src,
         *out = nullptr;
         auto h = make_handle( src_xyz, n_src, tgt_xyz, n_tgt, opts );
         StageEvents ev( h->stream );
-        dtkx::DeviceBuffer<unsigned long long> counters( 4, h->stream );
-        DTKX_HIP( hipMemsetAsync( counters.get(), 0, 4 * sizeof( uint64_t ),
-                                  h->stream ) );
+        dtkx::DeviceBuffer<unsigned long long> counters( kCounters,
+                                                         h->stream );
+        DTKX_HIP( hipMemsetAsync( counters.get(), 0,
+                                  kCounters * sizeof( uint64_t ), h->stream ) );
         if ( n_tgt > 0 )
         {
             ev.mark( 0 );
             dtkx::KnnInfo info;
             ev.pass_query_events( info );
+            info.n_overflow_dev =
+                reinterpret_cast<int32_t *>( counters.get() + 4 );
             int const k = opts->k > 0 ? opts->k : h->m;
             dtkx::KnnOutput kout;
             h->perm.resize( n_src, h->stream );
@@ -468,7 +478,6 @@ int dtkx_mls_create( const double *src_xyz, int64_t n_src,
             dtkx::invert_permutation( h->order.get(), n_tgt, h->trow.get(),
                                       h->stream );
             ev.mark( 1 );
-            h->stats.n_stack_overflow = info.n_overflow;
             run_setup( h.get(), counters.get() );
             h->row_max = {};
             ev.mark( 2 );
@@ -517,9 +526,10 @@ int dtkx_mls_create_with_neighbors( const double *src_xyz, int64_t n_src,
             DTKX_HIP( hipStreamSynchronize( h->stream ) );
         }
         StageEvents ev( h->stream );
-        dtkx::DeviceBuffer<unsigned long long> counters( 4, h->stream );
-        DTKX_HIP( hipMemsetAsync( counters.get(), 0, 4 * sizeof( uint64_t ),
-                                  h->stream ) );
+        dtkx::DeviceBuffer<unsigned long long> counters( kCounters,
+                                                         h->stream );
+        DTKX_HIP( hipMemsetAsync( counters.get(), 0,
+                                  kCounters * sizeof( uint64_t ), h->stream ) );
         if ( n_tgt > 0 )
         {
             ev.mark( 1 );

@@ -677,8 +677,9 @@ struct QueryArgs
     // are listed here (and written by the re-run) instead of finished
     int32_t *overflow_rows = nullptr;
     int32_t *n_overflow = nullptr;
-    // global-stack re-run: rows rows[i] for i < *n_rows_dev, a stack of
-    // kMaxDepth entries per lane laid out [level][lane] (stride gstride)
+    // global-stack re-run: rows rows[i] for i < *n_rows_dev, taken by the
+    // grid's n_rows lanes in turn, a stack of kMaxDepth entries per lane
+    // laid out [level][lane] (stride gstride)
     int32_t const *rows = nullptr;
     int32_t const *n_rows_dev = nullptr;
     int2 *gstack = nullptr;
@@ -689,27 +690,12 @@ struct QueryArgs
 // (a query that would need more is listed for the re-run); STACK == 0:
 // per-lane stack in global memory (gstack, laid out [level][lane]) deep
 // enough for any tree.
+// One row of a one-pass search: lane qi (its global-stack slot), row r;
+// st_node and st_d are the block's LDS stack.
 template <typename T, int K, int STACK>
-__global__ void __launch_bounds__( kQueryBlock )
-    knn_query_kernel( QueryArgs<T> const a )
+__device__ inline void query_row( QueryArgs<T> const &a, int64_t const qi,
+                                 int64_t const r, int32_t *st_node, T *st_d )
 {
-    constexpr int LS = STACK > 0 ? STACK : 1;
-    __shared__ int32_t st_node[LS * kQueryBlock];
-    __shared__ T st_d[LS * kQueryBlock];
-    int64_t const qi = xcd_tile() * kQueryBlock + threadIdx.x;
-    int64_t r;
-    if ( a.rows )
-    {
-        if ( qi >= *a.n_rows_dev )
-            return;
-        r = a.rows[qi];
-    }
-    else
-    {
-        if ( qi >= a.n_rows )
-            return;
-        r = a.row0 + qi;
-    }
     int32_t const t = a.order[r];
     int64_t so = 0, np = a.n_src;
     int32_t root = 0;
@@ -838,6 +824,30 @@ __global__ void __launch_bounds__( kQueryBlock )
             if ( a.row_max && j == kk - 1 )
                 a.row_max[r] = (float)best.distance( j );
         }
+}
+
+template <typename T, int K, int STACK>
+__global__ void __launch_bounds__( kQueryBlock )
+    knn_query_kernel( QueryArgs<T> const a )
+{
+    constexpr int LS = STACK > 0 ? STACK : 1;
+    __shared__ int32_t st_node[LS * kQueryBlock];
+    __shared__ T st_d[LS * kQueryBlock];
+    int64_t const qi = xcd_tile() * kQueryBlock + threadIdx.x;
+    if constexpr ( STACK == 0 )
+        if ( a.rows )
+        {
+            // re-run of listed rows: the grid's n_rows lanes stride over
+            // the list, whose length only the device knows
+            int64_t const n = *a.n_rows_dev;
+            for ( int64_t i = qi; i < n; i += a.n_rows )
+                query_row<T, K, STACK>( a, qi, a.rows[i], st_node, st_d );
+            return;
+        }
+    // LDS-stack launches run rows [row0, row0 + n_rows), one per lane
+    if ( qi >= a.n_rows )
+        return;
+    query_row<T, K, STACK>( a, qi, a.row0 + qi, st_node, st_d );
 }
 
 // ------------------------------------------------- two-pass k-best search
@@ -1018,28 +1028,13 @@ __device__ inline void sort_row( QueryArgs<T> const &a, T const *q,
         }
 }
 
+// One row of a two-pass search: lane qi (its global-stack slot), row r;
+// st_node and st_d are the block's LDS stack.
 template <typename T, int K, int STACK>
-__global__ void __launch_bounds__( kQueryBlock )
-    knn_query2_kernel( QueryArgs<T> const a )
+__device__ inline void query2_row( QueryArgs<T> const &a, int64_t const qi,
+                                  int64_t const r, int32_t *st_node, T *st_d )
 {
     using SK = SortKey<T>;
-    constexpr int LS = STACK > 0 ? STACK : 1;
-    __shared__ int32_t st_node[LS * kQueryBlock];
-    __shared__ T st_d[LS * kQueryBlock];
-    int64_t const qi = xcd_tile() * kQueryBlock + threadIdx.x;
-    int64_t r;
-    if ( a.rows )
-    {
-        if ( qi >= *a.n_rows_dev )
-            return;
-        r = a.rows[qi];
-    }
-    else
-    {
-        if ( qi >= a.n_rows )
-            return;
-        r = a.row0 + qi;
-    }
     int32_t const t = a.order[r];
     int64_t so = 0, np = a.n_src;
     int32_t root = 0;
@@ -1281,6 +1276,30 @@ __global__ void __launch_bounds__( kQueryBlock )
             row[j] = slot( j );
 }
 
+template <typename T, int K, int STACK>
+__global__ void __launch_bounds__( kQueryBlock )
+    knn_query2_kernel( QueryArgs<T> const a )
+{
+    constexpr int LS = STACK > 0 ? STACK : 1;
+    __shared__ int32_t st_node[LS * kQueryBlock];
+    __shared__ T st_d[LS * kQueryBlock];
+    int64_t const qi = xcd_tile() * kQueryBlock + threadIdx.x;
+    if constexpr ( STACK == 0 )
+        if ( a.rows )
+        {
+            // re-run of listed rows: the grid's n_rows lanes stride over
+            // the list, whose length only the device knows
+            int64_t const n = *a.n_rows_dev;
+            for ( int64_t i = qi; i < n; i += a.n_rows )
+                query2_row<T, K, STACK>( a, qi, a.rows[i], st_node, st_d );
+            return;
+        }
+    // LDS-stack launches run rows [row0, row0 + n_rows), one per lane
+    if ( qi >= a.n_rows )
+        return;
+    query2_row<T, K, STACK>( a, qi, a.row0 + qi, st_node, st_d );
+}
+
 // The row sort of a two-pass launch for K > kFusedSortK, whose sorting
 // network would take the traversal's registers.
 template <typename T, int K>
@@ -1362,6 +1381,16 @@ inline int query_bucket( int k )
     return 64;
 }
 
+// blocks of the global-stack re-run (16k lanes, 12.7 MB of stack)
+constexpr int64_t kRerunBlocks = 256;
+
+// *total += *count, on one lane
+__global__ void add_count_kernel( int32_t const *count, int32_t *total )
+{
+    if ( threadIdx.x == 0 )
+        atomicAdd( total, *count );
+}
+
 // The query launch with the LDS stack, then the re-run of its overflow.
 template <typename T>
 void run_queries( int kmax, QueryArgs<T> qa, int64_t n_rows, KnnInfo *info,
@@ -1374,25 +1403,33 @@ void run_queries( int kmax, QueryArgs<T> qa, int64_t n_rows, KnnInfo *info,
     qa.overflow_rows = rows.get();
     qa.n_overflow = count.get();
     QueryLauncher<T>::dispatch( kmax, kLdsStack, qa, s );
-    int32_t n_over = 0;
-    DTKX_HIP( hipMemcpyAsync( &n_over, count.get(), sizeof( int32_t ),
-                              hipMemcpyDeviceToHost, s ) );
-    DTKX_HIP( hipStreamSynchronize( s ) );
-    if ( info )
-        info->n_overflow += n_over;
-    if ( n_over > 0 )
+    // The re-run is launched without reading the list's length back (a
+    // stream synchronisation, 60 us of idle GPU at 1M x 1M): a bounded grid
+    // whose lanes stride over the list, each with its own global stack.
+    // Almost always the list is empty and the launch costs a few us.
+    int64_t const lanes =
+        std::min<int64_t>( ceil_div( n_rows, kQueryBlock ), kRerunBlocks ) *
+        kQueryBlock;
+    DeviceBuffer<int2> gstack( static_cast<std::size_t>( kMaxDepth ) * lanes,
+                               s );
+    QueryArgs<T> ra = qa;
+    ra.overflow_rows = nullptr;
+    ra.n_overflow = nullptr;
+    ra.rows = rows.get();
+    ra.n_rows_dev = count.get();
+    ra.n_rows = lanes;
+    ra.gstack = gstack.get();
+    ra.gstride = lanes;
+    QueryLauncher<T>::dispatch( kmax, 0, ra, s );
+    if ( info && info->n_overflow_dev )
+        add_count_kernel<<<1, 64, 0, s>>>( count.get(), info->n_overflow_dev );
+    else if ( info )
     {
-        DeviceBuffer<int2> gstack(
-            static_cast<std::size_t>( kMaxDepth ) * n_over, s );
-        QueryArgs<T> ra = qa;
-        ra.overflow_rows = nullptr;
-        ra.n_overflow = nullptr;
-        ra.rows = rows.get();
-        ra.n_rows_dev = count.get();
-        ra.n_rows = n_over;
-        ra.gstack = gstack.get();
-        ra.gstride = n_over;
-        QueryLauncher<T>::dispatch( kmax, 0, ra, s );
+        int32_t n_over = 0;
+        DTKX_HIP( hipMemcpyAsync( &n_over, count.get(), sizeof( int32_t ),
+                                  hipMemcpyDeviceToHost, s ) );
+        DTKX_HIP( hipStreamSynchronize( s ) );
+        info->n_overflow += n_over;
     }
     // two-pass rows of large K are written unordered: sort them (a no-op
     // launch mode for every other K)
@@ -1463,9 +1500,11 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     DeviceBuffer<int32_t> tids( n_tgt, s );
     std::size_t const ttemp_bytes = sort_pairs_bytes<uint32_t>( n_tgt, 30, s );
     DeviceBuffer<char> ttemp( ttemp_bytes, s );
-    // the side stream waits for the box only; its work is queued after the
-    // sources' chain below, since queueing (a few microseconds of host time
-    // per launch) paces this build and the sources' chain is the longer
+    // The side stream waits for the box only; its work is queued after the
+    // sources' chain below. Queueing (a few microseconds of host time per
+    // launch) paces this build, so the order only decides which chain's
+    // tail runs unqueued: queued first, the targets' chain made the search
+    // 40 us slower at 1M x 1M.
     Fork fork( s );
 
     // Morton codes and sort: sources

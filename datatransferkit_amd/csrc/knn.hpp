@@ -20,6 +20,9 @@ namespace dtkx
 struct KnnInfo
 {
     int n_overflow = 0; // queries re-run with the global traversal stack
+    // (optional) device counter the re-run count is added to instead,
+    // leaving n_overflow alone: the search then never waits on the stream
+    int32_t *n_overflow_dev = nullptr;
     // (optional) recorded on the stream around the query kernels
     hipEvent_t query_begin = nullptr, query_end = nullptr;
 };
