@@ -39,6 +39,8 @@ struct dtkx_mls
     dtkx::DeviceBuffer<float> coeffs32;
     // src_d in position order, tgt_d in caller order
     dtkx::DeviceBuffer<double4> src_d, tgt_d;
+    // create only, float geometry: the points rounded to float (caller order)
+    dtkx::DeviceBuffer<float4> src_f, tgt_f;
     mutable dtkx::CscIndex csc;
     mutable bool csc_built = false;
     // the graph in caller order, built on the first dtkx_mls_get_graph
@@ -342,9 +344,11 @@ void alloc_coeffs( dtkx_mls *h, int64_t nnz )
     }
 }
 
+// float_copies: also the float points the search of float geometry needs
 std::unique_ptr<dtkx_mls> make_handle( double const *src, int64_t n_src,
                                        double const *tgt, int64_t n_tgt,
-                                       dtkx_mls_options const *opts )
+                                       dtkx_mls_options const *opts,
+                                       bool float_copies = false )
 {
     REQ( opts != nullptr, DTKX_ERR_INVALID_ARGUMENT );
     validate_options( *opts );
@@ -366,10 +370,15 @@ std::unique_ptr<dtkx_mls> make_handle( double const *src, int64_t n_src,
     Staged<double> s_tgt( tgt, 3 * n_tgt, h->stream );
     h->src_d.resize( n_src, h->stream );
     h->tgt_d.resize( std::max<int64_t>( n_tgt, 1 ), h->stream );
+    if ( float_copies )
+    {
+        h->src_f.resize( n_src, h->stream );
+        h->tgt_f.resize( std::max<int64_t>( n_tgt, 1 ), h->stream );
+    }
     dtkx::points_to_double4( s_src.ptr, n_src, opts->layout, h->src_d.get(),
-                             h->stream );
+                             h->stream, h->src_f.get() );
     dtkx::points_to_double4( s_tgt.ptr, n_tgt, opts->layout, h->tgt_d.get(),
-                             h->stream );
+                             h->stream, h->tgt_f.get() );
     h->indices.resize( std::max<int64_t>( n_tgt * h->kk, 1 ), h->stream );
     alloc_coeffs( h.get(), n_tgt * h->kk );
     h->order.resize( std::max<int64_t>( n_tgt, 1 ), h->stream );
@@ -422,7 +431,10 @@ int dtkx_mls_create( const double *src_xyz, int64_t n_src,
     return guarded( [&]() {
         REQ( out != nullptr, DTKX_ERR_INVALID_ARGUMENT );
         *out = nullptr;
-        auto h = make_handle( src_xyz, n_src, tgt_xyz, n_tgt, opts );
+        REQ( opts != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        auto h = make_handle( src_xyz, n_src, tgt_xyz, n_tgt, opts,
+                              opts->geometry == DTKX_GEOMETRY_FLOAT &&
+                                  n_tgt > 0 );
         StageEvents ev( h->stream );
         dtkx::DeviceBuffer<unsigned long long> counters( kCounters,
                                                          h->stream );
@@ -453,15 +465,13 @@ int dtkx_mls_create( const double *src_xyz, int64_t n_src,
             kout.morton_layout = true;
             if ( opts->geometry == DTKX_GEOMETRY_FLOAT )
             {
-                // float points rounded from the caller's doubles
-                dtkx::DeviceBuffer<float4> sf( n_src, h->stream ),
-                    tf( n_tgt, h->stream );
-                dtkx::double4_to_float4( h->src_d.get(), n_src, sf.get(),
-                                         h->stream );
-                dtkx::double4_to_float4( h->tgt_d.get(), n_tgt, tf.get(),
-                                         h->stream );
-                dtkx::knn_search( opts->geometry, sf.get(), n_src, tf.get(),
-                                  n_tgt, k, kout, &info, h->stream );
+                // float points rounded from the caller's doubles (made
+                // with the double4 copies), freed after the search
+                dtkx::knn_search( opts->geometry, h->src_f.get(), n_src,
+                                  h->tgt_f.get(), n_tgt, k, kout, &info,
+                                  h->stream );
+                h->src_f = {};
+                h->tgt_f = {};
             }
             else
                 dtkx::knn_search( opts->geometry, h->src_d.get(), n_src,

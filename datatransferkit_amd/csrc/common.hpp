@@ -169,8 +169,10 @@ bool is_device_pointer( void const *p );
 // float4 for float geometry (kNN), double4 for double geometry and setup.
 void points_to_float4( double const *xyz, int64_t n, int layout, float4 *out,
                        hipStream_t s );
+// outf (nullable): float copies in the same pass (the values
+// double4_to_float4 gives)
 void points_to_double4( double const *xyz, int64_t n, int layout,
-                        double4 *out, hipStream_t s );
+                        double4 *out, hipStream_t s, float4 *outf = nullptr );
 
 void double4_to_float4( double4 const *in, int64_t n, float4 *out,
                         hipStream_t s );

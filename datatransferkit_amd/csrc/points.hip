@@ -41,9 +41,11 @@ namespace
 
 constexpr int kBlock = 256;
 
+// outf (nullable): the same points rounded to float as well
 template <typename V, typename T>
 __global__ void __launch_bounds__( kBlock )
-    to_vec4_kernel( double const *xyz, int64_t n, int layout, V *out )
+    to_vec4_kernel( double const *xyz, int64_t n, int layout, V *out,
+                    float4 *outf )
 {
     int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
     if ( i >= n )
@@ -67,6 +69,9 @@ __global__ void __launch_bounds__( kBlock )
     v.z = static_cast<T>( z );
     v.w = 0;
     out[i] = v;
+    if ( outf )
+        outf[i] = make_float4( static_cast<float>( x ), static_cast<float>( y ),
+                               static_cast<float>( z ), 0.f );
 }
 
 __global__ void __launch_bounds__( kBlock )
@@ -184,17 +189,17 @@ void points_to_float4( double const *xyz, int64_t n, int layout, float4 *out,
     if ( n == 0 )
         return;
     to_vec4_kernel<float4, float><<<ceil_div( n, kBlock ), kBlock, 0, s>>>(
-        xyz, n, layout, out );
+        xyz, n, layout, out, nullptr );
     DTKX_HIP( hipGetLastError() );
 }
 
 void points_to_double4( double const *xyz, int64_t n, int layout,
-                        double4 *out, hipStream_t s )
+                        double4 *out, hipStream_t s, float4 *outf )
 {
     if ( n == 0 )
         return;
     to_vec4_kernel<double4, double><<<ceil_div( n, kBlock ), kBlock, 0, s>>>(
-        xyz, n, layout, out );
+        xyz, n, layout, out, outf );
     DTKX_HIP( hipGetLastError() );
 }
 
