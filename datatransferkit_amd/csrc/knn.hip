@@ -686,12 +686,11 @@ struct QueryArgs
     int64_t gstride = 0;
 };
 
-// One lane per query. STACK > 0: LDS stack of that many entries per lane
-// (a query that would need more is listed for the re-run); STACK == 0:
-// per-lane stack in global memory (gstack, laid out [level][lane]) deep
-// enough for any tree.
 // One row of a one-pass search: lane qi (its global-stack slot), row r;
-// st_node and st_d are the block's LDS stack.
+// st_node and st_d are the block's LDS stack. STACK > 0: an LDS stack of
+// that many entries per lane (a query that would need more is listed for
+// the re-run); STACK == 0: a per-lane stack in global memory (gstack, laid
+// out [level][lane]) deep enough for any tree.
 template <typename T, int K, int STACK>
 __device__ inline void query_row( QueryArgs<T> const &a, int64_t const qi,
                                  int64_t const r, int32_t *st_node, T *st_d )
