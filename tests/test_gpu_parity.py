@@ -360,6 +360,44 @@ def test_apply_multifield_and_transpose(dtk, oracle_lib):
         assert np.array_equal(bits(xbar.cpu().numpy()), bits(ref))
 
 
+def test_apply_padded_leading_dimensions(dtk, oracle_lib):
+    """Fields with leading dimensions beyond n (the C ABI's ld_src, ld_tgt)
+    and rows staged in Morton order: each value lands at f * ld + i, bit for
+    bit the oracle's, and the gaps keep their contents, forward and
+    transpose."""
+    import ctypes
+
+    from datatransferkit_amd import _lib
+
+    src, tgt, _ = c1_inputs(5_000, seed=52)
+    op = dtk.MovingLeastSquaresOperator(cuda(src), cuda(tgt), basis="quadratic", k=16,
+                                        rbf="wendland2")
+    _, idx, coeffs = op.graph()
+    gi, gc = idx.cpu().numpy(), coeffs.cpu().numpy()
+    ns, nt, nf = len(src), len(tgt), 3
+    lds, ldt = ns + 37, nt + 11
+    rng = np.random.default_rng(6)
+    L = _lib.load()
+    vp = ctypes.c_void_p
+    x = rng.standard_normal((nf, ns))
+    xp = torch.full((nf, lds), 7.0, dtype=torch.float64, device="cuda")
+    xp[:, :ns] = cuda(x)
+    yp = torch.full((nf, ldt), -3.0, dtype=torch.float64, device="cuda")
+    _lib.check(L.dtkx_mls_apply(op._h, vp(xp.data_ptr()), vp(yp.data_ptr()), nf, lds, ldt))
+    y = yp.cpu().numpy()
+    assert np.array_equal(bits(y[:, :nt]), bits(oracle_lib.apply(gi, gc, x)))
+    assert np.all(y[:, nt:] == -3.0)
+    ybar = rng.standard_normal((nf, nt))
+    yb = torch.full((nf, ldt), 5.0, dtype=torch.float64, device="cuda")
+    yb[:, :nt] = cuda(ybar)
+    xb = torch.full((nf, lds), -1.0, dtype=torch.float64, device="cuda")
+    _lib.check(L.dtkx_mls_apply_transpose(op._h, vp(yb.data_ptr()), vp(xb.data_ptr()), nf,
+                                          ldt, lds))
+    xbar = xb.cpu().numpy()
+    assert np.array_equal(bits(xbar[:, :ns]), bits(oracle_lib.apply_transpose(gi, gc, ybar, ns)))
+    assert np.all(xbar[:, ns:] == -1.0)
+
+
 def test_errors_are_reported(dtk):
     src, tgt, f = c1_inputs(100, seed=1)
     op = dtk.MovingLeastSquaresOperator(cuda(src), cuda(tgt))
