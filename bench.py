@@ -191,7 +191,8 @@ def kernel_rooflines(form, r, n, m, kk, nnz, fields):
                     "peak = MI355X FP64 78.6 TFLOP/s",
         },
         "apply": {
-            "bound": "hbm", "kernel": "gather_fields_kernel + crs_rows_kernel",
+            "bound": "hbm",
+            "kernel": "scatter_fields_kernel + crs_rows_kernel + records_to_caller_kernel",
             "ms": r["ms_apply"], "achieved": round(apply, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(apply / HBM_PEAK_GBS, 5), "traffic": None,
             "algorithmic_per_unit": f"B_apply = {b_apply(nnz, n, n, fields, fp32)} bytes "
