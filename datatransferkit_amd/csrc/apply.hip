@@ -417,12 +417,15 @@ void transpose_pass( RowLayout const &g, CscIndex const &csc,
                      double *xscratch, hipStream_t s )
 {
     auto *ys = reinterpret_cast<Pack<double, NF> *>( scratch );
-    if ( g.order && g.trow )
+    // no targets: every column is empty and the transpose kernel writes
+    // zeros, so only the packing launch (it would have zero blocks) is
+    // skipped
+    if ( g.n_tgt > 0 && g.order && g.trow )
         // lanes over targets: coalesced field reads, whole-record stores
         scatter_fields_kernel<double, double, NF>
             <<<ceil_div( g.n_tgt, kBlock ), kBlock, 0, s>>>(
                 ybar, ld_tgt, nf, g.trow, g.n_tgt, ys );
-    else
+    else if ( g.n_tgt > 0 )
         gather_fields_kernel<double, double, NF>
             <<<ceil_div( g.n_tgt, kBlock ), kBlock, 0, s>>>(
                 ybar, ld_tgt, nf, g.order, g.n_tgt, ys );

@@ -118,6 +118,10 @@ void validate_options( dtkx_mls_options const &o )
          DTKX_ERR_INVALID_ARGUMENT );
     REQ( o.rbf >= DTKX_RBF_WENDLAND0 && o.rbf <= DTKX_RBF_BUHMANN4,
          DTKX_ERR_INVALID_ARGUMENT );
+    // Buhmann<4> as the reference writes it
+    // (DTK_CompactlySupportedRadialBasisFunctions.hpp:139-146) is -67.886 at
+    // x = 1, so it is not compactly supported and nothing pins its values
+    REQ( o.rbf != DTKX_RBF_BUHMANN4, DTKX_ERR_UNSUPPORTED );
     REQ( o.k >= 0, DTKX_ERR_INVALID_ARGUMENT );
     REQ( o.k <= dtkx::knn_max_k, DTKX_ERR_UNSUPPORTED );
     REQ( o.precision == DTKX_PRECISION_FP64 ||
@@ -1048,6 +1052,7 @@ int dtkx_spline_create( const double *src_xyz, int64_t n_src,
             o = *opts;
         REQ( o.rbf >= DTKX_RBF_WENDLAND0 && o.rbf <= DTKX_RBF_BUHMANN4,
              DTKX_ERR_INVALID_ARGUMENT );
+        REQ( o.rbf != DTKX_RBF_BUHMANN4, DTKX_ERR_UNSUPPORTED );
         REQ( o.geometry == DTKX_GEOMETRY_FLOAT ||
                  o.geometry == DTKX_GEOMETRY_DOUBLE,
              DTKX_ERR_INVALID_ARGUMENT );

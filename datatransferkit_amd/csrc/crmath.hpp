@@ -1,4 +1,5 @@
-// Correctly rounded double-precision atan2, sin and cos for host and device.
+// Correctly rounded double-precision atan2, sin, cos and log for host and
+// device.
 //
 // The classical Jacobi of the reference forms its rotations from
 // 0.5 * atan2(...), cos(...) and sin(...) (DTK_DetailsSVDImpl.hpp:129-140).
@@ -21,6 +22,9 @@
 //            (1 + t j/16)), Taylor series of atan on |u| <= 1/32, quadrant
 //            fix-up in double-double. C99 special cases (signed zeros,
 //            infinities, NaN) are handled explicitly.
+//   log:     x = 2^e m, m in [sqrt(1/2), sqrt(2)), the atanh series of
+//            s = (m - 1) / (m + 1) in double-double plus e ln 2 (Buhmann<2>
+//            RBF, DTK_CompactlySupportedRadialBasisFunctions.hpp:118).
 // Constants were produced with mpmath at 400-bit precision.
 //
 // Domain note: sin/cos reduce exactly for |x| < 2^20 * pi/2; larger
@@ -30,6 +34,7 @@
 #define DTKX_CRMATH_HPP
 
 #include <cmath>
+#include <limits>
 
 #if defined( __HIPCC__ )
 #define DTKX_HD __host__ __device__ inline
@@ -401,6 +406,68 @@ DTKX_HD double atan2( double y, double x )
         v = dd_add( dd{DTKX_PI_HI, DTKX_PI_LO}, dd_neg( v ) );
     double res = v.hi + v.lo;
     return std::copysign( res, y );
+}
+
+// ------------------------------------------------------------------ log
+// ln 2 = LN2_HI + LN2_MID + LN2_LO (+ < 2^-165)
+#define DTKX_LN2_HI 0x1.62e42fefa39efp-1
+#define DTKX_LN2_MID 0x1.abc9e3b39803fp-56
+#define DTKX_LN2_LO 0x1.7b57a079a1934p-111
+
+// 1/d as double-double for a small odd integer d (q1 + exact remainder / d)
+DTKX_HD dd dd_recip_int( double d )
+{
+    DTKX_NO_CONTRACT
+    double q1 = 1.0 / d;
+    double r = std::fma( -q1, d, 1.0 ); // exact
+    return fast_two_sum( q1, r / d );
+}
+
+// Natural logarithm. x = 2^e m with m in [sqrt(1/2), sqrt(2)); then
+// log(m) = 2 atanh(s) = 2 s sum_n s^2n / (2n+1), s = (m - 1) / (m + 1),
+// |s| < 0.1716 (s^2 < 0.0295). Terms n >= 11 are below 2^-54 relative to
+// the sum and use double; n <= 10 use double-double. e ln 2 is a
+// three-part product. The Buhmann<2> RBF is the only caller.
+DTKX_HD double log( double x )
+{
+    DTKX_NO_CONTRACT
+    if ( std::isnan( x ) )
+        return x + x;
+    if ( x < 0.0 )
+        return std::numeric_limits<double>::quiet_NaN();
+    if ( x == 0.0 )
+        return -std::numeric_limits<double>::infinity();
+    if ( std::isinf( x ) )
+        return x;
+    if ( x == 1.0 )
+        return 0.0;
+    int e;
+    double m = std::frexp( x, &e ); // m in [0.5, 1), exact for subnormals
+    if ( m < 0.70710678118654752 )
+    {
+        m *= 2.0;
+        --e;
+    }
+    // m - 1 is exact (Sterbenz); m + 1 is carried as a double-double
+    dd const s = dd_div( dd{m - 1.0, 0.0}, two_sum( m, 1.0 ) );
+    dd const w = dd_mul( s, s );
+    double const wd = w.hi;
+    double t = 1.0 / 45.0;
+    for ( int n = 21; n >= 11; --n )
+        t = t * wd + 1.0 / ( 2 * n + 1 );
+    dd acc = {t, 0.0};
+    for ( int n = 10; n >= 1; --n )
+        acc = dd_add( dd_mul( acc, w ), dd_recip_int( 2 * n + 1 ) );
+    acc = dd_add_d( dd_mul( acc, w ), 1.0 );
+    dd v = dd_mul( acc, dd{2.0 * s.hi, 2.0 * s.lo} );
+    if ( e != 0 )
+    {
+        double const ed = static_cast<double>( e );
+        dd l2 = two_prod( ed, DTKX_LN2_HI );
+        l2.lo += ed * DTKX_LN2_MID + ed * DTKX_LN2_LO;
+        v = dd_add( fast_two_sum( l2.hi, l2.lo ), v );
+    }
+    return v.hi + v.lo;
 }
 
 } // namespace crm

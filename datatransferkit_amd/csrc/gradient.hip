@@ -61,7 +61,7 @@ __device__ inline double rbf_deriv( int rbf, double x )
     case DTKX_RBF_BUHMANN2:
     {
         double const x2 = x * x, x3 = x2 * x;
-        return x > 0 ? 8.0 * x3 * log( x ) - 12.0 * x3 + 16.0 * x2 - 4.0 * x
+        return x > 0 ? 8.0 * x3 * crm::log( x ) - 12.0 * x3 + 16.0 * x2 - 4.0 * x
                      : 0.0;
     }
     case DTKX_RBF_BUHMANN3:

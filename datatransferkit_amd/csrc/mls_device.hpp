@@ -41,8 +41,8 @@ __device__ inline double rbf_eval( int rbf, double x )
                ( 1.0 - x ) * ( 1.0 - x ) *
                ( 5.0 * x * x * x * x * x + 30.0 * x * x * x * x +
                  72.0 * x * x * x + 82.0 * x * x + 36.0 * x + 6.0 );
-    case DTKX_RBF_BUHMANN2:
-        return 2.0 * x * x * x * x * log( x ) - 7.0 / 2.0 * x * x * x * x +
+    case DTKX_RBF_BUHMANN2: // x = 0 gives 0 * log(0) = NaN, as the reference
+        return 2.0 * x * x * x * x * crm::log( x ) - 7.0 / 2.0 * x * x * x * x +
                16 / 3.0 * x * x * x - 2 * x * x + 1.0 / 6.0;
     case DTKX_RBF_BUHMANN3:
         return x * x * x * x * x * x * x * x -
@@ -50,7 +50,7 @@ __device__ inline double rbf_eval( int rbf, double x )
                1024.0 / 5.0 * x * x * x * x * sqrt( x ) -
                378.0 * x * x * x * x + 1024.0 / 5.0 * x * x * x * sqrt( x ) -
                84.0 / 5.0 * x * x + 1.0;
-    default: // DTKX_RBF_BUHMANN4
+    default: // DTKX_RBF_BUHMANN4 (rejected at create; kept for the table)
         return 99.0 / 35.0 * x * x * x * x * x * x * x * x -
                132.0 * x * x * x * x * x * x +
                9216.0 / 35.0 * x * x * x * x * x * sqrt( x ) -

@@ -68,6 +68,9 @@ typedef enum dtkx_rbf {
     DTKX_RBF_WU4 = 5,
     DTKX_RBF_BUHMANN2 = 6,
     DTKX_RBF_BUHMANN3 = 7,
+    /* refused by every create call with DTKX_ERR_UNSUPPORTED: as the
+     * reference writes it (:139-146) it is -67.9 at x = 1, so it is not
+     * compactly supported and no test or identity pins its values */
     DTKX_RBF_BUHMANN4 = 8
 } dtkx_rbf;
 

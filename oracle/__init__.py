@@ -140,11 +140,11 @@ class libm_mode:
 
 
 def elementary(op: str, x, y=None) -> np.ndarray:
-    """atan2(y, x), sin(x) or cos(x) as the oracle evaluates them."""
+    """atan2(y, x), sin(x), cos(x) or log(x) as the oracle evaluates them."""
     x = _f64(x)
     y = _f64(y) if y is not None else x
     out = np.empty_like(x)
-    code = {"atan2": 0, "sin": 1, "cos": 2}[op]
+    code = {"atan2": 0, "sin": 1, "cos": 2, "log": 3}[op]
     lib().dtkx_oracle_elementary(code, _d(y), _d(x), _d(out), x.size)
     return out
 

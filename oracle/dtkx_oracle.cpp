@@ -74,6 +74,11 @@ inline double j_sin( double x )
 {
     return g_libm_mode ? static_cast<double>( sinq( x ) ) : std::sin( x );
 }
+// Buhmann<2>'s log (RBF.hpp:118) in the same two modes
+inline double j_log( double x )
+{
+    return g_libm_mode ? static_cast<double>( logq( x ) ) : std::log( x );
+}
 
 // ------------------------------------------------------------------ RBFs
 // RBF.hpp:45-147. Expressions keep the reference's evaluation order.
@@ -102,7 +107,7 @@ double rbf_eval( int rbf, double x )
                ( 5.0 * x * x * x * x * x + 30.0 * x * x * x * x +
                  72.0 * x * x * x + 82.0 * x * x + 36.0 * x + 6.0 );
     case DTKX_RBF_BUHMANN2: // RBF.hpp:116-120
-        return 2.0 * x * x * x * x * std::log( x ) -
+        return 2.0 * x * x * x * x * j_log( x ) -
                7.0 / 2.0 * x * x * x * x + 16 / 3.0 * x * x * x - 2 * x * x +
                1.0 / 6.0;
     case DTKX_RBF_BUHMANN3: // RBF.hpp:126-133
@@ -762,13 +767,16 @@ void dtkx_oracle_set_libm_mode( int mode ) { g_libm_mode = mode ? 1 : 0; }
 int dtkx_oracle_get_libm_mode( void ) { return g_libm_mode; }
 
 // Elementary functions as the oracle evaluates them in the current mode
-// (op: 0 atan2(y, x), 1 sin(x), 2 cos(x)); y is ignored for sin/cos.
+// (op: 0 atan2(y, x), 1 sin(x), 2 cos(x), 3 log(x)); y is ignored for the
+// one-argument functions.
 void dtkx_oracle_elementary( int op, double const *y, double const *x,
                              double *out, int64_t n )
 {
     for ( int64_t i = 0; i < n; ++i )
-        out[i] = op == 0 ? j_atan2( y[i], x[i] )
-                         : ( op == 1 ? j_sin( x[i] ) : j_cos( x[i] ) );
+        out[i] = op == 0   ? j_atan2( y[i], x[i] )
+                 : op == 1 ? j_sin( x[i] )
+                 : op == 2 ? j_cos( x[i] )
+                           : j_log( x[i] );
 }
 
 int dtkx_oracle_num_threads( void )

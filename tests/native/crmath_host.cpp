@@ -1,5 +1,5 @@
 // Host build of datatransferkit_amd/csrc/crmath.hpp for the CPU tests that
-// pin the correctly rounded atan2/sin/cos against mpmath and glibc.
+// pin the correctly rounded atan2/sin/cos/log against mpmath and glibc.
 // Build: g++ -O2 -std=c++17 -ffp-contract=off -shared -fPIC
 #include "../../datatransferkit_amd/csrc/crmath.hpp"
 #include <cmath>
@@ -25,6 +25,11 @@ void crm_sincos( const double *x, double *s, double *c, int64_t n )
 {
     for ( int64_t i = 0; i < n; ++i )
         dtkx::crm::sincos( x[i], s + i, c + i );
+}
+void crm_log( const double *x, double *out, int64_t n )
+{
+    for ( int64_t i = 0; i < n; ++i )
+        out[i] = dtkx::crm::log( x[i] );
 }
 void libm_atan2( const double *y, const double *x, double *out, int64_t n )
 {

@@ -97,6 +97,28 @@ def test_error_path_without_gpu(lib):
     assert ", failed in capi.hip, line" in L.dtkx_last_error().decode()
 
 
+def test_buhmann4_is_refused(lib):
+    # Buhmann<4> as the reference writes it is -67.9 at the edge of its
+    # support (DTK_CompactlySupportedRadialBasisFunctions.hpp:139-146)
+    from datatransferkit_amd import _lib
+
+    L = _lib.load()
+    h = ctypes.c_void_p()
+    o = _lib.default_options()
+    o.rbf = _lib.RBF["buhmann4"]
+    rc = L.dtkx_mls_create(None, 0, None, 0, ctypes.byref(o), ctypes.byref(h))
+    assert rc == _lib.ERR_UNSUPPORTED
+    assert "DTKX_RBF_BUHMANN4" in L.dtkx_last_error().decode()
+    o = _lib.default_spline_options()
+    o.rbf = _lib.RBF["buhmann4"]
+    assert L.dtkx_spline_create(None, 0, None, 0, ctypes.byref(o),
+                                ctypes.byref(h)) == _lib.ERR_UNSUPPORTED
+    o = _lib.default_options()
+    o.rbf = _lib.RBF["buhmann3"]
+    rc = L.dtkx_mls_create(None, 0, None, 0, ctypes.byref(o), ctypes.byref(h))
+    assert rc == _lib.ERR_EMPTY_SOURCE
+
+
 def test_spline_structs_and_defaults(tmp_path, lib):
     """dtkx_spline_options / dtkx_spline_info: the ctypes layouts match the
     header, and the defaults are the reference's (Wendland<0>, float

@@ -1,5 +1,5 @@
 """Pins datatransferkit_amd/csrc/crmath.hpp (the GPU kernels' correctly
-rounded atan2/sin/cos) on the CPU: the same header is compiled for the host
+rounded atan2/sin/cos/log) on the CPU: the same header is compiled for the host
 and compared bit-for-bit with the oracle's independent correctly rounded
 mode (libquadmath __float128, rounded once) on millions of arguments,
 including the special values of C99 atan2."""
@@ -83,6 +83,23 @@ def test_atan2_matches_quadmath(crm, oracle_lib):
         ref = oracle_lib.elementary("atan2", X, Y)
     bad = ~(((got == ref) & (np.signbit(got) == np.signbit(ref))) | (np.isnan(got) & np.isnan(ref)))
     assert bad.sum() == 0, (Y[bad][:5], X[bad][:5], got[bad][:5], ref[bad][:5])
+
+
+def test_log_matches_quadmath(crm, oracle_lib):
+    rng = np.random.default_rng(5)
+    x = np.concatenate([
+        rng.uniform(0.0, 1.0, 1_000_000),             # the RBF's x = d / R
+        10.0 ** rng.uniform(-307, 307, 200_000),
+        1.0 + rng.uniform(-1e-6, 1e-6, 20_000),       # near the zero of log
+        np.sqrt(0.5) * (1 + rng.integers(-64, 64, 2000) * 2.0 ** -52),
+        [1.0, 2.0, 0.5, 4.0, 1 - 2 ** -53, 1 + 2 ** -52, 5e-324, 2.2e-308,
+         1.7e308, np.inf, 0.0, -0.0, -1.0, np.nan],
+    ])
+    got = _call(crm, "crm_log", x)
+    with oracle_lib.libm_mode(1):
+        ref = oracle_lib.elementary("log", x)
+    bad = ~(((got == ref) & (np.signbit(got) == np.signbit(ref))) | (np.isnan(got) & np.isnan(ref)))
+    assert bad.sum() == 0, (x[bad][:5], got[bad][:5], ref[bad][:5])
 
 
 def test_glibc_is_not_correctly_rounded(crm, oracle_lib):

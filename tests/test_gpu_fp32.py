@@ -80,3 +80,63 @@ def test_fp32_requires_fp32_handle(dtk):
     with pytest.raises(dtk.DtkxError) as e:
         op.apply(cuda(f, torch.float32), y)
     assert e.value.status == 7
+
+
+def _shell_inputs(n, seed=21):
+    """The thin-shell cloud: sources on a spherical shell of radius 1 and
+    thickness 1e-3 (uniform directions), targets on the unit sphere jittered
+    +-2e-4 radially, four smooth fields."""
+    rng = np.random.default_rng(seed)
+
+    def sphere(m):
+        v = rng.standard_normal((m, 3))
+        return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+    src = sphere(n) * (1 + rng.uniform(-5e-4, 5e-4, (n, 1)))
+    tgt = sphere(n) * (1 + rng.uniform(-2e-4, 2e-4, (n, 1)))
+    x, y, z = src[:, 0], src[:, 1], src[:, 2]
+    vals = np.stack([x, y * z, x * x - y * y, np.sin(3 * x) * np.cos(2 * y) + z])
+    return src, tgt, vals
+
+
+@pytest.mark.parametrize("solver", ["cholesky", "cyclic"])
+def test_thin_shell_fp32_bulk_against_fp64(dtk, oracle_lib, solver):
+    """Mixed precision on the ill-conditioned cloud (100k -> 100k, the 2M
+    shell's density scaled down): quadratic, k = 32, Wendland<4>, relative
+    truncation at rcond = 1e-12 (DTK_DetailsSVDImpl.hpp:281-297 with the
+    threshold rcond * sigma_max). Every system is truncated there: the
+    points lie on a surface, so the quadratic moments are near-singular.
+    FP32 bulk must stay within 1e-6 per field of the oracle and of the
+    all-FP64 GPU form, which itself is bit-exact to the oracle and truncates
+    the same systems."""
+    n = 100_000
+    src, tgt, vals = _shell_inputs(n)
+    print(f"thin shell n={n} solver={solver}", flush=True)
+    kw = dict(basis="quadratic", rbf="wendland4", k=32, truncation="relative", rcond=1e-12)
+    with oracle_lib.libm_mode(1):
+        gi, gc, cnt = oracle_lib.mls_create(src, tgt, 10, k=32, rbf_name="wendland4",
+                                            truncation=oracle_lib.TRUNC_RELATIVE,
+                                            rcond=1e-12)
+    ref = oracle_lib.apply(gi, gc, vals)
+    assert cnt.n_underdetermined == n
+    op64 = dtk.MovingLeastSquaresOperator(cuda(src), cuda(tgt), precision="fp64",
+                                          solver="classical", **kw)
+    _, idx, c64 = op64.graph()
+    assert np.array_equal(idx.cpu().numpy(), gi)
+    assert np.array_equal(c64.cpu().numpy().view(np.uint64), gc.view(np.uint64))
+    assert op64.stats()["n_underdetermined"] == cnt.n_underdetermined
+    y64 = torch.zeros((4, n), dtype=torch.float64, device="cuda")
+    op64.apply(cuda(vals), y64)
+    y64 = y64.cpu().numpy()
+    op32 = dtk.MovingLeastSquaresOperator(cuda(src), cuda(tgt), precision="fp32_bulk",
+                                          solver=solver, **kw)
+    y32 = torch.zeros((4, n), dtype=torch.float32, device="cuda")
+    op32.apply(cuda(vals, torch.float32), y32)
+    y32 = y32.double().cpu().numpy()
+    st = op32.stats()
+    vs_oracle = np.abs(y32 - ref).max(axis=1) / np.abs(ref).max(axis=1)
+    vs_fp64 = np.abs(y32 - y64).max(axis=1) / np.abs(y64).max(axis=1)
+    print(f"  per-field rel_inf vs oracle {vs_oracle}, vs all-FP64 GPU {vs_fp64}, "
+          f"n_underdetermined {st['n_underdetermined']} (oracle {cnt.n_underdetermined})")
+    assert np.all(vs_oracle <= 1e-6) and np.all(vs_fp64 <= 1e-6)
+    assert st["n_underdetermined"] == cnt.n_underdetermined

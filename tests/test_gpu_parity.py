@@ -186,6 +186,40 @@ def test_knn_deep_tree_overflow_rerun(dtk, oracle_lib, geometry):
     assert np.array_equal(idx.cpu().numpy(), gi)
 
 
+@pytest.mark.parametrize("geometry", ["float", "double"])
+def test_knn_overflow_rerun_strides_over_many_rows(dtk, oracle_lib, geometry):
+    """More overflowing queries (20,000) than the re-run has lanes (16,384),
+    so lanes take several listed rows and reuse their global stack.
+
+    Sources: pairs at (2^-i, 2^-i, 2^-i), i = 1..21, and 16 copies of the
+    origin; k = all 58 sources; every query at the origin. The hierarchy is
+    a chain: each level splits off the pair i from a node holding the
+    origin. The query lies in that node's box (distance 0) and the k-th
+    distance covers every pair, so each of the 21 levels pushes its pair,
+    and the 16 duplicates (split by index, 4 levels at distance 0) push 3
+    more: 24 entries, past the 22-entry LDS stack, for every query. So one
+    query gives a count of 1 and 20,000 give 20,000; the rows must equal
+    the oracle's."""
+    chain = np.repeat(np.array([[2.0 ** -i] * 3 for i in range(1, 22)]), 2, axis=0)
+    src = np.concatenate([chain, np.zeros((16, 3))])
+    k = len(src)
+    g = 0 if geometry == "float" else 1
+    n = 20_000
+    print(f"overflow rerun geometry={geometry} n_tgt={n} k={k}", flush=True)
+
+    def make(t):
+        op = dtk.MovingLeastSquaresOperator(cuda(src), cuda(t), basis="constant", k=k,
+                                            geometry=geometry)
+        return op, op.stats()["n_stack_overflow"]
+
+    assert make(np.zeros((1, 3)))[1] == 1
+    op, n_all = make(np.zeros((n, 3)))
+    assert n_all == n
+    gi, _ = oracle_lib.knn(src, np.zeros((1, 3)), k, geometry=g)
+    _, idx, _ = op.graph()
+    assert np.array_equal(idx.cpu().numpy(), np.repeat(gi, n, axis=0))
+
+
 # --------------------------------------------------------------------- SVD
 @pytest.mark.parametrize("case,shape,deficient", [
     ("svd_full_rank", (10, 32, 32), ()),
@@ -250,6 +284,23 @@ def test_reference_corner_cases(dtk):
     assert y.size == 0
 
 
+def test_transpose_with_no_targets_zero_fills(dtk):
+    # C^T of an operator with no targets is the zero map: every column of
+    # src_bar (with its leading dimension) is zeroed, the gaps are left alone
+    import ctypes
+
+    from datatransferkit_amd import _lib
+
+    src, _, _ = c1_inputs(500, seed=3)
+    op = dtk.MovingLeastSquaresOperator(cuda(src), cuda(np.zeros((0, 3))), basis="linear")
+    lds = 512
+    xb = torch.full((3, lds), 9.0, dtype=torch.float64, device="cuda")
+    _lib.check(_lib.load().dtkx_mls_apply_transpose(
+        op._h, None, ctypes.c_void_p(xb.data_ptr()), 3, 0, lds))
+    got = xb.cpu().numpy()
+    assert np.all(got[:, :500] == 0.0) and np.all(got[:, 500:] == 9.0)
+
+
 # ---------------------------------------------------------- C1 parity
 def _oracle_full(oracle_lib, src, tgt, m, k, rbf, mode, geometry=0):
     with oracle_lib.libm_mode(mode):
@@ -276,10 +327,25 @@ def test_c1_plain_fp64_bit_exact(dtk, oracle_lib, group):
     assert st["n_unconverged"] == 0
 
 
+def _moment_cond(src, tgt, idx, t):
+    """cond(A) of target t's linear moment matrix A = P^T W P (Wendland<0>,
+    float relative coordinates, as the setup forms it)."""
+    x = (src[idx[t]] - tgt[t]).astype(np.float32).astype(np.float64)
+    d = np.sqrt((x ** 2).sum(1))
+    w = (1.0 - d / (1.1 * d.max())) ** 2
+    p = np.c_[np.ones(len(d)), x]
+    return np.linalg.cond(p.T @ (w[:, None] * p))
+
+
 def test_c1_against_platform_libm(dtk, oracle_lib):
-    """Against the oracle with glibc's atan2/sin/cos: the 1e-12 relative bar
-    holds for the infinity-norm error; pointwise, only systems whose condition
-    amplifies one-ulp trig differences deviate, and those are counted."""
+    """Against the oracle with glibc's atan2/sin/cos (what the reference's CPU
+    build computes here). The GPU rounds these functions correctly, glibc
+    misses by an ulp on about 0.1 % of arguments, and on C1's most
+    ill-conditioned systems that ulp shows above 1e-12. Measured on the CPU
+    between the oracle's two modes (the GPU equals the correctly rounded
+    one): infinity-norm 1.10e-12 relative; pointwise, 5 of 10,000 targets
+    above 1e-12, at most 1.29e-11, all with cond(A) between 3.7e6 and
+    1.1e9."""
     src, tgt, f = c1_inputs()
     op = dtk.MovingLeastSquaresOperator(cuda(src), cuda(tgt))
     y = _apply(op, f, len(tgt))
@@ -287,11 +353,15 @@ def test_c1_against_platform_libm(dtk, oracle_lib):
     yo = oracle_lib.apply(gi, gc, f)
     rel_inf = np.abs(y - yo).max() / np.abs(yo).max()
     pointwise = np.abs(y - yo) / np.abs(yo)
-    frac = (pointwise > 1e-12).mean()
+    over = np.nonzero(pointwise > 1e-12)[0]
     print(f"C1 vs glibc oracle: inf-norm rel {rel_inf:.3e}, "
-          f"pointwise>1e-12 fraction {frac:.4%}, max {pointwise.max():.3e}")
-    assert frac < 0.05
-    assert rel_inf < 1e-6
+          f"{over.size} of {len(tgt)} targets above 1e-12, max {pointwise.max():.3e}")
+    for t in over:
+        print(f"  target {t}: rel {pointwise[t]:.3e}, "
+              f"cond(A) {_moment_cond(src, tgt, gi, t):.3e}")
+    assert rel_inf <= 2e-12
+    assert over.size <= 5
+    assert pointwise.max() <= 1.3e-11
 
 
 @pytest.mark.parametrize("m,basis,k,rbf", [
@@ -301,6 +371,9 @@ def test_c1_against_platform_libm(dtk, oracle_lib):
     (10, "quadratic", 64, "wendland6"),
     (1, "constant", 8, "wu2"),
     (4, "linear", 12, "wu4"),
+    (10, "quadratic", 16, "buhmann2"),
+    (4, "linear", 12, "buhmann3"),
+    (10, "quadratic", 32, "buhmann3"),
 ])
 def test_mls_variants_bit_exact(dtk, oracle_lib, m, basis, k, rbf):
     src, tgt, f = c1_inputs(20_000, seed=7 + k)

@@ -19,10 +19,18 @@ the oracle's formulas:
                     and polynomial reproduction; k = m interpolation; an
                     independent numpy weighted least squares for k > m
   apply/transpose   numpy CSR product; <y, C x> = <C^T y, x>
-  libm modes        correctly rounded mode vs mpmath
-
-Buhmann RBFs have no reference test and no closed-form pin here beyond their
-formula: parity unpinned for those three functions.
+  Buhmann<2>, <3>   no reference test; pinned by what the mathematics fixes:
+                    Buhmann<2> is C^3 and Buhmann<3> C^5 across the edge of
+                    their support, so phi(1) = 0 and phi(1 - h) has the
+                    leading term phi^(p)(1)/p! h^p (p = 4: 16/24, p = 6:
+                    6048/720, derived by hand); Buhmann<2>(0+) = 1/6 and
+                    Buhmann<3>(0) = 1
+  Buhmann<4>        as the reference writes it (RBF.hpp:139-146) phi(1) =
+                    -67.886: not compactly supported and nothing pins it; the
+                    library rejects it (DTKX_ERR_UNSUPPORTED)
+  libm modes        correctly rounded mode vs mpmath (atan2, sin, cos, log);
+                    the two modes on C1 differ by the measured bound that the
+                    GPU test asserts
 """
 import numpy as np
 import pytest
@@ -46,6 +54,41 @@ def test_rbf_polynomial_forms(oracle_lib, name, factors):
     ref = poly(r)
     got = oracle_lib.rbf(name, 1.0, r)
     np.testing.assert_allclose(got, ref, rtol=1e-8, atol=0)
+
+
+@pytest.mark.parametrize("name,order,lead", [
+    ("buhmann2", 4, 16.0 / 24.0),     # 4th derivative at 1: 48 log 1 + 16
+    ("buhmann3", 6, 6048.0 / 720.0),  # 6th derivative at 1: 6048
+])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_buhmann_vanish_at_the_support_edge(oracle_lib, name, order, lead, mode):
+    # Buhmann<2> is C^3 and Buhmann<3> C^5 across x = 1: phi(1) = 0 to
+    # rounding (terms up to 378 cancel) and phi(1 - h) = lead h^order + O(h^(order+1)),
+    # lead = phi^(order)(1) / order! from differentiating the closed forms by
+    # hand. Richardson's 2 q(h/2) - q(h), q(h) = phi(1 - h) / h^order, removes
+    # the O(h) term. A wrong coefficient breaks phi(1) = 0 or the order.
+    with oracle_lib.libm_mode(mode):
+        assert abs(oracle_lib.rbf(name, 1.0, np.array([1.0]))[0]) <= 1e-12
+        h = np.array([0.04, 0.02])
+        v = oracle_lib.rbf(name, 1.0, 1.0 - h)
+    q = v / h ** order
+    assert np.all(v > 0)
+    assert abs((2 * q[1] - q[0]) / lead - 1.0) < 2e-3
+
+
+def test_buhmann_values_at_zero(oracle_lib):
+    # Buhmann<3>(0) = 1; Buhmann<2>(x) -> 1/6 as x -> 0 (x^4 log x -> 0), and at
+    # x = 0 exactly the reference's 0 * log(0) is NaN
+    assert oracle_lib.rbf("buhmann3", 1.0, np.array([0.0]))[0] == 1.0
+    v = oracle_lib.rbf("buhmann2", 1.0, np.array([1e-4, 0.0]))
+    assert abs(v[0] - 1.0 / 6.0) < 1e-7 and np.isnan(v[1])
+
+
+def test_buhmann4_as_written_is_not_compactly_supported(oracle_lib):
+    # the reason the library refuses Buhmann<4>
+    v = oracle_lib.rbf("buhmann4", 1.0, np.array([1.0]))[0]
+    assert abs(v - (99 / 35 - 132 + 9216 / 35 - 11264 / 35 + 198 - 396 / 5 + 1)) < 1e-12
+    assert v < -67.8
 
 
 def test_rbf_radius_wrapper(oracle_lib):
@@ -375,6 +418,28 @@ def test_correctly_rounded_mode_matches_mpmath(oracle_lib):
         assert s[i] == float(mp.sin(mp.mpf(x[i])))
         assert co[i] == float(mp.cos(mp.mpf(x[i])))
         assert at[i] == float(mp.atan2(mp.mpf(y[i]), mp.mpf(x[i])))
+    u = np.concatenate([rng.uniform(0.0, 1.0, 1500), 10.0 ** rng.uniform(-300, 300, 200),
+                        [1.0, 2.0, 0.5, 1 - 2 ** -53, 1 + 2 ** -52, 5e-324]])
+    with oracle_lib.libm_mode(1):
+        lg = oracle_lib.elementary("log", u)
+    for i in range(u.size):
+        assert lg[i] == float(mp.log(mp.mpf(u[i])))
+
+
+def test_c1_libm_modes_differ_by_the_measured_bound(oracle_lib):
+    # The GPU is bit-exact to the correctly rounded mode; this is the gap to
+    # what glibc gives the reference's CPU build on C1 (asserted on the GPU
+    # in test_gpu_parity.test_c1_against_platform_libm)
+    src, tgt, f = c1_inputs()
+    y = {}
+    for mode in (0, 1):
+        with oracle_lib.libm_mode(mode):
+            idx, c, _ = oracle_lib.mls_create(src, tgt, 4, k=0, rbf_name="wendland0")
+        y[mode] = oracle_lib.apply(idx, c, f)
+    rel_inf = np.abs(y[1] - y[0]).max() / np.abs(y[0]).max()
+    pw = np.abs(y[1] - y[0]) / np.abs(y[0])
+    assert 1e-12 < rel_inf <= 2e-12
+    assert (pw > 1e-12).sum() == 5 and pw.max() <= 1.3e-11
 
 
 # ------------------------------------------------ extended-precision values
