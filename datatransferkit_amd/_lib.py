@@ -41,7 +41,7 @@ EXPORTED = [
     "dtkx_last_error", "dtkx_version", "dtkx_release_cached_memory",
     "dtkx_mls_options_init",
     "dtkx_mls_create", "dtkx_mls_create_with_neighbors", "dtkx_mls_apply",
-    "dtkx_mls_apply_fp32", "dtkx_mls_apply_transpose",
+    "dtkx_mls_apply_fp32", "dtkx_mls_apply_noalloc", "dtkx_mls_apply_transpose",
     "dtkx_mls_target_gradient", "dtkx_mls_get_graph", "dtkx_mls_write_graph",
     "dtkx_mls_get_stats",
     "dtkx_mls_destroy", "dtkx_knn", "dtkx_svd_pseudo_inverse",
@@ -140,6 +140,7 @@ def load():
         ctypes.POINTER(_vp)]
     L.dtkx_mls_apply.argtypes = [_vp, _vp, _vp, _i32, _i64, _i64]
     L.dtkx_mls_apply_fp32.argtypes = [_vp, _vp, _vp, _i32, _i64, _i64]
+    L.dtkx_mls_apply_noalloc.argtypes = [_vp, _vp, _vp, _i32, _i64, _i64]
     L.dtkx_mls_apply_transpose.argtypes = [_vp, _vp, _vp, _i32, _i64, _i64]
     L.dtkx_mls_target_gradient.argtypes = [_vp, _vp, _vp, _i32, _i64, _i64, _vp]
     L.dtkx_mls_get_graph.argtypes = [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp),

@@ -228,19 +228,38 @@ void apply_pass( RowLayout const &g, C const *coeffs, X const *x,
     DTKX_HIP( hipGetLastError() );
 }
 
+inline std::size_t align256( std::size_t b ) { return ( b + 255 ) & ~std::size_t( 255 ); }
+
+// The apply's two scratch arrays: the position-ordered gathered fields
+// (n_src records of the widest pass) and, when rows are in Morton order,
+// the row results staged before they move to the targets.
+template <typename X, typename XS>
+void apply_scratch_sizes( RowLayout const &g, int n_fields, std::size_t &xs_bytes,
+                          std::size_t &ys_bytes )
+{
+    int const w = pass_width( n_fields );
+    xs_bytes = align256( (std::size_t)g.n_src * w * sizeof( XS ) );
+    bool const staged = g.order && g.trow;
+    ys_bytes = staged ? align256( (std::size_t)g.n_tgt * w * sizeof( X ) ) : 0;
+}
+
+// ws: caller-provided scratch of apply_scratch_sizes' total (no allocation),
+// or null to take it from the block cache
 template <typename C, typename X, typename XS, typename Acc>
 void launch_apply( RowLayout const &g, C const *coeffs, X const *x,
                    int64_t ld_src, X *y, int64_t ld_tgt, int n_fields,
-                   hipStream_t s )
+                   hipStream_t s, void *ws = nullptr )
 {
     if ( g.n_tgt == 0 || n_fields == 0 )
         return;
-    DeviceBuffer<XS> scratch(
-        (std::size_t)g.n_src * pass_width( n_fields ), s );
+    std::size_t xs_bytes, ys_bytes;
+    apply_scratch_sizes<X, XS>( g, n_fields, xs_bytes, ys_bytes );
+    DeviceBuffer<char> owned( ws ? 0 : xs_bytes + ys_bytes, s );
+    char *const base = ws ? static_cast<char *>( ws ) : owned.get();
+    XS *const scratch = reinterpret_cast<XS *>( base );
     // rows in Morton order: results staged by row, then moved to targets
-    bool const staged = g.order && g.trow;
-    DeviceBuffer<X> yscratch(
-        staged ? (std::size_t)g.n_tgt * pass_width( n_fields ) : 0, s );
+    X *const yscratch =
+        ys_bytes ? reinterpret_cast<X *>( base + xs_bytes ) : nullptr;
     for ( int f0 = 0; f0 < n_fields; )
     {
         int const w = pass_width( n_fields - f0 );
@@ -251,19 +270,19 @@ void launch_apply( RowLayout const &g, C const *coeffs, X const *x,
         {
         case 1:
             apply_pass<C, X, XS, Acc, 1>( g, coeffs, xf, ld_src, yf, ld_tgt,
-                                          nf, scratch.get(), yscratch.get(), s );
+                                          nf, scratch, yscratch, s );
             break;
         case 2:
             apply_pass<C, X, XS, Acc, 2>( g, coeffs, xf, ld_src, yf, ld_tgt,
-                                          nf, scratch.get(), yscratch.get(), s );
+                                          nf, scratch, yscratch, s );
             break;
         case 4:
             apply_pass<C, X, XS, Acc, 4>( g, coeffs, xf, ld_src, yf, ld_tgt,
-                                          nf, scratch.get(), yscratch.get(), s );
+                                          nf, scratch, yscratch, s );
             break;
         default:
             apply_pass<C, X, XS, Acc, 8>( g, coeffs, xf, ld_src, yf, ld_tgt,
-                                          nf, scratch.get(), yscratch.get(), s );
+                                          nf, scratch, yscratch, s );
         }
         f0 += nf;
     }
@@ -635,6 +654,36 @@ void crs_apply_f32_dvals( RowLayout const &g, float const *coeffs,
     // values are rounded to float by the gather, products and sums in float
     launch_apply<float, double, float, float>( g, coeffs, x, ld_src, y,
                                                ld_tgt, n_fields, s );
+}
+
+std::size_t crs_apply_scratch_bytes( RowLayout const &g, int n_fields,
+                                     bool fp32_coeffs )
+{
+    if ( g.n_tgt == 0 || n_fields == 0 )
+        return 0;
+    std::size_t xs, ys;
+    if ( fp32_coeffs )
+        apply_scratch_sizes<double, float>( g, n_fields, xs, ys );
+    else
+        apply_scratch_sizes<double, double>( g, n_fields, xs, ys );
+    return xs + ys;
+}
+
+void crs_apply_f64_ws( RowLayout const &g, double const *coeffs,
+                       double const *x, int64_t ld_src, double *y,
+                       int64_t ld_tgt, int n_fields, void *ws, hipStream_t s )
+{
+    launch_apply<double, double, double, double>( g, coeffs, x, ld_src, y,
+                                                  ld_tgt, n_fields, s, ws );
+}
+
+void crs_apply_f32_dvals_ws( RowLayout const &g, float const *coeffs,
+                             double const *x, int64_t ld_src, double *y,
+                             int64_t ld_tgt, int n_fields, void *ws,
+                             hipStream_t s )
+{
+    launch_apply<float, double, float, float>( g, coeffs, x, ld_src, y,
+                                               ld_tgt, n_fields, s, ws );
 }
 
 void build_csc( RowLayout const &g, double const *coeffs, CscIndex &csc,

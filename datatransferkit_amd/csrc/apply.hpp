@@ -58,6 +58,21 @@ void crs_apply_f32_dvals( RowLayout const &g, float const *coeffs,
                           double const *x, int64_t ld_src, double *y,
                           int64_t ld_tgt, int n_fields, hipStream_t s );
 
+// Device scratch bytes the two applies above need for n_fields fields
+// (FP64 coefficients, or FP32 coefficients with double values).
+std::size_t crs_apply_scratch_bytes( RowLayout const &g, int n_fields,
+                                     bool fp32_coeffs );
+
+// The same applies with their scratch in ws (at least
+// crs_apply_scratch_bytes, 256-byte aligned): they allocate nothing.
+void crs_apply_f64_ws( RowLayout const &g, double const *coeffs,
+                       double const *x, int64_t ld_src, double *y,
+                       int64_t ld_tgt, int n_fields, void *ws, hipStream_t s );
+void crs_apply_f32_dvals_ws( RowLayout const &g, float const *coeffs,
+                             double const *x, int64_t ld_src, double *y,
+                             int64_t ld_tgt, int n_fields, void *ws,
+                             hipStream_t s );
+
 struct CscIndex
 {
     DeviceBuffer<int32_t> col_offsets; // [n_src + 1], by source position

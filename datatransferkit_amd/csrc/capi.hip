@@ -19,6 +19,7 @@
 #include <cstring>
 #include <cstdio>
 #include <memory>
+#include <mutex>
 #include <string>
 
 struct dtkx_mls
@@ -582,6 +583,74 @@ int dtkx_mls_apply( const dtkx_mls *h, const double *src_vals,
             dtkx::crs_apply_f64( h->layout(), h->coeffs.get(), x.ptr, ld_src,
                                  y.ptr, ld_tgt, n_fields, h->stream );
         y.finish( h->stream );
+        DTKX_HIP( hipStreamSynchronize( h->stream ) );
+        return DTKX_SUCCESS;
+    } );
+}
+
+// ------------------------------------------------ apply without allocation
+// The scratch of dtkx_mls_apply_noalloc: one array per GPU, allocated with
+// the library's code object and held until the process ends. Host code
+// never passes the array's name (a host-side stand-in) to a kernel: it
+// takes the device address from hipGetSymbolAddress once per device and
+// uses that.
+namespace
+{
+constexpr std::size_t kNoAllocScratchBytes = DTKX_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__( 256 ) unsigned char
+    g_noalloc_scratch[kNoAllocScratchBytes];
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mutex;
+void *g_noalloc_ptr[kNoAllocMaxDevices] = {};
+
+// the current device's copy of g_noalloc_scratch (looked up on its first use)
+void *noalloc_scratch_ptr()
+{
+    int dev = 0;
+    DTKX_HIP( hipGetDevice( &dev ) );
+    DTKX_REQUIRE( dev >= 0 && dev < kNoAllocMaxDevices, DTKX_ERR_UNSUPPORTED );
+    std::lock_guard<std::mutex> lock( g_noalloc_mutex );
+    if ( !g_noalloc_ptr[dev] )
+    {
+        void *p = nullptr;
+        DTKX_HIP( hipGetSymbolAddress( &p, HIP_SYMBOL( g_noalloc_scratch ) ) );
+        DTKX_REQUIRE( p != nullptr, DTKX_ERR_HIP );
+        g_noalloc_ptr[dev] = p;
+    }
+    return g_noalloc_ptr[dev];
+}
+} // namespace
+
+int dtkx_mls_apply_noalloc( const dtkx_mls *h, const double *src_vals,
+                            double *tgt_vals, int32_t n_fields,
+                            int64_t ld_src, int64_t ld_tgt )
+{
+    return guarded( [&]() {
+        REQ( h != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( n_fields >= 1, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( ld_src >= h->n_src, DTKX_ERR_SIZE_MISMATCH );
+        REQ( ld_tgt >= h->n_tgt, DTKX_ERR_SIZE_MISMATCH );
+        REQ( src_vals != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        REQ( h->n_tgt == 0 || tgt_vals != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        if ( h->n_tgt == 0 )
+            return DTKX_SUCCESS;
+        // host arrays would need staging buffers
+        REQ( dtkx::is_device_pointer( src_vals ), DTKX_ERR_INVALID_ARGUMENT );
+        REQ( dtkx::is_device_pointer( tgt_vals ), DTKX_ERR_INVALID_ARGUMENT );
+        bool const fp32 = h->opts.precision == DTKX_PRECISION_FP32_BULK;
+        std::size_t const need =
+            dtkx::crs_apply_scratch_bytes( h->layout(), n_fields, fp32 );
+        REQ( need <= kNoAllocScratchBytes, DTKX_ERR_UNSUPPORTED );
+        void *const ws = noalloc_scratch_ptr();
+        if ( fp32 )
+            dtkx::crs_apply_f32_dvals_ws( h->layout(), h->coeffs32.get(),
+                                          src_vals, ld_src, tgt_vals, ld_tgt,
+                                          n_fields, ws, h->stream );
+        else
+            dtkx::crs_apply_f64_ws( h->layout(), h->coeffs.get(), src_vals,
+                                    ld_src, tgt_vals, ld_tgt, n_fields, ws,
+                                    h->stream );
         DTKX_HIP( hipStreamSynchronize( h->stream ) );
         return DTKX_SUCCESS;
     } );

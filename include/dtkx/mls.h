@@ -26,6 +26,7 @@
 #ifndef DTKX_MLS_H
 #define DTKX_MLS_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -204,6 +205,26 @@ int dtkx_mls_create_with_neighbors(const double *src_xyz, int64_t n_src,
 int dtkx_mls_apply(const dtkx_mls *h, const double *src_vals,
                    double *tgt_vals, int32_t n_fields, int64_t ld_src,
                    int64_t ld_tgt);
+
+/* dtkx_mls_apply without any allocation. No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as dtkx_mls_apply (bit for bit), with these
+ * limits:
+ *   - src_vals and tgt_vals must be device memory (DTKX_ERR_INVALID_ARGUMENT
+ *     otherwise: host arrays would need staging buffers);
+ *   - the apply's scratch (the gathered fields, n_src records of up to 8
+ *     fields, and the staged row results, n_tgt records) must fit in
+ *     DTKX_NOALLOC_SCRATCH_BYTES (DTKX_ERR_UNSUPPORTED otherwise): FP64 with
+ *     4 fields fits n_src + n_tgt <= 2,097,152;
+ *   - the scratch is one fixed device array per GPU, shared by every call
+ *     on that GPU and held until the process ends: calls that overlap on
+ *     one GPU (other threads, other handles' streams) race; serialise them.
+ * Call it with the handle's device current. */
+#define DTKX_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int dtkx_mls_apply_noalloc(const dtkx_mls *h, const double *src_vals,
+                           double *tgt_vals, int32_t n_fields, int64_t ld_src,
+                           int64_t ld_tgt);
 
 /* FP32 apply: float values, float coefficients, float accumulation.
  * Requires a handle created with DTKX_PRECISION_FP32_BULK. */
