@@ -111,42 +111,99 @@ def rel_inf(y, ref):
     return (np.abs(y - ref).max(axis=1) / np.abs(ref).max(axis=1)).tolist()
 
 
+def cpu_model():
+    """The host CPU's model name and logical CPU count (from /proc/cpuinfo,
+    as lscpu reports them)."""
+    name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return name, os.cpu_count() or 1
+
+
+def cpu_threads():
+    """Threads for the CPU baseline: the host's CPU share for one GPU
+    (OMP_NUM_THREADS, 16 on the GPU hosts), at most the logical CPUs."""
+    share = int(os.environ.get("DTKX_CPU_THREADS") or os.environ.get("OMP_NUM_THREADS") or 16)
+    return max(1, min(os.cpu_count() or 1, share))
+
+
 def cpu_baseline(src, tgt, vals, m, k, rbf, sample):
     """The oracle (C++ restatement of the reference, OpenMP) on the first
     `sample` targets against all sources, same stages: kNN, setup, apply.
-    Returns the record and the oracle's target values."""
+    Returns the record and the oracle's target values. The parity build
+    (-O3, no FP contraction) is the reported baseline; the -march=native
+    speed build is timed beside it and labelled as such."""
     import oracle
 
-    threads = min(os.cpu_count() or 1, int(os.environ.get("DTKX_CPU_THREADS", "16")))
-    oracle.set_num_threads(threads)
-    # elementary functions correctly rounded, as the GPU kernels evaluate them
-    with oracle.libm_mode(oracle.LIBM_CORRECTLY_ROUNDED):
-        t0 = time.perf_counter()
-        idx, coeffs, _ = oracle.mls_create(src, tgt[:sample], m, k=k, rbf_name=rbf)
-        ref = oracle.apply(idx, coeffs, vals)
-        dt = time.perf_counter() - t0
+    threads = cpu_threads()
+    model, logical = cpu_model()
+
+    def run():
+        oracle.set_num_threads(threads)
+        # elementary functions correctly rounded, as the GPU kernels evaluate them
+        with oracle.libm_mode(oracle.LIBM_CORRECTLY_ROUNDED):
+            t0 = time.perf_counter()
+            idx, coeffs, _ = oracle.mls_create(src, tgt[:sample], m, k=k, rbf_name=rbf)
+            y = oracle.apply(idx, coeffs, vals)
+            return y, time.perf_counter() - t0
+
+    ref, dt = run()
+    what = (f"first {sample} of {len(tgt)} targets, all {len(src)} sources, k-d tree kNN + "
+            f"classical-Jacobi setup (correctly rounded libm) + apply of {vals.shape[0]} fields")
     rec = {
         "value": sample / dt,
         "unit": "target pts/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"first {sample} of {len(tgt)} targets, all {len(src)} sources, "
-                  f"k-d tree kNN + classical-Jacobi setup (correctly rounded libm) + "
-                  f"apply of {vals.shape[0]} fields ({dt:.1f} s)",
+        "sample": f"{what} ({dt:.1f} s; parity build -O3 -ffp-contract=off)",
+        "cpu_model": model,
+        "host_logical_cpus": logical,
+        "threads_note": "the host's CPU share for one GPU (OMP_NUM_THREADS)",
     }
+    try:
+        with oracle.speed_build():
+            _, dt_n = run()
+        rec["speed_build"] = {"value": sample / dt_n, "unit": "target pts/s",
+                              "cores": threads,
+                              "build": "-O3 -march=native -ffp-contract=fast (timing only)",
+                              "seconds": round(dt_n, 2)}
+    except Exception as e:  # the speed build is a label, not the baseline
+        rec["speed_build"] = {"error": str(e)[:200]}
     return rec, ref
 
 
-def load_traffic(key):
-    """HBM bytes per launch of a kernel at this workload, from two
-    rocprofv3 --pmc runs (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB, the
-    factors a 1 GiB copy gives), if measured."""
+TRAFFIC_SOURCE = ("profiles/kernel_traffic.json (committed; from two rocprofv3 --pmc runs "
+                  "of this workload, FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB; not measured "
+                  "in this run)")
+
+
+def load_traffic(*keys):
+    """HBM bytes per launch of a kernel (the sum over several) at this
+    workload, from two rocprofv3 --pmc runs (FETCH_SIZE x 2.0 + WRITE_SIZE x
+    1.0 KiB, the factors a 1 GiB copy gives), if all were measured."""
     path = os.path.join(REPO, "profiles", "kernel_traffic.json")
     try:
         with open(path) as fh:
-            return json.load(fh).get(key)
+            table = json.load(fh)
     except (OSError, ValueError):
         return None
+    vals = [table.get(k) for k in keys]
+    return None if any(v is None for v in vals) else sum(vals)
+
+
+# the apply's three kernels (FP64 or FP32 values), as kernel_traffic.json keys them
+APPLY_KERNELS = {
+    False: ["scatter_fields_kernel<double,double,4>", "crs_rows_kernel<double,double,double,double,4>",
+            "records_to_caller_kernel<double,4>"],
+    True: ["scatter_fields_kernel<float,float,4>", "crs_rows_kernel<float,float,float,float,4>",
+           "records_to_caller_kernel<float,4>"],
+}
 
 
 def query_kernel(geo, k):
@@ -174,15 +231,17 @@ def kernel_rooflines(form, r, n, m, kk, nnz, fields):
             "ms": r["ms_knn_query"], "achieved": round(query, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(query / HBM_PEAK_GBS, 5),
             "traffic": load_traffic(f"{query_kernel(geo, kk)}@{n}x{n}"),
+            "traffic_source": TRAFFIC_SOURCE,
             "algorithmic_per_unit": f"B_knn = 24 N_s + N_t (24 + 4k) = {b_knn(n, n, kk)} bytes "
                                     f"per launch ({n} queries)",
         },
         "setup": {
-            "bound": "mfma", "kernel": SETUP_KERNEL[form].replace("{m}", str(m)),
+            "bound": "fp64_valu", "kernel": SETUP_KERNEL[form].replace("{m}", str(m)),
             "ms": r["ms_setup"], "achieved": round(setup, 3), "peak": FP64_PEAK_TFLOPS,
             "unit": "TFLOP/s", "frac": round(setup / FP64_PEAK_TFLOPS, 5),
             "traffic": load_traffic(
                 f"{SETUP_KERNEL[form].replace('{m}', str(m)).split(' ')[0]}@{n}x{n},k{kk}"),
+            "traffic_source": TRAFFIC_SOURCE,
             "algorithmic_per_unit": (f"F_cholesky(m={m},k={kk}) = {f_cholesky(m, kk)}"
                                      if solver == "cholesky" else
                                      f"F_setup(m={m},k={kk}) = {f_setup(m, kk)}")
@@ -194,7 +253,9 @@ def kernel_rooflines(form, r, n, m, kk, nnz, fields):
             "bound": "hbm",
             "kernel": "scatter_fields_kernel + crs_rows_kernel + records_to_caller_kernel",
             "ms": r["ms_apply"], "achieved": round(apply, 2), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(apply / HBM_PEAK_GBS, 5), "traffic": None,
+            "unit": "GB/s", "frac": round(apply / HBM_PEAK_GBS, 5),
+            "traffic": load_traffic(*[f"{kname}@{n}x{n},k{kk}" for kname in APPLY_KERNELS[fp32]]),
+            "traffic_source": TRAFFIC_SOURCE,
             "algorithmic_per_unit": f"B_apply = {b_apply(nnz, n, n, fields, fp32)} bytes "
                                     f"per apply of {fields} fields",
         },

@@ -125,6 +125,46 @@ def get_libm_mode() -> int:
     return lib().dtkx_oracle_get_libm_mode()
 
 
+def native_path() -> str:
+    """The speed build's path, keyed by this host's CPU model and flags (a
+    build for another host's CPU may use instructions this one lacks)."""
+    import hashlib
+
+    key = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith(("model name", "flags")):
+                    key += line
+                if line.strip() == "":
+                    break
+    except OSError:
+        pass
+    tag = hashlib.sha1(key.encode()).hexdigest()[:12]
+    return os.path.join(HERE, "_build", f"libdtkx_oracle_native_{tag}.so")
+
+
+class speed_build:
+    """Context manager routing every oracle call to the -march=native,
+    FP-contracting build (built on first use on this host). For timing the
+    CPU baseline only: it rounds differently, so it is never a checker."""
+
+    def __enter__(self):
+        global _lib
+        path = native_path()
+        if not os.path.exists(path):
+            subprocess.run(["make", "-s", "-C", HERE, "native", f"NATIVE_OUT={path}"],
+                           check=True)
+        self.prev = lib()
+        _lib = ctypes.CDLL(path)
+        _declare(_lib)
+        return self
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self.prev
+
+
 class libm_mode:
     """Context manager selecting the oracle's elementary-function mode."""
 
