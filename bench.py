@@ -151,9 +151,9 @@ def cpu_baseline(src, tgt, vals, m, k, rbf, sample):
             t0 = time.perf_counter()
             idx, coeffs, _ = oracle.mls_create(src, tgt[:sample], m, k=k, rbf_name=rbf)
             y = oracle.apply(idx, coeffs, vals)
-            return y, time.perf_counter() - t0
+            return (y, idx), time.perf_counter() - t0
 
-    ref, dt = run()
+    (ref, idx), dt = run()
     what = (f"first {sample} of {len(tgt)} targets, all {len(src)} sources, k-d tree kNN + "
             f"classical-Jacobi setup (correctly rounded libm) + apply of {vals.shape[0]} fields")
     rec = {
@@ -175,7 +175,20 @@ def cpu_baseline(src, tgt, vals, m, k, rbf, sample):
                               "seconds": round(dt_n, 2)}
     except Exception as e:  # the speed build is a label, not the baseline
         rec["speed_build"] = {"error": str(e)[:200]}
-    return rec, ref
+    return rec, ref, idx
+
+
+def extended_truth(src, tgt, vals, idx, m, rbf, n):
+    """The operator evaluated in extended precision (float-rounded geometry
+    as the forms use it) on the first n targets, and the mask of systems
+    whose smallest moment eigenvalue is 1000 eps clear of the reference's
+    truncation threshold (elsewhere truncation depends on rounding)."""
+    import numpy as np
+    from oracle import extended
+
+    ce, lam = extended.float_geometry_operator(src, tgt[:n], idx[:n], m, rbf)
+    truth = np.einsum("tk,ftk->ft", ce, vals[:, idx[:n]])
+    return truth, lam > 1e3 * np.finfo(np.float64).eps
 
 
 TRAFFIC_SOURCE = ("profiles/kernel_traffic.json (committed; from two rocprofv3 --pmc runs "
@@ -886,11 +899,21 @@ def main():
     }
     if not args.no_cpu_baseline:
         s = min(args.cpu_sample, n) if args.cpu_sample > 0 else n
-        rec, ref = cpu_baseline(src, tgt, vals, m, kk, args.rbf, s)
+        rec, ref, idx = cpu_baseline(src, tgt, vals, m, kk, args.rbf, s)
         out["cpu_baseline"] = rec
         for name, y in outputs.items():
             results[name]["rel_err_vs_oracle_sample"] = rel_inf(y[:, :s], ref)
             results[name]["bit_exact_vs_oracle_sample"] = bool(np.array_equal(y[:, :s], ref))
+        if args.rbf in ("wendland0", "wendland2", "wendland4", "wendland6"):
+            ne = min(2000, s)
+            truth, clear = extended_truth(src, tgt, vals, idx, m, args.rbf, ne)
+            out["extended_sample"] = (f"first {ne} targets; {int(clear.sum())} systems 1000 eps "
+                                      f"clear of the truncation threshold compared")
+            out["oracle_rel_err_vs_extended_sample"] = rel_inf(ref[:, :ne][:, clear],
+                                                               truth[:, clear])
+            for name, y in outputs.items():
+                results[name]["rel_err_vs_extended_sample"] = rel_inf(
+                    y[:, :ne][:, clear], truth[:, clear])
     print(json.dumps(out), flush=True)
 
 

@@ -183,17 +183,15 @@ __device__ inline bool converged( double const *A )
     return !( off > tol * diag );
 }
 
-// Certified Cholesky inverse. A (packed upper triangle, trace tr) becomes
-// V = U^-1 in place, where A = U^T U (left-looking, each pivot's
-// reciprocal kept so a column divides once; non-positive pivots are
-// replaced by one and reported). Returns whether every eigenvalue of A is
-// proven above the truncation threshold: lambda_min(A) = 1 / |V|_2^2 >=
-// 1 / |V|_F^2 > 4 thr + 64 m eps trace, thr = DBL_EPSILON or rcond * trace
-// (>= rcond * lambda_max), the margin covering the rounding of the
-// eigenvalues an eigen-solver would compute. Then A^+ = A^-1 = V V^T.
+// Cholesky inverse factor. A (packed upper triangle) becomes V = U^-1 in
+// place, where A = U^T U (left-looking, each pivot's reciprocal kept so a
+// column divides once; non-positive pivots are replaced by one and
+// reported: the return value is whether every pivot was positive). Then
+// A^-1 = V V^T. Cholesky's rounding error is bounded relative to
+// sqrt(a_ii a_jj), so its accuracy is that of the equilibrated matrix
+// D A D (D = diag(a_ii^-1/2)), whatever the monomials' scales.
 template <int M>
-__device__ inline bool cholesky_inverse( double *A, double tr, int truncation,
-                                         double rcond )
+__device__ inline bool cholesky_factor_inverse( double *A )
 {
     using S = SymIdx<M>;
     bool ok = true;
@@ -234,6 +232,21 @@ __device__ inline bool cholesky_inverse( double *A, double tr, int truncation,
         }
         A[S::at( j, j )] = rinv[j];
     }
+    return ok;
+}
+
+// Certified Cholesky inverse: cholesky_factor_inverse, then whether every
+// eigenvalue of A (trace tr) is proven above the truncation threshold:
+// lambda_min(A) = 1 / |V|_2^2 >= 1 / |V|_F^2 > 4 thr + 64 m eps trace,
+// thr = DBL_EPSILON or rcond * trace (>= rcond * lambda_max), the margin
+// covering the rounding of the eigenvalues an eigen-solver would compute.
+// Then A^+ = A^-1 = V V^T.
+template <int M>
+__device__ inline bool cholesky_inverse( double *A, double tr, int truncation,
+                                         double rcond )
+{
+    using S = SymIdx<M>;
+    bool const ok = cholesky_factor_inverse<M>( A );
     double fro = 0.0;
 #pragma unroll
     for ( int e = 0; e < S::size; ++e )

@@ -14,7 +14,9 @@
 // of its diagonal mass, or to the sweep cap (a converged system idles while
 // the rest of its wave finishes, so its bits do not depend on its wave).
 // Truncation keeps the reference's semantics: eigenvalues with |l| >=
-// DBL_EPSILON (or rcond * max|l|) contribute.
+// DBL_EPSILON (or rcond * max|l|) contribute. Rows listed by the Cholesky
+// setup (systems it could not certify) whose eigenvalues all pass are
+// solved again by Cholesky (setup_fast.hip).
 // Neighbour data is re-gathered from L2 in three passes (radius, moments,
 // coefficients) instead of being staged.
 #include "cyclic_device.hpp"
@@ -34,6 +36,9 @@ constexpr int kBlock = 64;
 // 256 VGPRs): every lane of the pair keeps the whole moment matrix and runs
 // the same rotations on it (identical values, so both agree), and owns
 // R = M / H rows of V.
+// refine_rows (the Cholesky setup's fallback rows, nullable): systems whose
+// eigenvalues all pass the truncation test are listed there, to be solved
+// again by Cholesky (setup_fast.hip, refine_kernel).
 template <int M, int H, bool F32, bool STORE>
 __device__ __forceinline__ void
     cyclic_system( int64_t const li, double4 const *__restrict__ src,
@@ -49,7 +54,9 @@ __device__ __forceinline__ void
                          float *__restrict__ coeffs32,
                          double *__restrict__ pinv_out,
                          unsigned long long *__restrict__ counters,
-                         float const *__restrict__ row_max )
+                         float const *__restrict__ row_max,
+                         int32_t *__restrict__ refine_rows,
+                         int32_t *__restrict__ n_refine )
 {
     using S = SymIdx<M>;
     constexpr int R = M / H;
@@ -247,6 +254,8 @@ __device__ __forceinline__ void
             coeffs[rbeg + l] = acc * phi;
         }
     }
+    if ( refine_rows && h == 0 && !under )
+        refine_rows[atomicAdd( n_refine, 1 )] = (int32_t)slot;
     if ( counters && h == 0 )
     {
         if ( under )
@@ -276,14 +285,17 @@ __global__ void __launch_bounds__( kBlock )
                          float *__restrict__ coeffs32,
                          double *__restrict__ pinv_out,
                          unsigned long long *__restrict__ counters,
-                         float const *__restrict__ row_max )
+                         float const *__restrict__ row_max,
+                         int32_t *__restrict__ refine_rows,
+                         int32_t *__restrict__ n_refine )
 {
     constexpr int SPB = kBlock / H;
     auto one = [&]( int64_t tile ) {
         cyclic_system<M, H, F32, STORE>(
             tile * SPB + threadIdx.x / H, src, tgt, n_tgt, indices, order, row0,
             row_off, rows, n_rows, kk_uniform, rbf, geometry, truncation, rcond,
-            max_sweeps, coeffs, coeffs32, pinv_out, counters, row_max );
+            max_sweeps, coeffs, coeffs32, pinv_out, counters, row_max,
+            refine_rows, n_refine );
     };
     if constexpr ( LIST )
     {
@@ -318,7 +330,7 @@ void launch_cyclic( double4 const *src, double4 const *tgt, int64_t n_tgt,
                                      p.row_off, p.rows, p.n_rows, p.kk,
                                      p.rbf, p.geometry, p.truncation, p.rcond,
                                      sweeps, coeffs, coeffs32, pinv, counters,
-                                     p.row_max );
+                                     p.row_max, p.refine_rows, p.n_refine );
     };
     using T = std::true_type;
     using F = std::false_type;

@@ -20,6 +20,7 @@
 #include "cyclic_device.hpp"
 #include "setup.hpp"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace dtkx
@@ -242,6 +243,98 @@ __global__ void __launch_bounds__( kBlock, 2 )
     }
 }
 
+// The eigen-solver's untruncated rows again, one lane per listed row: the
+// moments, their Cholesky inverse factor and the row's coefficients (a
+// non-positive pivot leaves the eigen-solver's coefficients in place). A
+// bounded grid strides over the list, whose length only the device knows.
+constexpr int kRefineBlocks = 1024;
+
+template <int M, bool F32>
+__global__ void __launch_bounds__( kBlock )
+    refine_kernel( double4 const *__restrict__ src,
+                   double4 const *__restrict__ tgt,
+                   int32_t const *__restrict__ indices,
+                   int32_t const *__restrict__ order,
+                   int64_t const *__restrict__ row_off, int kk_uniform,
+                   int rbf, int geometry, int32_t const *__restrict__ rows,
+                   int32_t const *__restrict__ n_rows,
+                   double *__restrict__ coeffs, float *__restrict__ coeffs32,
+                   float const *__restrict__ row_max )
+{
+    using S = SymIdx<M>;
+    int64_t const n = *n_rows;
+    for ( int64_t li = xcd_tile() * kBlock + threadIdx.x; li < n;
+          li += (int64_t)gridDim.x * kBlock )
+    {
+        int64_t const slot = rows[li];
+        int64_t const t = order ? order[slot] : slot;
+        int64_t const rbeg = row_off ? row_off[slot] : slot * kk_uniform;
+        int const kk =
+            row_off ? (int)( row_off[slot + 1] - rbeg ) : kk_uniform;
+        int32_t const *nbr = indices + rbeg;
+        double4 const tp = tgt[t];
+        double dmax = 10. * DBL_EPSILON;
+        if ( F32 && row_max )
+        {
+            double const d = (double)row_max[slot];
+            dmax = d > dmax ? d : dmax;
+        }
+        else
+            for ( int l = 0; l < kk; ++l )
+            {
+                double const d =
+                    nbr_distance<M, F32>( geometry, src[nbr[l]], tp );
+                dmax = d > dmax ? d : dmax;
+            }
+        double const radius =
+            F32 ? (double)( 1.1f * (float)dmax ) : 1.1 * dmax;
+        double A[S::size];
+#pragma unroll
+        for ( int e = 0; e < S::size; ++e )
+            A[e] = 0.0;
+        for ( int l = 0; l < kk; ++l )
+        {
+            double P[M];
+            double const phi =
+                nbr_row<M, F32>( geometry, rbf, radius, src[nbr[l]], tp, P );
+#pragma unroll
+            for ( int r = 0; r < M; ++r )
+            {
+                double const pr = P[r] * phi;
+#pragma unroll
+                for ( int c = r; c < M; ++c )
+                    A[S::at( r, c )] = fma( pr, P[c], A[S::at( r, c )] );
+            }
+        }
+        if ( !cholesky_factor_inverse<M>( A ) )
+            continue;
+        double x[M];
+#pragma unroll
+        for ( int i = 0; i < M; ++i )
+        {
+            double acc = 0.0;
+#pragma unroll
+            for ( int k = i; k < M; ++k )
+                acc = fma( A[S::at( i, k )], A[S::at( 0, k )], acc );
+            x[i] = acc;
+        }
+        for ( int l = 0; l < kk; ++l )
+        {
+            double P[M];
+            double const phi =
+                nbr_row<M, F32>( geometry, rbf, radius, src[nbr[l]], tp, P );
+            double acc = 0.0;
+#pragma unroll
+            for ( int q = 0; q < M; ++q )
+                acc = fma( x[q], P[q], acc );
+            if constexpr ( F32 )
+                coeffs32[rbeg + l] = (float)( acc * phi );
+            else
+                coeffs[rbeg + l] = acc * phi;
+        }
+    }
+}
+
 template <int M>
 void launch_fast( double4 const *src, double4 const *tgt, int64_t n_tgt,
                   int32_t const *indices, int32_t const *order,
@@ -290,12 +383,45 @@ void setup_fast( double4 const *src, double4 const *tgt, int64_t n_tgt,
         launch_fast<10>( src, tgt, n_tgt, indices, order, p, coeffs, coeffs32,
                          pinv, list.get(), count.get(), counters, s );
     // the listed systems through the eigen-solver (a grid for every row;
-    // blocks past the device-side count return at once)
+    // blocks past the device-side count return at once); with no stored
+    // pseudo-inverse to keep consistent, its untruncated rows are listed
+    // again and solved by Cholesky
     SetupParams q = p;
     q.rows = list.get();
     q.n_rows = count.get();
+    DeviceBuffer<int32_t> rlist( pinv ? 0 : n_tgt - p.row0, s ),
+        rcount( pinv ? 0 : 1, s );
+    if ( !pinv )
+    {
+        DTKX_HIP( hipMemsetAsync( rcount.get(), 0, sizeof( int32_t ), s ) );
+        q.refine_rows = rlist.get();
+        q.n_refine = rcount.get();
+    }
     setup_cyclic( src, tgt, n_tgt, indices, order, q, coeffs, coeffs32, pinv,
                   counters, s );
+    if ( pinv )
+        return;
+    int64_t const blocks =
+        std::min<int64_t>( ceil_div( n_tgt - p.row0, kBlock ), kRefineBlocks );
+    auto refine = [&]( auto m, auto f32 ) {
+        refine_kernel<decltype( m )::value, decltype( f32 )::value>
+            <<<blocks, kBlock, 0, s>>>( src, tgt, indices, order, p.row_off,
+                                         p.kk, p.rbf, p.geometry, rlist.get(),
+                                         rcount.get(), coeffs, coeffs32,
+                                         p.row_max );
+    };
+    using T = std::true_type;
+    using F = std::false_type;
+    using M1 = std::integral_constant<int, 1>;
+    using M4 = std::integral_constant<int, 4>;
+    using M10 = std::integral_constant<int, 10>;
+    if ( p.m == 1 )
+        coeffs32 ? refine( M1{}, T{} ) : refine( M1{}, F{} );
+    else if ( p.m == 4 )
+        coeffs32 ? refine( M4{}, T{} ) : refine( M4{}, F{} );
+    else
+        coeffs32 ? refine( M10{}, T{} ) : refine( M10{}, F{} );
+    DTKX_HIP( hipGetLastError() );
 }
 
 } // namespace dtkx

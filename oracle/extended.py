@@ -104,3 +104,31 @@ def target_gradient_fd(src, tgt, nbr, x, ybar, m, rbf="wendland2", h=1e-6):
         lm = (target_values(src, tm, nbr, x, m, rbf) * ybar).sum(0)
         g[:, c] = (lp - lm) / (2 * LD(h))
     return g.astype(np.float64)
+
+
+def float_geometry_operator(src, tgt, nbr, m, rbf="wendland2"):
+    """The operator's coefficients [n_t, k] for ArborXFloat geometry (the
+    library's default and the reference's): relative coordinates rounded to
+    float, float monomial products and float distances exactly as the
+    oracle forms them (numpy float32 arithmetic, no contraction), radius
+    R = 1.1 max(10 eps, max_j d_j) in double, then the moments and the solve
+    A^-1 e_0 in extended precision (no truncation). Returns (coeffs,
+    lambda_min), the smallest eigenvalue of the double moment matrix, by
+    which a caller can leave out the systems whose truncation is in doubt."""
+    src = np.asarray(src, dtype=np.float64)
+    tgt = np.asarray(tgt, dtype=np.float64)
+    nbr = np.asarray(nbr)
+    u = (src[nbr] - tgt[:, None, :]).astype(np.float32)
+    d = np.sqrt((u[..., 0] * u[..., 0] + u[..., 1] * u[..., 1]) + u[..., 2] * u[..., 2])
+    dmax = np.maximum(d.max(axis=1).astype(np.float64), 10 * np.finfo(np.float64).eps)
+    r = 1.1 * dmax
+    x = d.astype(np.float64) / r[:, None]
+    phi = _rbf(rbf, x.astype(LD))
+    p = _basis(m, u).astype(np.float64)  # float products, exact in double
+    a = np.einsum("tki,tk,tkj->tij", p.astype(LD), phi, p.astype(LD))
+    e0 = np.zeros((len(tgt), m, 1), dtype=LD)
+    e0[:, 0, 0] = 1
+    a0 = _solve(a, e0)[:, :, 0]
+    c = np.einsum("ti,tki->tk", a0, p.astype(LD)) * phi
+    lam = np.linalg.eigvalsh(a.astype(np.float64))[:, 0]
+    return c.astype(np.float64), lam
