@@ -105,7 +105,29 @@ DTKX_HD dd dd_add_d( dd a, double b )
     return fast_two_sum( s.hi, s.lo );
 }
 
+// a + b for |a| >= |b| (or a = 0): one fast_two_sum of the leading parts
+DTKX_HD dd dd_add_ordered( dd a, dd b )
+{
+    DTKX_NO_CONTRACT
+    dd s = fast_two_sum( a.hi, b.hi );
+    s.lo += a.lo + b.lo;
+    return fast_two_sum( s.hi, s.lo );
+}
+
 DTKX_HD dd dd_neg( dd a ) { return {-a.hi, -a.lo}; }
+
+// Ziv's rounding test: v approximates the exact value to a relative error
+// below rel; if both ends of the interval round to the same double, that
+// double is the correctly rounded result.
+DTKX_HD bool dd_round_certain( dd v, double rel, double *out )
+{
+    DTKX_NO_CONTRACT
+    double const e = rel * std::fabs( v.hi );
+    double const up = v.hi + ( v.lo + e );
+    double const dn = v.hi + ( v.lo - e );
+    *out = v.hi + v.lo;
+    return up == dn;
+}
 
 DTKX_HD dd dd_mul( dd a, dd b )
 {
@@ -293,34 +315,59 @@ DTKX_HD double sin_cos_q( double x, int shift )
     return v.hi + v.lo;
 }
 
-DTKX_HD double sin( double x ) { return sin_cos_q( x, 0 ); }
-DTKX_HD double cos( double x ) { return sin_cos_q( x, 1 ); }
 
-// sin and cos of one argument sharing the reduction; each result is the
-// same correctly rounded value sin() and cos() return.
-DTKX_HD void sincos( double x, double *s, double *c )
+// sin (shift = 0) or cos (shift = 1) of x from one polynomial, whatever the
+// quadrant: with w = -r^2, sin(r) = r sum_n w^n / (2n+1)! and cos(r) =
+// sum_n w^n / (2n)! differ only in their coefficients and the final factor
+// r, so lanes of one wave that need different functions (or quadrants) run
+// the same instructions. Terms n <= 4 in double-double (quick ordered
+// additions: each term is below the next lower one), n >= 5 in double:
+// the double tail's rounding is below 2^-77 of the result on |r| <= pi/4
+// (r^10 / 10! < 2.5e-8), so with the rounding test at 2^-70 the result is
+// the correctly rounded one, or the full evaluation (sin_cos_q) decides.
+DTKX_HD double sin_or_cos( double x, int shift )
 {
     DTKX_NO_CONTRACT
-    if ( !( std::fabs( x ) < 1.6e6 ) )
-    {
-        *s = std::sin( x );
-        *c = std::cos( x );
-        return;
-    }
+    if ( !( std::fabs( x ) < 1.6e6 ) || ( shift == 0 && std::fabs( x ) < 0x1p-27 ) )
+        return sin_cos_q( x, shift );
     int q;
     dd const r = reduce_pio2( x, q );
-    dd const ps = sin_poly( r );
-    dd const pc = cos_poly( r );
-    // sin(x) uses quadrant q, cos(x) = sin(x + pi/2) quadrant q + 1
-    int const qs = q & 3, qc = ( q + 1 ) & 3;
-    dd vs = ( qs & 1 ) ? pc : ps;
-    if ( qs & 2 )
-        vs = dd_neg( vs );
-    dd vc = ( qc & 1 ) ? pc : ps;
-    if ( qc & 2 )
-        vc = dd_neg( vc );
-    *s = std::fabs( x ) < 0x1p-27 ? x : vs.hi + vs.lo;
-    *c = vc.hi + vc.lo;
+    q = ( q + shift ) & 3;
+    bool const c = ( q & 1 ) != 0; // cos polynomial in this quadrant
+    dd const w = dd_neg( dd_mul( r, r ) );
+    double const wd = w.hi;
+    // coefficients picked by a select between two compile-time constants
+    // (a table index that varies across lanes would branch)
+    auto coef = [c]( int n ) -> dd {
+        dd const kc = inv_fact( 2 * n ), ks = inv_fact( 2 * n + 1 );
+        return {c ? kc.hi : ks.hi, c ? kc.lo : ks.lo};
+    };
+    double t = coef( 13 ).hi;
+#pragma unroll
+    for ( int n = 12; n >= 5; --n )
+        t = t * wd + coef( n ).hi;
+    dd acc = {t, 0.0};
+#pragma unroll
+    for ( int n = 4; n >= 0; --n )
+        acc = dd_add_ordered( coef( n ), dd_mul( acc, w ) );
+    dd v = c ? acc : dd_mul( acc, r );
+    if ( q & 2 )
+        v = dd_neg( v );
+    double res;
+    if ( dd_round_certain( v, 0x1p-70, &res ) )
+        return res;
+    return sin_cos_q( x, shift );
+}
+
+DTKX_HD double sin( double x ) { return sin_or_cos( x, 0 ); }
+DTKX_HD double cos( double x ) { return sin_or_cos( x, 1 ); }
+
+// sin and cos of one argument, each the correctly rounded value sin() and
+// cos() return
+DTKX_HD void sincos( double x, double *s, double *c )
+{
+    *s = sin_or_cos( x, 0 );
+    *c = sin_or_cos( x, 1 );
 }
 
 // ---------------------------------------------------------------- atan2
@@ -354,9 +401,43 @@ DTKX_HD dd atan_01( dd t )
     return dd_add( atan_j16( j ), at );
 }
 
-DTKX_HD double atan2( double y, double x )
+// atan(t) for a double-double 0 <= t <= 1 with fewer double-double terms:
+// atan(u) = u - u^3/3 + u^5 S(u^2), the last in double. |u| <= 1/32, so
+// u^5 S is below 2^-25 u and its rounding below 2^-76 of the result; the
+// caller's rounding test (2^-70) sends the rare undecided case to atan_01.
+DTKX_HD dd atan_01_quick( dd t )
 {
     DTKX_NO_CONTRACT
+    int j = static_cast<int>( t.hi * 16.0 + 0.5 );
+    j = j < 0 ? 0 : ( j > 16 ? 16 : j );
+    double c = j * 0.0625;
+    dd u;
+    if ( j == 0 )
+        u = t;
+    else
+    {
+        dd num = dd_add_d( t, -c );
+        dd den = dd_add_d( dd_mul_d( t, c ), 1.0 );
+        u = dd_div( num, den );
+    }
+    double const uh = u.hi, wd = uh * uh;
+    // s = sum_{n=2..11} (-1)^n w^(n-2) / (2n+1) = 1/5 - w/7 + w^2/9 - ...
+    double s = -1.0 / 23.0;
+#pragma unroll
+    for ( int n = 10; n >= 2; --n )
+        s = s * wd + ( ( n & 1 ) ? -1.0 : 1.0 ) / ( 2 * n + 1 );
+    double const tail = uh * wd * wd * s;
+    dd const u3 = dd_mul( dd_mul( u, u ), u );
+    dd const c3 = dd_mul( u3, inv_odd( 1 ) );
+    dd a = dd_add_ordered( dd_neg( c3 ), dd{tail, 0.0} );
+    a = dd_add_ordered( u, a );
+    return dd_add( atan_j16( j ), a );
+}
+
+DTKX_HD double atan2_core( double y, double x, bool quick, bool *certain )
+{
+    DTKX_NO_CONTRACT
+    *certain = true;
     if ( std::isnan( x ) || std::isnan( y ) )
         return x + y;
     bool const xneg = std::signbit( x );
@@ -398,14 +479,28 @@ DTKX_HD double atan2( double y, double x )
         double r = std::fma( -q1, den, num ); // exact remainder
         double q2 = r / den;
         dd t = fast_two_sum( q1, q2 );
-        v = atan_01( t );
+        v = quick ? atan_01_quick( t ) : atan_01( t );
     }
     if ( swap )
         v = dd_add( dd{DTKX_PIO2_HI, DTKX_PIO2_LO}, dd_neg( v ) );
     if ( xneg )
         v = dd_add( dd{DTKX_PI_HI, DTKX_PI_LO}, dd_neg( v ) );
     double res = v.hi + v.lo;
+    if ( quick )
+        *certain = dd_round_certain( v, 0x1p-70, &res );
     return std::copysign( res, y );
+}
+
+// Correctly rounded: the quick evaluation when its rounding test decides,
+// else the full one (both give the same double whenever the quick one
+// accepts).
+DTKX_HD double atan2( double y, double x )
+{
+    bool certain;
+    double const v = atan2_core( y, x, true, &certain );
+    if ( certain )
+        return v;
+    return atan2_core( y, x, false, &certain );
 }
 
 // ------------------------------------------------------------------ log

@@ -179,17 +179,28 @@ __global__ void __launch_bounds__( kWave )
     bool active = valid;
     long rot = 0;
     bool unconverged = false;
+    // the off-diagonal entries of this lane's row-major chunk (e < M*M and
+    // e != r (M + 1)), fixed for the whole iteration
+    int const e0 = lane * CS;
+    uint64_t offdiag = 0;
+#pragma unroll
+    for ( int i = 0; i < CS; ++i )
+        if ( e0 + i < MM && ( e0 + i ) % ( M + 1 ) != 0 )
+            offdiag |= uint64_t( 1 ) << i;
     while ( true )
     {
         // fused pass over this lane's row-major chunk: partial off-diagonal
-        // norm and the first strict maximum of |E(i,j)|, i != j
+        // norm and the first strict maximum of |E(i,j)|, i != j (ascending
+        // e with a strict compare keeps the first). Branch-free: diagonal
+        // and out-of-range entries add zero to the norm and enter the
+        // maximum as -1, its starting value. The partial norm only decides
+        // against a margin (below), so it may use fma.
         double part = 0.0, mx = -1.0;
         int mi = INT32_MAX;
         if ( active )
         {
             // compile-time trip count: all LDS loads of the chunk issue
             // back to back instead of one load latency per entry
-            int const e0 = lane * CS;
             double v[CS];
 #pragma unroll
             for ( int i = 0; i < CS; ++i )
@@ -197,18 +208,13 @@ __global__ void __launch_bounds__( kWave )
 #pragma unroll
             for ( int i = 0; i < CS; ++i )
             {
-                int const e = e0 + i;
-                // diagonal entries are e = r (M + 1)
-                if ( e < MM && e % ( M + 1 ) != 0 )
-                {
-                    part += v[i] * v[i];
-                    double const av = fabs( v[i] );
-                    if ( av > mx )
-                    {
-                        mx = av;
-                        mi = e;
-                    }
-                }
+                bool const od = ( offdiag >> i ) & 1;
+                double const vo = od ? v[i] : 0.0; // a NaN still propagates
+                part = fma( vo, vo, part );
+                double const av = od ? fabs( v[i] ) : -1.0;
+                bool const gt = av > mx;
+                mx = gt ? av : mx;
+                mi = gt ? e0 + i : mi;
             }
         }
         // group reductions: the sum in a fixed tree order; the maximum with
@@ -271,16 +277,26 @@ __global__ void __launch_bounds__( kWave )
             double const a00 = E[p * M + p], a01 = E[p * M + q];
             double const a10 = E[q * M + p], a11 = E[q * M + q];
             // the two angle chains of svd_2x2 run on lanes 0 and 1 of the
-            // group (identical code on different inputs)
+            // group (identical code on different inputs); with four lanes
+            // or more, lanes 2 and 3 repeat the angles and each of the four
+            // takes one of sin/cos (one polynomial, see crm::sin_or_cos)
             double y, x;
             dev::svd_2x2_angle_args( a00, a01, a10, a11, ( lane & 1 ) != 0, y,
                                      x );
             double const ang = 0.5 * crm::atan2( y, x );
-            double sa, ca;
-            crm::sincos( ang, &sa, &ca );
             double cphi, sphi, ctheta, stheta;
-            if ( G > 1 )
+            if ( G >= 4 )
             {
+                double const sc = crm::sin_or_cos( ang, ( lane >> 1 ) & 1 );
+                sphi = __shfl( sc, gbase, kWave );
+                stheta = __shfl( sc, gbase + 1, kWave );
+                cphi = __shfl( sc, gbase + 2, kWave );
+                ctheta = __shfl( sc, gbase + 3, kWave );
+            }
+            else if ( G > 1 )
+            {
+                double sa, ca;
+                crm::sincos( ang, &sa, &ca );
                 cphi = __shfl( ca, gbase, kWave );
                 sphi = __shfl( sa, gbase, kWave );
                 ctheta = __shfl( ca, gbase + 1, kWave );
@@ -288,6 +304,8 @@ __global__ void __launch_bounds__( kWave )
             }
             else
             {
+                double sa, ca;
+                crm::sincos( ang, &sa, &ca );
                 cphi = ca;
                 sphi = sa;
                 double y2, x2;
