@@ -157,6 +157,23 @@ DTKX_HD dd dd_div( dd a, dd b )
     return dd_add_d( q, q3 );
 }
 
+// a / b with one double division (a reciprocal): each correction term is
+// the remainder times 1/b.hi, so the terms' errors shrink geometrically as
+// in dd_div and the double-double is as accurate, with a third of the
+// (long, dependent) division sequences.
+DTKX_HD dd dd_div_rcp( dd a, dd b )
+{
+    DTKX_NO_CONTRACT
+    double const inv = 1.0 / b.hi;
+    double q1 = a.hi * inv;
+    dd r = dd_add( a, dd_neg( dd_mul_d( b, q1 ) ) );
+    double q2 = r.hi * inv;
+    r = dd_add( r, dd_neg( dd_mul_d( b, q2 ) ) );
+    double q3 = r.hi * inv;
+    dd q = fast_two_sum( q1, q2 );
+    return dd_add_d( q, q3 );
+}
+
 // ------------------------------------------------------------ constants
 // pi/2 = PIO2_1 + PIO2_2 + PIO2_3 + PIO2_4 (+ < 2^-176); the first two
 // carry 33 significant bits so k * PIO2_{1,2} is exact for |k| < 2^20.
@@ -342,10 +359,15 @@ DTKX_HD double sin_or_cos( double x, int shift )
         dd const kc = inv_fact( 2 * n ), ks = inv_fact( 2 * n + 1 );
         return {c ? kc.hi : ks.hi, c ? kc.lo : ks.lo};
     };
-    double t = coef( 13 ).hi;
-#pragma unroll
-    for ( int n = 12; n >= 5; --n )
-        t = t * wd + coef( n ).hi;
+    // the double tail sum_{n=5..13} c_n w^(n-5) in Estrin's order (depth 4
+    // instead of 9)
+    double const w2 = wd * wd, w4 = w2 * w2;
+    double const p56 = coef( 5 ).hi + coef( 6 ).hi * wd;
+    double const p78 = coef( 7 ).hi + coef( 8 ).hi * wd;
+    double const p910 = coef( 9 ).hi + coef( 10 ).hi * wd;
+    double const p1112 = coef( 11 ).hi + coef( 12 ).hi * wd;
+    double const t = ( p56 + p78 * w2 ) + ( p910 + p1112 * w2 ) * w4 +
+                     coef( 13 ).hi * ( w4 * w4 );
     dd acc = {t, 0.0};
 #pragma unroll
     for ( int n = 4; n >= 0; --n )
@@ -418,15 +440,17 @@ DTKX_HD dd atan_01_quick( dd t )
     {
         dd num = dd_add_d( t, -c );
         dd den = dd_add_d( dd_mul_d( t, c ), 1.0 );
-        u = dd_div( num, den );
+        u = dd_div_rcp( num, den );
     }
-    double const uh = u.hi, wd = uh * uh;
+    double const uh = u.hi, w = uh * uh, w2 = w * w, w4 = w2 * w2;
     // s = sum_{n=2..11} (-1)^n w^(n-2) / (2n+1) = 1/5 - w/7 + w^2/9 - ...
-    double s = -1.0 / 23.0;
-#pragma unroll
-    for ( int n = 10; n >= 2; --n )
-        s = s * wd + ( ( n & 1 ) ? -1.0 : 1.0 ) / ( 2 * n + 1 );
-    double const tail = uh * wd * wd * s;
+    // in Estrin's order (depth 4 instead of 10: the chain is the cost)
+    auto k = []( int n ) { return ( ( n & 1 ) ? -1.0 : 1.0 ) / ( 2 * n + 1 ); };
+    double const p23 = k( 2 ) + k( 3 ) * w, p45 = k( 4 ) + k( 5 ) * w;
+    double const p67 = k( 6 ) + k( 7 ) * w, p89 = k( 8 ) + k( 9 ) * w;
+    double const p1011 = k( 10 ) + k( 11 ) * w;
+    double const s = ( p23 + p45 * w2 ) + ( p67 + p89 * w2 ) * w4 + p1011 * ( w4 * w4 );
+    double const tail = uh * w2 * s;
     dd const u3 = dd_mul( dd_mul( u, u ), u );
     dd const c3 = dd_mul( u3, inv_odd( 1 ) );
     dd a = dd_add_ordered( dd_neg( c3 ), dd{tail, 0.0} );
@@ -473,13 +497,22 @@ DTKX_HD double atan2_core( double y, double x, bool quick, bool *certain )
         double q = ( swap ? ax : ay ) / ( swap ? ay : ax );
         v = {q, 0.0};
     }
+    else if ( quick )
+    {
+        // one reciprocal: q1 within two ulps, the fma remainder exact
+        double const inv = 1.0 / den;
+        double q1 = num * inv;
+        double r = std::fma( -q1, den, num );
+        double q2 = r * inv;
+        v = atan_01_quick( fast_two_sum( q1, q2 ) );
+    }
     else
     {
         double q1 = num / den;
         double r = std::fma( -q1, den, num ); // exact remainder
         double q2 = r / den;
         dd t = fast_two_sum( q1, q2 );
-        v = quick ? atan_01_quick( t ) : atan_01( t );
+        v = atan_01( t );
     }
     if ( swap )
         v = dd_add( dd{DTKX_PIO2_HI, DTKX_PIO2_LO}, dd_neg( v ) );
