@@ -149,3 +149,34 @@ def test_batch_errors(dtk):
     with pytest.raises(dtk.DtkxError) as e:
         dtk.MovingLeastSquaresBatch([(s, t, "linear", 99)])
     assert e.value.status == 7
+
+
+@pytest.mark.parametrize("form", FORMS[1:], ids=lambda f: f"{f['solver']}-{f['precision']}")
+def test_batch_fast_solvers_match_oracle(dtk, oracle_lib, form):
+    """The fast solvers in the batch against the FP64 oracle, problem by
+    problem, on overdetermined systems (at least 1.6 m neighbours, at least
+    2 m sources): within 1e-6 relative (infinity norm) for the FP32-bulk
+    form and 1e-9 for the FP64 forms (the oracle's own rounding on these
+    systems is below 1e-10)."""
+    rng = np.random.default_rng(31)
+    probs = []
+    for s, t, b, k, v in make_problems(40, seed=21, max_src=3000):
+        m = M[b]
+        if len(s) < 2 * m or len(t) == 0:
+            continue
+        k = int(rng.choice({1: [4, 8], 4: [8, 16, 32], 10: [16, 32, 64]}[m]))
+        probs.append((s, t, b, k, v))
+    dev = [(cuda(s), cuda(t), b, k) for s, t, b, k, _ in probs]
+    batch = dtk.MovingLeastSquaresBatch(dev, rbf="wendland2", **form)
+    xs = [cuda(v) for *_, v in probs]
+    ys = [torch.zeros(len(t), dtype=torch.float64, device="cuda") for _, t, *_ in probs]
+    batch.apply(xs, ys)
+    bar = 1e-6 if form["precision"] == "fp32_bulk" else 1e-9
+    worst = 0.0
+    for i, ((s, t, b, k, v), y) in enumerate(zip(probs, ys)):
+        idx, c, _ = oracle_lib.mls_create(s, t, M[b], k=k, rbf_name="wendland2")
+        ref = oracle_lib.apply(idx, c, v[None])[0]
+        rel = np.abs(y.cpu().numpy() - ref).max() / np.abs(ref).max()
+        worst = max(worst, rel)
+        assert rel <= bar, f"problem {i}: {b} k={k} n_src={len(s)} rel {rel:.2e}"
+    print(f"{form}: {len(probs)} problems, worst rel error vs oracle {worst:.2e}")
