@@ -468,6 +468,13 @@ int dtkx_mls_create( const double *src_xyz, int64_t n_src,
             kout.src_perm = h->perm.get();
             kout.src_pos = h->spos.get();
             kout.morton_layout = true;
+            // the search's last sort pass also writes the target rows and
+            // the sources in position order (for the setup's gathers)
+            h->trow.resize( n_tgt, h->stream );
+            kout.query_pos = h->trow.get();
+            dtkx::DeviceBuffer<double4> sorted( n_src, h->stream );
+            kout.src4 = h->src_d.get();
+            kout.sorted_src = sorted.get();
             if ( opts->geometry == DTKX_GEOMETRY_FLOAT )
             {
                 // float points rounded from the caller's doubles (made
@@ -482,16 +489,7 @@ int dtkx_mls_create( const double *src_xyz, int64_t n_src,
                 dtkx::knn_search( opts->geometry, h->src_d.get(), n_src,
                                   h->tgt_d.get(), n_tgt, k, kout, &info,
                                   h->stream );
-            // sources in position order for the setup's gathers
-            {
-                dtkx::DeviceBuffer<double4> sorted( n_src, h->stream );
-                dtkx::gather_points( h->src_d.get(), h->perm.get(), n_src,
-                                     sorted.get(), h->stream );
-                h->src_d = std::move( sorted );
-            }
-            h->trow.resize( n_tgt, h->stream );
-            dtkx::invert_permutation( h->order.get(), n_tgt, h->trow.get(),
-                                      h->stream );
+            h->src_d = std::move( sorted );
             ev.mark( 1 );
             run_setup( h.get(), counters.get() );
             h->row_max = {};

@@ -1,11 +1,14 @@
 // LBVH k-nearest-neighbour search for gfx950.
 //
-// Build: scene box -> 63-bit Morton codes (21 bits per axis) -> radix sort
-// of (code, index) -> Karras hierarchy (duplicate codes split by index), in
-// which each internal node also writes both child boxes: exact for small
-// children, the Morton cell of the child's common prefix for large ones (a
-// conservative box needs no bottom-up pass, whose device-scope fences are
-// costly across the XCDs' separate L2 caches).
+// Build: scene box -> 63-bit Morton codes (21 bits per axis) with every
+// radix pass's digit counts -> the hand-written radix sort of (code, index)
+// (radix_sort.hpp; the targets' 30-bit codes sort in the same launches, and
+// the last pass writes both inverse permutations and the points in sorted
+// order) -> Karras hierarchy (duplicate codes split by index), in which each
+// internal node also writes both child boxes: exact for small children, the
+// Morton cell of the child's common prefix for large ones (a conservative
+// box needs no bottom-up pass, whose device-scope fences are costly across
+// the XCDs' separate L2 caches).
 // Query: one lane per target, targets processed in Morton order so a wave
 // walks a compact region; a per-lane stack in LDS; the k best candidates
 // stay sorted in registers as 64-bit keys (float distance bits << 32 |
@@ -19,6 +22,7 @@
 // alone (one median-of-three per slot), then a fixed-radius pass that
 // collects the row, sorted at the end by (distance, index).
 #include "knn.hpp"
+#include "radix_sort.hpp"
 
 #include <hipcub/hipcub.hpp>
 
@@ -56,10 +60,16 @@ struct alignas( 16 ) Node
 constexpr int kBlock = 256;
 
 // ------------------------------------------------------------- scene box
+// Also clears zero[0, n_zero): counters that later kernels of the same
+// build accumulate into.
 template <typename T, typename V>
 __global__ void __launch_bounds__( kBlock )
-    bbox_partial( V const *pts, int64_t n, T *partial )
+    bbox_partial( V const *pts, int64_t n, T *partial, uint4 *zero,
+                  int64_t n_zero )
 {
+    for ( int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n_zero;
+          i += (int64_t)gridDim.x * kBlock )
+        zero[i] = make_uint4( 0u, 0u, 0u, 0u );
     T lo[3] = {INFINITY, INFINITY, INFINITY};
     T hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for ( int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n;
@@ -141,6 +151,24 @@ __device__ inline uint64_t expand21( uint32_t v )
     return x;
 }
 
+// 63-bit Morton code of p (21 bits per axis) in the scene box
+// box[0..2] = lower corner, box[3..5] = upper corner.
+template <typename T, typename V>
+__device__ inline uint64_t morton63( V p, T const *box )
+{
+    double c[3] = {(double)p.x, (double)p.y, (double)p.z};
+    uint32_t q[3];
+    for ( int d = 0; d < 3; ++d )
+    {
+        double lo = box[d], ext = (double)box[3 + d] - lo;
+        double u = ext > 0 ? ( c[d] - lo ) / ext * 2097151.0 : 0.0;
+        u = u < 0 ? 0 : ( u > 2097151.0 ? 2097151.0 : u );
+        q[d] = (uint32_t)u;
+    }
+    return ( expand21( q[0] ) << 2 ) | ( expand21( q[1] ) << 1 ) |
+           expand21( q[2] );
+}
+
 // prob (nullable): problem of each point, whose scene box is box + 6 prob
 // Key uint64_t: the 63-bit code; uint32_t: its top 30 bits (10 per axis),
 // the same order at a coarser grid.
@@ -153,20 +181,71 @@ __global__ void __launch_bounds__( kBlock )
     if ( i >= n )
         return;
     T const *box = prob ? boxes + 6 * prob[i] : boxes;
-    V p = pts[i];
-    double c[3] = {(double)p.x, (double)p.y, (double)p.z};
-    uint32_t q[3];
-    for ( int d = 0; d < 3; ++d )
-    {
-        double lo = box[d], ext = (double)box[3 + d] - lo;
-        double u = ext > 0 ? ( c[d] - lo ) / ext * 2097151.0 : 0.0;
-        u = u < 0 ? 0 : ( u > 2097151.0 ? 2097151.0 : u );
-        q[d] = (uint32_t)u;
-    }
-    uint64_t const code = ( expand21( q[0] ) << 2 ) |
-                          ( expand21( q[1] ) << 1 ) | expand21( q[2] );
+    uint64_t const code = morton63<T, V>( pts[i], box );
     codes[i] = sizeof( Key ) == 8 ? (Key)code : (Key)( code >> 33 );
     ids[i] = (int32_t)i;
+}
+
+// Radix passes of the two key kinds: the sources' 63-bit codes and the
+// targets' top 30 bits.
+constexpr int kSrcKeyBits = 63, kTgtKeyBits = 30;
+constexpr int kSrcPasses = ( kSrcKeyBits + 7 ) / 8;
+constexpr int kTgtPasses = ( kTgtKeyBits + 7 ) / 8;
+constexpr int kHistWords = ( kSrcPasses + kTgtPasses ) * rsort::kDigits;
+
+// Morton codes of the sources (63-bit) and of the targets (top 30 bits) in
+// one launch, 4096 points a block (blocks [0, src_blocks) take the sources),
+// with the digit counts of every radix pass added into hist (sources'
+// passes first, zero on entry).
+template <typename T, typename V>
+__global__ void __launch_bounds__( rsort::kThreads )
+    morton_hist_kernel( V const *src, int64_t n_src, V const *tgt,
+                        int64_t n_tgt, T const *box, int src_blocks,
+                        uint64_t *scodes, uint32_t *tcodes, uint32_t *hist )
+{
+    __shared__ uint32_t h[kSrcPasses * rsort::kDigits];
+    bool const is_src = static_cast<int>( blockIdx.x ) < src_blocks;
+    int const words = ( is_src ? kSrcPasses : kTgtPasses ) * rsort::kDigits;
+    for ( int i = threadIdx.x; i < words; i += rsort::kThreads )
+        h[i] = 0;
+    __syncthreads();
+    int64_t const c0 =
+        (int64_t)( is_src ? blockIdx.x : blockIdx.x - src_blocks ) *
+        rsort::kTile;
+    int64_t const n = is_src ? n_src : n_tgt;
+    V const *pts = is_src ? src : tgt;
+    for ( int s = 0; s < rsort::kItems; ++s )
+    {
+        int64_t const i = c0 + s * rsort::kThreads + threadIdx.x;
+        if ( i >= n )
+            break;
+        uint64_t const code = morton63<T, V>( pts[i], box );
+        if ( is_src )
+        {
+            scodes[i] = code;
+#pragma unroll
+            for ( int p = 0; p < kSrcPasses; ++p )
+                atomicAdd( &h[p * rsort::kDigits + ( ( code >> ( 8 * p ) ) & 255 )],
+                           1u );
+        }
+        else
+        {
+            uint32_t const c = static_cast<uint32_t>( code >> 33 );
+            tcodes[i] = c;
+#pragma unroll
+            for ( int p = 0; p < kTgtPasses; ++p )
+                atomicAdd( &h[p * rsort::kDigits + ( ( c >> ( 8 * p ) ) & 255 )],
+                           1u );
+        }
+    }
+    __syncthreads();
+    uint32_t *g = hist + ( is_src ? 0 : kSrcPasses * rsort::kDigits );
+    for ( int i = threadIdx.x; i < words; i += rsort::kThreads )
+    {
+        uint32_t const v = h[i];
+        if ( v )
+            atomicAdd( g + i, v );
+    }
 }
 
 // ------------------------------------------------------ Karras hierarchy
@@ -274,9 +353,9 @@ __device__ inline int node_problem( int32_t const *src_off, int n_prob,
 template <typename T, typename V>
 __global__ void __launch_bounds__( kBlock )
     karras_kernel( uint64_t const *codes_all, int32_t const *sorted_all,
-                   V const *pts, int64_t n_int, T const *scenes,
-                   int32_t const *src_off, int n_prob, Node<T> *nodes,
-                   int32_t *sizes )
+                   V const *pts, V const *sorted_pts_all, int64_t n_int,
+                   T const *scenes, int32_t const *src_off, int n_prob,
+                   Node<T> *nodes, int32_t *sizes )
 {
     int64_t const gi = blockIdx.x * (int64_t)kBlock + threadIdx.x;
     if ( gi >= n_int )
@@ -293,6 +372,11 @@ __global__ void __launch_bounds__( kBlock )
     }
     uint64_t const *codes = codes_all + so;
     int32_t const *sorted_ids = sorted_all + so;
+    // the points in sorted order: a copy when given, else through the ids
+    V const *sorted_pts = sorted_pts_all ? sorted_pts_all + so : nullptr;
+    auto point = [&]( int64_t q ) {
+        return sorted_pts ? sorted_pts[q] : pts[sorted_ids[q]];
+    };
     int64_t const i = gi - io;
     int d = ( prefix_delta( codes, n, i, i + 1 ) -
               prefix_delta( codes, n, i, i - 1 ) ) >= 0
@@ -327,7 +411,7 @@ __global__ void __launch_bounds__( kBlock )
         if ( b == e )
         {
             int32_t const orig = sorted_ids[b];
-            V const p = pts[orig];
+            V const p = point( b );
             nd.lo[c][0] = nd.hi[c][0] = p.x;
             nd.lo[c][1] = nd.hi[c][1] = p.y;
             nd.lo[c][2] = nd.hi[c][2] = p.z;
@@ -342,7 +426,7 @@ __global__ void __launch_bounds__( kBlock )
             T hi[3] = {-INFINITY, -INFINITY, -INFINITY};
             for ( int64_t q = b; q <= e; ++q )
             {
-                V const p = pts[sorted_ids[q]];
+                V const p = point( q );
                 lo[0] = fmin( lo[0], p.x );
                 lo[1] = fmin( lo[1], p.y );
                 lo[2] = fmin( lo[2], p.z );
@@ -1435,37 +1519,6 @@ void run_queries( int kmax, QueryArgs<T> qa, int64_t n_rows, KnnInfo *info,
     QueryLauncher<T>::dispatch( kmax, kSortRows, qa, s );
 }
 
-// Radix sort of (key, index) pairs over key bits [0, bits). rocprim picks a
-// merge sort below 2^20 + 1 keys, about 22 launches at 2^20 whose host-side
-// cost (some 9 us each) paces the whole build; the onesweep radix sort
-// needs a histogram launch and one launch per 8 bits.
-struct OnesweepConfig
-    : rocprim::radix_sort_config<rocprim::default_config,
-                                 rocprim::default_config,
-                                 rocprim::default_config, 0>
-{
-};
-
-template <typename Key>
-std::size_t sort_pairs_bytes( int64_t n, int bits, hipStream_t s )
-{
-    std::size_t bytes = 0;
-    DTKX_HIP( rocprim::radix_sort_pairs<OnesweepConfig>(
-        nullptr, bytes, (Key *)nullptr, (Key *)nullptr, (int32_t *)nullptr,
-        (int32_t *)nullptr, static_cast<std::size_t>( n ), 0, bits, s ) );
-    return bytes;
-}
-
-template <typename Key>
-void sort_pairs( void *temp, std::size_t bytes, Key const *keys, Key *keys_out,
-                 int32_t const *vals, int32_t *vals_out, int64_t n, int bits,
-                 hipStream_t s )
-{
-    DTKX_HIP( rocprim::radix_sort_pairs<OnesweepConfig>(
-        temp, bytes, keys, keys_out, vals, vals_out,
-        static_cast<std::size_t>( n ), 0, bits, s ) );
-}
-
 template <typename T>
 void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
                int64_t n_tgt, int k, KnnOutput const &out, KnnInfo *info,
@@ -1476,92 +1529,138 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     V const *tgt = static_cast<V const *>( tgt_v );
     int const kk = static_cast<int>( k < n_src ? k : n_src );
 
+    // Sort state, one buffer: the digit counts of every pass, the tile
+    // tickets and the look-back words of the first pass (all cleared by the
+    // scene-box launch), then those of the second pass (cleared by the
+    // first pass).
+    int const src_tiles = rsort::tiles_for( n_src );
+    int const tgt_tiles = rsort::tiles_for( n_tgt );
+    std::size_t const head = ( kHistWords + 4 * rsort::kMaxPasses ) * 4;
+    std::size_t const look_src = (std::size_t)src_tiles * rsort::kDigits * 8;
+    std::size_t const look_tgt = (std::size_t)tgt_tiles * rsort::kDigits * 8;
+    std::size_t const zero_bytes = head + look_src + look_tgt;
+    static_assert( ( kHistWords + 4 * rsort::kMaxPasses ) * 4 % 16 == 0,
+                   "uint4 clearing" );
+    DeviceBuffer<uint4> state( ( zero_bytes + look_src + look_tgt ) / 16, s );
+    char *const st = reinterpret_cast<char *>( state.get() );
+    uint32_t *const hist = reinterpret_cast<uint32_t *>( st );
+    uint32_t *const tickets = hist + kHistWords;
+    uint64_t *const look0 = reinterpret_cast<uint64_t *>( st + head );
+    uint64_t *const look1 = reinterpret_cast<uint64_t *>( st + zero_bytes );
+
     // scene box of the sources
     int nparts = static_cast<int>( std::min<int64_t>(
         1024, std::max<int64_t>( 1, ceil_div( n_src, kBlock ) ) ) );
     DeviceBuffer<T> partial( 6 * nparts, s ), box( 6, s );
-    bbox_partial<T, V><<<nparts, kBlock, 0, s>>>( src, n_src, partial.get() );
+    bbox_partial<T, V><<<nparts, kBlock, 0, s>>>(
+        src, n_src, partial.get(), state.get(),
+        static_cast<int64_t>( zero_bytes / 16 ) );
     bbox_final<T><<<1, kBlock, 0, s>>>( partial.get(), nparts, box.get() );
     DTKX_HIP( hipGetLastError() );
 
-    // Morton order of the targets (same scene box) on the side stream: it
-    // needs only the box, and both sorts are latency-bound at these sizes,
-    // so it runs beside the sources' sort and hierarchy build
-    DeviceBuffer<int32_t> order_buf;
-    int32_t *order = out.query_order;
-    if ( !order && n_tgt > 0 )
-    {
-        order_buf.resize( n_tgt, s );
-        order = order_buf.get();
-    }
-    // (the order only has to be local, so 30-bit keys do)
-    DeviceBuffer<uint32_t> tcodes( n_tgt, s ), tcodes_sorted( n_tgt, s );
-    DeviceBuffer<int32_t> tids( n_tgt, s );
-    std::size_t const ttemp_bytes = sort_pairs_bytes<uint32_t>( n_tgt, 30, s );
-    DeviceBuffer<char> ttemp( ttemp_bytes, s );
-    // The side stream waits for the box only; its work is queued after the
-    // sources' chain below. Queueing (a few microseconds of host time per
-    // launch) paces this build, so the order only decides which chain's
-    // tail runs unqueued: queued first, the targets' chain made the search
-    // 40 us slower at 1M x 1M.
-    Fork fork( s );
+    // Morton codes of both point sets (the targets' order only has to be
+    // local, so their top 30 bits do) with every pass's digit counts
+    DeviceBuffer<uint64_t> scodes( n_src, s ), scodes1( n_src, s );
+    DeviceBuffer<uint32_t> tcodes( n_tgt, s ), tcodes1( n_tgt, s );
+    morton_hist_kernel<T, V><<<src_tiles + tgt_tiles, rsort::kThreads, 0, s>>>(
+        src, n_src, tgt, n_tgt, box.get(), src_tiles, scodes.get(),
+        tcodes.get(), hist );
+    DTKX_HIP( hipGetLastError() );
 
-    // Morton codes and sort: sources
-    DeviceBuffer<uint64_t> codes( n_src, s ), codes_sorted( n_src, s );
-    DeviceBuffer<int32_t> ids( n_src, s ), ids_sorted_buf;
+    // outputs of the sort: the permutations, their inverses, the points in
+    // sorted order (the hierarchy's copy, and the caller's double4 copy,
+    // one buffer when they coincide)
+    DeviceBuffer<int32_t> ids_sorted_buf, pos_buf, order_buf;
     int32_t *ids_sorted = out.src_perm;
     if ( !ids_sorted )
     {
         ids_sorted_buf.resize( n_src, s );
         ids_sorted = ids_sorted_buf.get();
     }
-    morton_kernel<T, V><<<ceil_div( n_src, kBlock ), kBlock, 0, s>>>(
-        src, n_src, box.get(), codes.get(), ids.get(), nullptr );
-    DTKX_HIP( hipGetLastError() );
-    // 63-bit keys: rocprim's merge sort at these sizes (its onesweep radix
-    // sort takes longer on 64-bit keys below 2^21)
-    std::size_t temp_bytes = 0;
-    DTKX_HIP( hipcub::DeviceRadixSort::SortPairs(
-        nullptr, temp_bytes, codes.get(), codes_sorted.get(), ids.get(),
-        ids_sorted, static_cast<int>( n_src ), 0, 63, s ) );
-    DeviceBuffer<char> temp( temp_bytes, s );
-    DTKX_HIP( hipcub::DeviceRadixSort::SortPairs(
-        temp.get(), temp_bytes, codes.get(), codes_sorted.get(), ids.get(),
-        ids_sorted, static_cast<int>( n_src ), 0, 63, s ) );
-    DeviceBuffer<int32_t> pos_buf;
     int32_t *src_pos = out.src_pos;
     if ( out.morton_layout && !src_pos )
     {
         pos_buf.resize( n_src, s );
         src_pos = pos_buf.get();
     }
-    if ( src_pos )
+    int32_t *order = out.query_order;
+    if ( !order && n_tgt > 0 )
     {
-        invert_permutation( ids_sorted, n_src, src_pos, s );
+        order_buf.resize( n_tgt, s );
+        order = order_buf.get();
     }
+    bool const share = out.sorted_src &&
+                       static_cast<void const *>( out.src4 ) == src_v &&
+                       sizeof( V ) == sizeof( double4 );
+    DeviceBuffer<V> spts_buf;
+    V *spts = reinterpret_cast<V *>( out.sorted_src );
+    if ( !share )
+    {
+        spts_buf.resize( n_src, s );
+        spts = spts_buf.get();
+    }
+    DeviceBuffer<int32_t> svals( 2 * n_src, s ), tvals( 2 * n_tgt, s );
 
-    // hierarchy + refit
+    rsort::PassArgs pa;
+    pa.tickets = tickets;
+    rsort::Array &a = pa.a[0];
+    a.keys[0] = scodes.get();
+    a.keys[1] = scodes1.get();
+    a.vals[0] = svals.get();
+    a.vals[1] = svals.get() + n_src;
+    a.vals_out = ids_sorted;
+    a.pos_out = src_pos;
+    a.gather_in[0] = src;
+    a.gather_out[0] = spts;
+    a.gather_bytes[0] = sizeof( V );
+    if ( out.sorted_src && !share )
+    {
+        a.gather_in[1] = out.src4;
+        a.gather_out[1] = out.sorted_src;
+        a.gather_bytes[1] = sizeof( double4 );
+    }
+    a.hist = hist;
+    a.look[0] = look0;
+    a.look[1] = look1;
+    a.n = n_src;
+    a.tiles = src_tiles;
+    a.passes = kSrcPasses;
+    a.key_bits = kSrcKeyBits;
+    a.key_bytes = 8;
+    a.keep_keys = true;
+    rsort::Array &b = pa.a[1];
+    if ( n_tgt > 0 )
+    {
+        b.keys[0] = tcodes.get();
+        b.keys[1] = tcodes1.get();
+        b.vals[0] = tvals.get();
+        b.vals[1] = tvals.get() + n_tgt;
+        b.vals_out = order;
+        b.pos_out = out.query_pos;
+        b.hist = hist + kSrcPasses * rsort::kDigits;
+        b.look[0] = look0 + (std::size_t)src_tiles * rsort::kDigits;
+        b.look[1] = look1 + (std::size_t)src_tiles * rsort::kDigits;
+        b.n = n_tgt;
+        b.tiles = tgt_tiles;
+        b.passes = kTgtPasses;
+        b.key_bits = kTgtKeyBits;
+        b.key_bytes = 4;
+    }
+    rsort::sort( pa, s );
+    // the sorted codes are back in scodes after an even number of passes
+    static_assert( kSrcPasses % 2 == 0, "sorted codes in scodes" );
+
+    // hierarchy
     int64_t const n_int = n_src - 1;
     DeviceBuffer<Node<T>> nodes( std::max<int64_t>( n_int, 1 ), s );
     DeviceBuffer<int32_t> sizes( std::max<int64_t>( n_int, 1 ), s );
     if ( n_int > 0 )
     {
         karras_kernel<T, V><<<ceil_div( n_int, kBlock ), kBlock, 0, s>>>(
-            codes_sorted.get(), ids_sorted, src, n_int, box.get(), nullptr, 1,
+            scodes.get(), ids_sorted, src, spts, n_int, box.get(), nullptr, 1,
             nodes.get(), sizes.get() );
         DTKX_HIP( hipGetLastError() );
     }
-    if ( n_tgt > 0 )
-    {
-        hipStream_t const s2 = fork.side();
-        morton_kernel<T, V, uint32_t>
-            <<<ceil_div( n_tgt, kBlock ), kBlock, 0, s2>>>(
-                tgt, n_tgt, box.get(), tcodes.get(), tids.get(), nullptr );
-        DTKX_HIP( hipGetLastError() );
-        sort_pairs( ttemp.get(), ttemp_bytes, tcodes.get(), tcodes_sorted.get(),
-                    tids.get(), order, n_tgt, 30, s2 );
-    }
-    fork.join();
     if ( n_tgt == 0 )
         return;
 
@@ -1740,8 +1839,8 @@ void knn_batched_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     if ( n_int > 0 )
     {
         karras_kernel<T, V><<<ceil_div( n_int, kBlock ), kBlock, 0, s>>>(
-            codes_sorted.get(), out.src_perm, src, n_int, boxes.get(),
-            b.src_off, P, nodes.get(), sizes.get() );
+            codes_sorted.get(), out.src_perm, src, nullptr, n_int,
+            boxes.get(), b.src_off, P, nodes.get(), sizes.get() );
         DTKX_HIP( hipGetLastError() );
     }
     codes_sorted.release();

@@ -46,6 +46,12 @@ struct KnnOutput
     // nullable, Morton layout: each row's largest neighbour distance (the
     // k-th, as the search computes it)
     float *row_max = nullptr;
+    // nullable: the inverse of query_order (query_pos[target] = row)
+    int32_t *query_pos = nullptr;
+    // nullable: sorted_src[position] = src4[source], a double4 copy of the
+    // sources in Morton order (src4 may be the search's own points)
+    double4 const *src4 = nullptr;
+    double4 *sorted_src = nullptr;
 };
 
 // src/tgt: float4 (geometry FLOAT) or double4 (geometry DOUBLE) device
