@@ -288,26 +288,39 @@ def run_form(dtk, torch, name, d_src, d_tgt, d_vals, n_fields, args, stream):
     y = torch.zeros((n_fields, d_tgt.shape[0]), dtype=vdt, device=dev)
     kw = dict(basis=args.basis, rbf=args.rbf, k=args.k, stream=stream, **form)
 
-    def step():
+    def launch():
         op = dtk.MovingLeastSquaresOperator(d_src, d_tgt, **kw)
         a = torch.cuda.Event(enable_timing=True)
         b = torch.cuda.Event(enable_timing=True)
         a.record()
         op.apply(x, y)
         b.record()
+        return op, a, b
+
+    def collect(item):
+        op, a, b = item
+        st = op.stats()  # waits for this step's create only
         b.synchronize()
-        st = op.stats()
         st["ms_apply"] = a.elapsed_time(b)
-        del op
         return st
 
-    for _ in range(args.warmup):
-        step()
+    def run(n):
+        # Steps are queued one ahead: the host builds step i + 1 (create and
+        # apply return once queued) while the GPU runs step i, and a step's
+        # statistics are read, and its operator released, one step later.
+        out, prev = [], None
+        for _ in range(n):
+            cur = launch()
+            if prev is not None:
+                out.append(collect(prev))
+            prev = cur
+        out.append(collect(prev))
+        return out
+
+    run(args.warmup)
     torch.cuda.synchronize()
-    stats = []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        stats.append(step())
+    stats = run(args.steps)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     n = d_tgt.shape[0]

@@ -22,6 +22,53 @@
 #include <mutex>
 #include <string>
 
+// Events around the create stages (0..2 around the stages, 3..4 around
+// the query kernels), read when the statistics are first asked for, so
+// timing adds no host-device round trip to the create.
+struct StageEvents
+{
+    hipEvent_t e[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    bool marked[5] = {false, false, false, false, false};
+    hipEvent_t done = nullptr; // after the create's last launch
+    hipStream_t s;
+    explicit StageEvents( hipStream_t st )
+        : s( st )
+    {
+        for ( auto &x : e )
+            DTKX_HIP( hipEventCreate( &x ) );
+        DTKX_HIP( hipEventCreateWithFlags( &done, hipEventDisableTiming ) );
+    }
+    void mark( int i )
+    {
+        DTKX_HIP( hipEventRecord( e[i], s ) );
+        marked[i] = true;
+    }
+    // events 3 and 4 bracket the kNN query kernels (recorded by the search)
+    void pass_query_events( dtkx::KnnInfo &info )
+    {
+        info.query_begin = e[3];
+        info.query_end = e[4];
+        marked[3] = marked[4] = true;
+    }
+    double ms( int i, int j )
+    {
+        if ( !marked[i] || !marked[j] )
+            return 0.0;
+        DTKX_HIP( hipEventSynchronize( e[j] ) );
+        float t = 0;
+        DTKX_HIP( hipEventElapsedTime( &t, e[i], e[j] ) );
+        return t;
+    }
+    ~StageEvents()
+    {
+        for ( auto x : e )
+            if ( x )
+                (void)hipEventDestroy( x );
+        if ( done )
+            (void)hipEventDestroy( done );
+    }
+};
+
 struct dtkx_mls
 {
     dtkx_mls_options opts{};
@@ -49,7 +96,14 @@ struct dtkx_mls
     mutable dtkx::DeviceBuffer<double> graph_c64;
     mutable dtkx::DeviceBuffer<float> graph_c32;
     mutable bool graph_built = false;
-    dtkx_mls_stats stats{};
+    // The create returns without waiting for the GPU: the counters its
+    // setup and search accumulate and its stage events are read on the
+    // first dtkx_mls_get_stats (resolve_stats), which waits for them.
+    mutable dtkx_mls_stats stats{};
+    mutable dtkx::DeviceBuffer<unsigned long long> counters;
+    mutable std::unique_ptr<StageEvents> events;
+    mutable bool stats_pending = false;
+    void resolve_stats() const;
 
     dtkx::RowLayout layout() const
     {
@@ -208,6 +262,16 @@ struct StagedOut
     }
 };
 
+// End of an apply. With device arrays the work stays queued on the
+// handle's stream, in order with the caller's later work there (the
+// reference's apply launches Kokkos kernels, which do not wait either);
+// staged host arrays are complete when the call returns.
+void finish_apply( hipStream_t s, bool staged )
+{
+    if ( staged )
+        DTKX_HIP( hipStreamSynchronize( s ) );
+}
+
 long default_rotation_cap( int m ) { return 200L * m * m + 1000; }
 
 __global__ void offsets_kernel( int32_t *off, int64_t n, int kk )
@@ -223,48 +287,6 @@ __global__ void to_f32_kernel( double const *a, float *b, int64_t n )
     if ( i < n )
         b[i] = (float)a[i];
 }
-
-// Events around the create stages, read after the create's final
-// synchronisation so timing adds no host-device round trip.
-struct StageEvents
-{
-    hipEvent_t e[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-    bool marked[5] = {false, false, false, false, false};
-    hipStream_t s;
-    explicit StageEvents( hipStream_t st )
-        : s( st )
-    {
-        for ( auto &x : e )
-            DTKX_HIP( hipEventCreate( &x ) );
-    }
-    void mark( int i )
-    {
-        DTKX_HIP( hipEventRecord( e[i], s ) );
-        marked[i] = true;
-    }
-    // events 3 and 4 bracket the kNN query kernels (recorded by the search)
-    void pass_query_events( dtkx::KnnInfo &info )
-    {
-        info.query_begin = e[3];
-        info.query_end = e[4];
-        marked[3] = marked[4] = true;
-    }
-    double ms( int i, int j )
-    {
-        if ( !marked[i] || !marked[j] )
-            return 0.0;
-        DTKX_HIP( hipEventSynchronize( e[j] ) );
-        float t = 0;
-        DTKX_HIP( hipEventElapsedTime( &t, e[i], e[j] ) );
-        return t;
-    }
-    ~StageEvents()
-    {
-        for ( auto x : e )
-            if ( x )
-                (void)hipEventDestroy( x );
-    }
-};
 
 // Create counters: the setup's four (underdetermined, unconverged,
 // rotations, eigen fallbacks) and the search's re-run count, read back
@@ -310,29 +332,58 @@ void run_setup( dtkx_mls *h, unsigned long long *counters_d )
     h->stats.group_size_used = g;
 }
 
-void finish_create( dtkx_mls *h, unsigned long long *counters_d )
+// The create's last launch; its counters and events stay with the handle
+// until the statistics are asked for (no wait here).
+void finish_create( dtkx_mls *h,
+                    dtkx::DeviceBuffer<unsigned long long> &&counters,
+                    std::unique_ptr<StageEvents> &&events )
 {
-    int64_t const nnz = h->n_tgt * h->kk;
     h->offsets.resize( h->n_tgt + 1, h->stream );
     offsets_kernel<<<dtkx::ceil_div( h->n_tgt + 1, 256 ), 256, 0,
                      h->stream>>>( h->offsets.get(), h->n_tgt, h->kk );
     DTKX_HIP( hipGetLastError() );
-    unsigned long long c[kCounters] = {0, 0, 0, 0, 0};
-    DTKX_HIP( hipMemcpyAsync( c, counters_d, sizeof( c ),
-                              hipMemcpyDeviceToHost, h->stream ) );
-    DTKX_HIP( hipStreamSynchronize( h->stream ) );
     h->stats.n_source = h->n_src;
     h->stats.n_target = h->n_tgt;
-    h->stats.nnz = nnz;
+    h->stats.nnz = h->n_tgt * h->kk;
     h->stats.basis_size = h->m;
     h->stats.k = h->kk;
-    h->stats.n_underdetermined = (int64_t)c[0];
-    h->stats.n_unconverged = (int64_t)c[1];
-    h->stats.total_rotations = (int64_t)c[2];
-    h->stats.n_eigen_fallback = (int64_t)c[3];
-    // an int32 count in the low half of a zeroed slot
-    h->stats.n_stack_overflow = (int64_t)( c[4] & 0xffffffffull );
+    DTKX_HIP( hipEventRecord( events->done, h->stream ) );
+    h->counters = std::move( counters );
+    h->events = std::move( events );
+    h->stats_pending = true;
 }
+
+} // namespace
+
+void dtkx_mls::resolve_stats() const
+{
+    if ( !stats_pending )
+        return;
+    // read on the library's non-blocking side stream after the create's
+    // last event: the wait covers this create only, not work the caller
+    // has queued on the handle's stream since
+    unsigned long long c[kCounters] = {0, 0, 0, 0, 0};
+    hipStream_t const side = dtkx::side_stream();
+    DTKX_HIP( hipStreamWaitEvent( side, events->done, 0 ) );
+    DTKX_HIP( hipMemcpyAsync( c, counters.get(), sizeof( c ),
+                              hipMemcpyDeviceToHost, side ) );
+    DTKX_HIP( hipStreamSynchronize( side ) );
+    stats.n_underdetermined = (int64_t)c[0];
+    stats.n_unconverged = (int64_t)c[1];
+    stats.total_rotations = (int64_t)c[2];
+    stats.n_eigen_fallback = (int64_t)c[3];
+    // an int32 count in the low half of a zeroed slot
+    stats.n_stack_overflow = (int64_t)( c[4] & 0xffffffffull );
+    stats.ms_knn = events->ms( 0, 1 );
+    stats.ms_setup = events->ms( 1, 2 );
+    stats.ms_knn_query = events->ms( 3, 4 );
+    counters = {};
+    events.reset();
+    stats_pending = false;
+}
+
+namespace
+{
 
 // coefficient storage of the handle's precision only
 void alloc_coeffs( dtkx_mls *h, int64_t nnz )
@@ -387,7 +438,10 @@ std::unique_ptr<dtkx_mls> make_handle( double const *src, int64_t n_src,
     h->indices.resize( std::max<int64_t>( n_tgt * h->kk, 1 ), h->stream );
     alloc_coeffs( h.get(), n_tgt * h->kk );
     h->order.resize( std::max<int64_t>( n_tgt, 1 ), h->stream );
-    // the staging buffers are freed in stream order at scope exit
+    // host point arrays are read by the time the create returns; the
+    // staging buffers are freed in stream order at scope exit
+    if ( s_src.buf.get() || s_tgt.buf.get() )
+        DTKX_HIP( hipStreamSynchronize( h->stream ) );
     return h;
 }
 
@@ -440,7 +494,8 @@ int dtkx_mls_create( const double *src_xyz, int64_t n_src,
         auto h = make_handle( src_xyz, n_src, tgt_xyz, n_tgt, opts,
                               opts->geometry == DTKX_GEOMETRY_FLOAT &&
                                   n_tgt > 0 );
-        StageEvents ev( h->stream );
+        auto events = std::make_unique<StageEvents>( h->stream );
+        StageEvents &ev = *events;
         dtkx::DeviceBuffer<unsigned long long> counters( kCounters,
                                                          h->stream );
         DTKX_HIP( hipMemsetAsync( counters.get(), 0,
@@ -495,10 +550,7 @@ int dtkx_mls_create( const double *src_xyz, int64_t n_src,
             h->row_max = {};
             ev.mark( 2 );
         }
-        finish_create( h.get(), counters.get() );
-        h->stats.ms_knn = ev.ms( 0, 1 );
-        h->stats.ms_setup = ev.ms( 1, 2 );
-        h->stats.ms_knn_query = n_tgt > 0 ? ev.ms( 3, 4 ) : 0.0;
+        finish_create( h.get(), std::move( counters ), std::move( events ) );
         *out = h.release();
         return DTKX_SUCCESS;
     } );
@@ -538,19 +590,18 @@ int dtkx_mls_create_with_neighbors( const double *src_xyz, int64_t n_src,
                                       hipMemcpyHostToDevice, h->stream ) );
             DTKX_HIP( hipStreamSynchronize( h->stream ) );
         }
-        StageEvents ev( h->stream );
+        auto events = std::make_unique<StageEvents>( h->stream );
         dtkx::DeviceBuffer<unsigned long long> counters( kCounters,
                                                          h->stream );
         DTKX_HIP( hipMemsetAsync( counters.get(), 0,
                                   kCounters * sizeof( uint64_t ), h->stream ) );
         if ( n_tgt > 0 )
         {
-            ev.mark( 1 );
+            events->mark( 1 );
             run_setup( h.get(), counters.get() );
-            ev.mark( 2 );
+            events->mark( 2 );
         }
-        finish_create( h.get(), counters.get() );
-        h->stats.ms_setup = ev.ms( 1, 2 );
+        finish_create( h.get(), std::move( counters ), std::move( events ) );
         *out = h.release();
         return DTKX_SUCCESS;
     } );
@@ -581,7 +632,7 @@ int dtkx_mls_apply( const dtkx_mls *h, const double *src_vals,
             dtkx::crs_apply_f64( h->layout(), h->coeffs.get(), x.ptr, ld_src,
                                  y.ptr, ld_tgt, n_fields, h->stream );
         y.finish( h->stream );
-        DTKX_HIP( hipStreamSynchronize( h->stream ) );
+        finish_apply( h->stream, x.buf.get() || y.buf.get() );
         return DTKX_SUCCESS;
     } );
 }
@@ -676,7 +727,7 @@ int dtkx_mls_apply_fp32( const dtkx_mls *h, const float *src_vals,
         dtkx::crs_apply_f32( h->layout(), h->coeffs32.get(), x.ptr, ld_src,
                              y.ptr, ld_tgt, n_fields, h->stream );
         y.finish( h->stream );
-        DTKX_HIP( hipStreamSynchronize( h->stream ) );
+        finish_apply( h->stream, x.buf.get() || y.buf.get() );
         return DTKX_SUCCESS;
     } );
 }
@@ -863,6 +914,7 @@ int dtkx_mls_get_stats( const dtkx_mls *h, dtkx_mls_stats *stats )
 {
     return guarded( [&]() {
         REQ( h != nullptr && stats != nullptr, DTKX_ERR_INVALID_ARGUMENT );
+        h->resolve_stats();
         *stats = h->stats;
         return DTKX_SUCCESS;
     } );

@@ -63,6 +63,11 @@ void cache_release();
 // created on first use and kept: for work that cannot run on the null
 // stream, such as graph capture.
 hipStream_t library_stream();
+// A second stream of the library's own on the current device (default
+// priority, non-blocking: it does not wait for the null stream), created
+// on first use and kept: forked work, and reads that must wait for one
+// event only, not for later work queued on a caller's stream.
+hipStream_t side_stream();
 
 // Work forked from stream s onto a second stream of the library's own
 // (default priority, one per device): the side stream first waits for

@@ -139,8 +139,6 @@ hipStream_t library_stream()
     return st;
 }
 
-namespace
-{
 hipStream_t side_stream()
 {
     static std::mutex mu;
@@ -156,7 +154,6 @@ hipStream_t side_stream()
     streams[dev] = st;
     return st;
 }
-} // namespace
 
 Fork::Fork( hipStream_t s )
     : main_( s )

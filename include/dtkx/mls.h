@@ -184,7 +184,9 @@ int dtkx_mls_options_init(dtkx_mls_options *opts);
 /* MovingLeastSquaresOperator ctor (..._def.hpp:24-119): kNN over the source
  * points, then the fused setup (relative coordinates, Vandermonde, radius,
  * weights, moments, pseudo-inverse, coefficients). src/tgt are [n][3]
- * (AoS) or [3][n] (SoA) doubles. n_tgt may be 0; n_src must be > 0. */
+ * (AoS) or [3][n] (SoA) doubles. n_tgt may be 0; n_src must be > 0.
+ * Returns once the work is queued on opts->stream (host point arrays are
+ * copied before it returns); dtkx_mls_get_stats waits for it. */
 int dtkx_mls_create(const double *src_xyz, int64_t n_src,
                     const double *tgt_xyz, int64_t n_tgt,
                     const dtkx_mls_options *opts, dtkx_mls **out);
@@ -201,7 +203,11 @@ int dtkx_mls_create_with_neighbors(const double *src_xyz, int64_t n_src,
 /* MovingLeastSquaresOperator::apply (..._def.hpp:121-141):
  * tgt_vals[f*ld_tgt + i] = sum_j coeff(i,j) * src_vals[f*ld_src + idx(i,j)]
  * for f < n_fields. n_fields = 1, ld = n reproduces the reference call.
- * In FP32_BULK precision the values are still double at the boundary. */
+ * In FP32_BULK precision the values are still double at the boundary.
+ * With device arrays it returns once the work is queued on the handle's
+ * stream (later work on that stream sees the result; synchronise it before
+ * reading from the host or another stream), as the reference's Kokkos
+ * kernels do; with host arrays the result is complete on return. */
 int dtkx_mls_apply(const dtkx_mls *h, const double *src_vals,
                    double *tgt_vals, int32_t n_fields, int64_t ld_src,
                    int64_t ld_tgt);
@@ -227,7 +233,8 @@ int dtkx_mls_apply_noalloc(const dtkx_mls *h, const double *src_vals,
                            int64_t ld_tgt);
 
 /* FP32 apply: float values, float coefficients, float accumulation.
- * Requires a handle created with DTKX_PRECISION_FP32_BULK. */
+ * Requires a handle created with DTKX_PRECISION_FP32_BULK. Returns as
+ * dtkx_mls_apply does. */
 int dtkx_mls_apply_fp32(const dtkx_mls *h, const float *src_vals,
                         float *tgt_vals, int32_t n_fields, int64_t ld_src,
                         int64_t ld_tgt);
