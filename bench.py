@@ -210,6 +210,27 @@ def load_traffic(*keys):
     return None if any(v is None for v in vals) else sum(vals)
 
 
+def valu_issue(key, ms):
+    """The VALU-issue bound of a kernel from committed instruction counts
+    (profiles/kernel_instructions.json, SQ_INSTS_VALU and SQ_WAVES from
+    rocprofv3 --pmc): a wave64 VALU instruction occupies a 16-lane SIMD for
+    four cycles, so issuing them alone takes VALU x waves x 4 / (1024 SIMDs
+    x 2.4 GHz). frac = that time / the measured kernel time."""
+    path = os.path.join(REPO, "profiles", "kernel_instructions.json")
+    try:
+        with open(path) as fh:
+            rec = json.load(fh).get(key)
+    except (OSError, ValueError):
+        return None
+    if not rec:
+        return None
+    t = rec["valu_per_wave"] * rec["waves"] * 4 / (1024 * 2.4e9) * 1e3
+    return {"valu_per_wave": rec["valu_per_wave"], "salu_per_wave": rec["salu_per_wave"],
+            "waves": rec["waves"], "issue_ms": round(t, 3), "frac": round(t / ms, 3),
+            "source": "profiles/kernel_instructions.json (committed; rocprofv3 --pmc of "
+                      "this workload; not measured in this run)"}
+
+
 # the apply's three kernels (FP64 or FP32 values), as kernel_traffic.json keys them
 APPLY_KERNELS = {
     False: ["scatter_fields_kernel<double,double,4>", "crs_rows_kernel<double,double,double,double,4>",
@@ -247,6 +268,9 @@ def kernel_rooflines(form, r, n, m, kk, nnz, fields):
             "traffic_source": TRAFFIC_SOURCE,
             "algorithmic_per_unit": f"B_knn = 24 N_s + N_t (24 + 4k) = {b_knn(n, n, kk)} bytes "
                                     f"per launch ({n} queries)",
+            "valu_issue": valu_issue(f"{query_kernel(geo, kk)}@{n}x{n}", r["ms_knn_query"]),
+            "note": "bound by instruction issue and dependent-load latency, not HBM: "
+                    "valu_issue is the time its VALU instructions alone take to issue",
         },
         "setup": {
             "bound": "fp64_valu", "kernel": SETUP_KERNEL[form].replace("{m}", str(m)),
