@@ -24,7 +24,6 @@
 #include "knn.hpp"
 #include "radix_sort.hpp"
 
-#include <hipcub/hipcub.hpp>
 
 #include <cfloat>
 #include <cstring>
@@ -169,23 +168,6 @@ __device__ inline uint64_t morton63( V p, T const *box )
            expand21( q[2] );
 }
 
-// prob (nullable): problem of each point, whose scene box is box + 6 prob
-// Key uint64_t: the 63-bit code; uint32_t: its top 30 bits (10 per axis),
-// the same order at a coarser grid.
-template <typename T, typename V, typename Key = uint64_t>
-__global__ void __launch_bounds__( kBlock )
-    morton_kernel( V const *pts, int64_t n, T const *boxes, Key *codes,
-                   int32_t *ids, int32_t const *prob )
-{
-    int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-    if ( i >= n )
-        return;
-    T const *box = prob ? boxes + 6 * prob[i] : boxes;
-    uint64_t const code = morton63<T, V>( pts[i], box );
-    codes[i] = sizeof( Key ) == 8 ? (Key)code : (Key)( code >> 33 );
-    ids[i] = (int32_t)i;
-}
-
 // Radix passes of the two key kinds: the sources' 63-bit codes and the
 // targets' top 30 bits.
 constexpr int kSrcKeyBits = 63, kTgtKeyBits = 30;
@@ -196,11 +178,13 @@ constexpr int kHistWords = ( kSrcPasses + kTgtPasses ) * rsort::kDigits;
 // Morton codes of the sources (63-bit) and of the targets (top 30 bits) in
 // one launch, 4096 points a block (blocks [0, src_blocks) take the sources),
 // with the digit counts of every radix pass added into hist (sources'
-// passes first, zero on entry).
+// passes first, zero on entry). src_prob / tgt_prob (nullable, a batch):
+// each point's problem, whose scene box is box + 6 problem.
 template <typename T, typename V>
 __global__ void __launch_bounds__( rsort::kThreads )
     morton_hist_kernel( V const *src, int64_t n_src, V const *tgt,
-                        int64_t n_tgt, T const *box, int src_blocks,
+                        int64_t n_tgt, T const *box, int32_t const *src_prob,
+                        int32_t const *tgt_prob, int src_blocks,
                         uint64_t *scodes, uint32_t *tcodes, uint32_t *hist )
 {
     __shared__ uint32_t h[kSrcPasses * rsort::kDigits];
@@ -214,12 +198,14 @@ __global__ void __launch_bounds__( rsort::kThreads )
         rsort::kTile;
     int64_t const n = is_src ? n_src : n_tgt;
     V const *pts = is_src ? src : tgt;
+    int32_t const *prob = is_src ? src_prob : tgt_prob;
     for ( int s = 0; s < rsort::kItems; ++s )
     {
         int64_t const i = c0 + s * rsort::kThreads + threadIdx.x;
         if ( i >= n )
             break;
-        uint64_t const code = morton63<T, V>( pts[i], box );
+        uint64_t const code =
+            morton63<T, V>( pts[i], prob ? box + 6 * prob[i] : box );
         if ( is_src )
         {
             scodes[i] = code;
@@ -281,7 +267,7 @@ __device__ inline double round_down_d( double v ) { return v; }
 // `delta` bits of `code` (bit 63 is always 0): the quantisation cell of the
 // prefix, mapped back to world coordinates with a margin and outward
 // rounding, so it contains the points whatever the rounding of the forward
-// quantisation (morton_kernel).
+// quantisation (morton63).
 template <typename T>
 __device__ inline void cell_box( uint64_t code, int delta, T const *scene,
                                  T *lo, T *hi )
@@ -1563,8 +1549,8 @@ void knn_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     DeviceBuffer<uint64_t> scodes( n_src, s ), scodes1( n_src, s );
     DeviceBuffer<uint32_t> tcodes( n_tgt, s ), tcodes1( n_tgt, s );
     morton_hist_kernel<T, V><<<src_tiles + tgt_tiles, rsort::kThreads, 0, s>>>(
-        src, n_src, tgt, n_tgt, box.get(), src_tiles, scodes.get(),
-        tcodes.get(), hist );
+        src, n_src, tgt, n_tgt, box.get(), nullptr, nullptr, src_tiles,
+        scodes.get(), tcodes.get(), hist );
     DTKX_HIP( hipGetLastError() );
 
     // outputs of the sort: the permutations, their inverses, the points in
@@ -1730,25 +1716,58 @@ __global__ void __launch_bounds__( kBlock )
             boxes[pb * 6 + d] = sh[d][0];
 }
 
-// keys[i] = table[prob[ids[i]]] (table null: prob[ids[i]])
+// Clears [a, a + na) and [b, b + nb) (uint4 units).
 __global__ void __launch_bounds__( kBlock )
-    group_key_kernel( int32_t const *ids, int32_t const *prob,
-                      int32_t const *table, int64_t n, int32_t *keys )
+    clear2_kernel( uint4 *a, int64_t na, uint4 *b, int64_t nb )
 {
-    int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-    if ( i >= n )
-        return;
-    int32_t const pb = prob[ids[i]];
-    keys[i] = table ? table[pb] : pb;
+    for ( int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < na + nb;
+          i += (int64_t)gridDim.x * kBlock )
+        ( i < na ? a[i] : b[i - na] ) = make_uint4( 0u, 0u, 0u, 0u );
 }
 
-__global__ void __launch_bounds__( kBlock )
-    gather_codes_kernel( uint64_t const *codes, int32_t const *ids, int64_t n,
-                         uint64_t *out )
+// Second-stage keys of a batch's sorts, read through the first stage's
+// order: a source's problem, a target's row key (row_key[problem]), with
+// the digit counts of their passes added into hist (sources' first, zero on
+// entry); 4096 keys a block, blocks [0, src_blocks) take the sources.
+__global__ void __launch_bounds__( rsort::kThreads )
+    group_keys_kernel( int32_t const *src_ids, int32_t const *src_prob,
+                       int64_t n_src, int src_passes, int32_t const *tgt_ids,
+                       int32_t const *tgt_prob, int32_t const *row_key,
+                       int64_t n_tgt, int tgt_passes, int src_blocks,
+                       uint32_t *src_keys, uint32_t *tgt_keys, uint32_t *hist )
 {
-    int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
-    if ( i < n )
-        out[i] = codes[ids[i]];
+    __shared__ uint32_t h[4 * rsort::kDigits];
+    bool const is_src = static_cast<int>( blockIdx.x ) < src_blocks;
+    int const words = ( is_src ? src_passes : tgt_passes ) * rsort::kDigits;
+    for ( int i = threadIdx.x; i < words; i += rsort::kThreads )
+        h[i] = 0;
+    __syncthreads();
+    int64_t const c0 =
+        (int64_t)( is_src ? blockIdx.x : blockIdx.x - src_blocks ) *
+        rsort::kTile;
+    int64_t const n = is_src ? n_src : n_tgt;
+    int const passes = is_src ? src_passes : tgt_passes;
+    for ( int s = 0; s < rsort::kItems; ++s )
+    {
+        int64_t const i = c0 + s * rsort::kThreads + threadIdx.x;
+        if ( i >= n )
+            break;
+        uint32_t const key =
+            is_src ? static_cast<uint32_t>( src_prob[src_ids[i]] )
+                   : static_cast<uint32_t>( row_key[tgt_prob[tgt_ids[i]]] );
+        ( is_src ? src_keys : tgt_keys )[i] = key;
+        for ( int p = 0; p < passes; ++p )
+            atomicAdd( &h[p * rsort::kDigits + ( ( key >> ( 8 * p ) ) & 255 )],
+                       1u );
+    }
+    __syncthreads();
+    uint32_t *g = hist + ( is_src ? 0 : src_passes * rsort::kDigits );
+    for ( int i = threadIdx.x; i < words; i += rsort::kThreads )
+    {
+        uint32_t const v = h[i];
+        if ( v )
+            atomicAdd( g + i, v );
+    }
 }
 
 __global__ void __launch_bounds__( kBlock )
@@ -1769,40 +1788,6 @@ __global__ void __launch_bounds__( kBlock )
     row_off[r] = row_base[pb] + ( r - row_start[pb] ) * (int64_t)kk_prob[pb];
 }
 
-// Sort (codes, identity) by code, then stably by group key: the result is
-// ordered by (key, code). ids_out receives the permutation.
-template <typename T, typename V>
-void sort_by_group( V const *pts, int64_t n, T const *boxes,
-                    int32_t const *prob, int32_t const *table, int key_bits,
-                    DeviceBuffer<uint64_t> &codes, int32_t *ids_out,
-                    hipStream_t s )
-{
-    codes.resize( n, s );
-    DeviceBuffer<uint64_t> codes_sorted( n, s );
-    DeviceBuffer<int32_t> ids( n, s ), ids1( n, s ), keys( n, s ),
-        keys_sorted( n, s );
-    morton_kernel<T, V><<<ceil_div( n, kBlock ), kBlock, 0, s>>>(
-        pts, n, boxes, codes.get(), ids.get(), prob );
-    DTKX_HIP( hipGetLastError() );
-    std::size_t tb1 = 0, tb2 = 0;
-    DTKX_HIP( hipcub::DeviceRadixSort::SortPairs(
-        nullptr, tb1, codes.get(), codes_sorted.get(), ids.get(), ids1.get(),
-        static_cast<int>( n ), 0, 63, s ) );
-    DTKX_HIP( hipcub::DeviceRadixSort::SortPairs(
-        nullptr, tb2, keys.get(), keys_sorted.get(), ids1.get(), ids_out,
-        static_cast<int>( n ), 0, key_bits, s ) );
-    DeviceBuffer<char> temp( std::max( tb1, tb2 ), s );
-    DTKX_HIP( hipcub::DeviceRadixSort::SortPairs(
-        temp.get(), tb1, codes.get(), codes_sorted.get(), ids.get(),
-        ids1.get(), static_cast<int>( n ), 0, 63, s ) );
-    group_key_kernel<<<ceil_div( n, kBlock ), kBlock, 0, s>>>(
-        ids1.get(), prob, table, n, keys.get() );
-    DTKX_HIP( hipGetLastError() );
-    DTKX_HIP( hipcub::DeviceRadixSort::SortPairs(
-        temp.get(), tb2, keys.get(), keys_sorted.get(), ids1.get(), ids_out,
-        static_cast<int>( n ), 0, key_bits, s ) );
-}
-
 template <typename T>
 void knn_batched_impl( void const *src_v, int64_t n_src, void const *tgt_v,
                        int64_t n_tgt, KnnBatch const &b,
@@ -1821,15 +1806,132 @@ void knn_batched_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     bbox_segments<T, V><<<P, kBlock, 0, s>>>( src, b.src_off, boxes.get() );
     DTKX_HIP( hipGetLastError() );
 
-    // sources: problem-major, Morton order inside each problem
-    DeviceBuffer<uint64_t> codes;
-    sort_by_group<T, V>( src, n_src, boxes.get(), b.src_prob, nullptr,
-                         prob_bits, codes, out.src_perm, s );
-    DeviceBuffer<uint64_t> codes_sorted( n_src, s );
-    gather_codes_kernel<<<ceil_div( n_src, kBlock ), kBlock, 0, s>>>(
-        codes.get(), out.src_perm, n_src, codes_sorted.get() );
+    // Two stable radix sorts of both point sets, in shared launches
+    // (radix_sort.hpp): by Morton code in the problem's box (sources 63 bits,
+    // targets 30, as one problem's search orders them), then by the
+    // problem's key (sources: the problem; targets: its row key). The second
+    // sorts positions of the first, and its last pass gathers the first's
+    // indices (and the sources' codes) into the final order.
+    int const src_tiles = rsort::tiles_for( n_src );
+    int const tgt_tiles = rsort::tiles_for( n_tgt );
+    int const src_passes2 = rsort::passes_for( prob_bits );
+    int const tgt_passes2 = rsort::passes_for( b.key_bits );
+    int const hist1 = kHistWords;
+    int const hist2 = ( src_passes2 + tgt_passes2 ) * rsort::kDigits;
+    std::size_t const sb1 = rsort::state_bytes( hist1, src_tiles + tgt_tiles );
+    std::size_t const sb2 = rsort::state_bytes( hist2, src_tiles + tgt_tiles );
+    DeviceBuffer<uint4> state( ( sb1 + sb2 ) / 16, s );
+    rsort::State const st1 =
+        rsort::carve_state( state.get(), hist1, src_tiles + tgt_tiles );
+    rsort::State const st2 = rsort::carve_state(
+        reinterpret_cast<char *>( state.get() ) + sb1, hist2,
+        src_tiles + tgt_tiles );
+    clear2_kernel<<<std::min<int64_t>( 1024, ceil_div( sb1 + sb2, 16 * kBlock ) ),
+                    kBlock, 0, s>>>(
+        static_cast<uint4 *>( st1.zero ), st1.zero_bytes / 16,
+        static_cast<uint4 *>( st2.zero ), st2.zero_bytes / 16 );
     DTKX_HIP( hipGetLastError() );
-    codes.release();
+
+    DeviceBuffer<uint64_t> scodes( n_src, s ), scodes1( n_src, s ),
+        codes_sorted( n_src, s );
+    DeviceBuffer<uint32_t> tcodes( n_tgt, s ), tcodes1( n_tgt, s );
+    morton_hist_kernel<T, V><<<src_tiles + tgt_tiles, rsort::kThreads, 0, s>>>(
+        src, n_src, tgt, n_tgt, boxes.get(), b.src_prob, b.tgt_prob,
+        src_tiles, scodes.get(), tcodes.get(), st1.hist );
+    DTKX_HIP( hipGetLastError() );
+    DeviceBuffer<int32_t> svals( 2 * n_src, s ), tvals( 2 * n_tgt, s ),
+        sids1( n_src, s ), tids1( n_tgt, s );
+    auto first_stage = [&]( rsort::Array &a, rsort::Array &t ) {
+        a.keys[0] = scodes.get();
+        a.keys[1] = scodes1.get();
+        a.vals[0] = svals.get();
+        a.vals[1] = svals.get() + n_src;
+        a.vals_out = sids1.get();
+        a.hist = st1.hist;
+        a.look[0] = st1.look0;
+        a.look[1] = st1.look1;
+        a.n = n_src;
+        a.tiles = src_tiles;
+        a.passes = kSrcPasses;
+        a.key_bits = kSrcKeyBits;
+        a.key_bytes = 8;
+        a.keep_keys = true;
+        if ( n_tgt == 0 )
+            return;
+        t.keys[0] = tcodes.get();
+        t.keys[1] = tcodes1.get();
+        t.vals[0] = tvals.get();
+        t.vals[1] = tvals.get() + n_tgt;
+        t.vals_out = tids1.get();
+        t.hist = st1.hist + kSrcPasses * rsort::kDigits;
+        t.look[0] = st1.look0 + (std::size_t)src_tiles * rsort::kDigits;
+        t.look[1] = st1.look1 + (std::size_t)src_tiles * rsort::kDigits;
+        t.n = n_tgt;
+        t.tiles = tgt_tiles;
+        t.passes = kTgtPasses;
+        t.key_bits = kTgtKeyBits;
+        t.key_bytes = 4;
+    };
+    rsort::PassArgs pa1;
+    pa1.tickets = st1.tickets;
+    first_stage( pa1.a[0], pa1.a[1] );
+    rsort::sort( pa1, s );
+
+    // second stage: keys through the first order; scodes holds the sorted
+    // codes (an even number of passes), tcodes the targets' keys
+    DeviceBuffer<uint32_t> skeys( n_src, s ), skeys1( n_src, s ),
+        tkeys( n_tgt, s ), tkeys1( n_tgt, s );
+    group_keys_kernel<<<src_tiles + tgt_tiles, rsort::kThreads, 0, s>>>(
+        sids1.get(), b.src_prob, n_src, src_passes2, tids1.get(), b.tgt_prob,
+        b.row_key, n_tgt, tgt_passes2, src_tiles, skeys.get(), tkeys.get(),
+        st2.hist );
+    DTKX_HIP( hipGetLastError() );
+    DeviceBuffer<int32_t> spos1( n_src, s ), tpos1( n_tgt, s );
+    rsort::PassArgs pa2;
+    pa2.tickets = st2.tickets;
+    {
+        rsort::Array &a = pa2.a[0];
+        a.keys[0] = skeys.get();
+        a.keys[1] = skeys1.get();
+        a.vals[0] = svals.get();
+        a.vals[1] = svals.get() + n_src;
+        a.vals_out = spos1.get();
+        a.gather_in[0] = sids1.get();
+        a.gather_out[0] = out.src_perm;
+        a.gather_bytes[0] = 4;
+        a.gather_in[1] = scodes.get();
+        a.gather_out[1] = codes_sorted.get();
+        a.gather_bytes[1] = 8;
+        a.hist = st2.hist;
+        a.look[0] = st2.look0;
+        a.look[1] = st2.look1;
+        a.n = n_src;
+        a.tiles = src_tiles;
+        a.passes = src_passes2;
+        a.key_bits = prob_bits;
+        a.key_bytes = 4;
+        if ( n_tgt > 0 )
+        {
+            rsort::Array &t = pa2.a[1];
+            t.keys[0] = tkeys.get();
+            t.keys[1] = tkeys1.get();
+            t.vals[0] = tvals.get();
+            t.vals[1] = tvals.get() + n_tgt;
+            t.vals_out = tpos1.get();
+            t.gather_in[0] = tids1.get();
+            t.gather_out[0] = out.order;
+            t.gather_bytes[0] = 4;
+            t.hist = st2.hist + src_passes2 * rsort::kDigits;
+            t.look[0] = st2.look0 + (std::size_t)src_tiles * rsort::kDigits;
+            t.look[1] = st2.look1 + (std::size_t)src_tiles * rsort::kDigits;
+            t.n = n_tgt;
+            t.tiles = tgt_tiles;
+            t.passes = tgt_passes2;
+            t.key_bits = b.key_bits;
+            t.key_bytes = 4;
+        }
+    }
+    rsort::sort( pa2, s );
     invert_permutation( out.src_perm, n_src, out.src_pos, s );
 
     // one hierarchy per problem, all in one launch
@@ -1847,12 +1949,7 @@ void knn_batched_impl( void const *src_v, int64_t n_src, void const *tgt_v,
     if ( n_tgt == 0 )
         return;
 
-    // targets: rows ordered by (row key of the problem, Morton code)
-    {
-        DeviceBuffer<uint64_t> tcodes;
-        sort_by_group<T, V>( tgt, n_tgt, boxes.get(), b.tgt_prob, b.row_key,
-                             b.key_bits, tcodes, out.order, s );
-    }
+    // rows: ordered by (row key of the problem, Morton code)
     row_offsets_kernel<<<ceil_div( n_tgt + 1, kBlock ), kBlock, 0, s>>>(
         out.order, b.tgt_prob, b.row_base, b.row_start, b.kk_prob, n_tgt,
         b.nnz, out.row_off );
@@ -1865,7 +1962,6 @@ void knn_batched_impl( void const *src_v, int64_t n_src, void const *tgt_v,
         QueryArgs<T> qa;
         qa.nodes = nodes.get();
         qa.node_size = kSeed ? sizes.get() : nullptr;
-    qa.node_size = kSeed ? sizes.get() : nullptr;
         qa.src = src;
         qa.tgt = tgt;
         qa.order = out.order;

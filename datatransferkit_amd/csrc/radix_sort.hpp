@@ -62,7 +62,7 @@ struct Array
     int32_t *vals_out = nullptr;        // indices in sorted order (last pass)
     int32_t *pos_out = nullptr;         // nullable: pos_out[index] = position
     // nullable: gather_out[g][position] = gather_in[g][index] (last pass),
-    // records of 16 or 32 bytes
+    // records of 4, 8, 16 or 32 bytes
     void const *gather_in[2] = {nullptr, nullptr};
     void *gather_out[2] = {nullptr, nullptr};
     int gather_bytes[2] = {0, 0};
@@ -84,6 +84,43 @@ struct PassArgs
 };
 
 inline int passes_for( int key_bits ) { return ( key_bits + 7 ) / 8; }
+
+// The device state of one sort of up to two arrays, in one buffer: digit
+// counts (hist_words), tickets and the first pass's look-back words (the
+// region [0, zero_bytes), to be cleared before the first pass), then the
+// second pass's look-back words.
+struct State
+{
+    uint32_t *hist = nullptr, *tickets = nullptr;
+    uint64_t *look0 = nullptr, *look1 = nullptr; // [tiles0 + tiles1][256]
+    void *zero = nullptr;
+    std::size_t zero_bytes = 0;
+};
+
+inline std::size_t state_bytes( int hist_words, int tiles )
+{
+    std::size_t const head = ( (std::size_t)hist_words + 4 * kMaxPasses ) * 4;
+    std::size_t const look = (std::size_t)tiles * kDigits * 8;
+    return ( ( head + 15 ) & ~std::size_t( 15 ) ) + 2 * look;
+}
+
+// Carves base (state_bytes( hist_words, tiles ) bytes, 16-aligned).
+inline State carve_state( void *base, int hist_words, int tiles )
+{
+    State st;
+    char *const b = static_cast<char *>( base );
+    std::size_t const head =
+        ( ( (std::size_t)hist_words + 4 * kMaxPasses ) * 4 + 15 ) &
+        ~std::size_t( 15 );
+    std::size_t const look = (std::size_t)tiles * kDigits * 8;
+    st.hist = reinterpret_cast<uint32_t *>( b );
+    st.tickets = st.hist + hist_words;
+    st.look0 = reinterpret_cast<uint64_t *>( b + head );
+    st.look1 = reinterpret_cast<uint64_t *>( b + head + look );
+    st.zero = b;
+    st.zero_bytes = head + look;
+    return st;
+}
 inline int tiles_for( int64_t n )
 {
     return static_cast<int>( ( n + kTile - 1 ) / kTile );
@@ -150,7 +187,13 @@ __device__ inline void copy_record( void const *in, void *out, int bytes,
 {
     uint4 const *src = static_cast<uint4 const *>( in );
     uint4 *dst = static_cast<uint4 *>( out );
-    if ( bytes == 16 )
+    if ( bytes == 4 )
+        static_cast<uint32_t *>( out )[to] =
+            static_cast<uint32_t const *>( in )[from];
+    else if ( bytes == 8 )
+        static_cast<uint64_t *>( out )[to] =
+            static_cast<uint64_t const *>( in )[from];
+    else if ( bytes == 16 )
         dst[to] = src[from];
     else
     {
