@@ -191,6 +191,43 @@ def extended_truth(src, tgt, vals, idx, m, rbf, n):
     return truth, lam > 1e3 * np.finfo(np.float64).eps
 
 
+def c2a_block(dtk, torch, d_src, d_tgt, d_vals, src, tgt, vals, args, stream, ne=2000):
+    """The same 1M x 1M transfer at k = m = 10 (the square systems of the
+    k = 10 configuration): speed of the fast forms, and every form's error
+    against the oracle and against the extended-precision operator on the
+    first ne targets (systems 1000 eps clear of the truncation threshold)."""
+    import copy
+
+    import numpy as np
+    import oracle
+
+    m = BASIS_SIZE[args.basis]
+    a = copy.copy(args)
+    a.k, a.steps, a.warmup = m, 2, 1
+    rec = {"workload": f"as the headline with k = {m} (one square system per target)",
+           "forms": {}}
+    outs = {}
+    for name in ("fp32_bulk", "fp64_cholesky"):
+        r, y, _ = run_form(dtk, torch, name, d_src, d_tgt, d_vals, args.fields, a, stream)
+        rec["forms"][name] = {"value": r["value"], "ms_per_step": r["ms_per_step"]}
+        outs[name] = y
+    oracle.set_num_threads(cpu_threads())
+    with oracle.libm_mode(oracle.LIBM_CORRECTLY_ROUNDED):
+        idx, coeffs, _ = oracle.mls_create(src, tgt[:ne], m, k=m, rbf_name=args.rbf)
+        ref = oracle.apply(idx, coeffs, vals)
+    truth, clear = extended_truth(src, tgt, vals, idx, m, args.rbf, ne)
+    rec["sample"] = (f"first {ne} targets; {int(clear.sum())} systems 1000 eps clear of "
+                     f"the truncation threshold compared with the extended-precision operator")
+    rec["oracle_rel_err_vs_extended"] = rel_inf(ref[:, clear], truth[:, clear])
+    for name, y in outs.items():
+        rec["forms"][name]["rel_err_vs_oracle"] = rel_inf(y[:, :ne], ref)
+        rec["forms"][name]["rel_err_vs_extended"] = rel_inf(y[:, :ne][:, clear],
+                                                            truth[:, clear])
+    rec["note"] = ("the plain FP64 form equals the oracle bit for bit; at k = m the "
+                   "reference's solver carries eps cond(A), cond(A) up to ~1e13")
+    return rec
+
+
 TRAFFIC_SOURCE = ("profiles/kernel_traffic.json (committed; from two rocprofv3 --pmc runs "
                   "of this workload, FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 KiB; not measured "
                   "in this run)")
@@ -951,6 +988,9 @@ def main():
             for name, y in outputs.items():
                 results[name]["rel_err_vs_extended_sample"] = rel_inf(
                     y[:, :ne][:, clear], truth[:, clear])
+        if args.k == 32 and args.basis == "quadratic" and args.rbf.startswith("wendland"):
+            out["c2a_k_equals_m"] = c2a_block(dtk, torch, d_src, d_tgt, d_vals, src, tgt,
+                                              vals, args, stream)
     print(json.dumps(out), flush=True)
 
 

@@ -180,3 +180,76 @@ def test_batch_fast_solvers_match_oracle(dtk, oracle_lib, form):
         worst = max(worst, rel)
         assert rel <= bar, f"problem {i}: {b} k={k} n_src={len(s)} rel {rel:.2e}"
     print(f"{form}: {len(probs)} problems, worst rel error vs oracle {worst:.2e}")
+
+
+def test_batch_c3_full_size(dtk, oracle_lib):
+    """C3 at its full size, as the batched bench runs it: 4,096 uneven
+    problems (sources log-uniform in [64, 65536], 0.5-2x as many targets,
+    k in {10, 16, 32, 64}, linear or quadratic basis; ~47M targets, ~1.4G
+    entries), FP32 bulk with the certified Cholesky solver. Checks whose
+    meaning does not depend on the size:
+      * 48 random problems bit-identical to single operators of the same
+        form (which the single-operator tests pin to the oracle);
+      * MLS reproduces its basis on every row of the overdetermined problems
+        (linear basis, or quadratic with k >= 32): a constant field maps to
+        the constant, a linear field to its values at the targets. Square
+        quadratic systems (k = 10) are left out: there the reference's
+        absolute DBL_EPSILON rule truncates the ill-conditioned ones, and a
+        truncated pseudo-inverse does not reproduce its basis;
+      * a sample of problems (k >= 16) against the FP64 oracle at the form's
+        1e-6 bar."""
+    rng = np.random.default_rng(7)
+    probs = []
+    for _ in range(4096):
+        ns = int(round(np.exp(rng.uniform(np.log(64), np.log(65536)))))
+        nt = int(round(ns * rng.uniform(0.5, 2.0)))
+        k = int(rng.choice([10, 16, 32, 64]))
+        basis = "linear" if rng.random() < 0.5 else "quadratic"
+        a = np.eye(3) + 0.25 * rng.standard_normal((3, 3))
+        b = rng.standard_normal(3)
+        probs.append((rng.random((ns, 3)) @ a.T + b, rng.random((nt, 3)) @ a.T + b,
+                      basis, k))
+    n_tgt = sum(len(t) for _, t, _, _ in probs)
+    print(f"C3 full size: {len(probs)} problems, {n_tgt} targets", flush=True)
+    dev = [(cuda(s), cuda(t), bs, k) for s, t, bs, k in probs]
+    form = dict(rbf="wendland2", precision="fp32_bulk", solver="cholesky")
+    batch = dtk.MovingLeastSquaresBatch(dev, **form)
+    st = batch.stats()
+    assert st["n_target"] == n_tgt
+    coef = rng.standard_normal((len(probs), 4))
+    xs = [c[0] + s[:, 0] * c[1] + s[:, 1] * c[2] + s[:, 2] * c[3]
+          for (s, _, _, _), c in zip(dev, coef)]
+    ys = [torch.zeros(len(t), dtype=torch.float64, device="cuda") for _, t, _, _ in dev]
+    batch.apply(xs, ys)
+    for i in rng.choice(len(probs), size=48, replace=False):
+        s, t, bs, k = dev[i]
+        op = dtk.MovingLeastSquaresOperator(s, t, basis=bs, k=k, **form)
+        ref = torch.empty_like(ys[i])
+        op.apply(xs[i], ref)
+        assert torch.equal(ys[i], ref), f"problem {i}: {bs} k={k} n_src={len(s)}"
+    over = [i for i, (_, _, bs, k) in enumerate(probs) if bs == "linear" or k >= 32]
+    worst_lin = 0.0
+    for i in over:
+        s, t, _, _ = dev[i]
+        c = coef[i]
+        f = c[0] + t[:, 0] * c[1] + t[:, 1] * c[2] + t[:, 2] * c[3]
+        worst_lin = max(worst_lin, float((ys[i] - f).abs().max() / xs[i].abs().max()))
+    ones = [torch.full((len(s),), 3.5, dtype=torch.float64, device="cuda") for s, _, _, _ in dev]
+    batch.apply(ones, ys)
+    worst_const = max(float((ys[i] - 3.5).abs().max()) / 3.5 for i in over)
+    print(f"  {len(over)} overdetermined problems: linear reproduction {worst_lin:.2e}, "
+          f"constants {worst_const:.2e}", flush=True)
+    # measured: 6.3e-7 and 1.0e-6
+    assert worst_lin <= 5e-6
+    assert worst_const <= 5e-6
+    # oracle on a sample (sizes the CPU oracle does in seconds)
+    batch.apply(xs, ys)
+    small = [i for i, (s, t, _, k) in enumerate(probs) if len(s) <= 4000 and k >= 16]
+    worst = 0.0
+    for i in rng.choice(small, size=6, replace=False):
+        s, t, bs, k = probs[i]
+        idx, co, _ = oracle_lib.mls_create(s, t, M[bs], k=k, rbf_name="wendland2")
+        ref = oracle_lib.apply(idx, co, xs[i].cpu().numpy()[None])[0]
+        worst = max(worst, float(np.abs(ys[i].cpu().numpy() - ref).max() / np.abs(ref).max()))
+    print(f"  sampled problems vs oracle {worst:.2e}", flush=True)
+    assert worst <= 1e-6
