@@ -421,7 +421,9 @@ def make_batch_problems(n_problems, seed=7, max_src=65536):
 def bench_batched(args):
     """Batched uneven operators: one batch create (segmented kNN, one setup
     launch per basis) + one graph-replayed apply per step, against a loop of
-    single-operator creates and applies over the same problems."""
+    single-operator creates and applies over the same problems, and against
+    one batch rebuilt per step (the search and setup replayed from their
+    captured graph, as after the points move in place)."""
     import numpy as np
     import torch
 
@@ -462,6 +464,16 @@ def bench_batched(args):
 
     dt_b, st = timed(batch_step, args.steps)
     y_batch = [y.clone() for y in ys]
+    kept = dtk.MovingLeastSquaresBatch(dev, **opts)
+
+    def rebuild_step():
+        kept.rebuild()
+        kept.apply(xs, ys)
+        return kept.stats()
+
+    dt_r, _ = timed(rebuild_step, args.steps)
+    same_r = all(torch.equal(a, b) for a, b in zip(y_batch, ys))
+    del kept
     dt_l, _ = timed(loop_step, max(1, args.steps // 2))
     same = all(torch.equal(a, b) for a, b in zip(y_batch, ys))
     out = {
@@ -492,6 +504,9 @@ def bench_batched(args):
         "loop_of_single_operators": {"value": n_tgt / dt_l, "ms_per_step": 1000 * dt_l},
         "speedup_vs_loop": dt_l / dt_b,
         "batch_equals_loop_bitwise": bool(same),
+        "rebuild_graph": {"value": n_tgt / dt_r, "ms_per_step": 1000 * dt_r,
+                          "speedup_vs_create": dt_b / dt_r,
+                          "equals_create_bitwise": bool(same_r)},
     }
     print(json.dumps(out), flush=True)
 

@@ -45,7 +45,7 @@ EXPORTED = [
     "dtkx_mls_target_gradient", "dtkx_mls_get_graph", "dtkx_mls_write_graph",
     "dtkx_mls_get_stats",
     "dtkx_mls_destroy", "dtkx_knn", "dtkx_svd_pseudo_inverse",
-    "dtkx_mls_batch_create", "dtkx_mls_batch_apply",
+    "dtkx_mls_batch_create", "dtkx_mls_batch_apply", "dtkx_mls_batch_rebuild",
     "dtkx_mls_batch_get_stats", "dtkx_mls_batch_destroy",
     "dtkx_nn_create", "dtkx_nn_apply", "dtkx_nn_get_indices", "dtkx_nn_destroy",
     "dtkx_spline_options_init", "dtkx_spline_create", "dtkx_spline_apply",
@@ -154,6 +154,7 @@ def load():
     L.dtkx_mls_batch_create.argtypes = [_i32, ctypes.POINTER(Problem),
                                         ctypes.POINTER(Options), ctypes.POINTER(_vp)]
     L.dtkx_mls_batch_apply.argtypes = [_vp, _vp, _vp]
+    L.dtkx_mls_batch_rebuild.argtypes = [_vp]
     L.dtkx_mls_batch_get_stats.argtypes = [_vp, ctypes.POINTER(Stats)]
     L.dtkx_mls_batch_destroy.argtypes = [_vp]
     L.dtkx_nn_create.argtypes = [_vp, _i64, _vp, _i64, _i32, _i32, _vp,

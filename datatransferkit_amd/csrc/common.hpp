@@ -11,6 +11,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 namespace dtkx
 {
@@ -57,6 +58,45 @@ struct Error : std::runtime_error
 // in hipFreeAsync at a million targets.
 void *cache_alloc( std::size_t bytes, hipStream_t s );
 void cache_free( void *p, std::size_t bytes, hipStream_t s );
+
+// Blocks a captured graph uses. While a KeepBlocks installs a GraphBlocks
+// on the current thread (around a stream capture), every block released to
+// the cache is kept in it instead: the graph's nodes use those addresses on
+// each replay, so they must not be handed out again. release() (or the
+// destructor) returns them to the free lists of their streams; the graph's
+// launches must all be queued on those streams, so later users of a block
+// are ordered after them.
+struct GraphBlocks
+{
+    struct Block
+    {
+        void *p;
+        std::size_t bytes;
+        hipStream_t s;
+    };
+    std::vector<Block> blocks;
+    void release();
+    GraphBlocks() = default;
+    GraphBlocks( GraphBlocks const & ) = delete;
+    GraphBlocks &operator=( GraphBlocks const & ) = delete;
+    ~GraphBlocks() { release(); }
+};
+
+class KeepBlocks
+{
+  public:
+    explicit KeepBlocks( GraphBlocks *keep );
+    ~KeepBlocks();
+    KeepBlocks( KeepBlocks const & ) = delete;
+    KeepBlocks &operator=( KeepBlocks const & ) = delete;
+
+  private:
+    GraphBlocks *prev_;
+};
+
+// p[0, bytes) = 0 on stream s (bytes a multiple of 4) by a kernel, which,
+// unlike a memset node, replays correctly from a captured graph.
+void zero_async( void *p, std::size_t bytes, hipStream_t s );
 // Returns the cached blocks of the current device to the runtime.
 void cache_release();
 // A stream of the library's own on the current device (default priority),

@@ -1468,7 +1468,7 @@ void run_queries( int kmax, QueryArgs<T> qa, int64_t n_rows, KnnInfo *info,
     if ( n_rows == 0 )
         return;
     DeviceBuffer<int32_t> rows( n_rows, s ), count( 1, s );
-    DTKX_HIP( hipMemsetAsync( count.get(), 0, sizeof( int32_t ), s ) );
+    zero_async( count.get(), sizeof( int32_t ), s );
     qa.overflow_rows = rows.get();
     qa.n_overflow = count.get();
     QueryLauncher<T>::dispatch( kmax, kLdsStack, qa, s );

@@ -8,6 +8,7 @@
 // built again, finds exact fits.
 #include "common.hpp"
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <unordered_map>
@@ -108,11 +109,20 @@ void *cache_alloc( std::size_t bytes, hipStream_t s )
     return p;
 }
 
+namespace
+{
+thread_local GraphBlocks *t_keep = nullptr;
+}
+
 void cache_free( void *p, std::size_t bytes, hipStream_t s )
 {
-    (void)bytes;
     if ( !p )
         return;
+    if ( t_keep )
+    {
+        t_keep->blocks.push_back( {p, bytes, s} );
+        return;
+    }
     BlockCache &c = cache();
     std::lock_guard<std::mutex> lock( c.mu );
     auto it = c.live.find( p );
@@ -181,6 +191,43 @@ Fork::~Fork()
         (void)hipStreamWaitEvent( main_, join_, 0 );
     (void)hipEventDestroy( fork_ );
     (void)hipEventDestroy( join_ );
+}
+
+void GraphBlocks::release()
+{
+    for ( auto const &b : blocks )
+        cache_free( b.p, b.bytes, b.s );
+    blocks.clear();
+}
+
+KeepBlocks::KeepBlocks( GraphBlocks *keep )
+    : prev_( t_keep )
+{
+    t_keep = keep;
+}
+
+KeepBlocks::~KeepBlocks() { t_keep = prev_; }
+
+namespace
+{
+__global__ void zero_kernel( uint32_t *p, std::size_t n )
+{
+    for ( std::size_t i = blockIdx.x * (std::size_t)blockDim.x + threadIdx.x;
+          i < n; i += (std::size_t)gridDim.x * blockDim.x )
+        p[i] = 0u;
+}
+} // namespace
+
+void zero_async( void *p, std::size_t bytes, hipStream_t s )
+{
+    DTKX_REQUIRE( bytes % 4 == 0, DTKX_ERR_INVALID_ARGUMENT );
+    std::size_t const n = bytes / 4;
+    if ( n == 0 )
+        return;
+    std::size_t const blocks = std::min<std::size_t>( ( n + 255 ) / 256, 1024 );
+    zero_kernel<<<static_cast<unsigned>( blocks ), 256, 0, s>>>(
+        static_cast<uint32_t *>( p ), n );
+    DTKX_HIP( hipGetLastError() );
 }
 
 void cache_release()

@@ -372,7 +372,7 @@ void setup_fast( double4 const *src, double4 const *tgt, int64_t n_tgt,
     DTKX_REQUIRE( p.m == 1 || p.m == 4 || p.m == 10,
                   DTKX_ERR_INVALID_ARGUMENT );
     DeviceBuffer<int32_t> list( n_tgt - p.row0, s ), count( 1, s );
-    DTKX_HIP( hipMemsetAsync( count.get(), 0, sizeof( int32_t ), s ) );
+    zero_async( count.get(), sizeof( int32_t ), s );
     if ( p.m == 1 )
         launch_fast<1>( src, tgt, n_tgt, indices, order, p, coeffs, coeffs32,
                         pinv, list.get(), count.get(), counters, s );
@@ -393,7 +393,7 @@ void setup_fast( double4 const *src, double4 const *tgt, int64_t n_tgt,
         rcount( pinv ? 0 : 1, s );
     if ( !pinv )
     {
-        DTKX_HIP( hipMemsetAsync( rcount.get(), 0, sizeof( int32_t ), s ) );
+        zero_async( rcount.get(), sizeof( int32_t ), s );
         q.refine_rows = rlist.get();
         q.n_refine = rcount.get();
     }

@@ -219,7 +219,9 @@ class MovingLeastSquaresBatch:
     problems: sequence of (source_points [n_s, 3], target_points [n_t, 3],
     basis, k) with float64 CUDA tensors; options as for
     MovingLeastSquaresOperator (group_size 0 lets the classical solver pick
-    its lanes per system by timing)."""
+    its lanes per system by timing). The point tensors are held by the
+    batch: rebuild() reads them again after the caller has moved the points
+    in place."""
 
     def __init__(self, problems, *, rbf="wendland0", precision="fp64",
                  geometry="float", solver="classical", truncation="absolute",
@@ -253,7 +255,6 @@ class MovingLeastSquaresBatch:
         h = ctypes.c_void_p()
         check(_lib.load().dtkx_mls_batch_create(n, arr, ctypes.byref(o), ctypes.byref(h)))
         self._h = h
-        self._keep = []  # points are copied at create
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -281,6 +282,14 @@ class MovingLeastSquaresBatch:
             ys[i] = y.data_ptr()
         check(_lib.load().dtkx_mls_batch_apply(self._h, xs, ys))
         return target_values
+
+    def rebuild(self):
+        """Builds every operator again from the current contents of the
+        point tensors given at construction (same sizes, bases and k): one
+        replay of the search and setup captured on the first call, with
+        results bit-identical to a new batch over the moved points."""
+        check(_lib.load().dtkx_mls_batch_rebuild(self._h))
+        return self
 
     def stats(self) -> dict:
         s = _lib.Stats()

@@ -316,7 +316,8 @@ typedef struct dtkx_mls_batch dtkx_mls_batch;
 
 /* Build many independent operators, each with its own basis and k (other
  * options from opts; store_pinv is not supported). Every problem needs at
- * least one source; points are device pointers and are copied. All trees
+ * least one source; points are device pointers, copied here and again by
+ * each dtkx_mls_batch_rebuild. All trees
  * are built by one set of kernels over the concatenated, problem-sorted
  * sources; queries run in one launch per neighbour-list size and the setup
  * in one launch per basis. With the classical solver and group_size 0 the
@@ -333,6 +334,18 @@ int dtkx_mls_batch_create(int32_t n_problems, const dtkx_mls_problem *problems,
 int dtkx_mls_batch_apply(const dtkx_mls_batch *b,
                          const double *const *src_vals,
                          double *const *tgt_vals);
+/* Builds every operator of the batch again from the problems' point arrays
+ * at the addresses given to dtkx_mls_batch_create, whose contents may have
+ * changed since (moving points; sizes, bases and k stay as created, and the
+ * arrays must still be valid). The search and setup are captured in a HIP
+ * graph on the first call and replayed: no host bookkeeping, uploads or
+ * allocations after it. Each problem's result is bit-identical to a new
+ * dtkx_mls_batch_create on the current points (the classical solver keeps
+ * the lanes per system chosen at create; every choice gives the same bits).
+ * Statistics: counters of this build; ms_knn is the whole replay, ms_setup
+ * and ms_knn_query 0. Synchronous at return. No counterpart in the
+ * reference (a new MovingLeastSquaresOperator is constructed instead). */
+int dtkx_mls_batch_rebuild(dtkx_mls_batch *b);
 int dtkx_mls_batch_get_stats(const dtkx_mls_batch *b, dtkx_mls_stats *stats);
 int dtkx_mls_batch_destroy(dtkx_mls_batch *b);
 

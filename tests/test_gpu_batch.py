@@ -121,6 +121,77 @@ def test_graph_replay_reads_new_values(dtk):
             assert torch.equal(y, ref)
 
 
+def deep_chain_problem():
+    """Sources whose hierarchy is deeper than the query's LDS stack (pairs
+    halving their distance to the origin, then 65,536 copies of it), with
+    targets at and within 1e-3 of the origin: the chain pairs inside a near
+    target's k-th distance and the duplicates push past the LDS stack, and
+    the search re-runs those targets with the global stack."""
+    rng = np.random.default_rng(4)
+    chain = np.repeat(np.array([[2.0 ** -i] * 3 for i in range(1, 22)]), 2, axis=0)
+    src = np.concatenate([rng.random((3000, 3)), chain, np.zeros((65536, 3))])
+    tgt = np.concatenate([np.zeros((64, 3)), rng.random((500, 3)) * 1e-3,
+                          rng.random((300, 3))])
+    return src, tgt, "constant", 16, np.cos(3 * src[:, 0]) + src[:, 1]
+
+
+REBUILD_FORMS = [FORMS[0], FORMS[2], FORMS[4]]
+
+
+@pytest.mark.parametrize("form", REBUILD_FORMS,
+                         ids=lambda f: f"{f['solver']}-{f['precision']}")
+def test_batch_rebuild_equals_new_batch(dtk, form):
+    """rebuild() after the points move in place: the captured search and
+    setup, replayed, give bit for bit the operators (and counters) of a new
+    batch over the moved points. Replays alternate with eager work (new
+    batches and applies), and the last move returns to the first points,
+    which must give the first results again. The deep problem makes the
+    search's overflow re-run part of the graph."""
+    probs = make_problems(24, seed=41) + [deep_chain_problem()]
+    dev = [(cuda(s), cuda(t), b, k) for s, t, b, k, _ in probs]
+    orig = [(s.clone(), t.clone()) for s, t, _, _ in dev]
+    opts = dict(rbf="wendland2", geometry="float", **form)
+    batch = dtk.MovingLeastSquaresBatch(dev, **opts)
+    xs = [cuda(v) for *_, v in probs]
+
+    def run(b):
+        ys = [torch.full((len(t),), np.nan, dtype=torch.float64, device="cuda")
+              for _, t, _, _ in dev]
+        b.apply(xs, ys)
+        torch.cuda.synchronize()
+        return ys
+
+    first = run(batch)
+    assert batch.stats()["n_stack_overflow"] > 0
+    keys = ("n_underdetermined", "n_unconverged", "total_rotations",
+            "n_eigen_fallback", "n_stack_overflow")
+    rng = np.random.default_rng(5)
+    for rep in range(4):
+        print(f"rebuild {rep} {form}", flush=True)
+        for (s, t, _, _), (s0, t0) in zip(dev, orig):
+            if rep == 3:
+                s.copy_(s0)
+                t.copy_(t0)
+            else:
+                # scaled, shifted and jittered: the neighbour lists change
+                s.copy_(s0 * (1 + 0.1 * rep) + 0.3 * rep
+                        + cuda(1e-3 * rng.standard_normal(tuple(s.shape))))
+                t.copy_(t0 * (1 + 0.1 * rep) + 0.3 * rep
+                        + cuda(1e-3 * rng.standard_normal(tuple(t.shape))))
+        batch.rebuild()
+        got = run(batch)
+        fresh = dtk.MovingLeastSquaresBatch(dev, **opts)
+        ref = run(fresh)
+        for i, (a, r) in enumerate(zip(got, ref)):
+            assert torch.equal(a, r), f"rebuild {rep}, problem {i}"
+        st, sf = batch.stats(), fresh.stats()
+        for key in keys:
+            assert st[key] == sf[key], (rep, key, st[key], sf[key])
+        if rep == 3:
+            for a, r in zip(got, first):
+                assert torch.equal(a, r)
+
+
 def test_autotuned_group_gives_the_same_bits(dtk):
     probs = make_problems(16, seed=12)
     dev = [(cuda(s), cuda(t), b, k) for s, t, b, k, _ in probs]
