@@ -241,16 +241,41 @@ void invert_permutation( int32_t const *perm, int64_t n, int32_t *inv,
 
 // Work tile of the current block under the MI355X dispatch order, which
 // hands consecutive workgroups to the 8 XCDs in turn (each XCD with its own
-// L2): the blocks an XCD receives take one contiguous range of tiles, so
-// spatially adjacent work (Morton-ordered rows) shares an L2 instead of
-// being spread over all eight. A bijection of [0, gridDim.x).
+// L2). A bijection of [0, gridDim.x) in one of two shapes:
+//  * kChunk = 0: each XCD takes one contiguous eighth of the tiles, so the
+//    spatially adjacent work of Morton-ordered rows shares an L2. The kNN
+//    queries use this: an XCD's L2 then holds the subtrees of its own region
+//    only (2.03 ms for the 1M x 1M, k = 32 query, against 2.06-2.13 ms with
+//    chunks of 32-512 tiles).
+//  * kChunk > 0: the tiles are cut into chunks of kChunk consecutive tiles,
+//    and chunk j goes to XCD j mod 8, each XCD taking its chunks in order:
+//    adjacent work inside a chunk still shares an L2, and the eight XCDs
+//    sweep the whole range side by side, so work whose cost changes along
+//    the range stays balanced over them. The setup and apply use this: a
+//    batch's rows are ordered by neighbour count, and with contiguous
+//    eighths the XCDs holding its 64-neighbour rows had twice the average
+//    work (1M rows in four k groups: setup 0.99 ms contiguous, 0.46 ms with
+//    chunks of 512). Tiles past the last whole round of 8 chunks keep the
+//    dispatch order.
+constexpr int64_t kBalancedChunk = 512;
+template <int64_t kChunk = kBalancedChunk>
 __device__ inline int64_t xcd_tile()
 {
     constexpr int64_t kXcds = 8;
     int64_t const nb = gridDim.x, b = blockIdx.x;
-    int64_t const per = nb / kXcds, rem = nb % kXcds;
     int64_t const x = b % kXcds, i = b / kXcds;
-    return x * per + ( x < rem ? x : rem ) + i;
+    if constexpr ( kChunk == 0 )
+    {
+        int64_t const per = nb / kXcds, rem = nb % kXcds;
+        return x * per + ( x < rem ? x : rem ) + i;
+    }
+    else
+    {
+        constexpr int64_t kRound = kXcds * kChunk;
+        if ( b >= nb / kRound * kRound )
+            return b;
+        return i / kChunk * kRound + x * kChunk + i % kChunk;
+    }
 }
 
 inline unsigned ceil_div( int64_t a, int64_t b )

@@ -902,7 +902,7 @@ __global__ void __launch_bounds__( kQueryBlock )
     constexpr int LS = STACK > 0 ? STACK : 1;
     __shared__ int32_t st_node[LS * kQueryBlock];
     __shared__ T st_d[LS * kQueryBlock];
-    int64_t const qi = xcd_tile() * kQueryBlock + threadIdx.x;
+    int64_t const qi = xcd_tile<0>() * kQueryBlock + threadIdx.x;
     if constexpr ( STACK == 0 )
         if ( a.rows )
         {
@@ -1352,7 +1352,7 @@ __global__ void __launch_bounds__( kQueryBlock )
     constexpr int LS = STACK > 0 ? STACK : 1;
     __shared__ int32_t st_node[LS * kQueryBlock];
     __shared__ T st_d[LS * kQueryBlock];
-    int64_t const qi = xcd_tile() * kQueryBlock + threadIdx.x;
+    int64_t const qi = xcd_tile<0>() * kQueryBlock + threadIdx.x;
     if constexpr ( STACK == 0 )
         if ( a.rows )
         {
@@ -1375,7 +1375,7 @@ template <typename T, int K>
 __global__ void __launch_bounds__( kQueryBlock )
     knn_sort_rows_kernel( QueryArgs<T> const a )
 {
-    int64_t const qi = xcd_tile() * kQueryBlock + threadIdx.x;
+    int64_t const qi = xcd_tile<0>() * kQueryBlock + threadIdx.x;
     if ( qi >= a.n_rows )
         return;
     int64_t const r = a.row0 + qi;
