@@ -176,9 +176,9 @@ constexpr int kTgtPasses = ( kTgtKeyBits + 7 ) / 8;
 constexpr int kHistWords = ( kSrcPasses + kTgtPasses ) * rsort::kDigits;
 
 // Morton codes of the sources (63-bit) and of the targets (top 30 bits) in
-// one launch, 4096 points a block (blocks [0, src_blocks) take the sources),
-// with the digit counts of every radix pass added into hist (sources'
-// passes first, zero on entry). src_prob / tgt_prob (nullable, a batch):
+// one launch, one sort tile of points a block (blocks [0, src_blocks) take
+// the sources), with the digit counts of every radix pass added into hist
+// (sources' passes first, zero on entry). src_prob / tgt_prob (nullable, a batch):
 // each point's problem, whose scene box is box + 6 problem.
 template <typename T, typename V>
 __global__ void __launch_bounds__( rsort::kThreads )
@@ -1728,7 +1728,8 @@ __global__ void __launch_bounds__( kBlock )
 // Second-stage keys of a batch's sorts, read through the first stage's
 // order: a source's problem, a target's row key (row_key[problem]), with
 // the digit counts of their passes added into hist (sources' first, zero on
-// entry); 4096 keys a block, blocks [0, src_blocks) take the sources.
+// entry); one sort tile of keys a block, blocks [0, src_blocks) take the
+// sources.
 __global__ void __launch_bounds__( rsort::kThreads )
     group_keys_kernel( int32_t const *src_ids, int32_t const *src_prob,
                        int64_t n_src, int src_passes, int32_t const *tgt_ids,

@@ -1,7 +1,7 @@
 // Stable LSD radix sort of (key, index) pairs for the kNN build's Morton
 // codes, written for gfx950 wave64.
 //
-// One launch per 8-bit digit. A block takes a tile of 4096 keys (a ticket
+// One launch per 8-bit digit. A block takes a tile of 8192 keys (a ticket
 // from a counter fixes the tile order, so every tile it waits for is already
 // running), ranks them in LDS, publishes the tile's 256 digit counts and
 // reads the counts of the tiles before it from their published words
@@ -10,8 +10,8 @@
 // digit order to its place. A pass thus reads and writes each pair once,
 // with no separate histogram or scan launch.
 //
-// Ranking is stable: each of the block's 16 waves holds 256 consecutive
-// keys as 4 slots of 64, and ranks them slot by slot in index order. For one slot the lanes with
+// Ranking is stable: each of the block's 16 waves holds 512 consecutive
+// keys as 8 slots of 64, and ranks them slot by slot in index order. For one slot the lanes with
 // the same digit are found with one ballot per digit bit; the lowest of
 // them adds the group's size to the wave's count for that digit.
 //
@@ -32,17 +32,27 @@ namespace dtkx
 namespace rsort
 {
 
-// 16 waves of 4 keys a lane: at one block per tile, a shorter ranking
-// chain per wave and more waves to hide it. Measured at 2^20 source keys
-// (with the targets' 2^20 in the first four passes), per pass: 18-19 us
-// alone, 29-34 us with the targets; 4 waves of 16 keys took 29 / 38-46 us.
+// 16 waves of 8 keys a lane, 8192 keys a tile (112 KB of LDS, one block
+// per CU). Per pass at 2^20 source keys (with the targets' 2^20 in the
+// first four passes), with 4096-key tiles of 16 waves: 18-19 us alone,
+// 29-34 us with the targets; 4 waves of 16 keys took 29 / 38-46 us.
+// Fewer, larger tiles shorten the look-back chains, which outweighs the
+// CUs they leave idle: the kNN build (scene box to hierarchy, 1M x 1M)
+// took 353 us with 8192-key tiles, 369 with 4096, 416 with 2048 (512
+// threads) and 551 with 1024 (256 threads).
 #ifndef DTKX_RSORT_THREADS
 #define DTKX_RSORT_THREADS 1024
 #endif
+#ifndef DTKX_RSORT_TILE
+#define DTKX_RSORT_TILE 8192
+#endif
 constexpr int kThreads = DTKX_RSORT_THREADS;
 constexpr int kWaves = kThreads / 64;
-constexpr int kItems = 4096 / kThreads;
+constexpr int kItems = DTKX_RSORT_TILE / kThreads;
 constexpr int kTile = kThreads * kItems;
+static_assert( kThreads >= 256 && kThreads % 64 == 0 && kItems >= 1 &&
+                   kTile == DTKX_RSORT_TILE,
+               "a thread per digit, whole waves, whole slots" );
 constexpr int kDigits = 256;
 constexpr int kMaxPasses = 8;
 // look-back words read per round trip (32 took 3-5 us longer a pass: its
