@@ -255,8 +255,10 @@ void invert_permutation( int32_t const *perm, int64_t n, int32_t *inv,
 //    batch's rows are ordered by neighbour count, and with contiguous
 //    eighths the XCDs holding its 64-neighbour rows had twice the average
 //    work (1M rows in four k groups: setup 0.99 ms contiguous, 0.46 ms with
-//    chunks of 512). Tiles past the last whole round of 8 chunks keep the
-//    dispatch order.
+//    chunks of 512). Tiles past the last whole round of 8 chunks are split
+//    into contiguous eighths, as with kChunk = 0 (in dispatch order they
+//    would alternate between the XCDs tile by tile, and the 1M-row setup
+//    has a fifth of its tiles there).
 constexpr int64_t kBalancedChunk = 512;
 template <int64_t kChunk = kBalancedChunk>
 __device__ inline int64_t xcd_tile()
@@ -272,9 +274,13 @@ __device__ inline int64_t xcd_tile()
     else
     {
         constexpr int64_t kRound = kXcds * kChunk;
-        if ( b >= nb / kRound * kRound )
-            return b;
-        return i / kChunk * kRound + x * kChunk + i % kChunk;
+        int64_t const full = nb / kRound * kRound;
+        if ( b < full )
+            return i / kChunk * kRound + x * kChunk + i % kChunk;
+        // (full is a multiple of 8: the tail's block b - full is on XCD x)
+        int64_t const nt = nb - full, it = ( b - full ) / kXcds;
+        int64_t const per = nt / kXcds, rem = nt % kXcds;
+        return full + x * per + ( x < rem ? x : rem ) + it;
     }
 }
 
