@@ -38,6 +38,10 @@ struct SetupParams
     // pass the truncation test are appended here (device count n_refine)
     int32_t *refine_rows = nullptr;
     int32_t *n_refine = nullptr;
+    // cyclic solver over a list that setup_screen made: the sweeps and
+    // unconverged systems are counted for truncated rows only (the screen
+    // counted the others)
+    bool screened = false;
 };
 
 // counters[0] = n_underdetermined, [1] = n_unconverged, [2] = rotations
@@ -66,6 +70,18 @@ void setup_cyclic( double4 const *src, double4 const *tgt, int64_t n_tgt,
                    int32_t const *indices, int32_t const *order,
                    SetupParams const &p, double *coeffs, float *coeffs32,
                    double *pinv, unsigned long long *counters, hipStream_t s );
+
+// Eigenvalue screen of the listed rows p.rows[0 .. *p.n_rows): the cyclic
+// Jacobi of setup_cyclic without eigenvectors (the same rotations of A, so
+// the same eigenvalues, sweeps and convergence). Rows whose eigenvalues all
+// pass the truncation test are appended to p.refine_rows, the others to
+// full_rows; counters[1] and [2] are taken for the first kind. f32: the
+// FP32-bulk form's geometry and weights.
+void setup_screen( double4 const *src, double4 const *tgt, int64_t n_tgt,
+                   int32_t const *indices, int32_t const *order,
+                   SetupParams const &p, bool f32, int32_t *full_rows,
+                   int32_t *n_full, unsigned long long *counters,
+                   hipStream_t s );
 
 // Certified Cholesky fast path (DTKX_SOLVER_CHOLESKY): each system is
 // factored A = U^T U in FP64; when the bound lambda_min(A) >= 1 / |U^-1|_F^2

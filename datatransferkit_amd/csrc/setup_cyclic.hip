@@ -32,6 +32,98 @@ namespace
 using namespace cyc;
 constexpr int kBlock = 64;
 
+// One listed or ranged row's neighbour list, target and radius
+// (Impl.hpp:118-134: the largest neighbour distance, for the FP32-bulk form
+// possibly taken from the kNN), with its moments (upper triangle) written
+// to A. Shared by the eigen-solver and the eigenvalue screen, so both form
+// the same bits.
+struct SystemRow
+{
+    int64_t rbeg;
+    int kk;
+    int32_t const *nbr;
+    double4 tp;
+    double radius;
+};
+
+template <int M, bool F32>
+__device__ __forceinline__ SystemRow
+    system_moments( bool valid, int64_t slot, double4 const *__restrict__ src,
+                    double4 const *__restrict__ tgt,
+                    int32_t const *__restrict__ indices,
+                    int32_t const *__restrict__ order,
+                    int64_t const *__restrict__ row_off, int kk_uniform,
+                    int rbf, int geometry, float const *__restrict__ row_max,
+                    double *A )
+{
+    using S = SymIdx<M>;
+    int64_t const t = valid ? ( order ? order[slot] : slot ) : 0;
+    int64_t const rbeg =
+        valid ? ( row_off ? row_off[slot] : slot * kk_uniform ) : 0;
+    int const kk =
+        valid ? ( row_off ? (int)( row_off[slot + 1] - rbeg ) : kk_uniform )
+              : 0;
+    int32_t const *nbr = indices + rbeg;
+    double4 const tp = valid ? tgt[t] : double4{0, 0, 0, 0};
+
+    double dmax = 10. * DBL_EPSILON;
+    if ( F32 && row_max )
+    {
+        double const d = valid ? (double)row_max[slot] : 0.0;
+        dmax = d > dmax ? d : dmax;
+    }
+    else if ( valid )
+        for ( int l = 0; l < kk; ++l )
+        {
+            double const d = nbr_distance<M, F32>( geometry, src[nbr[l]], tp );
+            dmax = d > dmax ? d : dmax;
+        }
+    double const radius = F32 ? (double)( 1.1f * (float)dmax ) : 1.1 * dmax;
+
+#pragma unroll
+    for ( int e = 0; e < S::size; ++e )
+        A[e] = 0.0;
+    if ( valid )
+        for ( int l = 0; l < kk; ++l )
+        {
+            double P[M];
+            double const phi =
+                nbr_row<M, F32>( geometry, rbf, radius, src[nbr[l]], tp, P );
+#pragma unroll
+            for ( int r = 0; r < M; ++r )
+            {
+                double const pr = P[r] * phi;
+#pragma unroll
+                for ( int c = r; c < M; ++c )
+                    A[S::at( r, c )] = fma( pr, P[c], A[S::at( r, c )] );
+            }
+        }
+    return SystemRow{rbeg, kk, nbr, tp, radius};
+}
+
+// Truncation threshold of a system's eigenvalues (DBL_EPSILON, or rcond
+// times the largest |eigenvalue|) and whether any falls below it.
+template <int M>
+__device__ __forceinline__ bool truncated( double const *A, int truncation,
+                                           double rcond, double &thr )
+{
+    using S = SymIdx<M>;
+    thr = DBL_EPSILON;
+    if ( truncation == DTKX_TRUNCATION_RELATIVE )
+    {
+        double lmax = 0.;
+#pragma unroll
+        for ( int k = 0; k < M; ++k )
+            lmax = fmax( lmax, fabs( A[S::at( k, k )] ) );
+        thr = rcond * lmax;
+    }
+    bool under = false;
+#pragma unroll
+    for ( int k = 0; k < M; ++k )
+        under = under || !( fabs( A[S::at( k, k )] ) >= thr );
+    return under;
+}
+
 // H lanes per system (H = 1, or 2 for M = 10 so A and half of V fit the
 // 256 VGPRs): every lane of the pair keeps the whole moment matrix and runs
 // the same rotations on it (identical values, so both agree), and owns
@@ -56,7 +148,7 @@ __device__ __forceinline__ void
                          unsigned long long *__restrict__ counters,
                          float const *__restrict__ row_max,
                          int32_t *__restrict__ refine_rows,
-                         int32_t *__restrict__ n_refine )
+                         int32_t *__restrict__ n_refine, bool screened )
 {
     using S = SymIdx<M>;
     constexpr int R = M / H;
@@ -74,51 +166,15 @@ __device__ __forceinline__ void
         slot = row0 + li;
         valid = slot < n_tgt;
     }
-    int64_t const t = valid ? ( order ? order[slot] : slot ) : 0;
-    int64_t const rbeg =
-        valid ? ( row_off ? row_off[slot] : slot * kk_uniform ) : 0;
-    int const kk =
-        valid ? ( row_off ? (int)( row_off[slot + 1] - rbeg ) : kk_uniform )
-              : 0;
-    int32_t const *nbr = indices + rbeg;
-    double4 const tp = valid ? tgt[t] : double4{0, 0, 0, 0};
-
-    // radius (Impl.hpp:118-134): the largest neighbour distance, for the
-    // FP32-bulk form possibly taken from the kNN
-    double dmax = 10. * DBL_EPSILON;
-    if ( F32 && row_max )
-    {
-        double const d = valid ? (double)row_max[slot] : 0.0;
-        dmax = d > dmax ? d : dmax;
-    }
-    else if ( valid )
-        for ( int l = 0; l < kk; ++l )
-        {
-            double const d = nbr_distance<M, F32>( geometry, src[nbr[l]], tp );
-            dmax = d > dmax ? d : dmax;
-        }
-    double const radius = F32 ? (double)( 1.1f * (float)dmax ) : 1.1 * dmax;
-
-    // moments, upper triangle
     double A[S::size];
-#pragma unroll
-    for ( int e = 0; e < S::size; ++e )
-        A[e] = 0.0;
-    if ( valid )
-        for ( int l = 0; l < kk; ++l )
-        {
-            double P[M];
-            double const phi =
-                nbr_row<M, F32>( geometry, rbf, radius, src[nbr[l]], tp, P );
-#pragma unroll
-            for ( int r = 0; r < M; ++r )
-            {
-                double const pr = P[r] * phi;
-#pragma unroll
-                for ( int c = r; c < M; ++c )
-                    A[S::at( r, c )] = fma( pr, P[c], A[S::at( r, c )] );
-            }
-        }
+    SystemRow const sr =
+        system_moments<M, F32>( valid, slot, src, tgt, indices, order, row_off,
+                                kk_uniform, rbf, geometry, row_max, A );
+    int const kk = sr.kk;
+    int64_t const rbeg = sr.rbeg;
+    int32_t const *nbr = sr.nbr;
+    double4 const tp = sr.tp;
+    double const radius = sr.radius;
 
     // cyclic Jacobi; this lane owns rows h*R .. h*R + R - 1 of V
     double V[R * M];
@@ -256,13 +312,18 @@ __device__ __forceinline__ void
     }
     if ( refine_rows && h == 0 && !under )
         refine_rows[atomicAdd( n_refine, 1 )] = (int32_t)slot;
+    // (a screened list's untruncated rows had their sweeps counted by the
+    // screen)
     if ( counters && h == 0 )
     {
         if ( under )
             atomicAdd( counters + 0, 1ull );
-        if ( !done )
-            atomicAdd( counters + 1, 1ull );
-        atomicAdd( counters + 2, (unsigned long long)sweeps );
+        if ( !screened || under )
+        {
+            if ( !done )
+                atomicAdd( counters + 1, 1ull );
+            atomicAdd( counters + 2, (unsigned long long)sweeps );
+        }
     }
 }
 
@@ -287,7 +348,7 @@ __global__ void __launch_bounds__( kBlock )
                          unsigned long long *__restrict__ counters,
                          float const *__restrict__ row_max,
                          int32_t *__restrict__ refine_rows,
-                         int32_t *__restrict__ n_refine )
+                         int32_t *__restrict__ n_refine, bool screened )
 {
     constexpr int SPB = kBlock / H;
     auto one = [&]( int64_t tile ) {
@@ -295,7 +356,7 @@ __global__ void __launch_bounds__( kBlock )
             tile * SPB + threadIdx.x / H, src, tgt, n_tgt, indices, order, row0,
             row_off, rows, n_rows, kk_uniform, rbf, geometry, truncation, rcond,
             max_sweeps, coeffs, coeffs32, pinv_out, counters, row_max,
-            refine_rows, n_refine );
+            refine_rows, n_refine, screened );
     };
     if constexpr ( LIST )
     {
@@ -330,7 +391,8 @@ void launch_cyclic( double4 const *src, double4 const *tgt, int64_t n_tgt,
                                      p.row_off, p.rows, p.n_rows, p.kk,
                                      p.rbf, p.geometry, p.truncation, p.rcond,
                                      sweeps, coeffs, coeffs32, pinv, counters,
-                                     p.row_max, p.refine_rows, p.n_refine );
+                                     p.row_max, p.refine_rows, p.n_refine,
+                                     p.screened );
     };
     using T = std::true_type;
     using F = std::false_type;
@@ -341,7 +403,111 @@ void launch_cyclic( double4 const *src, double4 const *tgt, int64_t n_tgt,
     DTKX_HIP( hipGetLastError() );
 }
 
+// Eigenvalues only, one lane per listed row: the cyclic Jacobi on A
+// without eigenvectors. A rotation of A never reads V, and a converged
+// system's rotations are exact no-ops whatever its wave, so the
+// eigenvalues, the sweep count and the convergence flag are those
+// cyclic_system computes for the row. Rows whose eigenvalues all pass the
+// truncation test go to refine_rows (solved again by Cholesky, with their
+// sweeps and convergence counted here), the others to full_rows (the
+// eigen-solver with eigenvectors). Most rows the Cholesky setup lists are
+// of the first kind: on 10-neighbour quadratic systems (1M targets), 466k
+// of the 493k listed rows.
+template <int M, bool F32>
+__global__ void __launch_bounds__( kBlock )
+    eigen_screen_kernel( double4 const *__restrict__ src,
+                         double4 const *__restrict__ tgt,
+                         int32_t const *__restrict__ indices,
+                         int32_t const *__restrict__ order,
+                         int64_t const *__restrict__ row_off,
+                         int32_t const *__restrict__ rows,
+                         int32_t const *__restrict__ n_rows, int kk_uniform,
+                         int rbf, int geometry, int truncation, double rcond,
+                         int max_sweeps,
+                         unsigned long long *__restrict__ counters,
+                         float const *__restrict__ row_max,
+                         int32_t *__restrict__ refine_rows,
+                         int32_t *__restrict__ n_refine,
+                         int32_t *__restrict__ full_rows,
+                         int32_t *__restrict__ n_full )
+{
+    using S = SymIdx<M>;
+    // the trip count is uniform over the block (the wave-wide vote)
+    int64_t const n = *n_rows;
+    for ( int64_t tile = xcd_tile(); tile * kBlock < n; tile += gridDim.x )
+    {
+        int64_t const li = tile * kBlock + threadIdx.x;
+        bool const valid = li < n;
+        int64_t const slot = valid ? rows[li] : 0;
+        double A[S::size];
+        (void)system_moments<M, F32>( valid, slot, src, tgt, indices, order,
+                                      row_off, kk_uniform, rbf, geometry,
+                                      row_max, A );
+        double no_v[1];
+        int sweeps = 0;
+        bool done = !valid || converged<M>( A );
+        for ( int pass = 0; !__all( done ) && pass < max_sweeps; ++pass )
+        {
+            sweep_all<M, 0>( A, no_v, done );
+            sweeps += done ? 0 : 1;
+            done = done || converged<M>( A );
+        }
+        if ( !valid )
+            continue;
+        double thr;
+        if ( truncated<M>( A, truncation, rcond, thr ) )
+            full_rows[atomicAdd( n_full, 1 )] = (int32_t)slot;
+        else
+        {
+            refine_rows[atomicAdd( n_refine, 1 )] = (int32_t)slot;
+            if ( counters )
+            {
+                if ( !done )
+                    atomicAdd( counters + 1, 1ull );
+                atomicAdd( counters + 2, (unsigned long long)sweeps );
+            }
+        }
+    }
+}
+
 } // namespace
+
+void setup_screen( double4 const *src, double4 const *tgt, int64_t n_tgt,
+                   int32_t const *indices, int32_t const *order,
+                   SetupParams const &p, bool f32, int32_t *full_rows,
+                   int32_t *n_full, unsigned long long *counters,
+                   hipStream_t s )
+{
+    DTKX_REQUIRE( p.rows && p.n_rows && p.refine_rows && p.n_refine,
+                  DTKX_ERR_INVALID_ARGUMENT );
+    if ( n_tgt <= p.row0 )
+        return;
+    int const sweeps = p.max_rot > 0 ? static_cast<int>( p.max_rot ) : 30;
+    int64_t blocks = ceil_div( n_tgt - p.row0, kBlock );
+    blocks = blocks < kListBlocks ? blocks : kListBlocks;
+    auto run = [&]( auto m, auto f32 ) {
+        eigen_screen_kernel<decltype( m )::value, decltype( f32 )::value>
+            <<<blocks, kBlock, 0, s>>>(
+                src, tgt, indices, order, p.row_off, p.rows, p.n_rows, p.kk,
+                p.rbf, p.geometry, p.truncation, p.rcond, sweeps, counters,
+                p.row_max, p.refine_rows, p.n_refine, full_rows, n_full );
+    };
+    using T = std::true_type;
+    using F = std::false_type;
+    using M1 = std::integral_constant<int, 1>;
+    using M4 = std::integral_constant<int, 4>;
+    using M10 = std::integral_constant<int, 10>;
+    if ( p.m == 1 )
+        f32 ? run( M1{}, T{} ) : run( M1{}, F{} );
+    else if ( p.m == 4 )
+        f32 ? run( M4{}, T{} ) : run( M4{}, F{} );
+    else if ( p.m == 10 )
+        f32 ? run( M10{}, T{} ) : run( M10{}, F{} );
+    else
+        DTKX_REQUIRE( p.m == 1 || p.m == 4 || p.m == 10,
+                      DTKX_ERR_INVALID_ARGUMENT );
+    DTKX_HIP( hipGetLastError() );
+}
 
 void setup_cyclic( double4 const *src, double4 const *tgt, int64_t n_tgt,
                    int32_t const *indices, int32_t const *order,

@@ -259,7 +259,9 @@ __global__ void __launch_bounds__( kBlock )
                    int rbf, int geometry, int32_t const *__restrict__ rows,
                    int32_t const *__restrict__ n_rows,
                    double *__restrict__ coeffs, float *__restrict__ coeffs32,
-                   float const *__restrict__ row_max )
+                   float const *__restrict__ row_max,
+                   int32_t *__restrict__ full_rows,
+                   int32_t *__restrict__ n_full )
 {
     using S = SymIdx<M>;
     int64_t const n = *n_rows;
@@ -307,7 +309,11 @@ __global__ void __launch_bounds__( kBlock )
             }
         }
         if ( !cholesky_factor_inverse<M>( A ) )
+        {
+            // to the eigen-solver with eigenvectors, which runs next
+            full_rows[atomicAdd( n_full, 1 )] = (int32_t)slot;
             continue;
+        }
         double x[M];
 #pragma unroll
         for ( int i = 0; i < M; ++i )
@@ -382,33 +388,39 @@ void setup_fast( double4 const *src, double4 const *tgt, int64_t n_tgt,
     else
         launch_fast<10>( src, tgt, n_tgt, indices, order, p, coeffs, coeffs32,
                          pinv, list.get(), count.get(), counters, s );
-    // the listed systems through the eigen-solver (a grid for every row;
-    // blocks past the device-side count return at once); with no stored
-    // pseudo-inverse to keep consistent, its untruncated rows are listed
-    // again and solved by Cholesky
+    // the listed systems: with a stored pseudo-inverse to keep consistent,
+    // all through the eigen-solver (a grid for every row; blocks past the
+    // device-side count return at once)
     SetupParams q = p;
     q.rows = list.get();
     q.n_rows = count.get();
-    DeviceBuffer<int32_t> rlist( pinv ? 0 : n_tgt - p.row0, s ),
-        rcount( pinv ? 0 : 1, s );
-    if ( !pinv )
-    {
-        zero_async( rcount.get(), sizeof( int32_t ), s );
-        q.refine_rows = rlist.get();
-        q.n_refine = rcount.get();
-    }
-    setup_cyclic( src, tgt, n_tgt, indices, order, q, coeffs, coeffs32, pinv,
-                  counters, s );
     if ( pinv )
+    {
+        setup_cyclic( src, tgt, n_tgt, indices, order, q, coeffs, coeffs32,
+                      pinv, counters, s );
         return;
+    }
+    // otherwise their eigenvalues first: the rows with none below the
+    // threshold have A^+ = A^-1 and are solved again by Cholesky, the
+    // others (and any whose Cholesky meets a non-positive pivot) by the
+    // eigen-solver with eigenvectors; each row ends as the eigen-solver
+    // alone would leave it followed by the Cholesky of its untruncated rows
+    int64_t const nl = n_tgt - p.row0;
+    DeviceBuffer<int32_t> rlist( nl, s ), flist( nl, s ), counts( 2, s );
+    zero_async( counts.get(), 2 * sizeof( int32_t ), s );
+    q.refine_rows = rlist.get();
+    q.n_refine = counts.get();
+    setup_screen( src, tgt, n_tgt, indices, order, q, coeffs32 != nullptr,
+                  flist.get(), counts.get() + 1, counters, s );
     int64_t const blocks =
         std::min<int64_t>( ceil_div( n_tgt - p.row0, kBlock ), kRefineBlocks );
     auto refine = [&]( auto m, auto f32 ) {
         refine_kernel<decltype( m )::value, decltype( f32 )::value>
             <<<blocks, kBlock, 0, s>>>( src, tgt, indices, order, p.row_off,
                                          p.kk, p.rbf, p.geometry, rlist.get(),
-                                         rcount.get(), coeffs, coeffs32,
-                                         p.row_max );
+                                         counts.get(), coeffs, coeffs32,
+                                         p.row_max, flist.get(),
+                                         counts.get() + 1 );
     };
     using T = std::true_type;
     using F = std::false_type;
@@ -422,6 +434,12 @@ void setup_fast( double4 const *src, double4 const *tgt, int64_t n_tgt,
     else
         coeffs32 ? refine( M10{}, T{} ) : refine( M10{}, F{} );
     DTKX_HIP( hipGetLastError() );
+    SetupParams f = p;
+    f.rows = flist.get();
+    f.n_rows = counts.get() + 1;
+    f.screened = true;
+    setup_cyclic( src, tgt, n_tgt, indices, order, f, coeffs, coeffs32,
+                  nullptr, counters, s );
 }
 
 } // namespace dtkx
