@@ -87,12 +87,12 @@ void setup_screen( double4 const *src, double4 const *tgt, int64_t n_tgt,
 // factored A = U^T U in FP64; when the bound lambda_min(A) >= 1 / |U^-1|_F^2
 // proves that no eigenvalue reaches the truncation threshold (with a margin
 // for the eigen-solver's rounding), A^+ = A^-1 and a0 = U^-1 U^-T e_0 is
-// used. The other systems are listed and set up by the cyclic eigen-solver;
-// those of them with no eigenvalue below the threshold then have A^+ =
-// A^-1 as well and are solved again by Cholesky, whose rounding error
-// follows the equilibrated matrix, not the eigen-decomposition's
-// eps |A| / lambda (measured on 10-neighbour quadratic systems: 5e-10
-// against 2e-4 from the operator in extended precision).
+// used. The other systems are listed; those of them with no eigenvalue
+// below the threshold (setup_screen) have A^+ = A^-1 as well and are solved
+// again by Cholesky, whose rounding error follows the equilibrated matrix,
+// not the eigen-decomposition's eps |A| / lambda (measured on 10-neighbour
+// quadratic systems: 5e-10 against 2e-4 from the operator in extended
+// precision); the rest are set up by the cyclic eigen-solver.
 // Same outputs and counters as setup_cyclic.
 void setup_fast( double4 const *src, double4 const *tgt, int64_t n_tgt,
                  int32_t const *indices, int32_t const *order,
