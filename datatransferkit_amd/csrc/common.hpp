@@ -109,12 +109,12 @@ hipStream_t library_stream();
 // event only, not for later work queued on a caller's stream.
 hipStream_t side_stream();
 
-// Work forked from stream s onto a second stream of the library's own
-// (default priority, one per device): the side stream first waits for
-// everything queued on s so far, and join() (or the destructor) makes s
-// wait for everything queued on the side stream since. Buffers the side
-// work uses are allocated on s before the fork and released on s after the
-// join, so the block cache's stream order holds for them.
+// Work forked from stream s onto a stream of the library's own (default
+// priority, non-blocking, one per device, not side_stream()): it first
+// waits for everything queued on s so far, and join() (or the destructor)
+// makes s wait for everything queued on the forked stream since. Buffers
+// the forked work uses are allocated on s before the fork and released on
+// s after the join, so the block cache's stream order holds for them.
 class Fork
 {
   public:

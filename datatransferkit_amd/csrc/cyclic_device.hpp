@@ -113,10 +113,16 @@ __device__ inline void rotate( double *A, double *V, bool skip )
     double const aqq = A[S::at( Q_, Q_ )];
     double const theta = ( aqq - app ) / ( 2.0 * apq );
     double const at = fabs( theta );
-    double t = 1.0 / ( at + __builtin_sqrt( fma( theta, theta, 1.0 ) ) );
+    // huge |theta|: theta^2 overflows, t -> 1/(2 theta), formed as
+    // 0.5 / |theta| with the sign of theta (the same bits as 0.5 / theta).
+    // One division for both cases: a select between two divisions became a
+    // branch per rotation, and the unrolled sweep's registers tripled.
+    bool const huge = at > 1e150;
+    double const den =
+        huge ? at : at + __builtin_sqrt( fma( theta, theta, 1.0 ) );
+    double t = ( huge ? 0.5 : 1.0 ) / den;
     t = theta < 0 ? -t : t;
-    // huge |theta|: theta^2 overflows, t -> 1/(2 theta); apq == 0: no-op
-    t = at > 1e150 ? 0.5 / theta : t;
+    // apq == 0: no-op
     t = ( apq == 0.0 || skip ) ? 0.0 : t;
     double const c = 1.0 / __builtin_sqrt( fma( t, t, 1.0 ) );
     double const s = t * c;

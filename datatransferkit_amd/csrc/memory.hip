@@ -165,9 +165,30 @@ hipStream_t side_stream()
     return st;
 }
 
+namespace
+{
+// the forked work's own stream, apart from side_stream(), whose statistics
+// reads wait for a create's last event only
+hipStream_t fork_stream()
+{
+    static std::mutex mu;
+    static std::map<int, hipStream_t> streams;
+    int dev = 0;
+    DTKX_HIP( hipGetDevice( &dev ) );
+    std::lock_guard<std::mutex> lock( mu );
+    auto it = streams.find( dev );
+    if ( it != streams.end() )
+        return it->second;
+    hipStream_t st = nullptr;
+    DTKX_HIP( hipStreamCreateWithFlags( &st, hipStreamNonBlocking ) );
+    streams[dev] = st;
+    return st;
+}
+} // namespace
+
 Fork::Fork( hipStream_t s )
     : main_( s )
-    , side_( side_stream() )
+    , side_( fork_stream() )
 {
     DTKX_HIP( hipEventCreateWithFlags( &fork_, hipEventDisableTiming ) );
     DTKX_HIP( hipEventCreateWithFlags( &join_, hipEventDisableTiming ) );

@@ -152,7 +152,6 @@ __device__ __forceinline__ void
 {
     using S = SymIdx<M>;
     constexpr int R = M / H;
-    constexpr int SPB = kBlock / H; // systems per block
     int const h = threadIdx.x % H;
     bool valid;
     int64_t slot;
@@ -331,6 +330,7 @@ __device__ __forceinline__ void
 // device) runs a bounded grid that strides over the list, so an empty list
 // costs a few thousand empty blocks, not one per row of the operator.
 constexpr int kListBlocks = 2048;
+constexpr int kScreenBlocks = 16384;
 
 template <int M, int H, bool F32, bool STORE, bool LIST>
 __global__ void __launch_bounds__( kBlock )
@@ -363,9 +363,20 @@ __global__ void __launch_bounds__( kBlock )
         // the trip count is uniform over the block (the pair shuffles and
         // the wave-wide convergence vote need every lane); the loop costs
         // registers, so the whole-operator form below has none
+        // (a list's rows sit in its first tiles and are scattered over the
+        // operator: tiles in dispatch order spread them over the XCDs)
         int64_t const n = *n_rows;
-        for ( int64_t tile = xcd_tile(); tile * SPB < n; tile += gridDim.x )
+        for ( int64_t tile = blockIdx.x; tile * SPB < n; tile += gridDim.x )
             one( tile );
+    }
+    else if ( rows )
+    {
+        // a row list over a grid for its whole capacity: blocks past the
+        // device-side count return at once
+        int64_t const tile = blockIdx.x;
+        if ( tile * SPB >= *n_rows )
+            return;
+        one( tile );
     }
     else
         one( xcd_tile() );
@@ -378,15 +389,20 @@ void launch_cyclic( double4 const *src, double4 const *tgt, int64_t n_tgt,
                     double *pinv, unsigned long long *counters, hipStream_t s )
 {
     int const sweeps = p.max_rot > 0 ? static_cast<int>( p.max_rot ) : 30;
+    // A screened list (the rows the eigenvalue screen found truncated, a
+    // few percent of the listed rows) runs without the strided loop, whose
+    // registers spilled (364 bytes a lane for m = 10), over a grid for the
+    // list's whole capacity; other lists stride over a bounded grid.
+    bool const strided = p.rows && !p.screened;
     int64_t blocks = ceil_div( n_tgt - p.row0, kBlock / H );
-    if ( p.rows )
+    if ( strided )
         blocks = blocks < kListBlocks ? blocks : kListBlocks;
     dim3 const grid( blocks ), block( kBlock );
     auto run = [&]( auto f32, auto store ) {
-        auto kernel = p.rows ? setup_cyclic_kernel<M, H, decltype( f32 )::value,
-                                                   decltype( store )::value, true>
-                             : setup_cyclic_kernel<M, H, decltype( f32 )::value,
-                                                   decltype( store )::value, false>;
+        auto kernel = strided ? setup_cyclic_kernel<M, H, decltype( f32 )::value,
+                                                    decltype( store )::value, true>
+                              : setup_cyclic_kernel<M, H, decltype( f32 )::value,
+                                                    decltype( store )::value, false>;
         kernel<<<grid, block, 0, s>>>( src, tgt, n_tgt, indices, order, p.row0,
                                      p.row_off, p.rows, p.n_rows, p.kk,
                                      p.rbf, p.geometry, p.truncation, p.rcond,
@@ -432,9 +448,10 @@ __global__ void __launch_bounds__( kBlock )
                          int32_t *__restrict__ n_full )
 {
     using S = SymIdx<M>;
-    // the trip count is uniform over the block (the wave-wide vote)
+    // the trip count is uniform over the block (the wave-wide vote); tiles
+    // in dispatch order spread the list's first tiles over the XCDs
     int64_t const n = *n_rows;
-    for ( int64_t tile = xcd_tile(); tile * kBlock < n; tile += gridDim.x )
+    for ( int64_t tile = blockIdx.x; tile * kBlock < n; tile += gridDim.x )
     {
         int64_t const li = tile * kBlock + threadIdx.x;
         bool const valid = li < n;
@@ -483,8 +500,10 @@ void setup_screen( double4 const *src, double4 const *tgt, int64_t n_tgt,
     if ( n_tgt <= p.row0 )
         return;
     int const sweeps = p.max_rot > 0 ? static_cast<int>( p.max_rot ) : 30;
+    // one wave a block and up to three waves a SIMD: a grid of 2,048
+    // blocks would leave the SIMDs two waves each
     int64_t blocks = ceil_div( n_tgt - p.row0, kBlock );
-    blocks = blocks < kListBlocks ? blocks : kListBlocks;
+    blocks = blocks < kScreenBlocks ? blocks : kScreenBlocks;
     auto run = [&]( auto m, auto f32 ) {
         eigen_screen_kernel<decltype( m )::value, decltype( f32 )::value>
             <<<blocks, kBlock, 0, s>>>(

@@ -265,7 +265,7 @@ __global__ void __launch_bounds__( kBlock )
 {
     using S = SymIdx<M>;
     int64_t const n = *n_rows;
-    for ( int64_t li = xcd_tile() * kBlock + threadIdx.x; li < n;
+    for ( int64_t li = blockIdx.x * (int64_t)kBlock + threadIdx.x; li < n;
           li += (int64_t)gridDim.x * kBlock )
     {
         int64_t const slot = rows[li];
@@ -402,42 +402,52 @@ void setup_fast( double4 const *src, double4 const *tgt, int64_t n_tgt,
     }
     // otherwise their eigenvalues first: the rows with none below the
     // threshold have A^+ = A^-1 and are solved again by Cholesky, the
-    // others (and any whose Cholesky meets a non-positive pivot) by the
-    // eigen-solver with eigenvectors; each row ends as the eigen-solver
-    // alone would leave it followed by the Cholesky of its untruncated rows
+    // others by the eigen-solver with eigenvectors, on a forked stream at
+    // the same time (disjoint rows); rows whose Cholesky meets a
+    // non-positive pivot go to the eigen-solver afterwards. Each row ends
+    // as it did when the eigen-solver ran over the whole list before the
+    // Cholesky of its untruncated rows.
     int64_t const nl = n_tgt - p.row0;
-    DeviceBuffer<int32_t> rlist( nl, s ), flist( nl, s ), counts( 2, s );
-    zero_async( counts.get(), 2 * sizeof( int32_t ), s );
+    DeviceBuffer<int32_t> rlist( nl, s ), tlist( nl, s ), flist( nl, s ),
+        counts( 3, s ); // refine, truncated, refine failures
+    zero_async( counts.get(), 3 * sizeof( int32_t ), s );
     q.refine_rows = rlist.get();
     q.n_refine = counts.get();
     setup_screen( src, tgt, n_tgt, indices, order, q, coeffs32 != nullptr,
-                  flist.get(), counts.get() + 1, counters, s );
-    int64_t const blocks =
-        std::min<int64_t>( ceil_div( n_tgt - p.row0, kBlock ), kRefineBlocks );
-    auto refine = [&]( auto m, auto f32 ) {
-        refine_kernel<decltype( m )::value, decltype( f32 )::value>
-            <<<blocks, kBlock, 0, s>>>( src, tgt, indices, order, p.row_off,
-                                         p.kk, p.rbf, p.geometry, rlist.get(),
-                                         counts.get(), coeffs, coeffs32,
-                                         p.row_max, flist.get(),
-                                         counts.get() + 1 );
-    };
-    using T = std::true_type;
-    using F = std::false_type;
-    using M1 = std::integral_constant<int, 1>;
-    using M4 = std::integral_constant<int, 4>;
-    using M10 = std::integral_constant<int, 10>;
-    if ( p.m == 1 )
-        coeffs32 ? refine( M1{}, T{} ) : refine( M1{}, F{} );
-    else if ( p.m == 4 )
-        coeffs32 ? refine( M4{}, T{} ) : refine( M4{}, F{} );
-    else
-        coeffs32 ? refine( M10{}, T{} ) : refine( M10{}, F{} );
-    DTKX_HIP( hipGetLastError() );
+                  tlist.get(), counts.get() + 1, counters, s );
     SetupParams f = p;
-    f.rows = flist.get();
-    f.n_rows = counts.get() + 1;
     f.screened = true;
+    {
+        Fork fork( s );
+        f.rows = tlist.get();
+        f.n_rows = counts.get() + 1;
+        setup_cyclic( src, tgt, n_tgt, indices, order, f, coeffs, coeffs32,
+                      nullptr, counters, fork.side() );
+        int64_t const blocks = std::min<int64_t>(
+            ceil_div( n_tgt - p.row0, kBlock ), kRefineBlocks );
+        auto refine = [&]( auto m, auto f32 ) {
+            refine_kernel<decltype( m )::value, decltype( f32 )::value>
+                <<<blocks, kBlock, 0, s>>>(
+                    src, tgt, indices, order, p.row_off, p.kk, p.rbf,
+                    p.geometry, rlist.get(), counts.get(), coeffs, coeffs32,
+                    p.row_max, flist.get(), counts.get() + 2 );
+        };
+        using T = std::true_type;
+        using F = std::false_type;
+        using M1 = std::integral_constant<int, 1>;
+        using M4 = std::integral_constant<int, 4>;
+        using M10 = std::integral_constant<int, 10>;
+        if ( p.m == 1 )
+            coeffs32 ? refine( M1{}, T{} ) : refine( M1{}, F{} );
+        else if ( p.m == 4 )
+            coeffs32 ? refine( M4{}, T{} ) : refine( M4{}, F{} );
+        else
+            coeffs32 ? refine( M10{}, T{} ) : refine( M10{}, F{} );
+        DTKX_HIP( hipGetLastError() );
+        fork.join();
+    }
+    f.rows = flist.get();
+    f.n_rows = counts.get() + 2;
     setup_cyclic( src, tgt, n_tgt, indices, order, f, coeffs, coeffs32,
                   nullptr, counters, s );
 }
