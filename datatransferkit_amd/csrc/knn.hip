@@ -1345,8 +1345,14 @@ __device__ inline void query2_row( QueryArgs<T> const &a, int64_t const qi,
             row[j] = slot( j );
 }
 
+// Registers for four waves per SIMD (128): the float K = 32 kernel then
+// spills 52 bytes a lane, and its LDS stack admits 14 waves per CU against
+// the 12 that 135 registers allowed: 1.92 ms against 2.02 for the 1M x 1M,
+// k = 32 query. (Kernels whose LDS already allows fewer waves, the double
+// ones, keep their registers.)
 template <typename T, int K, int STACK>
 __global__ void __launch_bounds__( kQueryBlock )
+    __attribute__( ( amdgpu_waves_per_eu( 4 ) ) )
     knn_query2_kernel( QueryArgs<T> const a )
 {
     constexpr int LS = STACK > 0 ? STACK : 1;
