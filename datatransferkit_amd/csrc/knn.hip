@@ -673,6 +673,16 @@ constexpr bool kSeed = DTKX_KNN_SEED != 0;
 #define DTKX_KNN_LDS_STACK 22
 #endif
 constexpr int kLdsStack = DTKX_KNN_LDS_STACK;
+// The one-pass search (K below two_pass_k) keeps no row in its stack's
+// slots, so its stack could be shorter: 16 entries took 0.63 ms against
+// 0.72 at k = 10 and 0.90 against 1.00 at k = 16 (1M x 1M uniform), but on
+// 1M points in 200 clusters (sigma 1e-3) 15 and 65 queries overflowed and
+// their re-run's serial tail made the search 0.86 against 0.73 and 1.23
+// against 0.98 ms; 22 overflows none there.
+#ifndef DTKX_KNN_LDS_STACK1
+#define DTKX_KNN_LDS_STACK1 22
+#endif
+constexpr int kLdsStack1 = DTKX_KNN_LDS_STACK1;
 // stack entries of the fixed-radius pass of a two-pass search when its row
 // is collected in the LDS stack's other slots (its radius is final, so it
 // descends far less deep than the first pass); the row needs
@@ -1415,7 +1425,7 @@ struct QueryLauncher
                 knn_sort_rows_kernel<T, K><<<grid, block, 0, s>>>( a );
         }
         else if ( stack == kLdsStack )
-            knn_query_kernel<T, K, kLdsStack><<<grid, block, 0, s>>>( a );
+            knn_query_kernel<T, K, kLdsStack1><<<grid, block, 0, s>>>( a );
         else if ( stack == 0 )
             knn_query_kernel<T, K, 0><<<grid, block, 0, s>>>( a );
         DTKX_HIP( hipGetLastError() );
