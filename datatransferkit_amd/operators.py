@@ -20,6 +20,7 @@ CPU path.
 from __future__ import annotations
 
 import ctypes
+import sys
 
 import numpy as np
 
@@ -50,6 +51,40 @@ def _check_points(p, name):
     if _dtype_name(p) != "float64":
         raise DtkxError(_lib.ERR_INVALID_ARGUMENT, f"{name} must be float64")
     return shape
+
+
+def _torch_float64():
+    """torch.float64, or None when torch has not been imported (then no
+    argument is a torch tensor)."""
+    torch = sys.modules.get("torch")
+    return None if torch is None else torch.float64
+
+
+def _point_address(p, name, f64):
+    """(address, rows, keep-alive) of one [n, 3] float64 point array of a
+    batch, with the checks of _check_points and _ptr; f64 =
+    _torch_float64(). A contiguous float64 torch tensor takes four
+    attribute reads, which is what a batch of thousands of problems spends
+    its Python time on."""
+    if f64 is not None and getattr(p, "dtype", None) is f64 and p.is_contiguous():
+        shape = p.shape
+        if len(shape) == 2 and shape[1] == 3:
+            return p.data_ptr(), shape[0], p
+    shape = _check_points(p, name)
+    if shape[1] != 3:
+        raise DtkxError(_lib.ERR_DIMENSION, f"{name} must be [n, 3]")
+    ptr, keep = _ptr(p)
+    return ptr.value or 0, shape[0], keep
+
+
+# numpy image of _lib.Problem: a batch's problem records are filled column
+# by column and handed to the library as one buffer
+_PROBLEM_DTYPE = np.dtype({
+    "names": [f for f, _ in _lib.Problem._fields_],
+    "formats": [np.uint64, np.int64, np.uint64, np.int64, np.int32, np.int32],
+    "offsets": [getattr(_lib.Problem, f).offset for f, _ in _lib.Problem._fields_],
+    "itemsize": ctypes.sizeof(_lib.Problem),
+})
 
 
 def _fields(vals, n, name):
@@ -211,6 +246,24 @@ class MovingLeastSquaresOperator:
         return offsets, indices, coeffs
 
 
+def _batch_options(rbf, precision, geometry, solver, truncation, rcond,
+                   max_rotations, group_size, stream):
+    """The dtkx_mls_options of a batch (each problem names its own basis
+    and k)."""
+    o = _lib.default_options()
+    o.basis = _lib.BASIS["linear"]
+    o.rbf = _lib.RBF[rbf]
+    o.precision = _lib.PRECISION[precision]
+    o.geometry = _lib.GEOMETRY[geometry]
+    o.solver = _lib.SOLVER[solver]
+    o.truncation = _lib.TRUNCATION[truncation]
+    o.rcond = float(rcond)
+    o.max_rotations = int(max_rotations)
+    o.group_size = int(group_size)
+    o.stream = ctypes.c_void_p(stream) if stream else None
+    return o
+
+
 class MovingLeastSquaresBatch:
     """Many independent MLS operators built together: one segmented LBVH
     build and one query launch per kernel size for all problems, one setup
@@ -221,38 +274,96 @@ class MovingLeastSquaresBatch:
     MovingLeastSquaresOperator (group_size 0 lets the classical solver pick
     its lanes per system by timing). The point tensors are held by the
     batch: rebuild() reads them again after the caller has moved the points
-    in place."""
+    in place. from_concatenated() builds the same batch from points held in
+    two arrays."""
 
     def __init__(self, problems, *, rbf="wendland0", precision="fp64",
                  geometry="float", solver="classical", truncation="absolute",
                  rcond=1e-12, max_rotations=0, group_size=0, stream=None):
-        o = _lib.default_options()
-        o.basis = _lib.BASIS["linear"]
-        o.rbf = _lib.RBF[rbf]
-        o.precision = _lib.PRECISION[precision]
-        o.geometry = _lib.GEOMETRY[geometry]
-        o.solver = _lib.SOLVER[solver]
-        o.truncation = _lib.TRUNCATION[truncation]
-        o.rcond = float(rcond)
-        o.max_rotations = int(max_rotations)
-        o.group_size = int(group_size)
-        o.stream = ctypes.c_void_p(stream) if stream else None
-        n = len(problems)
-        arr = (_lib.Problem * n)()
-        self._keep = []
-        self._sizes = []
-        for i, (src, tgt, basis, k) in enumerate(problems):
-            ns, _ = _check_points(src, "source_points")
-            nt, _ = _check_points(tgt, "target_points")
-            sp, ka = _ptr(src)
-            tp, kb = _ptr(tgt)
-            self._keep += [src, tgt]
-            arr[i].src_xyz, arr[i].n_src = sp.value, ns
-            arr[i].tgt_xyz, arr[i].n_tgt = tp.value, nt
-            arr[i].basis, arr[i].k = _lib.BASIS[basis], int(k)
-            self._sizes.append((ns, nt))
+        f64 = _torch_float64()
+        bases = _lib.BASIS
+        sp, ns, tp, nt, basis_ids, ks, keep = [], [], [], [], [], [], []
+        for src, tgt, basis, k in problems:
+            a, rows, kp = _point_address(src, "source_points", f64)
+            sp.append(a)
+            ns.append(rows)
+            keep.append(kp)
+            a, rows, kp = _point_address(tgt, "target_points", f64)
+            tp.append(a)
+            nt.append(rows)
+            keep.append(kp)
+            basis_ids.append(bases[basis])
+            ks.append(int(k))
+        self._create(sp, ns, tp, nt, basis_ids, ks, keep, _batch_options(
+            rbf, precision, geometry, solver, truncation, rcond, max_rotations,
+            group_size, stream))
         self.precision = precision
+
+    @classmethod
+    def from_concatenated(cls, source_points, target_points, n_source, n_target,
+                          basis, k, *, rbf="wendland0", precision="fp64",
+                          geometry="float", solver="classical",
+                          truncation="absolute", rcond=1e-12, max_rotations=0,
+                          group_size=0, stream=None):
+        """The batch of the problems whose points are concatenated in
+        problem order: source_points [sum n_source, 3] and target_points
+        [sum n_target, 3] (float64 CUDA tensors), problem p owning
+        n_source[p] sources and n_target[p] targets; basis and k are one
+        value for every problem or one per problem. The same batch as the
+        constructor's over the per-problem slices, formed without a Python
+        step per problem."""
+        f64 = _torch_float64()
+        n_source = np.asarray(n_source, dtype=np.int64).reshape(-1)
+        n_target = np.asarray(n_target, dtype=np.int64).reshape(-1)
+        n = len(n_source)
+        if len(n_target) != n:
+            raise DtkxError(_lib.ERR_SIZE_MISMATCH, "n_source and n_target lengths")
+        if np.any(n_source < 0) or np.any(n_target < 0):
+            raise DtkxError(_lib.ERR_INVALID_ARGUMENT, "negative point count")
+        addr = []
+        for pts, counts, name in ((source_points, n_source, "source_points"),
+                                  (target_points, n_target, "target_points")):
+            a, rows, _ = _point_address(pts, name, f64)
+            if rows != int(counts.sum()):
+                raise DtkxError(_lib.ERR_SIZE_MISMATCH,
+                                f"{name}.extent(0) == {int(counts.sum())} (got {rows})")
+            off = np.zeros(n, np.uint64)
+            np.cumsum(counts[:-1], out=off[1:])
+            # a problem without points may point one past the end (never read)
+            addr.append(off * np.uint64(24) + np.uint64(a))
+        bases = [_lib.BASIS[basis]] * n if isinstance(basis, str) else \
+            [_lib.BASIS[b] for b in basis]
+        ks = np.broadcast_to(np.asarray(k, dtype=np.int64), (n,))
+        if len(bases) != n:
+            raise DtkxError(_lib.ERR_SIZE_MISMATCH, "one basis per problem")
+        self = cls.__new__(cls)
+        self._create(addr[0], n_source, addr[1], n_target, bases, ks,
+                     [source_points, target_points], _batch_options(
+                         rbf, precision, geometry, solver, truncation, rcond,
+                         max_rotations, group_size, stream))
+        self.precision = precision
+        return self
+
+    def _create(self, sp, ns, tp, nt, basis_ids, ks, keep, o):
+        n = len(ns)
+        rec = np.zeros(n, dtype=_PROBLEM_DTYPE)
+        rec["src_xyz"], rec["n_src"] = sp, ns
+        rec["tgt_xyz"], rec["n_tgt"] = tp, nt
+        rec["basis"], rec["k"] = basis_ids, ks
+        self._keep = keep
+        self._ns = rec["n_src"].tolist()
+        self._nt = rec["n_tgt"].tolist()
+        # byte offsets of each problem's values in concatenated arrays
+        self._x_off = np.zeros(n, np.uint64)
+        self._y_off = np.zeros(n, np.uint64)
+        np.cumsum(rec["n_src"][:-1], out=self._x_off[1:])
+        np.cumsum(rec["n_tgt"][:-1], out=self._y_off[1:])
+        self._x_off *= np.uint64(8)
+        self._y_off *= np.uint64(8)
+        self._n_src_total = int(rec["n_src"].sum())
+        self._n_tgt_total = int(rec["n_tgt"].sum())
         h = ctypes.c_void_p()
+        arr = (_lib.Problem * n).from_buffer(rec) if n else None
         check(_lib.load().dtkx_mls_batch_create(n, arr, ctypes.byref(o), ctypes.byref(h)))
         self._h = h
 
@@ -268,19 +379,48 @@ class MovingLeastSquaresBatch:
     def apply(self, source_values, target_values):
         """One field per problem: target_values[p] = C_p source_values[p]
         (float64 CUDA tensors of the problems' sizes)."""
-        n = len(self._sizes)
+        n = len(self._ns)
         if len(source_values) != n or len(target_values) != n:
             raise DtkxError(_lib.ERR_SIZE_MISMATCH, "one array per problem")
-        xs = (ctypes.c_void_p * n)()
-        ys = (ctypes.c_void_p * n)()
-        for i, ((ns, nt), x, y) in enumerate(zip(self._sizes, source_values, target_values)):
+        f64 = _torch_float64()
+        xs, ys = [], []
+        for i, ns, nt, x, y in zip(range(n), self._ns, self._nt,
+                                   source_values, target_values):
             if x.shape[-1] != ns or y.shape[-1] != nt:
                 raise DtkxError(_lib.ERR_SIZE_MISMATCH, f"problem {i}: value sizes")
-            if _dtype_name(x) != "float64" or _dtype_name(y) != "float64":
+            if (x.dtype is not f64 or y.dtype is not f64) and (
+                    _dtype_name(x) != "float64" or _dtype_name(y) != "float64"):
                 raise DtkxError(_lib.ERR_INVALID_ARGUMENT, "values must be float64")
-            xs[i] = x.data_ptr()
-            ys[i] = y.data_ptr()
-        check(_lib.load().dtkx_mls_batch_apply(self._h, xs, ys))
+            if not (x.is_contiguous() and y.is_contiguous()):
+                raise DtkxError(_lib.ERR_INVALID_ARGUMENT,
+                                f"problem {i}: values must be contiguous")
+            xs.append(x.data_ptr())
+            ys.append(y.data_ptr())
+        xs = np.array(xs, dtype=np.uint64)
+        ys = np.array(ys, dtype=np.uint64)
+        check(_lib.load().dtkx_mls_batch_apply(self._h, xs.ctypes.data, ys.ctypes.data))
+        return target_values
+
+    def apply_concatenated(self, source_values, target_values):
+        """apply() with the problems' values concatenated in problem order:
+        source_values [sum n_src] and target_values [sum n_tgt], contiguous
+        float64 CUDA tensors, problem p's values at offset sum_{q<p} n_q.
+        The same library call as apply(); its per-problem pointers are
+        formed in one array operation, so the host cost of a call does not
+        grow with the number of problems."""
+        for v, total, name in ((source_values, self._n_src_total, "source_values"),
+                               (target_values, self._n_tgt_total, "target_values")):
+            if tuple(v.shape) != (total,):
+                raise DtkxError(_lib.ERR_SIZE_MISMATCH,
+                                f"{name} must be [{total}] (got {tuple(v.shape)})")
+            if _dtype_name(v) != "float64":
+                raise DtkxError(_lib.ERR_INVALID_ARGUMENT, f"{name} must be float64")
+            if not v.is_contiguous():
+                raise DtkxError(_lib.ERR_INVALID_ARGUMENT, f"{name} must be contiguous")
+        xs = self._x_off + np.uint64(source_values.data_ptr())
+        # a problem with no targets may point one past the end (never read)
+        ys = self._y_off + np.uint64(target_values.data_ptr())
+        check(_lib.load().dtkx_mls_batch_apply(self._h, xs.ctypes.data, ys.ctypes.data))
         return target_values
 
     def rebuild(self):

@@ -72,6 +72,29 @@ int main(void) {
     assert int(a[5]) == ctypes.sizeof(_lib.Problem)
 
 
+def test_batch_problem_records_read_back_through_ctypes():
+    # the batch wrapper fills dtkx_mls_problem records as a numpy structured
+    # array: every field must land where the ctypes struct (checked against
+    # the header above) reads it
+    import numpy as np
+    from datatransferkit_amd import _lib, operators
+
+    n = 3
+    rec = np.zeros(n, dtype=operators._PROBLEM_DTYPE)
+    rec["src_xyz"] = [0x1000, 0x7FFF_FFFF_F000, 0xDEAD_BEE0]
+    rec["n_src"] = [1, 2**40, 7]
+    rec["tgt_xyz"] = [0x2000, 0x3000, 0xFFFF_FFFF_FFFF_FFF0]
+    rec["n_tgt"] = [5, 6, 2**33]
+    rec["basis"] = [1, 4, 10]
+    rec["k"] = [-1, 32, 2**31 - 1]
+    arr = (_lib.Problem * n).from_buffer(rec)
+    assert operators._PROBLEM_DTYPE.itemsize == ctypes.sizeof(_lib.Problem)
+    for i in range(n):
+        p = arr[i]
+        assert (p.src_xyz, p.n_src, p.tgt_xyz, p.n_tgt, p.basis, p.k) == tuple(
+            int(rec[f][i]) for f in ("src_xyz", "n_src", "tgt_xyz", "n_tgt", "basis", "k"))
+
+
 def test_options_defaults_are_reference_semantics(lib):
     from datatransferkit_amd import _lib
 

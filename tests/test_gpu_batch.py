@@ -121,6 +121,92 @@ def test_graph_replay_reads_new_values(dtk):
             assert torch.equal(y, ref)
 
 
+def test_apply_concatenated_equals_apply(dtk):
+    """Values concatenated over the problems give the bits of the per-problem
+    apply, with problems without targets inside and at the end of the batch
+    (their pointer lands on the next problem's values or one past the end,
+    and nothing is written there)."""
+    probs = make_problems(20, seed=19)
+    probs.append((probs[0][0], np.zeros((0, 3)), "linear", 4, probs[0][4]))
+    assert sum(len(t) == 0 for _, t, *_ in probs) >= 2 and len(probs[-1][1]) == 0
+    dev = [(cuda(s), cuda(t), b, k) for s, t, b, k, _ in probs]
+    batch = dtk.MovingLeastSquaresBatch(dev, rbf="wendland2", solver="cholesky",
+                                        precision="fp32_bulk")
+    xs = [cuda(v) for *_, v in probs]
+    ys = [torch.zeros(len(t), dtype=torch.float64, device="cuda") for _, t, *_ in probs]
+    batch.apply(xs, ys)
+    x_flat = torch.cat(xs)
+    # one sentinel past the targets: the last problem has none, so its
+    # pointer is one past y_flat's end, where nothing may be written
+    y_buf = torch.full((sum(len(y) for y in ys) + 1,), 7.0, dtype=torch.float64,
+                       device="cuda")
+    y_flat = y_buf[:-1]
+    for _ in range(2):
+        batch.apply_concatenated(x_flat, y_flat)
+        torch.cuda.synchronize()
+        assert torch.equal(y_flat, torch.cat(ys))
+        assert y_buf[-1].item() == 7.0
+    with pytest.raises(dtk.DtkxError) as e:
+        batch.apply_concatenated(x_flat[:-1], y_flat)
+    assert e.value.status == 2
+    with pytest.raises(dtk.DtkxError) as e:
+        batch.apply_concatenated(x_flat, y_flat.float())
+    assert e.value.status == 1
+    with pytest.raises(dtk.DtkxError) as e:
+        batch.apply_concatenated(torch.stack([x_flat, x_flat], 1)[:, 0], y_flat)
+    assert e.value.status == 1
+
+
+def test_from_concatenated_equals_constructor(dtk):
+    """The batch built from concatenated points is the constructor's batch
+    over the per-problem slices, bit for bit: per-problem basis and k, one
+    shared basis and k, problems without targets inside and at the end, and
+    a rebuild after the flat points move in place."""
+    probs = make_problems(24, seed=23)
+    probs.append((probs[1][0], np.zeros((0, 3)), "quadratic", 16, probs[1][4]))
+    src = cuda(np.concatenate([s for s, *_ in probs]))
+    tgt = cuda(np.concatenate([t for _, t, *_ in probs]))
+    ns = [len(s) for s, *_ in probs]
+    nt = [len(t) for _, t, *_ in probs]
+    views = list(zip(torch.split(src, ns), torch.split(tgt, nt)))
+    x_flat = cuda(np.concatenate([v for *_, v in probs]))
+    opts = dict(rbf="wendland2", solver="cholesky", precision="fp32_bulk")
+
+    def run(batch):
+        y = torch.full((sum(nt),), np.nan, dtype=torch.float64, device="cuda")
+        return batch.apply_concatenated(x_flat, y)
+
+    for bases, ks in (([b for *_, b, _, _ in probs], [k for *_, k, _ in probs]),
+                      ("linear", 16)):
+        each = [(s, t, bases if isinstance(bases, str) else bases[i],
+                 ks if isinstance(ks, int) else ks[i]) for i, (s, t) in enumerate(views)]
+        ref = dtk.MovingLeastSquaresBatch(each, **opts)
+        flat = dtk.MovingLeastSquaresBatch.from_concatenated(src, tgt, ns, nt, bases, ks,
+                                                             **opts)
+        assert torch.equal(run(flat), run(ref))
+        assert flat.stats()["nnz"] == ref.stats()["nnz"]
+    # move the points in place: a rebuild of the flat batch reads them again
+    src.mul_(1.5).add_(0.25)
+    tgt.mul_(1.5).add_(0.25)
+    torch.cuda.synchronize()
+    flat.rebuild()
+    assert torch.equal(run(flat), run(dtk.MovingLeastSquaresBatch(each, **opts)))
+    with pytest.raises(dtk.DtkxError) as e:
+        dtk.MovingLeastSquaresBatch.from_concatenated(src, tgt, ns[:-1], nt, "linear", 4)
+    assert e.value.status == 2
+    with pytest.raises(dtk.DtkxError) as e:
+        dtk.MovingLeastSquaresBatch.from_concatenated(src, tgt[:-1], ns, nt, "linear", 4)
+    assert e.value.status == 2
+    with pytest.raises(dtk.DtkxError) as e:
+        dtk.MovingLeastSquaresBatch.from_concatenated(src[:, :2].contiguous(), tgt, ns, nt,
+                                                      "linear", 4)
+    assert e.value.status == 3
+    with pytest.raises(dtk.DtkxError) as e:
+        dtk.MovingLeastSquaresBatch([(views[0][0][:, :2].contiguous(), views[0][1],
+                                      "linear", 4)])
+    assert e.value.status == 3
+
+
 def deep_chain_problem():
     """Sources whose hierarchy is deeper than the query's LDS stack (pairs
     halving their distance to the origin, then 65,536 copies of it), with
