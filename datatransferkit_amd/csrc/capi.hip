@@ -1933,6 +1933,11 @@ int dtkx_mls_batch_apply( const dtkx_mls_batch *b,
             REQ( src_vals[p] != nullptr &&
                      ( tgt_vals[p] != nullptr || b->tgt_count[p] == 0 ),
                  DTKX_ERR_INVALID_ARGUMENT );
+        // host arrays would fault the gather: the first problem's pair is
+        // checked (a runtime query per pointer, too slow for all of them)
+        REQ( dtkx::is_device_pointer( src_vals[0] ), DTKX_ERR_INVALID_ARGUMENT );
+        REQ( b->tgt_count[0] == 0 || dtkx::is_device_pointer( tgt_vals[0] ),
+             DTKX_ERR_INVALID_ARGUMENT );
         hipStream_t const s = b->stream;
         DTKX_HIP( hipMemcpyAsync( b->xptr.get(), src_vals,
                                   sizeof( double * ) * b->n_prob,

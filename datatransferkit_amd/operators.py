@@ -394,6 +394,9 @@ class MovingLeastSquaresBatch:
             if not (x.is_contiguous() and y.is_contiguous()):
                 raise DtkxError(_lib.ERR_INVALID_ARGUMENT,
                                 f"problem {i}: values must be contiguous")
+            if not (x.is_cuda and y.is_cuda):
+                raise DtkxError(_lib.ERR_INVALID_ARGUMENT,
+                                f"problem {i}: values must be CUDA tensors")
             xs.append(x.data_ptr())
             ys.append(y.data_ptr())
         xs = np.array(xs, dtype=np.uint64)
@@ -417,6 +420,8 @@ class MovingLeastSquaresBatch:
                 raise DtkxError(_lib.ERR_INVALID_ARGUMENT, f"{name} must be float64")
             if not v.is_contiguous():
                 raise DtkxError(_lib.ERR_INVALID_ARGUMENT, f"{name} must be contiguous")
+            if not v.is_cuda:
+                raise DtkxError(_lib.ERR_INVALID_ARGUMENT, f"{name} must be a CUDA tensor")
         xs = self._x_off + np.uint64(source_values.data_ptr())
         # a problem with no targets may point one past the end (never read)
         ys = self._y_off + np.uint64(target_values.data_ptr())

@@ -329,7 +329,8 @@ int dtkx_mls_batch_create(int32_t n_problems, const dtkx_mls_problem *problems,
                           const dtkx_mls_options *opts, dtkx_mls_batch **out);
 /* One field per problem: src_vals[p] (n_src_p doubles) and tgt_vals[p]
  * (n_tgt_p doubles) are device pointers in host arrays of n_problems
- * entries. The gather and row sums of all problems are captured in a HIP
+ * entries (the first problem's pair is checked, DTKX_ERR_INVALID_ARGUMENT
+ * for host memory; the others are not). The gather and row sums of all problems are captured in a HIP
  * graph on the first call and replayed; synchronous at return. */
 int dtkx_mls_batch_apply(const dtkx_mls_batch *b,
                          const double *const *src_vals,

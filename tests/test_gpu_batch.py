@@ -155,6 +155,21 @@ def test_apply_concatenated_equals_apply(dtk):
     with pytest.raises(dtk.DtkxError) as e:
         batch.apply_concatenated(torch.stack([x_flat, x_flat], 1)[:, 0], y_flat)
     assert e.value.status == 1
+    # host values are refused before any work is queued
+    with pytest.raises(dtk.DtkxError) as e:
+        batch.apply_concatenated(x_flat.cpu(), y_flat)
+    assert e.value.status == 1
+    with pytest.raises(dtk.DtkxError) as e:
+        batch.apply([x.cpu() for x in xs], ys)
+    assert e.value.status == 1
+    # the C ABI checks the first problem's pair itself: a CPU tensor, the
+    # memory the create's point check refuses in test_batch_errors
+    from datatransferkit_amd import _lib
+
+    host = torch.zeros(len(xs[0]), dtype=torch.float64)
+    xp = np.array([host.data_ptr()] + [x.data_ptr() for x in xs[1:]], dtype=np.uint64)
+    yp = np.array([y.data_ptr() for y in ys], dtype=np.uint64)
+    assert _lib.load().dtkx_mls_batch_apply(batch._h, xp.ctypes.data, yp.ctypes.data) == 1
 
 
 def test_from_concatenated_equals_constructor(dtk):
