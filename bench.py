@@ -747,6 +747,74 @@ def bench_nn(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_knn(args):
+    """The k-nearest-neighbour search on its own (the reference's ArborX
+    build and query, DTK_MovingLeastSquaresOperator_def.hpp:47-60): per
+    step, the hierarchy over n float sources and the query of n targets,
+    each row ascending by (distance, index); rows of more than 32
+    neighbours are collected unordered and sorted by a kernel of their
+    own. A sample of rows is checked against the oracle's k-d tree search,
+    which is also timed on the host as the baseline."""
+    import numpy as np
+    import torch
+
+    import datatransferkit_amd as dtk
+
+    n = args.n
+    rng = np.random.default_rng(11)
+    src = rng.random((n, 3))
+    tgt = 0.05 + 0.9 * rng.random((n, 3))
+    d_src, d_tgt = (torch.from_numpy(a).cuda() for a in (src, tgt))
+    stream = torch.cuda.current_stream().cuda_stream
+    ks = [int(v) for v in args.knn_k.split(",")]
+    per_k, rows = {}, {}
+    for k in ks:
+        for _ in range(args.warmup):
+            dtk.knn(d_src, d_tgt, k, stream=stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            idx = dtk.knn(d_src, d_tgt, k, stream=stream)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.steps
+        rows[k] = idx
+        per_k[str(k)] = {"value": n / dt, "ms_per_step": 1000 * dt,
+                         "algorithmic_bytes": b_knn(n, n, k),
+                         "algorithmic_gbs": round(b_knn(n, n, k) / dt / 1e9, 1)}
+    first = per_k[str(ks[0])]
+    out = {
+        "metric": "kNN search target pts/s (build + query)",
+        "value": first["value"], "unit": "target pts/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": first["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 search", "data": "synthetic",
+        "config": {"workload": f"kNN {n} sources U[0,1]^3 -> {n} targets "
+                               f"U[0.05,0.95]^3, k in {ks} (value at k = {ks[0]})",
+                   "n": n, "k": ks},
+        "per_k": per_k,
+    }
+    if not args.no_cpu_baseline:
+        import oracle
+
+        sample = min(n, args.cpu_sample or 20_000)
+        threads = min(os.cpu_count() or 1, int(os.environ.get("DTKX_CPU_THREADS", "16")))
+        oracle.set_num_threads(threads)
+        t0 = time.perf_counter()
+        gi, _ = oracle.knn(src, tgt[:sample], ks[0])
+        dtc = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": sample / dtc, "unit": "target pts/s", "cores": threads,
+            "kind": "port",
+            "sample": f"first {sample} of {n} targets, all {n} sources, k-d tree "
+                      f"search at k = {ks[0]} ({dtc:.2f} s)"}
+        same = {}
+        for k in ks:
+            g = gi if k == ks[0] else oracle.knn(src, tgt[:sample], k)[0]
+            same[str(k)] = bool(np.array_equal(rows[k][:sample].cpu().numpy(), g))
+        out["rows_equal_oracle_sample"] = same
+    print(json.dumps(out), flush=True)
+
+
 def spline_solve_bytes(n_src, info, restart=300):
     """Algorithmic HBM bytes of one GMRES solve, 8 bytes per vector entry,
     n = n_src + 4 entries. Per iteration with nv basis vectors (the newest
@@ -896,11 +964,14 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="transfer",
                     choices=["transfer", "batched", "adjoint", "shell", "nn", "spline",
-                             "copy"],
+                             "knn", "copy"],
                     help="transfer: one 1M x 1M operator (default); batched: many "
                          "uneven operators; adjoint: forward + backward; shell: "
                          "FP32 bulk on a thin shell; nn: the nearest-neighbour "
-                         "operator; spline: the RBF spline operator")
+                         "operator; spline: the RBF spline operator; knn: the "
+                         "k-nearest-neighbour search alone")
+    ap.add_argument("--knn-k", default="10,32,64",
+                    help="neighbour counts of the knn workload (the first is the value)")
     ap.add_argument("--problems", type=int, default=4096)
     ap.add_argument("--max-src", type=int, default=65536)
     args = ap.parse_args()
@@ -920,6 +991,9 @@ def main():
         return
     if args.workload == "nn":
         bench_nn(args)
+        return
+    if args.workload == "knn":
+        bench_knn(args)
         return
     if args.workload == "spline":
         bench_spline(args)
