@@ -79,7 +79,7 @@ struct dtkx_mls
     // CRS rows in the Morton order of the targets (row r: target order[r],
     // trow the inverse), columns as Morton positions of the sources
     // (position p: source perm[p]); empty buffers mean caller order
-    dtkx::DeviceBuffer<int32_t> offsets, indices, order, trow, perm, spos;
+    dtkx::DeviceBuffer<int32_t> indices, order, trow, perm, spos;
     // create only: each row's largest neighbour distance (FP32 bulk)
     dtkx::DeviceBuffer<float> row_max;
     dtkx::DeviceBuffer<double> coeffs;
@@ -97,6 +97,8 @@ struct dtkx_mls
     mutable dtkx::DeviceBuffer<double> graph_c64;
     mutable dtkx::DeviceBuffer<float> graph_c32;
     mutable bool graph_built = false;
+    // the CSR offsets of the rows (kk entries each), made on first request
+    mutable dtkx::DeviceBuffer<int32_t> graph_off;
     // The create returns without waiting for the GPU: the counters its
     // setup and search accumulate and its stage events are read on the
     // first dtkx_mls_get_stats (resolve_stats), which waits for them.
@@ -339,10 +341,6 @@ void finish_create( dtkx_mls *h,
                     dtkx::DeviceBuffer<unsigned long long> &&counters,
                     std::unique_ptr<StageEvents> &&events )
 {
-    h->offsets.resize( h->n_tgt + 1, h->stream );
-    offsets_kernel<<<dtkx::ceil_div( h->n_tgt + 1, 256 ), 256, 0,
-                     h->stream>>>( h->offsets.get(), h->n_tgt, h->kk );
-    DTKX_HIP( hipGetLastError() );
     h->stats.n_source = h->n_src;
     h->stats.n_target = h->n_tgt;
     h->stats.nnz = h->n_tgt * h->kk;
@@ -845,7 +843,18 @@ int dtkx_mls_get_graph( const dtkx_mls *h, const int32_t **offsets,
             h->graph_built = true;
         }
         if ( offsets )
-            *offsets = h->offsets.get();
+        {
+            if ( !h->graph_off.get() )
+            {
+                h->graph_off.resize( h->n_tgt + 1, h->stream );
+                offsets_kernel<<<dtkx::ceil_div( h->n_tgt + 1, 256 ), 256, 0,
+                                 h->stream>>>( h->graph_off.get(), h->n_tgt,
+                                               h->kk );
+                DTKX_HIP( hipGetLastError() );
+                DTKX_HIP( hipStreamSynchronize( h->stream ) );
+            }
+            *offsets = h->graph_off.get();
+        }
         if ( indices )
             *indices = permuted ? h->graph_idx.get() : h->indices.get();
         if ( coeffs )
