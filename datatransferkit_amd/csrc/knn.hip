@@ -762,6 +762,8 @@ struct QueryArgs
     // laid out [level][lane] (stride gstride)
     int32_t const *rows = nullptr;
     int32_t const *n_rows_dev = nullptr;
+    // nullable: the re-run adds the list's length here (one lane)
+    int32_t *n_rows_total = nullptr;
     int2 *gstack = nullptr;
     int64_t gstride = 0;
 };
@@ -919,6 +921,8 @@ __global__ void __launch_bounds__( kQueryBlock )
             // re-run of listed rows: the grid's n_rows lanes stride over
             // the list, whose length only the device knows
             int64_t const n = *a.n_rows_dev;
+            if ( a.n_rows_total && qi == 0 )
+                atomicAdd( a.n_rows_total, (int32_t)n );
             for ( int64_t i = qi; i < n; i += a.n_rows )
                 query_row<T, K, STACK>( a, qi, a.rows[i], st_node, st_d );
             return;
@@ -1375,6 +1379,8 @@ __global__ void __launch_bounds__( kQueryBlock )
             // re-run of listed rows: the grid's n_rows lanes stride over
             // the list, whose length only the device knows
             int64_t const n = *a.n_rows_dev;
+            if ( a.n_rows_total && qi == 0 )
+                atomicAdd( a.n_rows_total, (int32_t)n );
             for ( int64_t i = qi; i < n; i += a.n_rows )
                 query2_row<T, K, STACK>( a, qi, a.rows[i], st_node, st_d );
             return;
@@ -1617,13 +1623,6 @@ inline int query_bucket( int k )
 // blocks of the global-stack re-run (16k lanes, 12.7 MB of stack)
 constexpr int64_t kRerunBlocks = 256;
 
-// *total += *count, on one lane
-__global__ void add_count_kernel( int32_t const *count, int32_t *total )
-{
-    if ( threadIdx.x == 0 )
-        atomicAdd( total, *count );
-}
-
 // The query launch with the LDS stack, then the re-run of its overflow.
 template <typename T>
 void run_queries( int kmax, QueryArgs<T> qa, int64_t n_rows, KnnInfo *info,
@@ -1653,10 +1652,10 @@ void run_queries( int kmax, QueryArgs<T> qa, int64_t n_rows, KnnInfo *info,
     ra.n_rows = lanes;
     ra.gstack = gstack.get();
     ra.gstride = lanes;
-    QueryLauncher<T>::dispatch( kmax, 0, ra, s );
     if ( info && info->n_overflow_dev )
-        add_count_kernel<<<1, 64, 0, s>>>( count.get(), info->n_overflow_dev );
-    else if ( info )
+        ra.n_rows_total = info->n_overflow_dev;
+    QueryLauncher<T>::dispatch( kmax, 0, ra, s );
+    if ( info && !info->n_overflow_dev )
     {
         int32_t n_over = 0;
         DTKX_HIP( hipMemcpyAsync( &n_over, count.get(), sizeof( int32_t ),
